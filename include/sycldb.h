@@ -1,0 +1,293 @@
+/*
+ * sycldb.h -- C ABI of the MI355X-native SYCLDB physical-operator library.
+ *
+ * Drop-in boundary for the operator hot path of SYCLDB (reference files
+ * kernels/selection.hpp, kernels/projection.hpp, kernels/join.hpp,
+ * kernels/aggregation.hpp, kernels/common.hpp) and for the executor that
+ * drives them (models/transient_table.hpp, executor/ddor_executor.cpp).
+ *
+ * Conventions (replacing the reference's sycl::queue / sycl::event):
+ *   - every pointer argument of an operator is a device pointer unless the
+ *     comment says "host";
+ *   - columns are int32, selection flags are 1-byte booleans holding 0 or 1
+ *     (sizeof(bool) == 1 in the reference), sums are uint64 holding the
+ *     two's-complement int64 sum;
+ *   - `stream` is a hipStream_t (NULL = the default stream); operators are
+ *     asynchronous and ordered by the stream, as the reference's kernels are
+ *     ordered by event dependencies;
+ *   - lengths are int64 (the reference's `int col_len` is at most 2^28);
+ *   - the return value is a status: SDB_OK, or an SDB_ERR_* code whose text
+ *     sdb_last_error() returns (per thread).  No C++ exception crosses the
+ *     ABI.  Where the reference throws std::invalid_argument (unknown
+ *     arithmetic op) this returns SDB_ERR_INVALID_ARG; where it throws
+ *     std::bad_alloc this returns SDB_ERR_OOM.
+ */
+#ifndef SYCLDB_H
+#define SYCLDB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *sdb_stream; /* hipStream_t */
+
+enum sdb_status {
+    SDB_OK = 0,
+    SDB_ERR_INVALID_ARG = 1,
+    SDB_ERR_HIP = 2,
+    SDB_ERR_OOM = 3,
+    SDB_ERR_UNSUPPORTED = 4,
+    SDB_ERR_RUNTIME = 5
+};
+
+/* comp_op, kernels/selection.hpp:7-15 (same values) */
+enum sdb_comp_op { SDB_EQ = 0, SDB_NE, SDB_LT, SDB_LE, SDB_GT, SDB_GE };
+/* logical_op, kernels/selection.hpp:16-21 */
+enum sdb_logical_op { SDB_NONE = 0, SDB_AND, SDB_OR };
+/* BinaryOp, kernels/aggregation.hpp:11-17 */
+enum sdb_binop { SDB_MUL = 0, SDB_DIV, SDB_ADD, SDB_SUB };
+
+/* Text of the last error on this thread ("" if none). */
+const char *sdb_last_error(void);
+/* Library version string. */
+const char *sdb_version(void);
+
+/* get_comp_op, kernels/selection.hpp:23-39: unknown strings map to SDB_EQ. */
+int sdb_comp_op_from_string(const char *op);
+/* get_logical_op, kernels/selection.hpp:41-49: unknown / "" map to SDB_NONE. */
+int sdb_logical_op_from_string(const char *op);
+/* get_op_from_string, kernels/aggregation.hpp:37-48: -1 for an unknown op
+ * (the reference throws std::invalid_argument). */
+int sdb_binop_from_string(const char *op);
+
+/* ------------------------------------------------------------ selection */
+
+/* SelectionKernelLiteral, kernels/selection.hpp:157-186, and
+ * selection(flags, arr, op, value, parent_op, ...) 188-214:
+ *   out[i] = logical(logic, in[i], compare(cmp, col[i], value)).
+ * in == out is allowed (the single-pointer constructor, 170-172). */
+int sdb_select_lit(const int32_t *col, const uint8_t *in_flags, uint8_t *out_flags, int64_t n,
+                   int cmp, int logic, int32_t value, sdb_stream stream);
+
+/* SelectionKernelColumns, kernels/selection.hpp:127-155, and
+ * selection(flags, op1, op, op2, parent_op, ...) 216-242. */
+int sdb_select_col(const int32_t *a, const int32_t *b, const uint8_t *in_flags, uint8_t *out_flags,
+                   int64_t n, int cmp, int logic, sdb_stream stream);
+
+/* LogicalKernel, kernels/selection.hpp:106-125:
+ *   out[i] = logical(logic, in[i], local[i]). */
+int sdb_logical(const uint8_t *in_flags, const uint8_t *local_flags, uint8_t *out_flags, int64_t n,
+                int logic, sdb_stream stream);
+
+/* ----------------------------------------------------------- projection */
+
+/* FillKernel, kernels/projection.hpp:7-21: dst[i] = value. */
+int sdb_fill_i32(int32_t *dst, int32_t value, int64_t n, sdb_stream stream);
+
+/* PerformOperationKernelColumns + perform_operation(result, a[], b[], ...),
+ * kernels/aggregation.hpp:50-100: if (flags[i]) result[i] = a[i] op b[i]
+ * (int32, wrapping).  Rows whose flag is 0 are not written. */
+int sdb_binop_cc(int32_t *result, const int32_t *a, const int32_t *b, const uint8_t *flags,
+                 int64_t n, int op, sdb_stream stream);
+/* PerformOperationKernelLiteralFirst, kernels/aggregation.hpp:102-153:
+ * result[i] = a op b[i]. */
+int sdb_binop_lc(int32_t *result, int32_t a, const int32_t *b, const uint8_t *flags, int64_t n,
+                 int op, sdb_stream stream);
+/* PerformOperationKernelLiteralSecond, kernels/aggregation.hpp:155-206:
+ * result[i] = a[i] op b. */
+int sdb_binop_cl(int32_t *result, const int32_t *a, int32_t b, const uint8_t *flags, int64_t n,
+                 int op, sdb_stream stream);
+
+/* ----------------------------------------------------------------- join */
+
+/* BuildKeysHTKernel / build_keys_ht, kernels/join.hpp:17-58:
+ *   ht[key - ht_min] = flags[i] for every row (uint8 table of ht_len slots).
+ * Rows whose key falls outside [ht_min, ht_min + ht_len) are skipped (the
+ * reference writes out of bounds there). */
+int sdb_build_keys(const int32_t *keys, const uint8_t *flags, int64_t n, uint8_t *ht,
+                   int32_t ht_len, int32_t ht_min, sdb_stream stream);
+
+/* BuildKeyValsHTKernel / build_key_vals_ht, kernels/join.hpp:60-127:
+ * interleaved int32 pairs ht[2h] = flag ? 1 : 0, ht[2h+1] = vals[i] when the
+ * flag is set (the value slot is left alone otherwise). */
+int sdb_build_keyvals(const int32_t *keys, const int32_t *vals, const uint8_t *flags, int64_t n,
+                      int32_t *ht, int32_t ht_len, int32_t ht_min, sdb_stream stream);
+
+/* FilterJoinKernel / filter_join, kernels/join.hpp:129-208 (semi join):
+ *   out[i] = in[i] && min <= probe[i] <= max && ht[probe[i] - min]. */
+int sdb_probe_semi(const int32_t *probe, const uint8_t *in_flags, uint8_t *out_flags, int64_t n,
+                   const uint8_t *ht, int32_t ht_min, int32_t ht_max, sdb_stream stream);
+
+/* FullJoinKernel / full_join, kernels/join.hpp:256-355: when in[i] and the
+ * key is in range and ht[2h] == 1: val_out[i] = ht[2h+1], out[i] = 1;
+ * otherwise out[i] = 0 and val_out[i] is not written. */
+int sdb_probe_full(const int32_t *probe, int32_t *val_out, const uint8_t *in_flags,
+                   uint8_t *out_flags, int64_t n, const int32_t *ht, int32_t ht_min,
+                   int32_t ht_max, sdb_stream stream);
+
+/* ---------------------------------------------------------- aggregation */
+
+/* AggregateOperationKernel / aggregate_operation,
+ * kernels/aggregation.hpp:208-276: *acc += sum over flagged rows of
+ * (uint64)(int64)col[i].  *acc is NOT reset (the reference reduction has no
+ * initialize_to_identity). */
+int sdb_sum_flagged(const int32_t *col, const uint8_t *flags, int64_t n, uint64_t *acc,
+                    sdb_stream stream);
+
+/* GroupByAggregateKernel / group_by_aggregate (pointer API),
+ * kernels/aggregation.hpp:278-409.  `keys` and `out_keys` are HOST arrays of
+ * k (1..8) device pointers; `mins`/`maxs` are HOST arrays of k ints.  For
+ * each flagged row: slot = (sum_j (keys[j][i]-mins[j]) * prod_{l<j}
+ * (maxs[l]-mins[l]+1)) (int32 arithmetic) % prod; seen[slot] = 1 (the first
+ * writer stores the key values into out_keys[j][slot]); sums[slot] +=
+ * (uint64)(int64)agg[i].  sums/seen are accumulated into (come zeroed). */
+int sdb_groupby_dense(const int32_t *const *keys, int k, const int32_t *mins, const int32_t *maxs,
+                      const int32_t *agg, const uint8_t *flags, int64_t n, int32_t *const *out_keys,
+                      uint64_t *sums, uint32_t *seen, uint64_t prod, sdb_stream stream);
+
+/* Final flags of a group-by, kernels/aggregation.hpp:517-530 and
+ * models/transient_table.hpp:1544-1558: flags[g] = seen[g] != 0. */
+int sdb_seen_to_flags(const uint32_t *seen, uint8_t *flags, uint64_t prod, sdb_stream stream);
+
+/* count_true_flags (kernels/common.hpp:30-110): *out = number of set flags
+ * when accumulate == 0 (initialize_to_identity, common.hpp:317), or
+ * *out += that number when accumulate != 0 (the allocator overload). */
+int sdb_count_flags(const uint8_t *flags, int64_t n, uint64_t *out, int accumulate,
+                    sdb_stream stream);
+
+/* Stream compaction of flags into row ids, the device half of
+ * build_row_ids (models/transient_table.hpp:535-599).  The reference's order
+ * is unspecified; this one writes the ids ascending.  *count (device) gets
+ * the number of ids; row_ids needs room for n entries. */
+int sdb_compact(const uint8_t *flags, int64_t n, int32_t *row_ids, uint32_t *count,
+                sdb_stream stream);
+
+/* Gather of selected rows, the device half of Segment::compress_sync
+ * (models/models.hpp:1019-1035): dst[j] = src[row_ids[j]] for j < count
+ * (count read on the host by the caller). */
+int sdb_gather_i32(const int32_t *src, const int32_t *row_ids, int64_t count, int32_t *dst,
+                   sdb_stream stream);
+
+/* Per-column min and max (the load-time statistics of
+ * models/models.hpp:71-77), written to the host ints min_out and max_out.
+ * Synchronises the stream.  n must be > 0. */
+int sdb_minmax_i32(const int32_t *col, int64_t n, int32_t *min_out, int32_t *max_out,
+                   sdb_stream stream);
+
+/* ------------------------------------------------ synthetic data (seeded) */
+
+/* out[i] = lo + U(seed, stream_id, first_row + i) over [lo, hi]
+ * (SplitMix64 counter hash + multiply-shift range reduction). */
+int sdb_gen_uniform(int32_t *out, int64_t n, uint64_t seed, uint64_t stream_id, int64_t first_row,
+                    int32_t lo, int32_t hi, sdb_stream stream);
+/* out[i] = base[i] * mult + U(seed, stream_id, first_row + i) over [lo, hi]. */
+int sdb_gen_child(int32_t *out, const int32_t *base, int64_t n, uint64_t seed, uint64_t stream_id,
+                  int64_t first_row, int32_t mult, int32_t lo, int32_t hi, sdb_stream stream);
+/* out[i] = table[U(seed, stream_id, first_row + i) over [0, table_len-1]]. */
+int sdb_gen_lookup(int32_t *out, int64_t n, uint64_t seed, uint64_t stream_id, int64_t first_row,
+                   const int32_t *table, int32_t table_len, sdb_stream stream);
+/* out[i] = first_row + i + 1. */
+int sdb_gen_rowid(int32_t *out, int64_t n, int64_t first_row, sdb_stream stream);
+
+/* -------------------------------------------------------------- engine */
+
+/* The executor side: tables resident in HBM, plans run through the
+ * TransientTable / KernelBundle model (models/transient_table.hpp,
+ * models/execution.hpp) driven by execute_ddor_plan
+ * (executor/ddor_executor.cpp:22-245). */
+typedef struct sdb_engine sdb_engine;
+typedef struct sdb_result sdb_result;
+
+/* Engine options (all 0 = default). */
+typedef struct sdb_engine_options {
+    uint64_t device_budget_bytes; /* scratch budget of the device allocator
+                                     (memory_manager; default 8 GiB,
+                                     common.hpp:7) */
+    uint64_t segment_size;        /* rows per segment (default 2^28,
+                                     common.hpp:12) */
+    int disable_fusion;           /* 0 (default): each bundle runs as one
+                                     fused kernel; 1: one kernel per operator,
+                                     as the reference runs with
+                                     disable_fusion = true
+                                     (models/execution.hpp:546) */
+    int device;                   /* HIP device ordinal */
+} sdb_engine_options;
+
+int sdb_engine_create(const sdb_engine_options *opts, sdb_engine **out);
+void sdb_engine_destroy(sdb_engine *eng);
+/* Stream the engine launches on (hipStream_t). */
+sdb_stream sdb_engine_stream(sdb_engine *eng);
+
+/* Registers table `name` (one of the SSB tables) with `nrows` rows.  Column
+ * storage is allocated per column by sdb_engine_column_alloc. */
+int sdb_engine_table_create(sdb_engine *eng, const char *name, uint64_t nrows);
+/* Device buffer of column `col` of table `name` (allocated on first call,
+ * nrows int32).  The caller fills it (generator, copy). */
+int sdb_engine_column_alloc(sdb_engine *eng, const char *name, int col, int32_t **dev_ptr);
+/* Copies column `col` (nrows int32) into host memory. */
+int sdb_engine_column_download(sdb_engine *eng, const char *name, int col, int32_t *host);
+/* Copies host data (nrows int32) into column `col`. */
+int sdb_engine_column_upload(sdb_engine *eng, const char *name, int col, const int32_t *host);
+/* Loads the columns of `name` listed by table_column_indices
+ * (kernels/types.cpp:3-11) from `<dir>/<TABLE><col>` raw int32 files
+ * (models/models.hpp:1659-1708). */
+int sdb_engine_table_load_dir(sdb_engine *eng, const char *name, const char *dir);
+/* Computes per-segment min/max of every column of the table (the Segment
+ * constructor statistics); call after the columns are filled. */
+int sdb_engine_table_finalize(sdb_engine *eng, const char *name);
+
+/* Runs a plan (JSON mirror of calciteserver.thrift PlanResult) to
+ * completion and returns its result table. */
+int sdb_engine_execute(sdb_engine *eng, const char *plan_json, sdb_result **out);
+/* Device time (ms, hipEvents) and host wall time (ms) of the last execute. */
+int sdb_engine_last_timing(sdb_engine *eng, double *device_ms, double *wall_ms);
+/* Resets the scratch allocator (memory_manager::reset, re-zeroing the
+ * zero region), as the reference does between repetitions
+ * (runtime/runtime.cpp:331-345). */
+int sdb_engine_reset(sdb_engine *eng);
+/* Number of kernels launched by the last execute. */
+int sdb_engine_last_launches(sdb_engine *eng, int64_t *launches);
+
+/* Switches bundle fusion on (1) or off (0) for later executions. */
+int sdb_engine_set_fusion(sdb_engine *eng, int on);
+
+/* Per-launch timing: with profiling on, every kernel launch of a bundle is
+ * bracketed by HIP events on the engine stream and accumulated per name
+ * ("fused:<table>" for a fused bundle, "op:<KernelType>:<table>" for an
+ * operator launched alone) -- the counterpart of the reference's
+ * add_profiling_info (models/execution.hpp:18-28). */
+int sdb_engine_set_profiling(sdb_engine *eng, int on);
+int sdb_engine_profile_reset(sdb_engine *eng);
+int sdb_engine_profile_count(sdb_engine *eng);
+/* Entry idx (0..count-1): name, launches, rows processed, summed ms. */
+int sdb_engine_profile_get(sdb_engine *eng, int idx, char *name, int name_cap, int64_t *launches,
+                           int64_t *rows, double *ms);
+
+/* Result table: nrows rows, ncols non-null columns, row selection flags. */
+uint64_t sdb_result_nrows(const sdb_result *r);
+int sdb_result_ncols(const sdb_result *r);
+/* 1 when column c holds uint64 sums, 0 when it holds int32 values. */
+int sdb_result_col_is_u64(const sdb_result *r, int c);
+/* Copies column c (nrows values, int32 or uint64) into host memory. */
+int sdb_result_col_copy(const sdb_result *r, int c, void *host);
+/* Copies the nrows selection flags into host memory. */
+int sdb_result_flags_copy(const sdb_result *r, uint8_t *host);
+/* Number of selected rows. */
+uint64_t sdb_result_selected(const sdb_result *r);
+/* Writes the .res text of executor/result_writer.cpp:41-67 (selected rows,
+ * space-separated, int32 keys and uint64 sums) to `path`. */
+int sdb_result_write(const sdb_result *r, const char *path);
+void sdb_result_destroy(sdb_result *r);
+
+/* Execution order of a plan (dag_topological_sort,
+ * operations/preprocessing.cpp:14-102): writes up to cap rel ids into
+ * order[] (host) and their count into *count.  Host-only, no device use. */
+int sdb_plan_dag_order(const char *plan_json, int32_t *order, int cap, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SYCLDB_H */

@@ -1,0 +1,121 @@
+"""ctypes binding of libsycldb_hip.so (the C ABI declared in include/sycldb.h).
+
+The library is the product: there is no fallback.  Importing this module
+raises ImportError when the built library is missing.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsycldb_hip.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        "libsycldb_hip.so is not built (expected at %s); run __graft_entry__.build() "
+        "or `python -m sycldb_amd.build`" % LIB_PATH)
+
+lib = ctypes.CDLL(LIB_PATH)
+
+P = ctypes.c_void_p
+I64 = ctypes.c_int64
+I32 = ctypes.c_int32
+U64 = ctypes.c_uint64
+INT = ctypes.c_int
+CSTR = ctypes.c_char_p
+DBL = ctypes.c_double
+
+SDB_OK, SDB_ERR_INVALID_ARG, SDB_ERR_HIP, SDB_ERR_OOM, SDB_ERR_UNSUPPORTED, SDB_ERR_RUNTIME = range(6)
+EQ, NE, LT, LE, GT, GE = range(6)
+NONE, AND, OR = range(3)
+MUL, DIV, ADD, SUB = range(4)
+
+
+class EngineOptions(ctypes.Structure):
+    _fields_ = [("device_budget_bytes", ctypes.c_uint64),
+                ("segment_size", ctypes.c_uint64),
+                ("disable_fusion", ctypes.c_int),
+                ("device", ctypes.c_int)]
+
+
+# name: (restype, argtypes)
+SIGNATURES = {
+    "sdb_last_error": (CSTR, []),
+    "sdb_version": (CSTR, []),
+    "sdb_comp_op_from_string": (INT, [CSTR]),
+    "sdb_logical_op_from_string": (INT, [CSTR]),
+    "sdb_binop_from_string": (INT, [CSTR]),
+    "sdb_select_lit": (INT, [P, P, P, I64, INT, INT, I32, P]),
+    "sdb_select_col": (INT, [P, P, P, P, I64, INT, INT, P]),
+    "sdb_logical": (INT, [P, P, P, I64, INT, P]),
+    "sdb_fill_i32": (INT, [P, I32, I64, P]),
+    "sdb_binop_cc": (INT, [P, P, P, P, I64, INT, P]),
+    "sdb_binop_lc": (INT, [P, I32, P, P, I64, INT, P]),
+    "sdb_binop_cl": (INT, [P, P, I32, P, I64, INT, P]),
+    "sdb_build_keys": (INT, [P, P, I64, P, I32, I32, P]),
+    "sdb_build_keyvals": (INT, [P, P, P, I64, P, I32, I32, P]),
+    "sdb_probe_semi": (INT, [P, P, P, I64, P, I32, I32, P]),
+    "sdb_probe_full": (INT, [P, P, P, P, I64, P, I32, I32, P]),
+    "sdb_sum_flagged": (INT, [P, P, I64, P, P]),
+    "sdb_groupby_dense": (INT, [P, INT, P, P, P, P, I64, P, P, P, U64, P]),
+    "sdb_seen_to_flags": (INT, [P, P, U64, P]),
+    "sdb_count_flags": (INT, [P, I64, P, INT, P]),
+    "sdb_compact": (INT, [P, I64, P, P, P]),
+    "sdb_gather_i32": (INT, [P, P, I64, P, P]),
+    "sdb_minmax_i32": (INT, [P, I64, P, P, P]),
+    "sdb_gen_uniform": (INT, [P, I64, U64, U64, I64, I32, I32, P]),
+    "sdb_gen_child": (INT, [P, P, I64, U64, U64, I64, I32, I32, I32, P]),
+    "sdb_gen_lookup": (INT, [P, I64, U64, U64, I64, P, I32, P]),
+    "sdb_gen_rowid": (INT, [P, I64, I64, P]),
+    "sdb_engine_create": (INT, [ctypes.POINTER(EngineOptions), ctypes.POINTER(P)]),
+    "sdb_engine_destroy": (None, [P]),
+    "sdb_engine_stream": (P, [P]),
+    "sdb_engine_table_create": (INT, [P, CSTR, U64]),
+    "sdb_engine_column_alloc": (INT, [P, CSTR, INT, ctypes.POINTER(P)]),
+    "sdb_engine_column_upload": (INT, [P, CSTR, INT, P]),
+    "sdb_engine_column_download": (INT, [P, CSTR, INT, P]),
+    "sdb_engine_table_load_dir": (INT, [P, CSTR, CSTR]),
+    "sdb_engine_table_finalize": (INT, [P, CSTR]),
+    "sdb_engine_execute": (INT, [P, CSTR, ctypes.POINTER(P)]),
+    "sdb_engine_last_timing": (INT, [P, ctypes.POINTER(DBL), ctypes.POINTER(DBL)]),
+    "sdb_engine_reset": (INT, [P]),
+    "sdb_engine_last_launches": (INT, [P, ctypes.POINTER(I64)]),
+    "sdb_engine_set_fusion": (INT, [P, INT]),
+    "sdb_engine_set_profiling": (INT, [P, INT]),
+    "sdb_engine_profile_reset": (INT, [P]),
+    "sdb_engine_profile_count": (INT, [P]),
+    "sdb_engine_profile_get": (INT, [P, INT, CSTR, INT, ctypes.POINTER(I64), ctypes.POINTER(I64),
+                                      ctypes.POINTER(DBL)]),
+    "sdb_result_nrows": (U64, [P]),
+    "sdb_result_ncols": (INT, [P]),
+    "sdb_result_col_is_u64": (INT, [P, INT]),
+    "sdb_result_col_copy": (INT, [P, INT, P]),
+    "sdb_result_flags_copy": (INT, [P, P]),
+    "sdb_result_selected": (U64, [P]),
+    "sdb_result_write": (INT, [P, CSTR]),
+    "sdb_result_destroy": (None, [P]),
+    "sdb_plan_dag_order": (INT, [CSTR, P, INT, ctypes.POINTER(INT)]),
+}
+
+for _name, (_res, _args) in SIGNATURES.items():
+    _fn = getattr(lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+
+class SdbError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("sycldb error %d: %s" % (code, msg))
+        self.code = code
+
+
+def check(rc):
+    """Raise the Python counterpart of the C++ exception the reference would
+    throw: ValueError for invalid_argument, MemoryError for bad_alloc."""
+    if rc == SDB_OK:
+        return
+    msg = (lib.sdb_last_error() or b"").decode(errors="replace")
+    if rc == SDB_ERR_INVALID_ARG:
+        raise ValueError(msg)
+    if rc == SDB_ERR_OOM:
+        raise MemoryError(msg)
+    raise SdbError(rc, msg)

@@ -1,0 +1,309 @@
+// Host side of the MI355X SYCLDB engine: the reference's data model and
+// dependency model (models/models.hpp, models/execution.hpp,
+// models/transient_table.hpp, operations/memory_manager.*) on one HIP
+// device, with HIP streams instead of SYCL queues/events.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "fused.hpp"
+#include "plan.hpp"
+#include "sdb_common.hpp"
+
+namespace sdb {
+
+// ---------------------------------------------------------------- launchers
+// (ops.hip)
+void launch_select_lit(const int32_t *col, const uint8_t *in, uint8_t *out, int64_t n, int cmp, int logic,
+                       int32_t v, hipStream_t s);
+void launch_select_col(const int32_t *a, const int32_t *b, const uint8_t *in, uint8_t *out, int64_t n, int cmp,
+                       int logic, hipStream_t s);
+void launch_logical(const uint8_t *in, const uint8_t *local, uint8_t *out, int64_t n, int logic, hipStream_t s);
+void launch_fill_i32(int32_t *dst, int32_t v, int64_t n, hipStream_t s);
+void launch_fill_u8(uint8_t *dst, uint8_t v, int64_t n, hipStream_t s);
+void launch_binop(int kind, int op, int32_t *r, const int32_t *a, const int32_t *b, int32_t lit, const uint8_t *f,
+                  int64_t n, hipStream_t s);
+void launch_build_keys(const int32_t *keys, const uint8_t *flags, int64_t n, uint8_t *ht, int32_t len, int32_t min,
+                       hipStream_t s);
+void launch_build_keyvals(const int32_t *keys, const int32_t *vals, const uint8_t *flags, int64_t n, int32_t *ht,
+                          int32_t len, int32_t min, hipStream_t s);
+void launch_probe_semi(const int32_t *probe, const uint8_t *in, uint8_t *out, int64_t n, const uint8_t *ht,
+                       int32_t min, int32_t max, hipStream_t s);
+void launch_probe_full(const int32_t *probe, int32_t *val_out, const uint8_t *in, uint8_t *out, int64_t n,
+                       const int32_t *ht, int32_t min, int32_t max, hipStream_t s);
+void launch_sum_flagged(const int32_t *col, const uint8_t *flags, int64_t n, uint64_t *acc, hipStream_t s);
+void launch_count_flags(const uint8_t *flags, int64_t n, uint64_t *out, bool accumulate, hipStream_t s);
+void launch_groupby_dense(const int32_t *const *keys, int k, const int32_t *mins, const int32_t *maxs,
+                          const int32_t *agg, const uint8_t *flags, int64_t n, int32_t *const *out_keys,
+                          uint64_t *sums, uint32_t *seen, uint64_t prod, hipStream_t s, int strategy);
+void launch_seen_to_flags(const uint32_t *seen, uint8_t *flags, uint64_t prod, hipStream_t s);
+void minmax_i32(const int32_t *col, int64_t n, int32_t *mn, int32_t *mx, hipStream_t s);
+uint64_t groupby_lds_max_slots();
+
+// ----------------------------------------------------------- memory manager
+
+// Bump allocator over device regions (operations/memory_manager.hpp:61-164).
+// alloc_zero is served from regions that are zero when handed out; reset()
+// rewinds every region and re-zeroes the used part of the zero regions
+// (memory_manager.cpp:42-53, 71-90).  Throws std::bad_alloc past the budget.
+class MemoryManager {
+  public:
+    MemoryManager(uint64_t budget, hipStream_t stream, uint64_t max_region = 1ull << 30);
+    ~MemoryManager();
+    MemoryManager(const MemoryManager &) = delete;
+    MemoryManager &operator=(const MemoryManager &) = delete;
+
+    template <class T> T *alloc(uint64_t count) { return static_cast<T *>(raw(count * sizeof(T), false)); }
+    template <class T> T *alloc_zero(uint64_t count) { return static_cast<T *>(raw(count * sizeof(T), true)); }
+    void reset(hipStream_t s);
+    uint64_t used_bytes() const;
+
+  private:
+    struct Region {
+        uint8_t *base;
+        uint64_t size, used;
+    };
+    void *raw(uint64_t bytes, bool zero);
+    std::vector<Region> regions_, zero_regions_;
+    hipStream_t stream_;
+    uint64_t budget_, budget_zero_, reserved_ = 0, reserved_zero_ = 0, max_region_;
+};
+
+// ------------------------------------------------------------------ columns
+
+// A column of int32 (or, for aggregate results, uint64) values split into
+// segments of `segment_size` rows with per-segment min/max
+// (models/models.hpp:20-1649).  Storage is one device buffer; segment s is
+// the view starting at row s * segment_size.
+struct Column {
+    uint64_t nrows = 0;
+    uint64_t segment_size = 1;
+    bool loaded = false;        // false: a column the base table does not hold
+    bool is_aggregate = false;  // values are uint64
+    void *data = nullptr;       // device storage
+    std::vector<int32_t> seg_min, seg_max;
+
+    uint64_t nseg() const { return nrows == 0 ? 0 : (nrows + segment_size - 1) / segment_size; }
+    uint64_t seg_rows(uint64_t s) const { return std::min(segment_size, nrows - s * segment_size); }
+    int32_t *i32(uint64_t seg = 0) const { return static_cast<int32_t *>(data) + seg * segment_size; }
+    uint64_t *u64(uint64_t seg = 0) const { return static_cast<uint64_t *>(data) + seg * segment_size; }
+    std::pair<int32_t, int32_t> get_min_max() const; // models.hpp:1257-1272
+};
+
+struct Table {
+    std::string name;
+    uint64_t nrows = 0;
+    std::vector<Column> columns;            // table_column_numbers entries
+    std::vector<void *> owned;              // device buffers to free
+};
+
+// ------------------------------------------------------------ kernel bundles
+
+// KernelType, models/execution.hpp:30-46
+enum class KernelType : uint8_t {
+    EmptyKernel,
+    LogicalKernel,
+    SelectionKernelColumns,
+    SelectionKernelLiteral,
+    FillKernel,
+    PerformOperationKernelColumns,
+    PerformOperationKernelLiteralFirst,
+    PerformOperationKernelLiteralSecond,
+    BuildKeysHTKernel,
+    FilterJoinKernel,
+    BuildKeyValsHTKernel,
+    FullJoinKernel,
+    AggregateOperationKernel,
+    GroupByAggregateKernel,
+};
+
+struct GroupBySpec {
+    std::vector<const int32_t *> keys;   // per key column (segment view)
+    std::vector<int32_t> mins, maxs;
+    std::vector<int32_t *> out_keys;
+    uint64_t *sums = nullptr;
+    uint32_t *seen = nullptr;
+    uint64_t prod = 0;
+};
+
+// One functor of the reference (the constructor arguments of the classes in
+// kernels/*.hpp), with device pointers already offset to its segment.
+struct KernelData {
+    KernelType type = KernelType::EmptyKernel;
+    int64_t n = 0;
+    int cmp = 0, logic = 0, op = 0;
+    int32_t literal = 0;
+    const uint8_t *in_flags = nullptr;
+    uint8_t *out_flags = nullptr;
+    const uint8_t *local_flags = nullptr;
+    const int32_t *a = nullptr, *b = nullptr;
+    int32_t *result = nullptr;
+    void *ht = nullptr;
+    int32_t ht_len = 0, ht_min = 0, ht_max = 0;
+    uint64_t *agg_res = nullptr;
+    std::shared_ptr<GroupBySpec> gb;
+};
+
+struct Engine;
+
+// Buffers still referenced after a bundle: their final contents must be
+// written back by a fused execution.
+struct LiveSet {
+    std::vector<std::pair<uintptr_t, uintptr_t>> ranges; // [begin, end)
+    bool contains(const void *p) const
+    {
+        const uintptr_t a = (uintptr_t)p;
+        for (auto &r : ranges)
+            if (a >= r.first && a < r.second) return true;
+        return false;
+    }
+    void add(const void *p, uint64_t bytes)
+    {
+        if (p && bytes) ranges.emplace_back((uintptr_t)p, (uintptr_t)p + bytes);
+    }
+};
+
+struct ExecContext {
+    Engine *eng;
+    hipStream_t stream;
+    const LiveSet *live;
+    const std::string *table = nullptr; // base table the bundle runs over
+    // flags buffer the table has not written yet: logically all true
+    const uint8_t *virgin_begin = nullptr, *virgin_end = nullptr;
+    mutable bool virgin_filled = false; // set once an unfused bundle filled it
+};
+
+// KernelBundle, models/execution.hpp:500-651: the operators queued for one
+// segment, run in order on one stream.  With fusion on, the bundle becomes a
+// single fused kernel when its operators allow it.
+class KernelBundle {
+  public:
+    void add_kernel(const KernelData &k) { kernels_.push_back(k); }
+    const std::vector<KernelData> &get_kernels() const { return kernels_; }
+    void execute(const ExecContext &ctx) const;
+
+  private:
+    void execute_unfused(const ExecContext &ctx) const;
+    bool build_program(const ExecContext &ctx, FProgram &p) const;
+    std::vector<KernelData> kernels_;
+};
+
+// ------------------------------------------------------------------- engine
+
+struct Engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t segment_size = 1ull << 28; // common.hpp:12
+    bool fusion = true;
+    std::unique_ptr<MemoryManager> mm;
+    std::map<std::string, std::unique_ptr<Table>> tables;
+    int64_t launches = 0;
+    double last_device_ms = 0, last_wall_ms = 0;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+
+    // Optional per-launch timing with HIP events on the engine stream.
+    struct KernelStat {
+        int64_t launches = 0, rows = 0;
+        double ms = 0;
+    };
+    bool profile = false;
+    std::map<std::string, KernelStat> prof_stats;
+    struct ProfRec {
+        std::string name;
+        int64_t rows;
+        hipEvent_t a, b;
+    };
+    std::vector<ProfRec> prof_pending;
+    std::vector<hipEvent_t> event_pool;
+    void prof_begin(const std::string &name, int64_t rows, ProfRec &rec);
+    void prof_end(ProfRec &rec);
+    void prof_collect(); // after the stream is synchronised
+
+    Table *find_table(const std::string &name);
+};
+
+// -------------------------------------------------------- transient table
+
+// TransientTable, models/transient_table.hpp:20-1928, for one device.
+class TransientTable {
+  public:
+    TransientTable(Engine &eng, Table *base);
+
+    void set_group_by_column(uint64_t col); // 189-193
+    void apply_filter(const ExprType &expr, const std::string &parent_op);
+    void apply_project(const std::vector<ExprType> &exprs);
+    void apply_aggregate(const AggType &agg, const std::vector<int64_t> &group);
+    void apply_join(TransientTable &right, const RelNode &rel);
+    // Runs the queued bundles (383-506).  `dead_after`: the caller replaces
+    // the table's columns and flags right after, so nothing needs writing
+    // back.
+    void execute_pending_kernels(bool dead_after = false);
+
+    uint64_t get_nrows() const { return nrows_; }
+    const std::vector<Column *> &get_columns() const { return current_columns_; }
+    // Device flags of the table (materialised if still all-true).
+    const uint8_t *flags_device();
+
+  private:
+    uint64_t segment_num() const { return nrows_ == 0 ? 0 : (nrows_ + eng_.segment_size - 1) / eng_.segment_size; }
+    uint64_t seg_rows(uint64_t s) const { return std::min(eng_.segment_size, nrows_ - s * eng_.segment_size); }
+    void add_pending_kernels(std::vector<KernelBundle> ops); // 347-381
+    void ensure_secondary_flags();
+    void swap_flag_buffers();
+    Column &new_column(uint64_t nrows, bool aggregate = false);
+    Column *col(int64_t idx, const char *what) const;
+    std::tuple<uint8_t *, int32_t, int32_t> build_keys_hash_table(int column);     // 707-726
+    std::tuple<int32_t *, int32_t, int32_t> build_key_vals_hash_table(int column); // 728-748
+    void apply_search(const ExprType &expr, const std::string &parent_op);
+
+    Engine &eng_;
+    uint64_t nrows_;
+    uint8_t *flags_ = nullptr, *flags_secondary_ = nullptr;
+    uint8_t *virgin_ = nullptr;          // flags buffer that is logically all
+                                         // true and not yet written
+    std::vector<Column *> current_columns_;
+    std::deque<Column> materialized_columns_;
+    Column *group_by_column_ = nullptr;
+    uint64_t group_by_column_index_ = 0;
+    std::vector<std::vector<KernelBundle>> pending_kernels_; // [phase][segment]
+    std::string name_;
+};
+
+// ------------------------------------------------------------------ executor
+
+struct ExecutionInfo {
+    std::map<std::string, std::vector<int>> loaded_columns;
+    std::map<std::string, int> group_by_columns;
+    std::vector<int> dag_order;
+};
+
+std::vector<int> dag_topological_sort(const PlanResult &plan);   // preprocessing.cpp:14-102
+ExecutionInfo parse_execution_info(const PlanResult &plan);     // preprocessing.cpp:104-275
+
+// Host copy of a final table.
+struct ResultTable {
+    uint64_t nrows = 0;
+    std::vector<uint8_t> flags;
+    struct Col {
+        bool is_u64;
+        std::vector<int32_t> i32;
+        std::vector<uint64_t> u64;
+    };
+    std::vector<Col> cols;
+};
+
+// execute_ddor_plan, executor/ddor_executor.cpp:22-245
+ResultTable execute_ddor_plan(Engine &eng, const PlanResult &plan);
+
+extern const std::map<std::string, int> TABLE_COLUMN_NUMBERS;   // kernels/types.cpp:13-21
+extern const std::map<std::string, std::vector<int>> TABLE_COLUMN_INDICES; // types.cpp:3-11
+
+} // namespace sdb

@@ -1,0 +1,442 @@
+// C ABI of the engine: resident tables, plan execution, results.
+#include <chrono>
+#include <cstring>
+#include <cstdio>
+#include <fstream>
+
+#include "capi_util.hpp"
+#include "engine.hpp"
+
+using namespace sdb;
+
+struct sdb_engine {
+    Engine eng;
+};
+
+struct sdb_result {
+    ResultTable t;
+};
+
+namespace sdb {
+
+Table *Engine::find_table(const std::string &name)
+{
+    auto it = tables.find(name);
+    return it == tables.end() ? nullptr : it->second.get();
+}
+
+void Engine::prof_begin(const std::string &name, int64_t rows, ProfRec &rec)
+{
+    auto take = [&]() {
+        hipEvent_t e;
+        if (!event_pool.empty()) {
+            e = event_pool.back();
+            event_pool.pop_back();
+        } else {
+            SDB_HIP(hipEventCreate(&e));
+        }
+        return e;
+    };
+    rec.name = name;
+    rec.rows = rows;
+    rec.a = take();
+    rec.b = take();
+    SDB_HIP(hipEventRecord(rec.a, stream));
+}
+
+void Engine::prof_end(ProfRec &rec)
+{
+    SDB_HIP(hipEventRecord(rec.b, stream));
+    prof_pending.push_back(rec);
+}
+
+void Engine::prof_collect()
+{
+    for (ProfRec &r : prof_pending) {
+        SDB_HIP(hipEventSynchronize(r.b));
+        float ms = 0;
+        SDB_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+        KernelStat &st = prof_stats[r.name];
+        st.launches++;
+        st.rows += r.rows;
+        st.ms += ms;
+        event_pool.push_back(r.a);
+        event_pool.push_back(r.b);
+    }
+    prof_pending.clear();
+}
+
+} // namespace sdb
+
+namespace {
+
+Table &table_of(sdb_engine *e, const char *name)
+{
+    require(e && name, "null engine or name");
+    Table *t = e->eng.find_table(name);
+    if (!t) fail(SDB_ERR_INVALID_ARG, std::string("unknown table ") + name);
+    return *t;
+}
+
+Column &column_of(sdb_engine *e, const char *name, int col)
+{
+    Table &t = table_of(e, name);
+    require(col >= 0 && col < (int)t.columns.size(), "column index out of range");
+    return t.columns[col];
+}
+
+void ensure_storage(Engine &eng, Table &t, Column &c)
+{
+    if (c.data) return;
+    void *p = nullptr;
+    SDB_HIP(hipMalloc(&p, std::max<uint64_t>(t.nrows, 1) * sizeof(int32_t)));
+    t.owned.push_back(p);
+    c.data = p;
+    c.loaded = true;
+    c.nrows = t.nrows;
+    c.segment_size = eng.segment_size;
+}
+
+} // namespace
+
+extern "C" {
+
+int sdb_engine_create(const sdb_engine_options *opts, sdb_engine **out)
+{
+    return guarded([&] {
+        require(out, "null output");
+        *out = nullptr;
+        auto e = std::make_unique<sdb_engine>();
+        Engine &eng = e->eng;
+        uint64_t budget = 16ull << 30;
+        if (opts) {
+            if (opts->device_budget_bytes) budget = opts->device_budget_bytes;
+            if (opts->segment_size) eng.segment_size = opts->segment_size;
+            eng.fusion = opts->disable_fusion == 0;
+            eng.device = opts->device;
+        }
+        require(eng.segment_size >= 1 && eng.segment_size <= (1ull << 31), "segment size must be in [1, 2^31]");
+        SDB_HIP(hipSetDevice(eng.device));
+        SDB_HIP(hipStreamCreateWithFlags(&eng.stream, hipStreamNonBlocking));
+        SDB_HIP(hipEventCreate(&eng.ev_start));
+        SDB_HIP(hipEventCreate(&eng.ev_stop));
+        eng.mm = std::make_unique<MemoryManager>(budget, eng.stream);
+        *out = e.release();
+    });
+}
+
+void sdb_engine_destroy(sdb_engine *e)
+{
+    if (!e) return;
+    Engine &eng = e->eng;
+    (void)hipSetDevice(eng.device);
+    if (eng.stream) (void)hipStreamSynchronize(eng.stream);
+    eng.mm.reset();
+    for (auto &kv : eng.tables)
+        for (void *p : kv.second->owned) (void)hipFree(p);
+    for (auto &r : eng.prof_pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (hipEvent_t ev : eng.event_pool) (void)hipEventDestroy(ev);
+    if (eng.ev_start) (void)hipEventDestroy(eng.ev_start);
+    if (eng.ev_stop) (void)hipEventDestroy(eng.ev_stop);
+    if (eng.stream) (void)hipStreamDestroy(eng.stream);
+    delete e;
+}
+
+sdb_stream sdb_engine_stream(sdb_engine *e) { return e ? reinterpret_cast<sdb_stream>(e->eng.stream) : nullptr; }
+
+int sdb_engine_table_create(sdb_engine *e, const char *name, uint64_t nrows)
+{
+    return guarded([&] {
+        require(e && name, "null engine or name");
+        auto it = TABLE_COLUMN_NUMBERS.find(name);
+        if (it == TABLE_COLUMN_NUMBERS.end()) fail(SDB_ERR_INVALID_ARG, std::string("unknown table ") + name);
+        Engine &eng = e->eng;
+        SDB_HIP(hipSetDevice(eng.device));
+        auto old = eng.tables.find(name);
+        if (old != eng.tables.end()) {
+            SDB_HIP(hipStreamSynchronize(eng.stream));
+            for (void *p : old->second->owned) SDB_HIP(hipFree(p));
+            eng.tables.erase(old);
+        }
+        auto t = std::make_unique<Table>();
+        t->name = name;
+        t->nrows = nrows;
+        t->columns.resize(it->second);
+        for (Column &c : t->columns) {
+            c.nrows = nrows;
+            c.segment_size = eng.segment_size;
+        }
+        eng.tables[name] = std::move(t);
+    });
+}
+
+int sdb_engine_column_alloc(sdb_engine *e, const char *name, int col, int32_t **dev_ptr)
+{
+    return guarded([&] {
+        require(dev_ptr, "null output");
+        Table &t = table_of(e, name);
+        Column &c = column_of(e, name, col);
+        SDB_HIP(hipSetDevice(e->eng.device));
+        ensure_storage(e->eng, t, c);
+        *dev_ptr = c.i32();
+    });
+}
+
+int sdb_engine_column_upload(sdb_engine *e, const char *name, int col, const int32_t *host)
+{
+    return guarded([&] {
+        require(host, "null host data");
+        Table &t = table_of(e, name);
+        Column &c = column_of(e, name, col);
+        SDB_HIP(hipSetDevice(e->eng.device));
+        ensure_storage(e->eng, t, c);
+        if (t.nrows)
+            SDB_HIP(hipMemcpyAsync(c.data, host, t.nrows * 4, hipMemcpyHostToDevice, e->eng.stream));
+        SDB_HIP(hipStreamSynchronize(e->eng.stream));
+    });
+}
+
+int sdb_engine_column_download(sdb_engine *e, const char *name, int col, int32_t *host)
+{
+    return guarded([&] {
+        require(host, "null host buffer");
+        Table &t = table_of(e, name);
+        Column &c = column_of(e, name, col);
+        require(c.data != nullptr, "column has no storage");
+        SDB_HIP(hipSetDevice(e->eng.device));
+        if (t.nrows)
+            SDB_HIP(hipMemcpyAsync(host, c.data, t.nrows * 4, hipMemcpyDeviceToHost, e->eng.stream));
+        SDB_HIP(hipStreamSynchronize(e->eng.stream));
+    });
+}
+
+int sdb_engine_table_load_dir(sdb_engine *e, const char *name, const char *dir)
+{
+    return guarded([&] {
+        require(e && name && dir, "null argument");
+        auto idx = TABLE_COLUMN_INDICES.find(name);
+        if (idx == TABLE_COLUMN_INDICES.end()) fail(SDB_ERR_INVALID_ARG, std::string("unknown table ") + name);
+        // models/models.hpp:1659-1708: <DIR>/<TABLE><col>, raw int32; the
+        // first file fixes the row count; a file of another length leaves
+        // that column unloaded
+        std::string upper = name;
+        for (char &ch : upper) ch = (char)toupper((unsigned char)ch);
+        std::vector<std::pair<int, std::vector<int32_t>>> cols;
+        int64_t nrows = -1;
+        for (int ci : idx->second) {
+            const std::string path = std::string(dir) + "/" + upper + std::to_string(ci);
+            std::ifstream f(path, std::ios::binary | std::ios::ate);
+            if (!f) fail(SDB_ERR_INVALID_ARG, "cannot open column file " + path);
+            const int64_t bytes = (int64_t)f.tellg();
+            const int64_t n = bytes / 4;
+            if (nrows < 0) nrows = n;
+            if (n != nrows) {
+                fprintf(stderr, "Warning: Column length mismatch in %s\n", path.c_str());
+                continue;
+            }
+            std::vector<int32_t> data((size_t)n);
+            f.seekg(0);
+            f.read(reinterpret_cast<char *>(data.data()), n * 4);
+            cols.emplace_back(ci, std::move(data));
+        }
+        if (nrows < 0) nrows = 0;
+        int rc = sdb_engine_table_create(e, name, (uint64_t)nrows);
+        if (rc) fail(rc, sdb_last_error());
+        for (auto &c : cols) {
+            rc = sdb_engine_column_upload(e, name, c.first, c.second.data());
+            if (rc) fail(rc, sdb_last_error());
+        }
+        rc = sdb_engine_table_finalize(e, name);
+        if (rc) fail(rc, sdb_last_error());
+    });
+}
+
+int sdb_engine_table_finalize(sdb_engine *e, const char *name)
+{
+    return guarded([&] {
+        Table &t = table_of(e, name);
+        Engine &eng = e->eng;
+        SDB_HIP(hipSetDevice(eng.device));
+        for (Column &c : t.columns) {
+            if (!c.loaded || !c.data) continue;
+            const uint64_t ns = c.nseg();
+            c.seg_min.assign(ns, 0);
+            c.seg_max.assign(ns, 0);
+            for (uint64_t s = 0; s < ns; s++)
+                minmax_i32(c.i32(s), (int64_t)c.seg_rows(s), &c.seg_min[s], &c.seg_max[s], eng.stream);
+        }
+    });
+}
+
+int sdb_engine_execute(sdb_engine *e, const char *plan_json, sdb_result **out)
+{
+    return guarded([&] {
+        require(e && plan_json && out, "null argument");
+        *out = nullptr;
+        Engine &eng = e->eng;
+        SDB_HIP(hipSetDevice(eng.device));
+        const PlanResult plan = parse_plan(plan_json);
+        // scratch of the previous query is released (runtime/runtime.cpp:331-345)
+        eng.mm->reset(eng.stream);
+        eng.launches = 0;
+        const auto w0 = std::chrono::steady_clock::now();
+        SDB_HIP(hipEventRecord(eng.ev_start, eng.stream));
+        auto r = std::make_unique<sdb_result>();
+        r->t = execute_ddor_plan(eng, plan);
+        SDB_HIP(hipEventRecord(eng.ev_stop, eng.stream));
+        SDB_HIP(hipEventSynchronize(eng.ev_stop));
+        if (eng.profile) eng.prof_collect();
+        const auto w1 = std::chrono::steady_clock::now();
+        float ms = 0;
+        SDB_HIP(hipEventElapsedTime(&ms, eng.ev_start, eng.ev_stop));
+        eng.last_device_ms = ms;
+        eng.last_wall_ms = std::chrono::duration<double, std::milli>(w1 - w0).count();
+        *out = r.release();
+    });
+}
+
+int sdb_engine_last_timing(sdb_engine *e, double *device_ms, double *wall_ms)
+{
+    return guarded([&] {
+        require(e, "null engine");
+        if (device_ms) *device_ms = e->eng.last_device_ms;
+        if (wall_ms) *wall_ms = e->eng.last_wall_ms;
+    });
+}
+
+int sdb_engine_last_launches(sdb_engine *e, int64_t *launches)
+{
+    return guarded([&] {
+        require(e && launches, "null argument");
+        *launches = e->eng.launches;
+    });
+}
+
+int sdb_engine_reset(sdb_engine *e)
+{
+    return guarded([&] {
+        require(e, "null engine");
+        SDB_HIP(hipSetDevice(e->eng.device));
+        e->eng.mm->reset(e->eng.stream);
+        SDB_HIP(hipStreamSynchronize(e->eng.stream));
+    });
+}
+
+int sdb_engine_set_fusion(sdb_engine *e, int on)
+{
+    return guarded([&] {
+        require(e, "null engine");
+        e->eng.fusion = on != 0;
+    });
+}
+
+int sdb_engine_set_profiling(sdb_engine *e, int on)
+{
+    return guarded([&] {
+        require(e, "null engine");
+        e->eng.profile = on != 0;
+    });
+}
+
+int sdb_engine_profile_reset(sdb_engine *e)
+{
+    return guarded([&] {
+        require(e, "null engine");
+        e->eng.prof_stats.clear();
+    });
+}
+
+int sdb_engine_profile_count(sdb_engine *e)
+{
+    return e ? (int)e->eng.prof_stats.size() : 0;
+}
+
+int sdb_engine_profile_get(sdb_engine *e, int idx, char *name, int name_cap, int64_t *launches, int64_t *rows,
+                           double *ms)
+{
+    return guarded([&] {
+        require(e && name && name_cap > 0 && launches && rows && ms, "null argument");
+        require(idx >= 0 && idx < (int)e->eng.prof_stats.size(), "profile index out of range");
+        auto it = e->eng.prof_stats.begin();
+        std::advance(it, idx);
+        std::snprintf(name, (size_t)name_cap, "%s", it->first.c_str());
+        *launches = it->second.launches;
+        *rows = it->second.rows;
+        *ms = it->second.ms;
+    });
+}
+
+uint64_t sdb_result_nrows(const sdb_result *r) { return r ? r->t.nrows : 0; }
+int sdb_result_ncols(const sdb_result *r) { return r ? (int)r->t.cols.size() : 0; }
+
+int sdb_result_col_is_u64(const sdb_result *r, int c)
+{
+    if (!r || c < 0 || c >= (int)r->t.cols.size()) return -1;
+    return r->t.cols[c].is_u64 ? 1 : 0;
+}
+
+int sdb_result_col_copy(const sdb_result *r, int c, void *host)
+{
+    return guarded([&] {
+        require(r && host, "null argument");
+        require(c >= 0 && c < (int)r->t.cols.size(), "column out of range");
+        const auto &col = r->t.cols[c];
+        if (col.is_u64) std::memcpy(host, col.u64.data(), col.u64.size() * 8);
+        else std::memcpy(host, col.i32.data(), col.i32.size() * 4);
+    });
+}
+
+int sdb_result_flags_copy(const sdb_result *r, uint8_t *host)
+{
+    return guarded([&] {
+        require(r && host, "null argument");
+        std::memcpy(host, r->t.flags.data(), r->t.flags.size());
+    });
+}
+
+uint64_t sdb_result_selected(const sdb_result *r)
+{
+    if (!r) return 0;
+    uint64_t n = 0;
+    for (uint8_t f : r->t.flags) n += f != 0;
+    return n;
+}
+
+int sdb_result_write(const sdb_result *r, const char *path)
+{
+    return guarded([&] {
+        require(r && path, "null argument");
+        std::ofstream out(path);
+        if (!out) fail(SDB_ERR_INVALID_ARG, std::string("cannot open ") + path);
+        const ResultTable &t = r->t;
+        for (uint64_t row = 0; row < t.nrows; row++) {
+            if (!t.flags[row]) continue;
+            bool first = true;
+            for (const auto &c : t.cols) {
+                if (!first) out << ' ';
+                if (c.is_u64) out << c.u64[row];
+                else out << c.i32[row];
+                first = false;
+            }
+            out << '\n';
+        }
+    });
+}
+
+void sdb_result_destroy(sdb_result *r) { delete r; }
+
+int sdb_plan_dag_order(const char *plan_json, int32_t *order, int cap, int *count)
+{
+    return guarded([&] {
+        require(plan_json && count, "null argument");
+        const PlanResult plan = parse_plan(plan_json);
+        const std::vector<int> o = dag_topological_sort(plan);
+        *count = (int)o.size();
+        for (int i = 0; i < (int)o.size() && i < cap; i++) order[i] = o[i];
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,241 @@
+// Plan analysis and the plan walk: operations/preprocessing.cpp:14-275 and
+// executor/ddor_executor.cpp:22-245.
+#include <deque>
+#include <iostream>
+#include <set>
+
+#include "engine.hpp"
+
+namespace sdb {
+
+// kernels/types.cpp:3-21
+const std::map<std::string, int> TABLE_COLUMN_NUMBERS = {
+    {"lineorder", 17}, {"part", 9}, {"supplier", 7}, {"customer", 8}, {"ddate", 17}};
+const std::map<std::string, std::vector<int>> TABLE_COLUMN_INDICES = {
+    {"lineorder", {2, 3, 4, 5, 8, 9, 11, 12, 13}},
+    {"part", {0, 2, 3, 4}},
+    {"supplier", {0, 3, 4, 5}},
+    {"customer", {0, 3, 4, 5}},
+    {"ddate", {0, 4, 5}}};
+
+namespace {
+
+int column_count(const std::string &table)
+{
+    auto it = TABLE_COLUMN_NUMBERS.find(table);
+    if (it == TABLE_COLUMN_NUMBERS.end()) throw std::invalid_argument("unknown table " + table);
+    return it->second;
+}
+
+void expression_columns(const ExprType &e, std::set<int64_t> &out)
+{
+    if (e.exprType == ExprOption::COLUMN) out.insert(e.input);
+    else if (e.exprType == ExprOption::EXPR)
+        for (const ExprType &o : e.operands) expression_columns(o, out);
+}
+
+} // namespace
+
+std::vector<int> dag_topological_sort(const PlanResult &plan)
+{
+    const size_t n = plan.rels.size();
+    std::vector<std::vector<char>> dag(n, std::vector<char>(n, 0));
+    std::deque<int> S;
+    auto edge = [&](int64_t a, int64_t b) {
+        if (a < 0 || b < 0 || a >= (int64_t)n || b >= (int64_t)n) throw std::invalid_argument("plan: bad rel input");
+        dag[a][b] = 1;
+    };
+    for (const RelNode &rel : plan.rels) {
+        switch (rel.relOp) {
+        case RelNodeType::TABLE_SCAN: S.push_front((int)rel.id); break;
+        case RelNodeType::FILTER:
+        case RelNodeType::PROJECT:
+        case RelNodeType::AGGREGATE:
+        case RelNodeType::SORT: edge(rel.id - 1, rel.id); break;
+        case RelNodeType::JOIN:
+            if (rel.inputs.size() != 2) return {};
+            edge(rel.inputs[0], rel.id);
+            edge(rel.inputs[1], rel.id);
+            break;
+        default: return {};
+        }
+    }
+    std::vector<int> sorted;
+    while (!S.empty()) {
+        const int node = S.back();
+        S.pop_back();
+        const RelNode &r = plan.rels[node];
+        // the lineorder scan waits until it is the only source left (60-66)
+        if (!S.empty() && r.relOp == RelNodeType::TABLE_SCAN && r.tables.size() > 1 && r.tables[1] == "lineorder") {
+            S.push_front(node);
+            continue;
+        }
+        sorted.push_back(node);
+        for (size_t i = 0; i < n; i++) {
+            if (!dag[node][i]) continue;
+            dag[node][i] = 0;
+            bool inbound = false;
+            for (size_t j = 0; j < n && !inbound; j++) inbound = dag[j][i] != 0;
+            if (!inbound) S.push_front((int)i);
+        }
+    }
+    if (sorted.size() != n) return {};
+    return sorted;
+}
+
+ExecutionInfo parse_execution_info(const PlanResult &plan)
+{
+    ExecutionInfo info;
+    using Info = std::vector<std::pair<std::string, int>>;
+    std::vector<Info> ops;
+    auto at = [](const Info &v, int64_t i) -> const std::pair<std::string, int> & {
+        if (i < 0 || i >= (int64_t)v.size()) throw std::invalid_argument("plan: column reference out of range");
+        return v[i];
+    };
+    for (const RelNode &rel : plan.rels) {
+        switch (rel.relOp) {
+        case RelNodeType::TABLE_SCAN: {
+            if (rel.tables.size() < 2) throw std::invalid_argument("plan: TABLE_SCAN needs [schema, table]");
+            Info t;
+            for (int i = 0; i < column_count(rel.tables[1]); i++) t.emplace_back(rel.tables[1], i);
+            info.loaded_columns[rel.tables[1]];
+            ops.push_back(t);
+            break;
+        }
+        case RelNodeType::FILTER: {
+            if (ops.empty()) throw std::invalid_argument("plan: FILTER without input");
+            Info t = ops.back();
+            std::set<int64_t> cols;
+            expression_columns(rel.condition, cols);
+            for (int64_t c : cols) info.loaded_columns[at(t, c).first].push_back(at(t, c).second);
+            ops.push_back(t);
+            break;
+        }
+        case RelNodeType::PROJECT: {
+            if (ops.empty()) throw std::invalid_argument("plan: PROJECT without input");
+            const Info last = ops.back();
+            Info t;
+            for (const ExprType &e : rel.exprs) {
+                std::set<int64_t> cols;
+                expression_columns(e, cols);
+                for (int64_t c : cols) info.loaded_columns[at(last, c).first].push_back(at(last, c).second);
+                if (!cols.empty()) t.push_back(at(last, *cols.begin()));
+            }
+            ops.push_back(t);
+            break;
+        }
+        case RelNodeType::AGGREGATE: {
+            if (ops.empty()) throw std::invalid_argument("plan: AGGREGATE without input");
+            const Info last = ops.back();
+            Info t;
+            for (int64_t g : rel.group) {
+                const auto &tc = at(last, g);
+                info.loaded_columns[tc.first].push_back(tc.second);
+                t.push_back(tc);
+                info.group_by_columns[tc.first] = tc.second;
+            }
+            for (const AggType &a : rel.aggs) {
+                for (int64_t c : a.operands) info.loaded_columns[at(last, c).first].push_back(at(last, c).second);
+                if (!a.operands.empty()) t.push_back(at(last, a.operands[0]));
+            }
+            ops.push_back(t);
+            break;
+        }
+        case RelNodeType::JOIN: {
+            if (rel.inputs.size() != 2) throw std::invalid_argument("plan: JOIN needs two inputs");
+            for (int64_t in : rel.inputs)
+                if (in < 0 || in >= (int64_t)ops.size()) throw std::invalid_argument("plan: JOIN input out of range");
+            Info t = ops[rel.inputs[0]];
+            const Info &r = ops[rel.inputs[1]];
+            t.insert(t.end(), r.begin(), r.end());
+            ops.push_back(t);
+            break;
+        }
+        case RelNodeType::SORT: {
+            if (ops.empty()) throw std::invalid_argument("plan: SORT without input");
+            ops.push_back(ops.back());
+            break;
+        }
+        default: std::cerr << "Unsupported RelNodeType" << std::endl; break;
+        }
+    }
+    info.dag_order = dag_topological_sort(plan);
+    return info;
+}
+
+ResultTable execute_ddor_plan(Engine &eng, const PlanResult &plan)
+{
+    const ExecutionInfo info = parse_execution_info(plan);
+    if (info.dag_order.empty()) throw std::runtime_error("DAG topological sort failed");
+    std::vector<int> output_table(plan.rels.size(), -1);
+    std::deque<TransientTable> tts;
+    for (const RelNode &rel : plan.rels) {
+        if (rel.relOp != RelNodeType::TABLE_SCAN) continue;
+        Table *t = eng.find_table(rel.tables[1]);
+        if (!t) throw std::invalid_argument("Table " + rel.tables[1] + " not found among loaded tables.");
+        TransientTable &tt = tts.emplace_back(eng, t);
+        auto gb = info.group_by_columns.find(rel.tables[1]);
+        if (gb != info.group_by_columns.end()) tt.set_group_by_column(gb->second);
+        output_table[rel.id] = (int)tts.size() - 1;
+    }
+    auto input_of = [&](int64_t id) -> TransientTable & {
+        if (id < 0 || id >= (int64_t)output_table.size() || output_table[id] < 0)
+            throw std::invalid_argument("plan: operator input has no table");
+        return tts[output_table[id]];
+    };
+    for (int id : info.dag_order) {
+        const RelNode &rel = plan.rels[id];
+        switch (rel.relOp) {
+        case RelNodeType::TABLE_SCAN: break;
+        case RelNodeType::FILTER:
+            input_of(id - 1).apply_filter(rel.condition, "");
+            output_table[id] = output_table[id - 1];
+            break;
+        case RelNodeType::PROJECT:
+            input_of(id - 1).apply_project(rel.exprs);
+            output_table[id] = output_table[id - 1];
+            break;
+        case RelNodeType::AGGREGATE:
+            if (rel.aggs.empty()) throw std::invalid_argument("plan: AGGREGATE without aggs");
+            input_of(id - 1).apply_aggregate(rel.aggs[0], rel.group);
+            output_table[id] = output_table[id - 1];
+            break;
+        case RelNodeType::JOIN:
+            input_of(rel.inputs[0]).apply_join(input_of(rel.inputs[1]), rel);
+            output_table[id] = output_table[rel.inputs[0]];
+            break;
+        case RelNodeType::SORT: output_table[id] = output_table[id - 1]; break;
+        default: std::cerr << "RelNodeType not yet supported in DDOR." << std::endl; break;
+        }
+    }
+    TransientTable &final_table = input_of((int64_t)plan.rels.size() - 1);
+    final_table.execute_pending_kernels(false);
+
+    // host copy of the result (the materialize_host_view + save_result path,
+    // executor/result_writer.cpp:28-67)
+    ResultTable res;
+    res.nrows = final_table.get_nrows();
+    res.flags.resize(res.nrows);
+    const uint8_t *flags = final_table.flags_device();
+    if (res.nrows)
+        SDB_HIP(hipMemcpyAsync(res.flags.data(), flags, res.nrows, hipMemcpyDeviceToHost, eng.stream));
+    for (Column *c : final_table.get_columns()) {
+        if (!c || !c->loaded || !c->data) continue;
+        ResultTable::Col rc;
+        rc.is_u64 = c->is_aggregate;
+        if (rc.is_u64) {
+            rc.u64.resize(res.nrows);
+            if (res.nrows)
+                SDB_HIP(hipMemcpyAsync(rc.u64.data(), c->data, res.nrows * 8, hipMemcpyDeviceToHost, eng.stream));
+        } else {
+            rc.i32.resize(res.nrows);
+            if (res.nrows)
+                SDB_HIP(hipMemcpyAsync(rc.i32.data(), c->data, res.nrows * 4, hipMemcpyDeviceToHost, eng.stream));
+        }
+        res.cols.push_back(std::move(rc));
+    }
+    SDB_HIP(hipStreamSynchronize(eng.stream));
+    return res;
+}
+
+} // namespace sdb

@@ -1,0 +1,402 @@
+// The fused bundle kernel: one launch runs a whole per-segment operator
+// sequence over the segment's rows (see fused.hpp for the program format).
+//
+// Each thread executes the program on chunks of R rows (R = 4: 16-byte column
+// loads; R = 1: any alignment / the tail).  Flag registers of a chunk live in
+// one 32-bit word (4 bits per register), value registers in VGPRs.  The
+// instruction stream is uniform across the grid, so the dispatch on the
+// instruction kind is a scalar branch; per-row work follows the reference
+// functors exactly.  A column operand is only loaded for chunks in which some
+// row still needs it.  Terminal reductions (sum / group-by) are privatised
+// per thread (sum) or per block in LDS (dense group-by) and flushed once.
+#include "capi_util.hpp"
+#include "fused.hpp"
+
+#include <algorithm>
+
+namespace sdb {
+
+// from ops.hip
+uint64_t groupby_lds_max_slots();
+
+namespace {
+
+constexpr int BLOCK = 256;
+
+template <int R> __device__ __forceinline__ uint32_t flags_in(const uint8_t *p)
+{
+    if constexpr (R == 4) {
+        const uint32_t w = *reinterpret_cast<const uint32_t *>(p);
+        return ((w & 0xffu) != 0) | (((w >> 8) & 0xffu) != 0) << 1 | (((w >> 16) & 0xffu) != 0) << 2 |
+               (((w >> 24) & 0xffu) != 0) << 3;
+    } else {
+        return p[0] != 0;
+    }
+}
+
+template <int R> __device__ __forceinline__ void flags_out(uint8_t *p, uint32_t m)
+{
+    if constexpr (R == 4) {
+        *reinterpret_cast<uint32_t *>(p) = (m & 1u) | ((m >> 1) & 1u) << 8 | ((m >> 2) & 1u) << 16 | ((m >> 3) & 1u) << 24;
+    } else {
+        p[0] = (uint8_t)(m & 1u);
+    }
+}
+
+// Value registers live in LDS: register j of thread t is vregs[j * BLOCK + t]
+// (one int4 = the 4 rows of a chunk).  The register number is uniform, so an
+// access is a single ds_read/ds_write_b128 at a computed address; keeping them
+// in VGPRs would need a dynamically indexed array, which the compiler places
+// in scratch memory.
+struct VRegs {
+    int4 *base; // &vregs[threadIdx.x]
+};
+
+template <int R> __device__ __forceinline__ void vget(const VRegs &V, int r, int32_t (&out)[R])
+{
+    const int4 t = V.base[r * BLOCK];
+    out[0] = t.x;
+    if constexpr (R == 4) {
+        out[1] = t.y;
+        out[2] = t.z;
+        out[3] = t.w;
+    }
+}
+
+template <int R> __device__ __forceinline__ void vset(const VRegs &V, int r, const int32_t (&in)[R], uint32_t mask)
+{
+    constexpr uint32_t M = (1u << R) - 1;
+    if constexpr (R == 4) {
+        if ((mask & M) == M) {
+            V.base[r * BLOCK] = make_int4(in[0], in[1], in[2], in[3]);
+            return;
+        }
+        int4 t = V.base[r * BLOCK];
+        if (mask & 1u) t.x = in[0];
+        if (mask & 2u) t.y = in[1];
+        if (mask & 4u) t.z = in[2];
+        if (mask & 8u) t.w = in[3];
+        V.base[r * BLOCK] = t;
+    } else {
+        if (mask & 1u) V.base[r * BLOCK].x = in[0];
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void operand(int reg, const int32_t *mem, int64_t row0, const VRegs &V, int32_t (&x)[R])
+{
+    if (reg >= 0) {
+        vget<R>(V, reg, x);
+    } else if constexpr (R == 4) {
+        const int4 t = *reinterpret_cast<const int4 *>(mem + row0);
+        x[0] = t.x;
+        x[1] = t.y;
+        x[2] = t.z;
+        x[3] = t.w;
+    } else {
+        x[0] = mem[row0];
+    }
+}
+
+template <int R> __device__ __forceinline__ void store_masked(int32_t *dst, int64_t row0, const int32_t (&x)[R], uint32_t m)
+{
+    constexpr uint32_t M = (1u << R) - 1;
+    if constexpr (R == 4) {
+        if (m == M) {
+            *reinterpret_cast<int4 *>(dst + row0) = make_int4(x[0], x[1], x[2], x[3]);
+            return;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < R; k++)
+        if ((m >> k) & 1u) dst[row0 + k] = x[k];
+}
+
+struct GbState {
+    unsigned long long *lsum;
+    uint32_t *lbits;
+};
+
+__device__ __forceinline__ void gb_row(const FGroup &g, const GbState &st, const int32_t *kv, int32_t agg)
+{
+    bool in_range = true;
+    uint32_t hash = 0, mult = 1;
+#pragma unroll
+    for (int j = 0; j < F_MAX_KEYS; j++) {
+        if (j >= g.k) break;
+        in_range &= kv[j] >= g.mins[j] && kv[j] <= g.maxs[j];
+        hash += ((uint32_t)kv[j] - (uint32_t)g.mins[j]) * mult;
+        mult *= g.range[j];
+    }
+    const uint32_t s = (uint32_t)((uint64_t)(int64_t)(int32_t)hash % g.prod);
+    if (g.lds && in_range) {
+        atomicAdd(&st.lsum[s], (unsigned long long)(int64_t)agg);
+        atomicOr(&st.lbits[s >> 5], 1u << (s & 31));
+        return;
+    }
+    if (atomicExch(&g.seen[s], 1u) == 0u) {
+#pragma unroll
+        for (int j = 0; j < F_MAX_KEYS; j++)
+            if (j < g.k) g.out_keys[j][s] = kv[j];
+    }
+    atomicAdd(reinterpret_cast<unsigned long long *>(&g.sums[s]), (unsigned long long)(int64_t)agg);
+}
+
+template <int R>
+__device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64_t &acc, const GbState &gst,
+                                          const VRegs &V)
+{
+    constexpr uint32_t M = (1u << R) - 1;
+    uint32_t F = 0;
+#pragma unroll
+    for (int j = 0; j < F_MAX_FLAGS; j++) {
+        const uint8_t init = P.flag_init[j];
+        if (init == FF_LOAD) F |= flags_in<R>(P.flag_load[j] + row0) << (4 * j);
+        else if (init == FF_TRUE) F |= M << (4 * j);
+    }
+
+    for (int i = 0; i < P.ninstr; i++) {
+        const FInstr &I = P.ins[i];
+        const uint32_t in = (F >> (4 * I.fin)) & M;
+        switch (I.kind) {
+        case FK_SEL_LIT:
+        case FK_SEL_COL: {
+            const uint32_t need = I.logic == SDB_NONE ? M : (I.logic == SDB_AND ? in : (~in & M));
+            uint32_t cm = 0;
+            if (need) {
+                int32_t x[R], y[R];
+                operand<R>(I.va, I.a, row0, V, x);
+                if (I.kind == FK_SEL_COL) operand<R>(I.vb, I.b, row0, V, y);
+#pragma unroll
+                for (int k = 0; k < R; k++)
+                    cm |= (uint32_t)compare(I.cmp, x[k], I.kind == FK_SEL_COL ? y[k] : I.lit) << k;
+            }
+            const uint32_t out = I.logic == SDB_NONE ? cm : (I.logic == SDB_AND ? (in & cm) : (in | cm));
+            F = (F & ~(0xFu << (4 * I.fout))) | (out << (4 * I.fout));
+            break;
+        }
+        case FK_LOGICAL: {
+            const uint32_t loc = (F >> (4 * I.floc)) & M;
+            const uint32_t out = I.logic == SDB_NONE ? loc : (I.logic == SDB_AND ? (in & loc) : (in | loc));
+            F = (F & ~(0xFu << (4 * I.fout))) | (out << (4 * I.fout));
+            break;
+        }
+        case FK_FILL: {
+            int32_t x[R];
+#pragma unroll
+            for (int k = 0; k < R; k++) x[k] = I.lit;
+            vset<R>(V, I.vout, x, M);
+            if (I.store) store_masked<R>(I.store, row0, x, M);
+            break;
+        }
+        case FK_BINOP_CC:
+        case FK_BINOP_LC:
+        case FK_BINOP_CL: {
+            if (!in) break;
+            int32_t x[R], y[R], r[R];
+            if (I.kind != FK_BINOP_LC) operand<R>(I.va, I.a, row0, V, x);
+            if (I.kind != FK_BINOP_CL) operand<R>(I.vb, I.b, row0, V, y);
+#pragma unroll
+            for (int k = 0; k < R; k++)
+                r[k] = element_op(I.op, I.kind == FK_BINOP_LC ? I.lit : x[k], I.kind == FK_BINOP_CL ? I.lit : y[k]);
+            vset<R>(V, I.vout, r, in);
+            if (I.store) store_masked<R>(I.store, row0, r, in);
+            break;
+        }
+        case FK_PROBE_SEMI: {
+            uint32_t out = 0;
+            if (in) {
+                int32_t x[R];
+                operand<R>(I.va, I.a, row0, V, x);
+                const uint8_t *ht = reinterpret_cast<const uint8_t *>(I.ht);
+#pragma unroll
+                for (int k = 0; k < R; k++)
+                    if (((in >> k) & 1u) && x[k] >= I.ht_min && x[k] <= I.ht_max && ht[slot_of(x[k], I.ht_min)] != 0)
+                        out |= 1u << k;
+            }
+            F = (F & ~(0xFu << (4 * I.fout))) | (out << (4 * I.fout));
+            break;
+        }
+        case FK_PROBE_FULL: {
+            uint32_t out = 0;
+            if (in) {
+                int32_t x[R], r[R];
+                operand<R>(I.va, I.a, row0, V, x);
+                const int2 *ht = reinterpret_cast<const int2 *>(I.ht);
+#pragma unroll
+                for (int k = 0; k < R; k++) {
+                    r[k] = 0;
+                    if (((in >> k) & 1u) && x[k] >= I.ht_min && x[k] <= I.ht_max) {
+                        const int2 e = ht[slot_of(x[k], I.ht_min)];
+                        if (e.x == 1) {
+                            r[k] = e.y;
+                            out |= 1u << k;
+                        }
+                    }
+                }
+                vset<R>(V, I.vout, r, out);
+                if (I.store && out) store_masked<R>(I.store, row0, r, out);
+            }
+            F = (F & ~(0xFu << (4 * I.fout))) | (out << (4 * I.fout));
+            break;
+        }
+        case FK_BUILD_KEYS: {
+            int32_t x[R];
+            operand<R>(I.va, I.a, row0, V, x);
+            uint8_t *ht = reinterpret_cast<uint8_t *>(const_cast<void *>(I.ht));
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                const uint32_t s = slot_of(x[k], I.ht_min);
+                if (s < (uint32_t)I.ht_len) ht[s] = (in >> k) & 1u;
+            }
+            break;
+        }
+        case FK_BUILD_KEYVALS: {
+            int32_t x[R], y[R];
+            operand<R>(I.va, I.a, row0, V, x);
+            if (in) operand<R>(I.vb, I.b, row0, V, y);
+            int32_t *ht = reinterpret_cast<int32_t *>(const_cast<void *>(I.ht));
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                const uint32_t s = slot_of(x[k], I.ht_min);
+                if (s >= (uint32_t)I.ht_len) continue;
+                if ((in >> k) & 1u)
+                    reinterpret_cast<int2 *>(ht)[s] = make_int2(1, y[k]);
+                else
+                    ht[2 * (int64_t)s] = 0;
+            }
+            break;
+        }
+        case FK_SUM: {
+            if (!in) break;
+            int32_t x[R];
+            operand<R>(I.va, I.a, row0, V, x);
+#pragma unroll
+            for (int k = 0; k < R; k++)
+                if ((in >> k) & 1u) acc += x[k];
+            break;
+        }
+        case FK_GROUPBY: {
+            if (!in) break;
+            const FGroup &g = P.gb;
+            int32_t keys[F_MAX_KEYS][R];
+            int32_t agg[R];
+#pragma unroll
+            for (int j = 0; j < F_MAX_KEYS; j++) {
+                int32_t t[R];
+                if (j < g.k) operand<R>(g.key_reg[j], g.key_mem[j], row0, V, t);
+                else
+#pragma unroll
+                    for (int k = 0; k < R; k++) t[k] = 0;
+#pragma unroll
+                for (int k = 0; k < R; k++) keys[j][k] = t[k];
+            }
+            operand<R>(g.agg_reg, g.agg_mem, row0, V, agg);
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                if (!((in >> k) & 1u)) continue;
+                int32_t kv[F_MAX_KEYS];
+#pragma unroll
+                for (int j = 0; j < F_MAX_KEYS; j++) kv[j] = keys[j][k];
+                gb_row(g, gst, kv, agg[k]);
+            }
+            break;
+        }
+        default: break;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < F_MAX_FLAGS; j++)
+        if (P.flag_store[j]) flags_out<R>(P.flag_store[j] + row0, (F >> (4 * j)) & M);
+}
+
+// The program is the kernel's first argument; it is read in place from the
+// kernel-argument segment (constant address space, scalar loads) instead of
+// through the by-value parameter, whose dynamic indexing would force a copy
+// to scratch.  The magic word guards the layout assumption.
+__global__ void __launch_bounds__(BLOCK) k_fused(const FProgram P_unused, int64_t n)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const FProgram &P = *(const FProgram __attribute__((address_space(4))) *)__builtin_amdgcn_kernarg_segment_ptr();
+#else
+    const FProgram &P = P_unused;
+#endif
+    if (P.magic != F_MAGIC) return;
+    extern __shared__ unsigned long long lds[];
+    __shared__ int4 vregs[F_MAX_VALS * BLOCK];
+    const VRegs V{&vregs[threadIdx.x]};
+    GbState gst{nullptr, nullptr};
+    const uint32_t prod = (uint32_t)P.gb.prod;
+    const bool use_lds = P.gb.lds != 0;
+    if (use_lds) {
+        gst.lsum = lds;
+        gst.lbits = reinterpret_cast<uint32_t *>(lds + prod);
+        for (uint32_t s = threadIdx.x; s < prod; s += BLOCK) gst.lsum[s] = 0;
+        for (uint32_t w = threadIdx.x; w < (prod + 31) / 32; w += BLOCK) gst.lbits[w] = 0;
+        __syncthreads();
+    }
+    int64_t acc = 0;
+    if (P.vec) {
+        const int64_t nchunks = n >> 2;
+        for (int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x; c < nchunks; c += (int64_t)gridDim.x * BLOCK)
+            run_chunk<4>(P, c << 2, acc, gst, V);
+        if (blockIdx.x == gridDim.x - 1 && threadIdx.x < (n & 3))
+            run_chunk<1>(P, (nchunks << 2) + threadIdx.x, acc, gst, V);
+    } else {
+        for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+            run_chunk<1>(P, i, acc, gst, V);
+    }
+    if (P.sum_acc) {
+        __shared__ int64_t part[BLOCK / 64];
+        uint64_t v = (uint64_t)acc;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = (int64_t)v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t s = 0;
+            for (int w = 0; w < BLOCK / 64; w++) s += (uint64_t)part[w];
+            if (s) atomicAdd(reinterpret_cast<unsigned long long *>(P.sum_acc), (unsigned long long)s);
+        }
+    }
+    if (use_lds) {
+        __syncthreads();
+        const FGroup &g = P.gb;
+        for (uint32_t s = threadIdx.x; s < prod; s += BLOCK) {
+            if (!((gst.lbits[s >> 5] >> (s & 31)) & 1u)) continue;
+            atomicAdd(reinterpret_cast<unsigned long long *>(&g.sums[s]), gst.lsum[s]);
+            if (atomicExch(&g.seen[s], 1u) == 0u) {
+                uint32_t rest = s;
+                for (int j = 0; j < g.k; j++) {
+                    g.out_keys[j][s] = (int32_t)((uint32_t)g.mins[j] + rest % g.range[j]);
+                    rest /= g.range[j];
+                }
+            }
+        }
+    }
+}
+
+} // namespace
+
+// The fused kernel also holds the value registers in LDS; the dense group-by
+// gets what is left of a 64 KiB workgroup allocation.
+uint64_t fused_groupby_lds_max_slots()
+{
+    constexpr uint64_t total = 64 * 1024;
+    constexpr uint64_t regs = (uint64_t)F_MAX_VALS * BLOCK * 16 + 64;
+    return std::min<uint64_t>(groupby_lds_max_slots(), ((total - regs) * 8) / 65);
+}
+
+void launch_fused(const FProgram &p, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return;
+    if (p.magic != F_MAGIC) fail(SDB_ERR_RUNTIME, "fused program without magic");
+    size_t lds = 0;
+    if (p.gb.lds) lds = p.gb.prod * 8 + ((p.gb.prod + 31) / 32) * 4;
+    const int64_t units = p.vec ? (n + 3) / 4 : n;
+    const int per_cu = lds > 32 * 1024 ? 2 : 8;
+    hipLaunchKernelGGL(k_fused, dim3(stream_grid(units, BLOCK, per_cu)), dim3(BLOCK), lds, s, p, n);
+    hip_check(hipGetLastError(), "fused bundle");
+}
+
+} // namespace sdb

@@ -1,0 +1,1180 @@
+// Operator kernels of the drop-in per-operator path (one launch per
+// reference functor) and their C-ABI entry points.
+//
+// Layout: int32 columns and 1-byte flags in HBM.  A thread handles chunks of
+// 4 consecutive rows (one 16-byte column load, one 32-bit flag word), a block
+// handles 256 x UNROLL chunks per step, the grid strides over the column.  All
+// loads of a step are issued before any store so each wave keeps UNROLL
+// vector loads in flight.  Comparison / logic / arithmetic variants are
+// template parameters so the row loop carries no data-independent branches.
+// Pointers that are not 16-byte (columns) or 4-byte (flags) aligned take a
+// one-row-per-thread path instead.
+#include "capi_util.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace sdb {
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int UNROLL = 4;
+constexpr int64_t ROWS_PER_STEP = (int64_t)BLOCK * UNROLL * 4;
+
+inline bool aligned(const void *p, size_t a) { return ((uintptr_t)p % a) == 0; }
+
+__device__ __forceinline__ uint32_t byte_of(uint32_t w, int k) { return (w >> (8 * k)) & 0xffu; }
+
+// ------------------------------------------------------------- selection
+
+template <int CMP, int LOGIC>
+__global__ void __launch_bounds__(BLOCK) k_select_lit(const int32_t *__restrict__ col,
+                                                      const uint8_t *in, uint8_t *out, int64_t n,
+                                                      int32_t v)
+{
+    const int64_t nchunks = n >> 2;
+    const int4 *c4 = reinterpret_cast<const int4 *>(col);
+    const uint32_t *in32 = reinterpret_cast<const uint32_t *>(in);
+    uint32_t *out32 = reinterpret_cast<uint32_t *>(out);
+    for (int64_t base = (int64_t)blockIdx.x * BLOCK * UNROLL; base < nchunks;
+         base += (int64_t)gridDim.x * BLOCK * UNROLL) {
+        int4 x[UNROLL];
+        uint32_t f[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            const int64_t c = base + u * BLOCK + threadIdx.x;
+            if (c < nchunks) {
+                x[u] = c4[c];
+                f[u] = (LOGIC == SDB_NONE) ? 0u : in32[c];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            const int64_t c = base + u * BLOCK + threadIdx.x;
+            if (c < nchunks) {
+                const int32_t xs[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+                uint32_t r = 0;
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    r |= (uint32_t)logical(LOGIC, byte_of(f[u], k) != 0, compare(CMP, xs[k], v)) << (8 * k);
+                out32[c] = r;
+            }
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+        const int64_t i = (nchunks << 2) + threadIdx.x;
+        out[i] = logical(LOGIC, in[i] != 0, compare(CMP, col[i], v));
+    }
+}
+
+template <int CMP, int LOGIC>
+__global__ void __launch_bounds__(BLOCK) k_select_lit_row(const int32_t *col, const uint8_t *in,
+                                                          uint8_t *out, int64_t n, int32_t v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+        out[i] = logical(LOGIC, in[i] != 0, compare(CMP, col[i], v));
+}
+
+template <int CMP, int LOGIC>
+__global__ void __launch_bounds__(BLOCK) k_select_col(const int32_t *__restrict__ a,
+                                                      const int32_t *__restrict__ b,
+                                                      const uint8_t *in, uint8_t *out, int64_t n)
+{
+    const int64_t nchunks = n >> 2;
+    const int4 *a4 = reinterpret_cast<const int4 *>(a);
+    const int4 *b4 = reinterpret_cast<const int4 *>(b);
+    const uint32_t *in32 = reinterpret_cast<const uint32_t *>(in);
+    uint32_t *out32 = reinterpret_cast<uint32_t *>(out);
+    for (int64_t base = (int64_t)blockIdx.x * BLOCK * UNROLL; base < nchunks;
+         base += (int64_t)gridDim.x * BLOCK * UNROLL) {
+        int4 x[UNROLL], y[UNROLL];
+        uint32_t f[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            const int64_t c = base + u * BLOCK + threadIdx.x;
+            if (c < nchunks) {
+                x[u] = a4[c];
+                y[u] = b4[c];
+                f[u] = (LOGIC == SDB_NONE) ? 0u : in32[c];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            const int64_t c = base + u * BLOCK + threadIdx.x;
+            if (c < nchunks) {
+                const int32_t xs[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+                const int32_t ys[4] = {y[u].x, y[u].y, y[u].z, y[u].w};
+                uint32_t r = 0;
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    r |= (uint32_t)logical(LOGIC, byte_of(f[u], k) != 0, compare(CMP, xs[k], ys[k])) << (8 * k);
+                out32[c] = r;
+            }
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+        const int64_t i = (nchunks << 2) + threadIdx.x;
+        out[i] = logical(LOGIC, in[i] != 0, compare(CMP, a[i], b[i]));
+    }
+}
+
+template <int CMP, int LOGIC>
+__global__ void __launch_bounds__(BLOCK) k_select_col_row(const int32_t *a, const int32_t *b,
+                                                          const uint8_t *in, uint8_t *out, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+        out[i] = logical(LOGIC, in[i] != 0, compare(CMP, a[i], b[i]));
+}
+
+template <int LOGIC>
+__global__ void __launch_bounds__(BLOCK) k_logical(const uint8_t *in, const uint8_t *local,
+                                                   uint8_t *out, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+        out[i] = logical(LOGIC, in[i] != 0, local[i] != 0);
+}
+
+// ------------------------------------------------------------ projection
+
+__global__ void __launch_bounds__(BLOCK) k_fill_i32(int32_t *dst, int32_t v, int64_t n)
+{
+    const int64_t head = ((16 - ((uintptr_t)dst & 15)) & 15) / 4; // rows before 16B boundary
+    const int64_t h = head < n ? head : n;
+    if (blockIdx.x == 0 && threadIdx.x < h) dst[threadIdx.x] = v;
+    int32_t *body = dst + h;
+    const int64_t m = n - h;
+    const int64_t nchunks = m >> 2;
+    int4 *d4 = reinterpret_cast<int4 *>(body);
+    const int4 vv = make_int4(v, v, v, v);
+    for (int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x; c < nchunks; c += (int64_t)gridDim.x * BLOCK)
+        d4[c] = vv;
+    if (blockIdx.x == 0 && threadIdx.x < (m & 3)) body[(nchunks << 2) + threadIdx.x] = v;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_fill_u8(uint8_t *dst, uint8_t v, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+        dst[i] = v;
+}
+
+__global__ void k_zero_u64(uint64_t *p) { *p = 0; }
+
+// kind: 0 = column op column, 1 = literal op column, 2 = column op literal
+template <int OP, int KIND>
+__global__ void __launch_bounds__(BLOCK) k_binop(int32_t *result, const int32_t *a, const int32_t *b,
+                                                 int32_t lit, const uint8_t *flags, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        if (flags[i]) {
+            const int32_t x = KIND == 1 ? lit : a[i];
+            const int32_t y = KIND == 2 ? lit : b[i];
+            result[i] = element_op(OP, x, y);
+        }
+    }
+}
+
+// Vector form: 4 rows per chunk, rows whose flag is 0 keep their content.
+template <int OP, int KIND>
+__global__ void __launch_bounds__(BLOCK) k_binop_vec(int32_t *result, const int32_t *a,
+                                                     const int32_t *b, int32_t lit,
+                                                     const uint8_t *flags, int64_t n)
+{
+    const int64_t nchunks = n >> 2;
+    const uint32_t *f32 = reinterpret_cast<const uint32_t *>(flags);
+    for (int64_t base = (int64_t)blockIdx.x * BLOCK * UNROLL; base < nchunks;
+         base += (int64_t)gridDim.x * BLOCK * UNROLL) {
+        uint32_t f[UNROLL];
+        int4 x[UNROLL], y[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            const int64_t c = base + u * BLOCK + threadIdx.x;
+            f[u] = c < nchunks ? f32[c] : 0u;
+            if (f[u]) {
+                if (KIND != 1) x[u] = reinterpret_cast<const int4 *>(a)[c];
+                if (KIND != 2) y[u] = reinterpret_cast<const int4 *>(b)[c];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            if (!f[u]) continue;
+            const int64_t c = base + u * BLOCK + threadIdx.x;
+            const int32_t xs[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+            const int32_t ys[4] = {y[u].x, y[u].y, y[u].z, y[u].w};
+            int32_t r[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                r[k] = element_op(OP, KIND == 1 ? lit : xs[k], KIND == 2 ? lit : ys[k]);
+            if (f[u] == 0x01010101u) {
+                reinterpret_cast<int4 *>(result)[c] = make_int4(r[0], r[1], r[2], r[3]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (byte_of(f[u], k)) result[4 * c + k] = r[k];
+            }
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+        const int64_t i = (nchunks << 2) + threadIdx.x;
+        if (flags[i]) result[i] = element_op(OP, KIND == 1 ? lit : a[i], KIND == 2 ? lit : b[i]);
+    }
+}
+
+// ------------------------------------------------------------------ join
+
+__global__ void __launch_bounds__(BLOCK) k_build_keys(const int32_t *keys, const uint8_t *flags,
+                                                      int64_t n, uint8_t *ht, int32_t len,
+                                                      int32_t min)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        const uint32_t s = slot_of(keys[i], min);
+        if (s < (uint32_t)len) ht[s] = flags[i] ? 1 : 0;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_build_keyvals(const int32_t *keys, const int32_t *vals,
+                                                         const uint8_t *flags, int64_t n,
+                                                         int32_t *ht, int32_t len, int32_t min)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        const uint32_t s = slot_of(keys[i], min);
+        if (s >= (uint32_t)len) continue;
+        if (flags[i])
+            reinterpret_cast<int2 *>(ht)[s] = make_int2(1, vals[i]);
+        else
+            ht[2 * (int64_t)s] = 0;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_probe_semi(const int32_t *__restrict__ probe,
+                                                      const uint8_t *in, uint8_t *out, int64_t n,
+                                                      const uint8_t *__restrict__ ht, int32_t min,
+                                                      int32_t max, int vec)
+{
+    if (vec) {
+        const int64_t nchunks = n >> 2;
+        const int4 *p4 = reinterpret_cast<const int4 *>(probe);
+        const uint32_t *in32 = reinterpret_cast<const uint32_t *>(in);
+        uint32_t *out32 = reinterpret_cast<uint32_t *>(out);
+        for (int64_t base = (int64_t)blockIdx.x * BLOCK * UNROLL; base < nchunks;
+             base += (int64_t)gridDim.x * BLOCK * UNROLL) {
+            uint32_t f[UNROLL];
+            int4 x[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) {
+                const int64_t c = base + u * BLOCK + threadIdx.x;
+                f[u] = c < nchunks ? in32[c] : 0u;
+                if (f[u]) x[u] = p4[c];
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) {
+                const int64_t c = base + u * BLOCK + threadIdx.x;
+                if (c >= nchunks) continue;
+                uint32_t r = 0;
+                if (f[u]) {
+                    const int32_t xs[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const bool hit = byte_of(f[u], k) && xs[k] >= min && xs[k] <= max &&
+                                         ht[slot_of(xs[k], min)] != 0;
+                        r |= (uint32_t)hit << (8 * k);
+                    }
+                }
+                out32[c] = r;
+            }
+        }
+        if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+            const int64_t i = (nchunks << 2) + threadIdx.x;
+            const int32_t x = probe[i];
+            out[i] = in[i] && x >= min && x <= max && ht[slot_of(x, min)] != 0;
+        }
+        return;
+    }
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        const bool f = in[i] != 0;
+        bool r = false;
+        if (f) {
+            const int32_t x = probe[i];
+            r = x >= min && x <= max && ht[slot_of(x, min)] != 0;
+        }
+        out[i] = r;
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_probe_full(const int32_t *__restrict__ probe,
+                                                      int32_t *val_out, const uint8_t *in,
+                                                      uint8_t *out, int64_t n,
+                                                      const int32_t *__restrict__ ht, int32_t min,
+                                                      int32_t max, int vec)
+{
+    const int2 *ht2 = reinterpret_cast<const int2 *>(ht);
+    if (vec) {
+        const int64_t nchunks = n >> 2;
+        const int4 *p4 = reinterpret_cast<const int4 *>(probe);
+        const uint32_t *in32 = reinterpret_cast<const uint32_t *>(in);
+        uint32_t *out32 = reinterpret_cast<uint32_t *>(out);
+        for (int64_t base = (int64_t)blockIdx.x * BLOCK * UNROLL; base < nchunks;
+             base += (int64_t)gridDim.x * BLOCK * UNROLL) {
+            uint32_t f[UNROLL];
+            int4 x[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) {
+                const int64_t c = base + u * BLOCK + threadIdx.x;
+                f[u] = c < nchunks ? in32[c] : 0u;
+                if (f[u]) x[u] = p4[c];
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) {
+                const int64_t c = base + u * BLOCK + threadIdx.x;
+                if (c >= nchunks) continue;
+                uint32_t r = 0;
+                if (f[u]) {
+                    const int32_t xs[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        if (byte_of(f[u], k) && xs[k] >= min && xs[k] <= max) {
+                            const int2 e = ht2[slot_of(xs[k], min)];
+                            if (e.x == 1) {
+                                val_out[4 * c + k] = e.y;
+                                r |= 1u << (8 * k);
+                            }
+                        }
+                    }
+                }
+                out32[c] = r;
+            }
+        }
+        if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+            const int64_t i = (nchunks << 2) + threadIdx.x;
+            bool r = false;
+            if (in[i]) {
+                const int32_t x = probe[i];
+                if (x >= min && x <= max) {
+                    const int2 e = ht2[slot_of(x, min)];
+                    if (e.x == 1) {
+                        val_out[i] = e.y;
+                        r = true;
+                    }
+                }
+            }
+            out[i] = r;
+        }
+        return;
+    }
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        bool r = false;
+        if (in[i]) {
+            const int32_t x = probe[i];
+            if (x >= min && x <= max) {
+                const int2 e = ht2[slot_of(x, min)];
+                if (e.x == 1) {
+                    val_out[i] = e.y;
+                    r = true;
+                }
+            }
+        }
+        out[i] = r;
+    }
+}
+
+// ----------------------------------------------------------- aggregation
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block-wide sum; one atomic per block into *acc.
+__device__ __forceinline__ void block_add_u64(uint64_t v, uint64_t *acc)
+{
+    __shared__ uint64_t part[BLOCK / 64];
+    v = wave_sum_u64(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) part[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t s = 0;
+#pragma unroll
+        for (int w = 0; w < BLOCK / 64; w++) s += part[w];
+        if (s) atomicAdd(reinterpret_cast<unsigned long long *>(acc), (unsigned long long)s);
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_sum_flagged(const int32_t *__restrict__ col,
+                                                       const uint8_t *__restrict__ flags, int64_t n,
+                                                       uint64_t *acc, int vec)
+{
+    int64_t s = 0; // int64 partial; the uint64 total is its two's-complement image
+    if (vec) {
+        const int64_t nchunks = n >> 2;
+        const int4 *c4 = reinterpret_cast<const int4 *>(col);
+        const uint32_t *f32 = reinterpret_cast<const uint32_t *>(flags);
+        for (int64_t base = (int64_t)blockIdx.x * BLOCK * UNROLL; base < nchunks;
+             base += (int64_t)gridDim.x * BLOCK * UNROLL) {
+            uint32_t f[UNROLL];
+            int4 x[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) {
+                const int64_t c = base + u * BLOCK + threadIdx.x;
+                f[u] = c < nchunks ? f32[c] : 0u;
+                if (f[u]) x[u] = c4[c];
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) {
+                if (!f[u]) continue;
+                s += byte_of(f[u], 0) ? (int64_t)x[u].x : 0;
+                s += byte_of(f[u], 1) ? (int64_t)x[u].y : 0;
+                s += byte_of(f[u], 2) ? (int64_t)x[u].z : 0;
+                s += byte_of(f[u], 3) ? (int64_t)x[u].w : 0;
+            }
+        }
+        if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+            const int64_t i = (nchunks << 2) + threadIdx.x;
+            if (flags[i]) s += col[i];
+        }
+    } else {
+        for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+            if (flags[i]) s += col[i];
+    }
+    block_add_u64((uint64_t)s, acc);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_count_flags(const uint8_t *flags, int64_t n, uint64_t *acc,
+                                                       int vec)
+{
+    uint64_t s = 0;
+    if (vec) {
+        const int64_t nw = n >> 2;
+        const uint32_t *f32 = reinterpret_cast<const uint32_t *>(flags);
+        for (int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x; c < nw; c += (int64_t)gridDim.x * BLOCK) {
+            const uint32_t w = f32[c];
+            // one bit per non-zero byte
+            const uint32_t nz = (((w & 0x7f7f7f7fu) + 0x7f7f7f7fu) | w) & 0x80808080u;
+            s += __popc(nz);
+        }
+        if (blockIdx.x == 0 && threadIdx.x < (n & 3)) s += flags[(nw << 2) + threadIdx.x] != 0;
+    } else {
+        for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
+            s += flags[i] != 0;
+    }
+    block_add_u64(s, acc);
+}
+
+constexpr int GB_MAX_KEYS = 8;
+struct GroupArgs {
+    const int32_t *keys[GB_MAX_KEYS];
+    int32_t *out_keys[GB_MAX_KEYS];
+    int32_t mins[GB_MAX_KEYS];
+    int32_t maxs[GB_MAX_KEYS];
+    uint32_t range[GB_MAX_KEYS]; // max - min + 1 (uint32 image of the int)
+    int k;
+};
+
+// Dense slot of row i exactly as GroupByAggregateKernel computes it
+// (int32 wrapping accumulation, then % prod on the uint64 image).
+__device__ __forceinline__ uint32_t group_slot(const GroupArgs &g, const int32_t *kv, uint64_t prod)
+{
+    uint32_t hash = 0, mult = 1;
+    for (int j = 0; j < g.k; j++) {
+        hash += ((uint32_t)kv[j] - (uint32_t)g.mins[j]) * mult;
+        mult *= g.range[j];
+    }
+    const uint64_t h64 = (uint64_t)(int64_t)(int32_t)hash;
+    return (uint32_t)(h64 % prod);
+}
+
+// Global path: one exchange + one atomic add per flagged row.
+__device__ __forceinline__ void group_row_global(const GroupArgs &g, const int32_t *kv, int32_t agg,
+                                                 uint64_t *sums, uint32_t *seen, uint64_t prod)
+{
+    const uint32_t s = group_slot(g, kv, prod);
+    if (atomicExch(&seen[s], 1u) == 0u)
+        for (int j = 0; j < g.k; j++) g.out_keys[j][s] = kv[j];
+    atomicAdd(reinterpret_cast<unsigned long long *>(&sums[s]), (unsigned long long)(int64_t)agg);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_groupby_global(GroupArgs g, const int32_t *agg,
+                                                          const uint8_t *flags, int64_t n,
+                                                          uint64_t *sums, uint32_t *seen, uint64_t prod)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        if (!flags[i]) continue;
+        int32_t kv[GB_MAX_KEYS];
+        for (int j = 0; j < g.k; j++) kv[j] = g.keys[j][i];
+        group_row_global(g, kv, agg[i], sums, seen, prod);
+    }
+}
+
+// LDS-privatised dense group-by: each block accumulates its rows into an LDS
+// copy of the dense sum array (ds_add_u64) plus an occupancy bitmap, then
+// flushes the occupied slots with one global atomic each.  Rows whose keys
+// all lie in [min, max] map to a unique slot whose key values are recovered
+// from the slot index at flush; any other row takes the global path so the
+// key it stores is its own.
+__global__ void __launch_bounds__(BLOCK) k_groupby_lds(GroupArgs g, const int32_t *agg,
+                                                       const uint8_t *flags, int64_t n,
+                                                       uint64_t *sums, uint32_t *seen,
+                                                       uint32_t prod)
+{
+    extern __shared__ unsigned long long lds[];
+    unsigned long long *lsum = lds;
+    uint32_t *lbits = reinterpret_cast<uint32_t *>(lds + prod);
+    const uint32_t nwords = (prod + 31) / 32;
+    for (uint32_t s = threadIdx.x; s < prod; s += BLOCK) lsum[s] = 0;
+    for (uint32_t w = threadIdx.x; w < nwords; w += BLOCK) lbits[w] = 0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        if (!flags[i]) continue;
+        int32_t kv[GB_MAX_KEYS];
+        bool in_range = true;
+        for (int j = 0; j < g.k; j++) {
+            kv[j] = g.keys[j][i];
+            in_range &= kv[j] >= g.mins[j] && kv[j] <= g.maxs[j];
+        }
+        if (!in_range) {
+            group_row_global(g, kv, agg[i], sums, seen, prod);
+            continue;
+        }
+        const uint32_t s = group_slot(g, kv, prod);
+        atomicAdd(&lsum[s], (unsigned long long)(int64_t)agg[i]);
+        atomicOr(&lbits[s >> 5], 1u << (s & 31));
+    }
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < prod; s += BLOCK) {
+        if (!((lbits[s >> 5] >> (s & 31)) & 1u)) continue;
+        atomicAdd(reinterpret_cast<unsigned long long *>(&sums[s]), lsum[s]);
+        if (atomicExch(&seen[s], 1u) == 0u) {
+            uint32_t rest = s;
+            for (int j = 0; j < g.k; j++) {
+                g.out_keys[j][s] = (int32_t)((uint32_t)g.mins[j] + rest % g.range[j]);
+                rest /= g.range[j];
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_seen_to_flags(const uint32_t *seen, uint8_t *flags, uint64_t prod)
+{
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < (int64_t)prod; i += (int64_t)gridDim.x * BLOCK)
+        flags[i] = seen[i] != 0;
+}
+
+// ------------------------------------------------------------ compaction
+// Three passes over tiles of COMPACT_TILE rows: count per tile, exclusive
+// scan of the counts (one block), then write ids in row order.
+constexpr int COMPACT_PER_THREAD = 16;
+constexpr int64_t COMPACT_TILE = (int64_t)BLOCK * COMPACT_PER_THREAD;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *total)
+{
+    __shared__ uint32_t wsum[BLOCK / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; w++) {
+        if (w < wave) before += wsum[w];
+        all += wsum[w];
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_compact_count(const uint8_t *flags, int64_t n, uint32_t *counts)
+{
+    const int64_t t0 = (int64_t)blockIdx.x * COMPACT_TILE + (int64_t)threadIdx.x * COMPACT_PER_THREAD;
+    uint32_t c = 0;
+    for (int k = 0; k < COMPACT_PER_THREAD; k++)
+        if (t0 + k < n) c += flags[t0 + k] != 0;
+    uint32_t total;
+    block_exclusive_scan(c, &total);
+    if (threadIdx.x == 0) counts[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_compact_scan(uint32_t *counts, int64_t ntiles, uint32_t *count_out)
+{
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int64_t base = 0; base < ntiles; base += BLOCK) {
+        const int64_t t = base + threadIdx.x;
+        const uint32_t v = t < ntiles ? counts[t] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan(v, &total);
+        if (t < ntiles) counts[t] = carry + ex;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *count_out = carry;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_compact_write(const uint8_t *flags, int64_t n,
+                                                         const uint32_t *offsets, int32_t *row_ids)
+{
+    const int64_t t0 = (int64_t)blockIdx.x * COMPACT_TILE + (int64_t)threadIdx.x * COMPACT_PER_THREAD;
+    uint32_t c = 0;
+    for (int k = 0; k < COMPACT_PER_THREAD; k++)
+        if (t0 + k < n) c += flags[t0 + k] != 0;
+    uint32_t total;
+    uint32_t pos = offsets[blockIdx.x] + block_exclusive_scan(c, &total);
+    for (int k = 0; k < COMPACT_PER_THREAD; k++)
+        if (t0 + k < n && flags[t0 + k]) row_ids[pos++] = (int32_t)(t0 + k);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_gather_i32(const int32_t *src, const int32_t *ids, int64_t count,
+                                                      int32_t *dst)
+{
+    for (int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x; j < count; j += (int64_t)gridDim.x * BLOCK)
+        dst[j] = src[ids[j]];
+}
+
+__global__ void __launch_bounds__(BLOCK) k_minmax(const int32_t *col, int64_t n, int32_t *mm)
+{
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        const int32_t v = col[i];
+        lo = v < lo ? v : lo;
+        hi = v > hi ? v : hi;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const int32_t a = __shfl_xor(lo, off, 64), b = __shfl_xor(hi, off, 64);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&mm[0], lo);
+        atomicMax(&mm[1], hi);
+    }
+}
+
+__global__ void k_minmax_init(int32_t *mm)
+{
+    mm[0] = INT32_MAX;
+    mm[1] = INT32_MIN;
+}
+
+// --------------------------------------------------------- dispatch helpers
+
+void check_launch(const char *what)
+{
+    hip_check(hipGetLastError(), what);
+}
+
+unsigned grid_rows(int64_t n) { return stream_grid(n, ROWS_PER_STEP); }
+unsigned grid_one(int64_t n) { return stream_grid(n, BLOCK); }
+
+template <int CMP, int LOGIC>
+void run_select_lit(const int32_t *col, const uint8_t *in, uint8_t *out, int64_t n, int32_t v, hipStream_t s)
+{
+    if (aligned(col, 16) && aligned(in, 4) && aligned(out, 4))
+        hipLaunchKernelGGL((k_select_lit<CMP, LOGIC>), dim3(grid_rows(n)), dim3(BLOCK), 0, s, col, in, out, n, v);
+    else
+        hipLaunchKernelGGL((k_select_lit_row<CMP, LOGIC>), dim3(grid_one(n)), dim3(BLOCK), 0, s, col, in, out, n, v);
+}
+
+template <int CMP, int LOGIC>
+void run_select_col(const int32_t *a, const int32_t *b, const uint8_t *in, uint8_t *out, int64_t n, hipStream_t s)
+{
+    if (aligned(a, 16) && aligned(b, 16) && aligned(in, 4) && aligned(out, 4))
+        hipLaunchKernelGGL((k_select_col<CMP, LOGIC>), dim3(grid_rows(n)), dim3(BLOCK), 0, s, a, b, in, out, n);
+    else
+        hipLaunchKernelGGL((k_select_col_row<CMP, LOGIC>), dim3(grid_one(n)), dim3(BLOCK), 0, s, a, b, in, out, n);
+}
+
+template <int CMP>
+void sel_lit_logic(int logic, const int32_t *col, const uint8_t *in, uint8_t *out, int64_t n, int32_t v, hipStream_t s)
+{
+    switch (logic) {
+    case SDB_NONE: run_select_lit<CMP, SDB_NONE>(col, in, out, n, v, s); break;
+    case SDB_AND: run_select_lit<CMP, SDB_AND>(col, in, out, n, v, s); break;
+    default: run_select_lit<CMP, SDB_OR>(col, in, out, n, v, s); break;
+    }
+}
+
+template <int CMP>
+void sel_col_logic(int logic, const int32_t *a, const int32_t *b, const uint8_t *in, uint8_t *out, int64_t n, hipStream_t s)
+{
+    switch (logic) {
+    case SDB_NONE: run_select_col<CMP, SDB_NONE>(a, b, in, out, n, s); break;
+    case SDB_AND: run_select_col<CMP, SDB_AND>(a, b, in, out, n, s); break;
+    default: run_select_col<CMP, SDB_OR>(a, b, in, out, n, s); break;
+    }
+}
+
+template <int KIND>
+void run_binop(int op, int32_t *r, const int32_t *a, const int32_t *b, int32_t lit, const uint8_t *f, int64_t n,
+               hipStream_t s)
+{
+    const bool vec = aligned(r, 16) && aligned(f, 4) && (KIND == 1 || aligned(a, 16)) && (KIND == 2 || aligned(b, 16));
+#define SDB_BINOP_CASE(OPV)                                                                              \
+    case OPV:                                                                                            \
+        if (vec)                                                                                         \
+            hipLaunchKernelGGL((k_binop_vec<OPV, KIND>), dim3(grid_rows(n)), dim3(BLOCK), 0, s, r, a, b, lit, f, n); \
+        else                                                                                             \
+            hipLaunchKernelGGL((k_binop<OPV, KIND>), dim3(grid_one(n)), dim3(BLOCK), 0, s, r, a, b, lit, f, n); \
+        break;
+    switch (op) {
+        SDB_BINOP_CASE(SDB_MUL)
+        SDB_BINOP_CASE(SDB_DIV)
+        SDB_BINOP_CASE(SDB_ADD)
+        SDB_BINOP_CASE(SDB_SUB)
+    }
+#undef SDB_BINOP_CASE
+}
+
+} // namespace
+
+// ------------------------------------------------------ internal launchers
+// (used by the engine's unfused bundle path as well as the C ABI)
+
+void launch_select_lit(const int32_t *col, const uint8_t *in, uint8_t *out, int64_t n, int cmp, int logic,
+                       int32_t v, hipStream_t s)
+{
+    if (n <= 0) return;
+    switch (cmp) {
+    case SDB_EQ: sel_lit_logic<SDB_EQ>(logic, col, in, out, n, v, s); break;
+    case SDB_NE: sel_lit_logic<SDB_NE>(logic, col, in, out, n, v, s); break;
+    case SDB_LT: sel_lit_logic<SDB_LT>(logic, col, in, out, n, v, s); break;
+    case SDB_LE: sel_lit_logic<SDB_LE>(logic, col, in, out, n, v, s); break;
+    case SDB_GT: sel_lit_logic<SDB_GT>(logic, col, in, out, n, v, s); break;
+    default: sel_lit_logic<SDB_GE>(logic, col, in, out, n, v, s); break;
+    }
+    check_launch("select_lit");
+}
+
+void launch_select_col(const int32_t *a, const int32_t *b, const uint8_t *in, uint8_t *out, int64_t n, int cmp,
+                       int logic, hipStream_t s)
+{
+    if (n <= 0) return;
+    switch (cmp) {
+    case SDB_EQ: sel_col_logic<SDB_EQ>(logic, a, b, in, out, n, s); break;
+    case SDB_NE: sel_col_logic<SDB_NE>(logic, a, b, in, out, n, s); break;
+    case SDB_LT: sel_col_logic<SDB_LT>(logic, a, b, in, out, n, s); break;
+    case SDB_LE: sel_col_logic<SDB_LE>(logic, a, b, in, out, n, s); break;
+    case SDB_GT: sel_col_logic<SDB_GT>(logic, a, b, in, out, n, s); break;
+    default: sel_col_logic<SDB_GE>(logic, a, b, in, out, n, s); break;
+    }
+    check_launch("select_col");
+}
+
+void launch_logical(const uint8_t *in, const uint8_t *local, uint8_t *out, int64_t n, int logic, hipStream_t s)
+{
+    if (n <= 0) return;
+    switch (logic) {
+    case SDB_NONE: hipLaunchKernelGGL((k_logical<SDB_NONE>), dim3(grid_one(n)), dim3(BLOCK), 0, s, in, local, out, n); break;
+    case SDB_AND: hipLaunchKernelGGL((k_logical<SDB_AND>), dim3(grid_one(n)), dim3(BLOCK), 0, s, in, local, out, n); break;
+    default: hipLaunchKernelGGL((k_logical<SDB_OR>), dim3(grid_one(n)), dim3(BLOCK), 0, s, in, local, out, n); break;
+    }
+    check_launch("logical");
+}
+
+void launch_fill_i32(int32_t *dst, int32_t v, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_i32, dim3(stream_grid(n, BLOCK * 4)), dim3(BLOCK), 0, s, dst, v, n);
+    check_launch("fill_i32");
+}
+
+void launch_fill_u8(uint8_t *dst, uint8_t v, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_u8, dim3(grid_one(n)), dim3(BLOCK), 0, s, dst, v, n);
+    check_launch("fill_u8");
+}
+
+void launch_binop(int kind, int op, int32_t *r, const int32_t *a, const int32_t *b, int32_t lit,
+                  const uint8_t *f, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return;
+    if (kind == 0) run_binop<0>(op, r, a, b, lit, f, n, s);
+    else if (kind == 1) run_binop<1>(op, r, a, b, lit, f, n, s);
+    else run_binop<2>(op, r, a, b, lit, f, n, s);
+    check_launch("binop");
+}
+
+void launch_build_keys(const int32_t *keys, const uint8_t *flags, int64_t n, uint8_t *ht, int32_t len,
+                       int32_t min, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_build_keys, dim3(grid_one(n)), dim3(BLOCK), 0, s, keys, flags, n, ht, len, min);
+    check_launch("build_keys");
+}
+
+void launch_build_keyvals(const int32_t *keys, const int32_t *vals, const uint8_t *flags, int64_t n, int32_t *ht,
+                          int32_t len, int32_t min, hipStream_t s)
+{
+    if (n <= 0) return;
+    if (!aligned(ht, 8)) fail(SDB_ERR_INVALID_ARG, "key-value hash table must be 8-byte aligned");
+    hipLaunchKernelGGL(k_build_keyvals, dim3(grid_one(n)), dim3(BLOCK), 0, s, keys, vals, flags, n, ht, len, min);
+    check_launch("build_keyvals");
+}
+
+void launch_probe_semi(const int32_t *probe, const uint8_t *in, uint8_t *out, int64_t n, const uint8_t *ht,
+                       int32_t min, int32_t max, hipStream_t s)
+{
+    if (n <= 0) return;
+    const int vec = aligned(probe, 16) && aligned(in, 4) && aligned(out, 4);
+    hipLaunchKernelGGL(k_probe_semi, dim3(vec ? grid_rows(n) : grid_one(n)), dim3(BLOCK), 0, s, probe, in, out, n,
+                       ht, min, max, vec);
+    check_launch("probe_semi");
+}
+
+void launch_probe_full(const int32_t *probe, int32_t *val_out, const uint8_t *in, uint8_t *out, int64_t n,
+                       const int32_t *ht, int32_t min, int32_t max, hipStream_t s)
+{
+    if (n <= 0) return;
+    if (!aligned(ht, 8)) fail(SDB_ERR_INVALID_ARG, "key-value hash table must be 8-byte aligned");
+    const int vec = aligned(probe, 16) && aligned(in, 4) && aligned(out, 4);
+    hipLaunchKernelGGL(k_probe_full, dim3(vec ? grid_rows(n) : grid_one(n)), dim3(BLOCK), 0, s, probe, val_out, in,
+                       out, n, ht, min, max, vec);
+    check_launch("probe_full");
+}
+
+void launch_sum_flagged(const int32_t *col, const uint8_t *flags, int64_t n, uint64_t *acc, hipStream_t s)
+{
+    if (n <= 0) return;
+    if (!aligned(acc, 8)) fail(SDB_ERR_INVALID_ARG, "accumulator must be 8-byte aligned");
+    const int vec = aligned(col, 16) && aligned(flags, 4);
+    hipLaunchKernelGGL(k_sum_flagged, dim3(vec ? grid_rows(n) : grid_one(n)), dim3(BLOCK), 0, s, col, flags, n, acc,
+                       vec);
+    check_launch("sum_flagged");
+}
+
+void launch_count_flags(const uint8_t *flags, int64_t n, uint64_t *out, bool accumulate, hipStream_t s)
+{
+    if (!aligned(out, 8)) fail(SDB_ERR_INVALID_ARG, "counter must be 8-byte aligned");
+    if (!accumulate) {
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, out);
+        check_launch("count_flags init");
+    }
+    if (n <= 0) return;
+    const int vec = aligned(flags, 4);
+    hipLaunchKernelGGL(k_count_flags, dim3(stream_grid(n, BLOCK * 4)), dim3(BLOCK), 0, s, flags, n, out, vec);
+    check_launch("count_flags");
+}
+
+// LDS budget of the privatised group-by (bytes); the slot array is u64 plus a
+// one-bit occupancy map.
+constexpr uint64_t GROUPBY_LDS_BYTES = 64 * 1024;
+
+uint64_t groupby_lds_max_slots() { return (GROUPBY_LDS_BYTES * 8) / (64 + 1); }
+
+void launch_groupby_dense(const int32_t *const *keys, int k, const int32_t *mins, const int32_t *maxs,
+                          const int32_t *agg, const uint8_t *flags, int64_t n, int32_t *const *out_keys,
+                          uint64_t *sums, uint32_t *seen, uint64_t prod, hipStream_t s, int strategy)
+{
+    if (k < 1 || k > GB_MAX_KEYS) fail(SDB_ERR_INVALID_ARG, "group-by supports 1..8 key columns");
+    if (prod == 0) fail(SDB_ERR_INVALID_ARG, "group-by range product is 0");
+    if (n <= 0) return;
+    GroupArgs g{};
+    g.k = k;
+    for (int j = 0; j < k; j++) {
+        g.keys[j] = keys[j];
+        g.out_keys[j] = out_keys[j];
+        g.mins[j] = mins[j];
+        g.maxs[j] = maxs[j];
+        g.range[j] = (uint32_t)maxs[j] - (uint32_t)mins[j] + 1u;
+    }
+    // the slot decode at flush is exact when the in-range index cannot wrap
+    uint64_t exact = 1;
+    bool fits = true;
+    for (int j = 0; j < k; j++) {
+        if (maxs[j] < mins[j]) fits = false;
+        exact *= (uint64_t)((int64_t)maxs[j] - (int64_t)mins[j] + 1);
+        if (exact > prod) fits = false;
+    }
+    fits = fits && exact == prod;
+    const bool lds = strategy != 1 && fits && prod <= groupby_lds_max_slots();
+    if (lds) {
+        const size_t bytes = prod * 8 + ((prod + 31) / 32) * 4;
+        hipLaunchKernelGGL(k_groupby_lds, dim3(stream_grid(n, BLOCK, 2)), dim3(BLOCK), bytes, s, g, agg, flags, n,
+                           sums, seen, (uint32_t)prod);
+    } else {
+        hipLaunchKernelGGL(k_groupby_global, dim3(grid_one(n)), dim3(BLOCK), 0, s, g, agg, flags, n, sums, seen,
+                           prod);
+    }
+    check_launch("groupby_dense");
+}
+
+void launch_seen_to_flags(const uint32_t *seen, uint8_t *flags, uint64_t prod, hipStream_t s)
+{
+    if (prod == 0) return;
+    hipLaunchKernelGGL(k_seen_to_flags, dim3(grid_one((int64_t)prod)), dim3(BLOCK), 0, s, seen, flags, prod);
+    check_launch("seen_to_flags");
+}
+
+void launch_compact(const uint8_t *flags, int64_t n, int32_t *row_ids, uint32_t *count, hipStream_t s)
+{
+    const int64_t ntiles = n > 0 ? (n + COMPACT_TILE - 1) / COMPACT_TILE : 0;
+    uint32_t *counts = reinterpret_cast<uint32_t *>(scratch_buffer((size_t)std::max<int64_t>(ntiles, 1) * 4, s));
+    if (ntiles > 0)
+        hipLaunchKernelGGL(k_compact_count, dim3((unsigned)ntiles), dim3(BLOCK), 0, s, flags, n, counts);
+    hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(BLOCK), 0, s, counts, ntiles, count);
+    if (ntiles > 0)
+        hipLaunchKernelGGL(k_compact_write, dim3((unsigned)ntiles), dim3(BLOCK), 0, s, flags, n, counts, row_ids);
+    check_launch("compact");
+}
+
+void launch_gather_i32(const int32_t *src, const int32_t *ids, int64_t count, int32_t *dst, hipStream_t s)
+{
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_gather_i32, dim3(grid_one(count)), dim3(BLOCK), 0, s, src, ids, count, dst);
+    check_launch("gather_i32");
+}
+
+void minmax_i32(const int32_t *col, int64_t n, int32_t *mn, int32_t *mx, hipStream_t s)
+{
+    if (n <= 0) fail(SDB_ERR_INVALID_ARG, "min/max of an empty column");
+    int32_t *mm = reinterpret_cast<int32_t *>(scratch_buffer(8, s));
+    hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(1), 0, s, mm);
+    hipLaunchKernelGGL(k_minmax, dim3(grid_one(n)), dim3(BLOCK), 0, s, col, n, mm);
+    check_launch("minmax");
+    int32_t host[2];
+    SDB_HIP(hipMemcpyAsync(host, mm, 8, hipMemcpyDeviceToHost, s));
+    SDB_HIP(hipStreamSynchronize(s));
+    *mn = host[0];
+    *mx = host[1];
+}
+
+} // namespace sdb
+
+// =================================================================== C ABI
+
+using namespace sdb;
+
+namespace {
+inline void check_n(int64_t n) { require(n >= 0, "negative length"); }
+inline void check_cmp(int cmp) { require(cmp >= SDB_EQ && cmp <= SDB_GE, "comparison op out of range"); }
+inline void check_logic(int l) { require(l >= SDB_NONE && l <= SDB_OR, "logical op out of range"); }
+inline void check_binop(int op) { require(op >= SDB_MUL && op <= SDB_SUB, "arithmetic op out of range"); }
+} // namespace
+
+extern "C" {
+
+int sdb_comp_op_from_string(const char *op)
+{
+    const std::string s = op ? op : "";
+    if (s == "==") return SDB_EQ;
+    if (s == ">=") return SDB_GE;
+    if (s == "<=") return SDB_LE;
+    if (s == "<") return SDB_LT;
+    if (s == ">") return SDB_GT;
+    if (s == "!=" || s == "<>") return SDB_NE;
+    return SDB_EQ;
+}
+
+int sdb_logical_op_from_string(const char *op)
+{
+    const std::string s = op ? op : "";
+    if (s == "AND") return SDB_AND;
+    if (s == "OR") return SDB_OR;
+    return SDB_NONE;
+}
+
+int sdb_binop_from_string(const char *op)
+{
+    const std::string s = op ? op : "";
+    if (s == "*") return SDB_MUL;
+    if (s == "/") return SDB_DIV;
+    if (s == "+") return SDB_ADD;
+    if (s == "-") return SDB_SUB;
+    return -1;
+}
+
+int sdb_select_lit(const int32_t *col, const uint8_t *in_flags, uint8_t *out_flags, int64_t n, int cmp, int logic,
+                   int32_t value, sdb_stream stream)
+{
+    return guarded([&] {
+        check_n(n), check_cmp(cmp), check_logic(logic);
+        require(n == 0 || (col && in_flags && out_flags), "null pointer");
+        launch_select_lit(col, in_flags, out_flags, n, cmp, logic, value, as_stream(stream));
+    });
+}
+
+int sdb_select_col(const int32_t *a, const int32_t *b, const uint8_t *in_flags, uint8_t *out_flags, int64_t n,
+                   int cmp, int logic, sdb_stream stream)
+{
+    return guarded([&] {
+        check_n(n), check_cmp(cmp), check_logic(logic);
+        require(n == 0 || (a && b && in_flags && out_flags), "null pointer");
+        launch_select_col(a, b, in_flags, out_flags, n, cmp, logic, as_stream(stream));
+    });
+}
+
+int sdb_logical(const uint8_t *in_flags, const uint8_t *local_flags, uint8_t *out_flags, int64_t n, int logic,
+                sdb_stream stream)
+{
+    return guarded([&] {
+        check_n(n), check_logic(logic);
+        require(n == 0 || (in_flags && local_flags && out_flags), "null pointer");
+        launch_logical(in_flags, local_flags, out_flags, n, logic, as_stream(stream));
+    });
+}
+
+int sdb_fill_i32(int32_t *dst, int32_t value, int64_t n, sdb_stream stream)
+{
+    return guarded([&] {
+        check_n(n);
+        require(n == 0 || dst, "null pointer");
+        require(((uintptr_t)dst & 3) == 0, "int32 column must be 4-byte aligned");
+        launch_fill_i32(dst, value, n, as_stream(stream));
+    });
+}
+
+int sdb_binop_cc(int32_t *result, const int32_t *a, const int32_t *b, const uint8_t *flags, int64_t n, int op,
+                 sdb_stream stream)
+{
+    return guarded([&] {
+        check_n(n), check_binop(op);
+        require(n == 0 || (result && a && b && flags), "null pointer");
+        launch_binop(0, op, result, a, b, 0, flags, n, as_stream(stream));
+    });
+}
+
+int sdb_binop_lc(int32_t *result, int32_t a, const int32_t *b, const uint8_t *flags, int64_t n, int op,
+                 sdb_stream stream)
+{
+    return guarded([&] {
+        check_n(n), check_binop(op);
+        require(n == 0 || (result && b && flags), "null pointer");
+        launch_binop(1, op, result, nullptr, b, a, flags, n, as_stream(stream));
+    });
+}
+
+int sdb_binop_cl(int32_t *result, const int32_t *a, int32_t b, const uint8_t *flags, int64_t n, int op,
+                 sdb_stream stream)
+{
+    return guarded([&] {
+        check_n(n), check_binop(op);
+        require(n == 0 || (result && a && flags), "null pointer");
+        launch_binop(2, op, result, a, nullptr, b, flags, n, as_stream(stream));
+    });
+}
+
+int sdb_build_keys(const int32_t *keys, const uint8_t *flags, int64_t n, uint8_t *ht, int32_t ht_len,
+                   int32_t ht_min, sdb_stream stream)
+{
+    return guarded([&] {
+        check_n(n);
+        require(ht_len > 0, "hash table length must be positive");
+        require(n == 0 || (keys && flags && ht), "null pointer");
+        launch_build_keys(keys, flags, n, ht, ht_len, ht_min, as_stream(stream));
+    });
+}
+
+int sdb_build_keyvals(const int32_t *keys, const int32_t *vals, const uint8_t *flags, int64_t n, int32_t *ht,
+                      int32_t ht_len, int32_t ht_min, sdb_stream stream)
+{
+    return guarded([&] {
+        check_n(n);
+        require(ht_len > 0, "hash table length must be positive");
+        require(n == 0 || (keys && vals && flags && ht), "null pointer");
+        launch_build_keyvals(keys, vals, flags, n, ht, ht_len, ht_min, as_stream(stream));
+    });
+}
+
+int sdb_probe_semi(const int32_t *probe, const uint8_t *in_flags, uint8_t *out_flags, int64_t n, const uint8_t *ht,
+                   int32_t ht_min, int32_t ht_max, sdb_stream stream)
+{
+    return guarded([&] {
+        check_n(n);
+        require(ht_max >= ht_min, "ht_max < ht_min");
+        require(n == 0 || (probe && in_flags && out_flags && ht), "null pointer");
+        launch_probe_semi(probe, in_flags, out_flags, n, ht, ht_min, ht_max, as_stream(stream));
+    });
+}
+
+int sdb_probe_full(const int32_t *probe, int32_t *val_out, const uint8_t *in_flags, uint8_t *out_flags, int64_t n,
+                   const int32_t *ht, int32_t ht_min, int32_t ht_max, sdb_stream stream)
+{
+    return guarded([&] {
+        check_n(n);
+        require(ht_max >= ht_min, "ht_max < ht_min");
+        require(n == 0 || (probe && val_out && in_flags && out_flags && ht), "null pointer");
+        launch_probe_full(probe, val_out, in_flags, out_flags, n, ht, ht_min, ht_max, as_stream(stream));
+    });
+}
+
+int sdb_sum_flagged(const int32_t *col, const uint8_t *flags, int64_t n, uint64_t *acc, sdb_stream stream)
+{
+    return guarded([&] {
+        check_n(n);
+        require(acc, "null accumulator");
+        require(n == 0 || (col && flags), "null pointer");
+        launch_sum_flagged(col, flags, n, acc, as_stream(stream));
+    });
+}
+
+int sdb_groupby_dense(const int32_t *const *keys, int k, const int32_t *mins, const int32_t *maxs,
+                      const int32_t *agg, const uint8_t *flags, int64_t n, int32_t *const *out_keys,
+                      uint64_t *sums, uint32_t *seen, uint64_t prod, sdb_stream stream)
+{
+    return guarded([&] {
+        check_n(n);
+        require(keys && mins && maxs && out_keys && sums && seen, "null pointer");
+        require(n == 0 || (agg && flags), "null pointer");
+        for (int j = 0; j < k && j < 8; j++) require(keys[j] && out_keys[j], "null key column");
+        launch_groupby_dense(keys, k, mins, maxs, agg, flags, n, out_keys, sums, seen, prod, as_stream(stream), 0);
+    });
+}
+
+int sdb_seen_to_flags(const uint32_t *seen, uint8_t *flags, uint64_t prod, sdb_stream stream)
+{
+    return guarded([&] {
+        require(prod == 0 || (seen && flags), "null pointer");
+        launch_seen_to_flags(seen, flags, prod, as_stream(stream));
+    });
+}
+
+int sdb_count_flags(const uint8_t *flags, int64_t n, uint64_t *out, int accumulate, sdb_stream stream)
+{
+    return guarded([&] {
+        check_n(n);
+        require(out, "null output");
+        require(n == 0 || flags, "null pointer");
+        launch_count_flags(flags, n, out, accumulate != 0, as_stream(stream));
+    });
+}
+
+int sdb_compact(const uint8_t *flags, int64_t n, int32_t *row_ids, uint32_t *count, sdb_stream stream)
+{
+    return guarded([&] {
+        check_n(n);
+        require(n <= INT32_MAX, "compaction is limited to 2^31-1 rows (int32 row ids)");
+        require(count, "null count");
+        require(n == 0 || (flags && row_ids), "null pointer");
+        launch_compact(flags, n, row_ids, count, as_stream(stream));
+    });
+}
+
+int sdb_gather_i32(const int32_t *src, const int32_t *row_ids, int64_t count, int32_t *dst, sdb_stream stream)
+{
+    return guarded([&] {
+        check_n(count);
+        require(count == 0 || (src && row_ids && dst), "null pointer");
+        launch_gather_i32(src, row_ids, count, dst, as_stream(stream));
+    });
+}
+
+int sdb_minmax_i32(const int32_t *col, int64_t n, int32_t *min_out, int32_t *max_out, sdb_stream stream)
+{
+    return guarded([&] {
+        require(n > 0, "min/max needs at least one row");
+        require(col && min_out && max_out, "null pointer");
+        minmax_i32(col, n, min_out, max_out, as_stream(stream));
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,212 @@
+"""Engine API: HBM-resident star-schema tables and plan execution.
+
+Wraps the C ABI's engine (include/sycldb.h): tables are created on the device,
+filled by the seeded device generator (or uploaded / loaded from the
+reference's columnar binary files), and plans -- JSON documents mirroring
+calciteserver.thrift -- run through the TransientTable / KernelBundle model
+(models/transient_table.hpp) to completion.
+"""
+import ctypes
+import json
+
+import numpy as np
+
+from . import _lib as L
+from . import schema as S
+
+
+class Result:
+    """Host copy of a plan's final table (TransientTable after the last
+    flush): ``nrows`` rows, selection ``flags``, non-null ``columns``."""
+
+    def __init__(self, handle):
+        lib = L.lib
+        try:
+            self.nrows = int(lib.sdb_result_nrows(handle))
+            self.flags = np.zeros(self.nrows, dtype=np.uint8)
+            if self.nrows:
+                L.check(lib.sdb_result_flags_copy(handle, self.flags.ctypes.data_as(ctypes.c_void_p)))
+            self.columns = []
+            for c in range(lib.sdb_result_ncols(handle)):
+                is_u64 = lib.sdb_result_col_is_u64(handle, c) == 1
+                arr = np.zeros(self.nrows, dtype=np.uint64 if is_u64 else np.int32)
+                if self.nrows:
+                    L.check(lib.sdb_result_col_copy(handle, c, arr.ctypes.data_as(ctypes.c_void_p)))
+                self.columns.append((is_u64, arr))
+        finally:
+            lib.sdb_result_destroy(handle)
+
+    def rows(self):
+        """Selected rows as tuples of Python ints (result_writer.cpp:41-67)."""
+        idx = np.nonzero(self.flags)[0]
+        cols = [arr[idx] for _, arr in self.columns]
+        return [tuple(int(c[r]) for c in cols) for r in range(len(idx))]
+
+    def text(self):
+        return "".join(" ".join(str(v) for v in row) + "\n" for row in self.rows())
+
+
+class Engine:
+    def __init__(self, device_budget_bytes=0, segment_size=0, fusion=True, device=0):
+        opts = L.EngineOptions(device_budget_bytes, segment_size, 0 if fusion else 1, device)
+        h = ctypes.c_void_p()
+        L.check(L.lib.sdb_engine_create(ctypes.byref(opts), ctypes.byref(h)))
+        self._h = h
+        self.segment_size = segment_size or (1 << 28)
+        self.fusion = fusion
+
+    def close(self):
+        if self._h:
+            L.lib.sdb_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def stream(self):
+        return L.lib.sdb_engine_stream(self._h)
+
+    # ---------------------------------------------------------------- tables
+    def create_table(self, name, nrows):
+        L.check(L.lib.sdb_engine_table_create(self._h, name.encode(), int(nrows)))
+
+    def column_ptr(self, name, col):
+        p = ctypes.c_void_p()
+        L.check(L.lib.sdb_engine_column_alloc(self._h, name.encode(), int(col), ctypes.byref(p)))
+        return p.value
+
+    def upload(self, name, col, data):
+        a = np.ascontiguousarray(data, dtype=np.int32)
+        L.check(L.lib.sdb_engine_column_upload(self._h, name.encode(), int(col), a.ctypes.data_as(ctypes.c_void_p)))
+
+    def download(self, name, col, nrows):
+        a = np.empty(int(nrows), dtype=np.int32)
+        L.check(L.lib.sdb_engine_column_download(self._h, name.encode(), int(col),
+                                                 a.ctypes.data_as(ctypes.c_void_p)))
+        return a
+
+    def finalize(self, name):
+        L.check(L.lib.sdb_engine_table_finalize(self._h, name.encode()))
+
+    def load_dir(self, name, directory):
+        L.check(L.lib.sdb_engine_table_load_dir(self._h, name.encode(), directory.encode()))
+
+    def load_tables(self, tables):
+        """tables: {name: {col: int32 array}} (host data)."""
+        for name, cols in tables.items():
+            n = len(next(iter(cols.values())))
+            self.create_table(name, n)
+            for c, data in cols.items():
+                self.upload(name, c, data)
+            self.finalize(name)
+
+    def generate(self, fact_rows, seed, sizes=None, fact_columns=None, tables=None):
+        """Generates the seeded star schema on the device (schema.column_spec)."""
+        sizes = dict(S.DIM_SIZES, **(sizes or {}))
+        s = self.stream
+        tables = tables or ("ddate", "customer", "supplier", "part", "lineorder")
+        for t in tables:
+            n = fact_rows if t == "lineorder" else sizes[t]
+            self.create_table(t, n)
+            cols = S.generation_order(t)
+            if t == "lineorder" and fact_columns is not None:
+                cols = [c for c in cols if c in fact_columns]
+            for c in cols:
+                kind = S.column_spec(t, c, sizes)
+                out = ctypes.c_void_p(self.column_ptr(t, c))
+                sid = S.stream_id(t, c)
+                if kind[0] == "rowid":
+                    L.check(L.lib.sdb_gen_rowid(out, n, 0, s))
+                elif kind[0] == "uniform":
+                    L.check(L.lib.sdb_gen_uniform(out, n, seed, sid, 0, kind[1], kind[2], s))
+                elif kind[0] == "child":
+                    base = ctypes.c_void_p(self.column_ptr(t, kind[1]))
+                    L.check(L.lib.sdb_gen_child(out, base, n, seed, sid, 0, kind[2], kind[3], kind[4], s))
+                elif kind[0] == "lookup":
+                    table = ctypes.c_void_p(self.column_ptr(kind[1], kind[2]))
+                    L.check(L.lib.sdb_gen_lookup(out, n, seed, sid, 0, table, sizes[kind[1]], s))
+                elif kind[0] == "date":
+                    pass  # filled below from the host calendar
+                else:
+                    raise ValueError(kind)
+            if t == "ddate":
+                dk, yr, ym = date_columns(sizes["ddate"])
+                self.upload(t, S.D_DATEKEY, dk)
+                self.upload(t, S.D_YEAR, yr)
+                self.upload(t, S.D_YEARMONTHNUM, ym)
+            self.finalize(t)
+
+    # ----------------------------------------------------------- execution
+    def execute(self, plan):
+        text = plan if isinstance(plan, str) else json.dumps(plan)
+        h = ctypes.c_void_p()
+        L.check(L.lib.sdb_engine_execute(self._h, text.encode(), ctypes.byref(h)))
+        return Result(h)
+
+    def last_timing(self):
+        d, w = ctypes.c_double(), ctypes.c_double()
+        L.check(L.lib.sdb_engine_last_timing(self._h, ctypes.byref(d), ctypes.byref(w)))
+        return d.value, w.value
+
+    def last_launches(self):
+        n = ctypes.c_int64()
+        L.check(L.lib.sdb_engine_last_launches(self._h, ctypes.byref(n)))
+        return n.value
+
+    def reset(self):
+        L.check(L.lib.sdb_engine_reset(self._h))
+
+    def set_fusion(self, on=True):
+        L.check(L.lib.sdb_engine_set_fusion(self._h, 1 if on else 0))
+        self.fusion = bool(on)
+
+    def set_profiling(self, on=True):
+        L.check(L.lib.sdb_engine_set_profiling(self._h, 1 if on else 0))
+
+    def profile_reset(self):
+        L.check(L.lib.sdb_engine_profile_reset(self._h))
+
+    def profile(self):
+        """{name: (launches, rows, ms)} accumulated since profile_reset()."""
+        out = {}
+        buf = ctypes.create_string_buffer(256)
+        for i in range(L.lib.sdb_engine_profile_count(self._h)):
+            la, ro, ms = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
+            L.check(L.lib.sdb_engine_profile_get(self._h, i, buf, 256, ctypes.byref(la), ctypes.byref(ro),
+                                                 ctypes.byref(ms)))
+            out[buf.value.decode()] = (la.value, ro.value, ms.value)
+        return out
+
+
+def date_columns(ndays):
+    """Date dimension from 1992-01-01: datekey yyyymmdd, year, yearmonthnum."""
+    import datetime
+    d0 = datetime.date(1992, 1, 1)
+    dk = np.empty(ndays, np.int32)
+    yr = np.empty(ndays, np.int32)
+    ym = np.empty(ndays, np.int32)
+    for i in range(ndays):
+        d = d0 + datetime.timedelta(days=i)
+        dk[i] = d.year * 10000 + d.month * 100 + d.day
+        yr[i] = d.year
+        ym[i] = d.year * 100 + d.month
+    return dk, yr, ym
+
+
+def dag_order(plan):
+    """dag_topological_sort of the plan (host only)."""
+    text = plan if isinstance(plan, str) else json.dumps(plan)
+    order = (ctypes.c_int32 * 4096)()
+    count = ctypes.c_int()
+    L.check(L.lib.sdb_plan_dag_order(text.encode(), order, 4096, ctypes.byref(count)))
+    return list(order[:count.value])

@@ -1,0 +1,69 @@
+"""The C-ABI library loads here (no GPU), exports every function that
+include/sycldb.h declares, and its host-only entry points (operator-string
+parsing, plan parsing and DAG ordering) agree with the oracle."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from oracle import plan_oracle as PO
+from sycldb_amd import _lib as L
+from sycldb_amd import engine as E
+from sycldb_amd import plans as PL
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "sycldb.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(sdb_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_functions()
+    assert len(names) >= 40
+    for n in names:
+        assert hasattr(L.lib, n), n
+    # and every binding in _lib is declared in the header
+    assert set(L.SIGNATURES) <= set(names)
+
+
+def test_op_strings_follow_reference():
+    f = lambda s: L.lib.sdb_comp_op_from_string(s.encode())
+    assert [f(s) for s in ["==", "!=", "<>", "<", "<=", ">", ">=", "=", "?"]] == [0, 1, 1, 2, 3, 4, 5, 0, 0]
+    g = lambda s: L.lib.sdb_logical_op_from_string(s.encode())
+    assert [g(s) for s in ["AND", "OR", "", "NONE", "NOT"]] == [1, 2, 0, 0, 0]
+    h = lambda s: L.lib.sdb_binop_from_string(s.encode())
+    assert [h(s) for s in ["*", "/", "+", "-", "%"]] == [0, 1, 2, 3, -1]
+
+
+@pytest.mark.parametrize("tmpl", ["t1", "t2", "t3", "t4"])
+def test_dag_order_matches_oracle(tmpl):
+    plan = PL.build(tmpl)
+    assert E.dag_order(plan) == PO.dag_topological_sort(plan["rels"])
+
+
+def test_bad_plans_raise_value_error():
+    with pytest.raises(ValueError):
+        E.dag_order("{not json")
+    with pytest.raises(ValueError):
+        E.dag_order('{"rels": [{"id": 3, "relOp": "TABLE_SCAN", "tables": ["ssb", "ddate"]}]}')
+    with pytest.raises(ValueError):
+        E.dag_order('{"rels": [{"id": 0, "relOp": "FILTER", "condition": {"exprType": "EXPR", "op": "=", '
+                    '"operands": []}}]}')
+
+
+def test_version_and_last_error():
+    assert b"sycldb" in L.lib.sdb_version()
+    assert isinstance(L.lib.sdb_last_error(), bytes)
+
+
+def test_date_dimension_matches_oracle_calendar():
+    import numpy as np
+    from oracle import oracle as O
+    dk, yr, ym = E.date_columns(2556)
+    odk, oyr, oym = O.gen_dates(2556)
+    assert (dk == odk).all() and (yr == oyr).all() and (ym == oym).all()

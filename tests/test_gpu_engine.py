@@ -1,0 +1,128 @@
+"""Plans end to end on the GPU engine (TransientTable model, fused and
+unfused bundles, several segment sizes) against the plan oracle on the same
+seeded tables; the device generator against the CPU one; reference error
+behaviour."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import datagen, plan_oracle as PO  # noqa: E402
+from sycldb_amd import plans as PL  # noqa: E402
+from sycldb_amd import schema as S  # noqa: E402
+
+FACT_ROWS = 400_003
+SEED = 45
+CASES = [
+    ("t1", {}),
+    ("t1", {"date": ("yearmonthnum", 199401), "disc": (4, 6), "qty": ("range", 26, 35)}),
+    ("t1", {"date": ("year_range_and", 1992, 1993), "disc": (0, 10), "qty": ("lt", 51)}),
+    ("t2", {}),
+    ("t2", {"part": ("brand", 260), "s_region": 3}),
+    ("t2", {"part": ("brand_range", 221, 228), "s_region": 2}),
+    ("t3", {}),
+    ("t3", {"cust": ("nation", 14), "supp": ("nation", 14), "date": ("year_range_and", 1992, 1997), "group": "city"}),
+    ("t3", {"cust": ("city2", 141, 145), "supp": ("region", 2), "date": ("yearmonthnum", 199712), "group": "city"}),
+    ("t4", {}),
+    ("t4", {"c_region": 3, "s_region": 0, "mfgr": (2, 4)}),
+]
+
+
+@pytest.fixture(scope="module")
+def cpu_tables():
+    return datagen.generate_tables(S, FACT_ROWS, seed=SEED)
+
+
+@pytest.fixture(scope="module")
+def expected(cpu_tables):
+    return {i: PO.execute_plan(PL.build(t, **p), cpu_tables).rows() for i, (t, p) in enumerate(CASES)}
+
+
+def make_engine(**kw):
+    from sycldb_amd.engine import Engine
+    eng = Engine(**kw)
+    eng.generate(FACT_ROWS, SEED)
+    return eng
+
+
+@pytest.fixture(scope="module")
+def fused():
+    e = make_engine()
+    yield e
+    e.close()
+
+
+def test_device_generator_matches_cpu(fused, cpu_tables):
+    for t, cols in cpu_tables.items():
+        for c, want in cols.items():
+            got = fused.download(t, c, len(want))
+            assert (got == want).all(), (t, c)
+
+
+@pytest.mark.parametrize("idx", range(len(CASES)))
+def test_fused_matches_oracle(fused, expected, idx):
+    tmpl, params = CASES[idx]
+    print("case", idx, tmpl, params, flush=True)
+    res = fused.execute(PL.build(tmpl, **params))
+    assert res.rows() == expected[idx]
+
+
+@pytest.mark.parametrize("kw", [dict(fusion=False), dict(segment_size=50_000), dict(segment_size=77_777),
+                                dict(segment_size=77_777, fusion=False)])
+def test_unfused_and_segmented_match_oracle(expected, kw):
+    print("engine", kw, flush=True)
+    eng = make_engine(**kw)
+    try:
+        for idx, (tmpl, params) in enumerate(CASES):
+            res = eng.execute(PL.build(tmpl, **params))
+            assert res.rows() == expected[idx], (kw, idx)
+    finally:
+        eng.close()
+
+
+def test_fusion_launches_fewer_kernels(fused):
+    from sycldb_amd.engine import Engine
+    plan = PL.build("t2")
+    fused.execute(plan)
+    n_fused = fused.last_launches()
+    with Engine(fusion=False) as u:
+        u.generate(FACT_ROWS, SEED)
+        u.execute(plan)
+        n_unfused = u.last_launches()
+    assert n_fused < n_unfused
+
+
+def test_repeated_execution_is_stable(fused, expected):
+    plan = PL.build("t3")
+    for _ in range(3):
+        assert fused.execute(plan).rows() == expected[6]
+
+
+def test_result_text_matches_oracle(fused, cpu_tables, tmp_path):
+    import ctypes
+    from sycldb_amd import _lib as L
+    plan = PL.build("t2")
+    want = PO.execute_plan(plan, cpu_tables).text()
+    res = fused.execute(plan)
+    assert res.text() == want
+
+
+def test_errors_follow_reference(fused):
+    bad_op = PL.build("t1")
+    bad_op["rels"][5]["exprs"][0]["op"] = "%"     # unknown arithmetic op -> invalid_argument
+    with pytest.raises(ValueError):
+        fused.execute(bad_op)
+    bad_join = PL.build("t1")
+    bad_join["rels"][4]["condition"]["operands"][1]["input"] = 999
+    with pytest.raises(ValueError):
+        fused.execute(bad_join)
+    # the engine keeps working after errors
+    assert len(fused.execute(PL.build("t1")).rows()) == 1
+
+
+def test_small_budget_raises_memory_error():
+    from sycldb_amd.engine import Engine
+    with Engine(device_budget_bytes=1 << 20) as e:
+        e.generate(FACT_ROWS, SEED)
+        with pytest.raises(MemoryError):
+            e.execute(PL.build("t3"))
