@@ -48,6 +48,7 @@ void KernelBundle::execute(const ExecContext &ctx) const
     if (eng.fusion) {
         FProgram p;
         if (build_program(ctx, p)) {
+            validate_program(p, kernels_.front().n, eng.registry);
             Engine::ProfRec rec;
             if (eng.profile) eng.prof_begin("fused:" + (ctx.table ? *ctx.table : std::string("?")), kernels_.front().n, rec);
             launch_fused(p, kernels_.front().n, ctx.stream);
@@ -68,12 +69,14 @@ void KernelBundle::execute_unfused(const ExecContext &ctx) const
     for (const KernelData &k : kernels_)
         need_fill |= reads_virgin(ctx, k.in_flags) || reads_virgin(ctx, k.local_flags);
     if (need_fill && !ctx.virgin_filled) {
+        eng.registry.check(ctx.virgin_begin, (uint64_t)(ctx.virgin_end - ctx.virgin_begin), "initial flags");
         launch_fill_u8(const_cast<uint8_t *>(ctx.virgin_begin), 1, ctx.virgin_end - ctx.virgin_begin, s);
         eng.launches++;
         ctx.virgin_filled = true;
     }
     for (const KernelData &k : kernels_) {
         if (k.type == KernelType::EmptyKernel) continue;
+        validate_kernel(k, eng.registry);
         Engine::ProfRec rec;
         if (eng.profile)
             eng.prof_begin(std::string("op:") + kernel_type_name(k.type) + ":" + (ctx.table ? *ctx.table : std::string("?")),

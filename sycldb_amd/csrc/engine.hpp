@@ -48,6 +48,23 @@ void launch_seen_to_flags(const uint32_t *seen, uint8_t *flags, uint64_t prod, h
 void minmax_i32(const int32_t *col, int64_t n, int32_t *mn, int32_t *mx, hipStream_t s);
 uint64_t groupby_lds_max_slots();
 
+// --------------------------------------------------------- allocation map
+
+// Every device allocation the engine hands to kernels (table columns, memory
+// manager regions) is registered here; each launch's buffers are checked
+// against it on the host before the launch, so a wrong pointer or a too
+// short buffer becomes an error instead of a GPU memory fault.
+class AllocRegistry {
+  public:
+    void add(const void *p, uint64_t bytes);
+    void remove(const void *p);
+    // throws unless [p, p + bytes) lies inside one registered allocation
+    void check(const void *p, uint64_t bytes, const char *what) const;
+
+  private:
+    std::map<uintptr_t, uint64_t> allocs_;
+};
+
 // ----------------------------------------------------------- memory manager
 
 // Bump allocator over device regions (operations/memory_manager.hpp:61-164).
@@ -56,7 +73,7 @@ uint64_t groupby_lds_max_slots();
 // (memory_manager.cpp:42-53, 71-90).  Throws std::bad_alloc past the budget.
 class MemoryManager {
   public:
-    MemoryManager(uint64_t budget, hipStream_t stream, uint64_t max_region = 1ull << 30);
+    MemoryManager(uint64_t budget, hipStream_t stream, AllocRegistry *registry, uint64_t max_region = 1ull << 30);
     ~MemoryManager();
     MemoryManager(const MemoryManager &) = delete;
     MemoryManager &operator=(const MemoryManager &) = delete;
@@ -74,6 +91,7 @@ class MemoryManager {
     void *raw(uint64_t bytes, bool zero);
     std::vector<Region> regions_, zero_regions_;
     hipStream_t stream_;
+    AllocRegistry *registry_;
     uint64_t budget_, budget_zero_, reserved_ = 0, reserved_zero_ = 0, max_region_;
 };
 
@@ -203,6 +221,7 @@ struct Engine {
     hipStream_t stream = nullptr;
     uint64_t segment_size = 1ull << 28; // common.hpp:12
     bool fusion = true;
+    AllocRegistry registry;
     std::unique_ptr<MemoryManager> mm;
     std::map<std::string, std::unique_ptr<Table>> tables;
     int64_t launches = 0;
@@ -276,6 +295,10 @@ class TransientTable {
     std::vector<std::vector<KernelBundle>> pending_kernels_; // [phase][segment]
     std::string name_;
 };
+
+// Host-side bounds checks of one launch (validate.cpp).
+void validate_kernel(const KernelData &k, const AllocRegistry &reg);
+void validate_program(const FProgram &p, int64_t n, const AllocRegistry &reg);
 
 // ------------------------------------------------------------------ executor
 

@@ -90,6 +90,7 @@ void ensure_storage(Engine &eng, Table &t, Column &c)
     if (c.data) return;
     void *p = nullptr;
     SDB_HIP(hipMalloc(&p, std::max<uint64_t>(t.nrows, 1) * sizeof(int32_t)));
+    eng.registry.add(p, std::max<uint64_t>(t.nrows, 1) * sizeof(int32_t));
     t.owned.push_back(p);
     c.data = p;
     c.loaded = true;
@@ -120,7 +121,7 @@ int sdb_engine_create(const sdb_engine_options *opts, sdb_engine **out)
         SDB_HIP(hipStreamCreateWithFlags(&eng.stream, hipStreamNonBlocking));
         SDB_HIP(hipEventCreate(&eng.ev_start));
         SDB_HIP(hipEventCreate(&eng.ev_stop));
-        eng.mm = std::make_unique<MemoryManager>(budget, eng.stream);
+        eng.mm = std::make_unique<MemoryManager>(budget, eng.stream, &eng.registry);
         *out = e.release();
     });
 }
@@ -158,7 +159,10 @@ int sdb_engine_table_create(sdb_engine *e, const char *name, uint64_t nrows)
         auto old = eng.tables.find(name);
         if (old != eng.tables.end()) {
             SDB_HIP(hipStreamSynchronize(eng.stream));
-            for (void *p : old->second->owned) SDB_HIP(hipFree(p));
+            for (void *p : old->second->owned) {
+                eng.registry.remove(p);
+                SDB_HIP(hipFree(p));
+            }
             eng.tables.erase(old);
         }
         auto t = std::make_unique<Table>();

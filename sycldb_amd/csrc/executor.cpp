@@ -217,20 +217,22 @@ ResultTable execute_ddor_plan(Engine &eng, const PlanResult &plan)
     res.nrows = final_table.get_nrows();
     res.flags.resize(res.nrows);
     const uint8_t *flags = final_table.flags_device();
-    if (res.nrows)
+    if (res.nrows) {
+        eng.registry.check(flags, res.nrows, "result flags");
         SDB_HIP(hipMemcpyAsync(res.flags.data(), flags, res.nrows, hipMemcpyDeviceToHost, eng.stream));
+    }
     for (Column *c : final_table.get_columns()) {
         if (!c || !c->loaded || !c->data) continue;
         ResultTable::Col rc;
         rc.is_u64 = c->is_aggregate;
-        if (rc.is_u64) {
-            rc.u64.resize(res.nrows);
-            if (res.nrows)
-                SDB_HIP(hipMemcpyAsync(rc.u64.data(), c->data, res.nrows * 8, hipMemcpyDeviceToHost, eng.stream));
-        } else {
-            rc.i32.resize(res.nrows);
-            if (res.nrows)
-                SDB_HIP(hipMemcpyAsync(rc.i32.data(), c->data, res.nrows * 4, hipMemcpyDeviceToHost, eng.stream));
+        const uint64_t rows = std::min(res.nrows, c->nrows);
+        const uint64_t elem = rc.is_u64 ? 8 : 4;
+        if (rc.is_u64) rc.u64.assign(res.nrows, 0);
+        else rc.i32.assign(res.nrows, 0);
+        if (rows) {
+            eng.registry.check(c->data, rows * elem, "result column");
+            void *dst = rc.is_u64 ? (void *)rc.u64.data() : (void *)rc.i32.data();
+            SDB_HIP(hipMemcpyAsync(dst, c->data, rows * elem, hipMemcpyDeviceToHost, eng.stream));
         }
         res.cols.push_back(std::move(rc));
     }

@@ -151,7 +151,7 @@ __device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64
 #pragma unroll
     for (int j = 0; j < F_MAX_FLAGS; j++) {
         const uint8_t init = P.flag_init[j];
-        if (init == FF_LOAD) F |= flags_in<R>(P.flag_load[j] + row0) << (4 * j);
+        if (init == FF_LOAD && P.flag_load[j]) F |= flags_in<R>(P.flag_load[j] + row0) << (4 * j);
         else if (init == FF_TRUE) F |= M << (4 * j);
     }
 

@@ -68,7 +68,9 @@ struct FGroup {
     uint64_t prod;
 };
 
-enum FFlagInit : uint8_t { FF_LOAD = 0, FF_TRUE = 1, FF_UNDEF = 2 };
+// Initial value of a flag register.  FF_UNDEF is 0 so that registers a
+// program does not use (zero-initialised) are never loaded.
+enum FFlagInit : uint8_t { FF_UNDEF = 0, FF_LOAD = 1, FF_TRUE = 2 };
 
 constexpr uint32_t F_MAGIC = 0x5DB0F05Eu;
 

@@ -12,15 +12,21 @@ uint64_t round_up(uint64_t x) { return (x + ALIGN - 1) & ~(ALIGN - 1); }
 } // namespace
 
 // zero budget = budget / 64 as in memory_manager.cpp:55-69
-MemoryManager::MemoryManager(uint64_t budget, hipStream_t stream, uint64_t max_region)
-    : stream_(stream), budget_(budget), budget_zero_(budget >> 6), max_region_(max_region)
+MemoryManager::MemoryManager(uint64_t budget, hipStream_t stream, AllocRegistry *registry, uint64_t max_region)
+    : stream_(stream), registry_(registry), budget_(budget), budget_zero_(budget >> 6), max_region_(max_region)
 {
 }
 
 MemoryManager::~MemoryManager()
 {
-    for (auto &r : regions_) (void)hipFree(r.base);
-    for (auto &r : zero_regions_) (void)hipFree(r.base);
+    for (auto &r : regions_) {
+        if (registry_) registry_->remove(r.base);
+        (void)hipFree(r.base);
+    }
+    for (auto &r : zero_regions_) {
+        if (registry_) registry_->remove(r.base);
+        (void)hipFree(r.base);
+    }
 }
 
 void *MemoryManager::raw(uint64_t bytes, bool zero)
@@ -41,6 +47,7 @@ void *MemoryManager::raw(uint64_t bytes, bool zero)
     const uint64_t size = std::min(remaining, std::max(bytes, max_region_));
     Region r{nullptr, size, 0};
     SDB_HIP(hipMalloc(reinterpret_cast<void **>(&r.base), size));
+    if (registry_) registry_->add(r.base, size);
     if (zero) {
         // zeroed once here, in stream order before any use; reset()
         // re-zeroes the part handed out since

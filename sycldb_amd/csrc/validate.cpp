@@ -1,0 +1,219 @@
+// Host-side bounds checks run before every engine launch: each buffer a
+// kernel will touch must lie inside one registered device allocation and be
+// long enough for the rows (or hash-table slots, group slots) it is indexed
+// with.  A violation throws instead of launching.
+#include <string>
+
+#include "engine.hpp"
+
+namespace sdb {
+
+void AllocRegistry::add(const void *p, uint64_t bytes)
+{
+    if (p) allocs_[(uintptr_t)p] = bytes;
+}
+
+void AllocRegistry::remove(const void *p) { allocs_.erase((uintptr_t)p); }
+
+void AllocRegistry::check(const void *p, uint64_t bytes, const char *what) const
+{
+    if (bytes == 0) return;
+    if (!p) fail(SDB_ERR_RUNTIME, std::string("launch check: null ") + what);
+    const uintptr_t a = (uintptr_t)p;
+    auto it = allocs_.upper_bound(a);
+    if (it == allocs_.begin())
+        fail(SDB_ERR_RUNTIME, std::string("launch check: ") + what + " is not in a device allocation");
+    --it;
+    if (a < it->first || a + bytes > it->first + it->second || a + bytes < a)
+        fail(SDB_ERR_RUNTIME, std::string("launch check: ") + what + " (" + std::to_string(bytes) +
+                                  " bytes) overruns its device allocation");
+}
+
+namespace {
+
+uint64_t range_len(int32_t mn, int32_t mx)
+{
+    if (mx < mn) fail(SDB_ERR_RUNTIME, "launch check: hash table max < min");
+    return (uint64_t)((int64_t)mx - (int64_t)mn + 1);
+}
+
+} // namespace
+
+void validate_kernel(const KernelData &k, const AllocRegistry &reg)
+{
+    const uint64_t n = (uint64_t)k.n;
+    if (k.n < 0) fail(SDB_ERR_RUNTIME, "launch check: negative row count");
+    if (n == 0) return;
+    const uint64_t col = 4 * n;
+    switch (k.type) {
+    case KernelType::EmptyKernel: return;
+    case KernelType::LogicalKernel:
+        reg.check(k.in_flags, n, "input flags");
+        reg.check(k.local_flags, n, "local flags");
+        reg.check(k.out_flags, n, "output flags");
+        return;
+    case KernelType::SelectionKernelColumns: reg.check(k.b, col, "second column"); [[fallthrough]];
+    case KernelType::SelectionKernelLiteral:
+        reg.check(k.a, col, "column");
+        reg.check(k.in_flags, n, "input flags");
+        reg.check(k.out_flags, n, "output flags");
+        return;
+    case KernelType::FillKernel: reg.check(k.result, col, "fill target"); return;
+    case KernelType::PerformOperationKernelColumns:
+    case KernelType::PerformOperationKernelLiteralFirst:
+    case KernelType::PerformOperationKernelLiteralSecond:
+        if (k.type != KernelType::PerformOperationKernelLiteralFirst) reg.check(k.a, col, "first operand");
+        if (k.type != KernelType::PerformOperationKernelLiteralSecond) reg.check(k.b, col, "second operand");
+        reg.check(k.result, col, "result column");
+        reg.check(k.in_flags, n, "flags");
+        return;
+    case KernelType::BuildKeysHTKernel:
+        if (k.ht_len <= 0) fail(SDB_ERR_RUNTIME, "launch check: hash table length");
+        reg.check(k.a, col, "build keys");
+        reg.check(k.in_flags, n, "build flags");
+        reg.check(k.ht, (uint64_t)k.ht_len, "key hash table");
+        return;
+    case KernelType::BuildKeyValsHTKernel:
+        if (k.ht_len <= 0) fail(SDB_ERR_RUNTIME, "launch check: hash table length");
+        reg.check(k.a, col, "build keys");
+        reg.check(k.b, col, "build values");
+        reg.check(k.in_flags, n, "build flags");
+        reg.check(k.ht, 8 * (uint64_t)k.ht_len, "key-value hash table");
+        return;
+    case KernelType::FilterJoinKernel:
+        reg.check(k.a, col, "probe keys");
+        reg.check(k.in_flags, n, "input flags");
+        reg.check(k.out_flags, n, "output flags");
+        reg.check(k.ht, range_len(k.ht_min, k.ht_max), "key hash table");
+        return;
+    case KernelType::FullJoinKernel:
+        reg.check(k.a, col, "probe keys");
+        reg.check(k.result, col, "join value column");
+        reg.check(k.in_flags, n, "input flags");
+        reg.check(k.out_flags, n, "output flags");
+        reg.check(k.ht, 8 * range_len(k.ht_min, k.ht_max), "key-value hash table");
+        return;
+    case KernelType::AggregateOperationKernel:
+        reg.check(k.a, col, "aggregated column");
+        reg.check(k.in_flags, n, "flags");
+        reg.check(k.agg_res, 8, "accumulator");
+        return;
+    case KernelType::GroupByAggregateKernel: {
+        if (!k.gb) fail(SDB_ERR_RUNTIME, "launch check: group-by without spec");
+        const GroupBySpec &g = *k.gb;
+        if (g.prod == 0 || g.keys.empty() || g.keys.size() != g.out_keys.size())
+            fail(SDB_ERR_RUNTIME, "launch check: malformed group-by");
+        reg.check(k.a, col, "aggregated column");
+        reg.check(k.in_flags, n, "flags");
+        for (size_t j = 0; j < g.keys.size(); j++) {
+            reg.check(g.keys[j], col, "group key column");
+            reg.check(g.out_keys[j], 4 * g.prod, "group key output");
+        }
+        reg.check(g.sums, 8 * g.prod, "group sums");
+        reg.check(g.seen, 4 * g.prod, "group seen flags");
+        return;
+    }
+    }
+}
+
+void validate_program(const FProgram &p, int64_t n_rows, const AllocRegistry &reg)
+{
+    if (p.magic != F_MAGIC) fail(SDB_ERR_RUNTIME, "launch check: program magic");
+    if (p.ninstr < 1 || p.ninstr > F_MAX_INSTR) fail(SDB_ERR_RUNTIME, "launch check: instruction count");
+    if (n_rows <= 0) return;
+    const uint64_t n = (uint64_t)n_rows, col = 4 * n;
+    bool flag_used[F_MAX_FLAGS] = {};
+    for (int j = 0; j < F_MAX_FLAGS; j++) {
+        if (p.flag_init[j] > FF_TRUE) fail(SDB_ERR_RUNTIME, "launch check: flag init");
+        if (p.flag_init[j] == FF_LOAD) reg.check(p.flag_load[j], n, "loaded flags");
+        if (p.flag_store[j]) reg.check(p.flag_store[j], n, "stored flags");
+    }
+    auto fl = [&](uint8_t r) {
+        if (r >= F_MAX_FLAGS) fail(SDB_ERR_RUNTIME, "launch check: flag register");
+        flag_used[r] = true;
+    };
+    auto operand = [&](int8_t r, const int32_t *mem, const char *what) {
+        if (r >= F_MAX_VALS) fail(SDB_ERR_RUNTIME, "launch check: value register");
+        if (r < 0) reg.check(mem, col, what);
+    };
+    bool has_sum = false, has_gb = false;
+    for (int i = 0; i < p.ninstr; i++) {
+        const FInstr &I = p.ins[i];
+        fl(I.fin);
+        switch (I.kind) {
+        case FK_SEL_LIT:
+        case FK_SEL_COL:
+            if (I.cmp > SDB_GE || I.logic > SDB_OR) fail(SDB_ERR_RUNTIME, "launch check: selection op");
+            operand(I.va, I.a, "selection column");
+            if (I.kind == FK_SEL_COL) operand(I.vb, I.b, "selection column");
+            fl(I.fout);
+            break;
+        case FK_LOGICAL:
+            if (I.logic > SDB_OR) fail(SDB_ERR_RUNTIME, "launch check: logical op");
+            fl(I.floc);
+            fl(I.fout);
+            break;
+        case FK_FILL:
+            if (I.vout < 0 || I.vout >= F_MAX_VALS) fail(SDB_ERR_RUNTIME, "launch check: fill register");
+            if (I.store) reg.check(I.store, col, "fill store");
+            break;
+        case FK_BINOP_CC:
+        case FK_BINOP_LC:
+        case FK_BINOP_CL:
+            if (I.op > SDB_SUB) fail(SDB_ERR_RUNTIME, "launch check: arithmetic op");
+            if (I.kind != FK_BINOP_LC) operand(I.va, I.a, "arithmetic operand");
+            if (I.kind != FK_BINOP_CL) operand(I.vb, I.b, "arithmetic operand");
+            if (I.vout < 0 || I.vout >= F_MAX_VALS) fail(SDB_ERR_RUNTIME, "launch check: result register");
+            if (I.store) reg.check(I.store, col, "arithmetic store");
+            break;
+        case FK_PROBE_SEMI:
+            operand(I.va, I.a, "probe keys");
+            reg.check(I.ht, range_len(I.ht_min, I.ht_max), "semi-join table");
+            fl(I.fout);
+            break;
+        case FK_PROBE_FULL:
+            operand(I.va, I.a, "probe keys");
+            reg.check(I.ht, 8 * range_len(I.ht_min, I.ht_max), "join table");
+            if (I.vout < 0 || I.vout >= F_MAX_VALS) fail(SDB_ERR_RUNTIME, "launch check: join register");
+            if (I.store) reg.check(I.store, col, "join value store");
+            fl(I.fout);
+            break;
+        case FK_BUILD_KEYS:
+            operand(I.va, I.a, "build keys");
+            if (I.ht_len <= 0) fail(SDB_ERR_RUNTIME, "launch check: table length");
+            reg.check(I.ht, (uint64_t)I.ht_len, "key table");
+            break;
+        case FK_BUILD_KEYVALS:
+            operand(I.va, I.a, "build keys");
+            operand(I.vb, I.b, "build values");
+            if (I.ht_len <= 0) fail(SDB_ERR_RUNTIME, "launch check: table length");
+            reg.check(I.ht, 8 * (uint64_t)I.ht_len, "key-value table");
+            break;
+        case FK_SUM:
+            operand(I.va, I.a, "summed column");
+            reg.check(p.sum_acc, 8, "sum accumulator");
+            has_sum = true;
+            break;
+        case FK_GROUPBY: {
+            const FGroup &g = p.gb;
+            if (g.k < 1 || g.k > F_MAX_KEYS || g.prod == 0) fail(SDB_ERR_RUNTIME, "launch check: group-by shape");
+            if (g.lds && g.prod > fused_groupby_lds_max_slots()) fail(SDB_ERR_RUNTIME, "launch check: group-by LDS");
+            for (int j = 0; j < g.k; j++) {
+                operand(g.key_reg[j], g.key_mem[j], "group key");
+                reg.check(g.out_keys[j], 4 * g.prod, "group key output");
+            }
+            operand(g.agg_reg, g.agg_mem, "aggregated column");
+            reg.check(g.sums, 8 * g.prod, "group sums");
+            reg.check(g.seen, 4 * g.prod, "group seen flags");
+            has_gb = true;
+            break;
+        }
+        default: fail(SDB_ERR_RUNTIME, "launch check: unknown instruction");
+        }
+    }
+    if (p.sum_acc && !has_sum) fail(SDB_ERR_RUNTIME, "launch check: accumulator without sum");
+    if (p.gb.prod && !has_gb) fail(SDB_ERR_RUNTIME, "launch check: group spec without group-by");
+    (void)flag_used;
+}
+
+} // namespace sdb
