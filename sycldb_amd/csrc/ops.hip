@@ -1123,9 +1123,10 @@ int sdb_groupby_dense(const int32_t *const *keys, int k, const int32_t *mins, co
 {
     return guarded([&] {
         check_n(n);
-        require(keys && mins && maxs && out_keys && sums && seen, "null pointer");
+        require(keys && mins && maxs && out_keys, "null pointer");
+        require(prod == 0 || (sums && seen), "null pointer");
         require(n == 0 || (agg && flags), "null pointer");
-        for (int j = 0; j < k && j < 8; j++) require(keys[j] && out_keys[j], "null key column");
+        for (int j = 0; j < k && j < 8; j++) require(n == 0 || (keys[j] && out_keys[j]), "null key column");
         launch_groupby_dense(keys, k, mins, maxs, agg, flags, n, out_keys, sums, seen, prod, as_stream(stream), 0);
     });
 }
