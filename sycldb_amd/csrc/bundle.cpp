@@ -3,6 +3,9 @@
 // kernels ordered on one stream instead of by waited events), or the whole
 // bundle as one fused kernel.
 #include <algorithm>
+#include <array>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 
 #include "engine.hpp"
@@ -41,6 +44,33 @@ const char *kernel_type_name(KernelType t)
     return "?";
 }
 
+static const char *fkind_name(uint8_t k)
+{
+    static const char *names[] = {"SEL_LIT",  "SEL_COL",    "LOGICAL", "FILL",       "BINOP_CC",
+                                  "BINOP_LC", "BINOP_CL",   "PROBE_SEMI", "PROBE_FULL", "BUILD_KEYS",
+                                  "BUILD_KEYVALS", "SUM",   "GROUPBY", "SEL_RANGE",  "SEL_IN2", "PROBE_BITS"};
+    return k < sizeof(names) / sizeof(names[0]) ? names[k] : "?";
+}
+
+// SDB_DEBUG_FUSED=1 prints each fused program to stderr.
+static void dump_program(const FProgram &p, int64_t n, const std::string &table)
+{
+    static const bool on = [] {
+        const char *e = getenv("SDB_DEBUG_FUSED");
+        return e && *e && *e != '0';
+    }();
+    if (!on) return;
+    fprintf(stderr, "[fused %s] rows=%lld vec=%d instr=%d vals=%d blobs=%d lds=%u group=%d\n", table.c_str(),
+            (long long)n, p.vec, p.ninstr, p.nvals, p.nblobs, p.lds_bytes, p.gb.lds);
+    for (int i = 0; i < p.ninstr; i++) {
+        const FInstr &I = p.ins[i];
+        fprintf(stderr, "  %2d %-12s in=%d out=%d logic=%d va=%d vout=%d lds=%d vkind=%d store=%d\n", i,
+                fkind_name(I.kind), I.fin, I.fout, I.logic, I.va, I.vout, I.in_lds, I.vkind, I.store != nullptr);
+    }
+    for (int j = 0; j < F_MAX_FLAGS; j++)
+        if (p.flag_init[j] || p.flag_store[j]) fprintf(stderr, "  flag %d init=%d store=%d\n", j, p.flag_init[j], p.flag_store[j] != nullptr);
+}
+
 void KernelBundle::execute(const ExecContext &ctx) const
 {
     if (kernels_.empty()) return;
@@ -49,6 +79,7 @@ void KernelBundle::execute(const ExecContext &ctx) const
         FProgram p;
         if (build_program(ctx, p)) {
             validate_program(p, kernels_.front().n, eng.registry);
+            dump_program(p, kernels_.front().n, ctx.table ? *ctx.table : std::string("?"));
             Engine::ProfRec rec;
             if (eng.profile) eng.prof_begin("fused:" + (ctx.table ? *ctx.table : std::string("?")), kernels_.front().n, rec);
             launch_fused(p, kernels_.front().n, ctx.stream);
@@ -127,8 +158,159 @@ void KernelBundle::execute_unfused(const ExecContext &ctx) const
     }
 }
 
-// Translates the bundle into a fused program.  Returns false when the bundle
-// uses something the fused kernel does not cover (then it runs unfused).
+// ------------------------------------------------------------------ fusion
+//
+// build_program translates the bundle into a fused program in three passes:
+//   1. translation: each functor becomes one instruction; every distinct flag
+//      buffer becomes a flag register, every column produced inside the bundle
+//      a value register (probes use the compact bitmap tables when present);
+//   2. optimisation, each step preserving every row's result:
+//      - SEARCH triples (x >= lo into a local buffer, x <= hi AND, then the
+//        logical merge) fold into one range / two-point predicate;
+//      - a NONE predicate whose input is still the all-true initial flags
+//        becomes AND (identical on an all-true input);
+//      - a run of conjunctive instructions (out = in AND pred(row)) whose
+//        intermediate flags nobody else reads is reordered cheapest and most
+//        selective first; values a full join produces there are only read
+//        under the run's final flag, so rows that reach them are unchanged;
+//   3. layout: LDS register file, staged bitmaps, LDS aggregation area, and
+//      the write-back of buffers still referenced after the bundle.
+// It returns false when the bundle needs something the fused kernel does
+// not cover (the bundle then runs unfused).
+
+namespace {
+
+struct Draft {
+    FInstr I{};
+    float sel = 1.0f;   // fraction of rows a predicate keeps
+    float cost = 1.0f;  // relative per-row cost
+    bool store = false; // writes a value back to memory
+};
+
+bool is_predicate(uint8_t k)
+{
+    return k == FK_SEL_LIT || k == FK_SEL_COL || k == FK_SEL_RANGE || k == FK_SEL_IN2 || k == FK_LOGICAL ||
+           k == FK_PROBE_SEMI || k == FK_PROBE_FULL || k == FK_PROBE_BITS;
+}
+
+bool writes_flag(uint8_t k) { return is_predicate(k); }
+
+bool is_conjunctive(const FInstr &I)
+{
+    switch (I.kind) {
+    case FK_PROBE_SEMI:
+    case FK_PROBE_FULL:
+    case FK_PROBE_BITS: return true;
+    case FK_SEL_LIT:
+    case FK_SEL_COL:
+    case FK_SEL_RANGE:
+    case FK_SEL_IN2:
+    case FK_LOGICAL: return I.logic == SDB_AND;
+    default: return false;
+    }
+}
+
+bool has_vout(const FInstr &I)
+{
+    switch (I.kind) {
+    case FK_FILL:
+    case FK_BINOP_CC:
+    case FK_BINOP_LC:
+    case FK_BINOP_CL:
+    case FK_PROBE_FULL: return true;
+    case FK_PROBE_BITS: return I.vkind != FV_NONE;
+    default: return false;
+    }
+}
+
+// value registers an instruction reads
+void value_reads(const FInstr &I, const FGroup &g, std::vector<int> &out)
+{
+    out.clear();
+    switch (I.kind) {
+    case FK_SEL_LIT:
+    case FK_SEL_RANGE:
+    case FK_SEL_IN2:
+    case FK_PROBE_SEMI:
+    case FK_PROBE_FULL:
+    case FK_PROBE_BITS:
+    case FK_BUILD_KEYS:
+    case FK_SUM:
+    case FK_BINOP_CL:
+        if (I.va >= 0) out.push_back(I.va);
+        break;
+    case FK_SEL_COL:
+    case FK_BINOP_CC:
+    case FK_BUILD_KEYVALS:
+        if (I.va >= 0) out.push_back(I.va);
+        if (I.vb >= 0) out.push_back(I.vb);
+        break;
+    case FK_BINOP_LC:
+        if (I.vb >= 0) out.push_back(I.vb);
+        break;
+    case FK_GROUPBY:
+        for (int j = 0; j < g.k; j++)
+            if (g.key_reg[j] >= 0) out.push_back(g.key_reg[j]);
+        if (g.agg_reg >= 0) out.push_back(g.agg_reg);
+        break;
+    default: break;
+    }
+}
+
+// flag registers an instruction reads (fin always, floc for LOGICAL)
+int flag_reads(const FInstr &I, int out[2])
+{
+    int n = 0;
+    switch (I.kind) {
+    case FK_FILL: return 0; // writes every row, reads no flags
+    case FK_LOGICAL: out[n++] = I.fin; out[n++] = I.floc; return n;
+    default: out[n++] = I.fin; return n;
+    }
+}
+
+// SEARCH fold: [x >= lo (NONE) into L][x <= hi AND into L][logic(in, L)]
+// or the two-point form [x == a (NONE)][x == b OR][logic(in, L)].
+void fold_search(std::vector<Draft> &d)
+{
+    for (size_t i = 0; i + 2 < d.size(); i++) {
+        FInstr &a = d[i].I, &b = d[i + 1].I, &c = d[i + 2].I;
+        if (a.kind != FK_SEL_LIT || b.kind != FK_SEL_LIT || c.kind != FK_LOGICAL) continue;
+        const int L = a.fout;
+        if (a.fin != L || b.fin != L || b.fout != L || c.floc != L || c.fin == L || c.fout == L) continue;
+        if (a.va != b.va || a.a != b.a) continue;
+        const bool range = a.cmp == SDB_GE && a.logic == SDB_NONE && b.cmp == SDB_LE && b.logic == SDB_AND;
+        const bool pts = a.cmp == SDB_EQ && a.logic == SDB_NONE && b.cmp == SDB_EQ && b.logic == SDB_OR;
+        if (!range && !pts) continue;
+        // the local buffer must not be read after the merge
+        bool read_later = false;
+        for (size_t k = i + 3; k < d.size() && !read_later; k++) {
+            int r[2];
+            const int nr = flag_reads(d[k].I, r);
+            for (int q = 0; q < nr; q++) read_later |= r[q] == L;
+            if (writes_flag(d[k].I.kind) && d[k].I.fout == L) break;
+        }
+        if (read_later) continue;
+        FInstr f{};
+        f.kind = range ? FK_SEL_RANGE : FK_SEL_IN2;
+        f.logic = c.logic;
+        f.fin = c.fin;
+        f.fout = c.fout;
+        f.va = a.va;
+        f.a = a.a;
+        f.vb = f.vout = -1;
+        f.lit = a.lit;
+        f.lit2 = b.lit;
+        Draft nd;
+        nd.I = f;
+        nd.sel = d[i + 2].sel;
+        nd.cost = 1.0f;
+        d.erase(d.begin() + i, d.begin() + i + 3);
+        d.insert(d.begin() + i, nd);
+    }
+}
+
+} // namespace
+
 bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
 {
     p = FProgram{};
@@ -138,37 +320,22 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
         if (k.n != n) return false;
 
     std::vector<const uint8_t *> fptr;     // flag register -> buffer
-    std::vector<bool> fwritten;
+    std::vector<uint8_t> finit;            // FFlagInit per register
     std::map<const int32_t *, int> vmap;   // column produced in this bundle -> value register
-    bool vec = true;
-    bool terminal_seen = false;
+    std::vector<int32_t *> vptr;           // value register -> its column
     bool ok = true;
 
-    auto freg = [&](const uint8_t *ptr, bool write) -> int {
+    auto freg = [&](const uint8_t *ptr, bool write) -> uint8_t {
         for (size_t j = 0; j < fptr.size(); j++)
-            if (fptr[j] == ptr) {
-                if (write) fwritten[j] = true;
-                return (int)j;
-            }
+            if (fptr[j] == ptr) return (uint8_t)j;
         if ((int)fptr.size() >= F_MAX_FLAGS) {
             ok = false;
             return 0;
         }
-        const int j = (int)fptr.size();
         fptr.push_back(ptr);
-        fwritten.push_back(write);
-        if (write) {
-            p.flag_init[j] = FF_UNDEF;
-        } else if (reads_virgin(ctx, ptr)) {
-            p.flag_init[j] = FF_TRUE;
-        } else {
-            p.flag_init[j] = FF_LOAD;
-            p.flag_load[j] = ptr;
-            vec &= aligned(ptr, 4);
-        }
-        return j;
+        finit.push_back(write ? FF_UNDEF : (reads_virgin(ctx, ptr) ? FF_TRUE : FF_LOAD));
+        return (uint8_t)(fptr.size() - 1);
     };
-    // operand: value register when produced earlier in the bundle, else memory
     auto src = [&](const int32_t *ptr, int8_t &reg, const int32_t *&mem) {
         auto it = vmap.find(ptr);
         if (it != vmap.end()) {
@@ -177,34 +344,37 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
         } else {
             reg = -1;
             mem = ptr;
-            vec &= aligned(ptr, 16);
         }
     };
-    auto dst = [&](int32_t *ptr, FInstr &I) {
+    auto dst = [&](int32_t *ptr, FInstr &I, bool &store) {
         auto it = vmap.find(ptr);
         int r;
         if (it != vmap.end()) {
             r = it->second;
         } else {
-            r = (int)vmap.size();
+            r = (int)vptr.size();
             if (r >= F_MAX_VALS) {
                 ok = false;
                 r = 0;
             }
             vmap[ptr] = r;
+            vptr.push_back(ptr);
         }
         I.vout = (int8_t)r;
-        I.store = ctx.live->contains(ptr) ? ptr : nullptr;
-        if (I.store) vec &= aligned(ptr, 16);
+        store = ctx.live->contains(ptr);
+        I.store = store ? ptr : nullptr;
     };
 
-    int ni = 0;
+    // ---------------------------------------------------------- translation
+    std::vector<Draft> d;
+    bool terminal_seen = false;
     for (const KernelData &k : kernels_) {
         if (k.type == KernelType::EmptyKernel) continue;
-        if (ni >= F_MAX_INSTR || terminal_seen) return false;
-        FInstr &I = p.ins[ni];
-        I = FInstr{};
+        if (terminal_seen) return false;
+        Draft D;
+        FInstr &I = D.I;
         I.va = I.vb = I.vout = -1;
+        D.sel = k.sel;
         switch (k.type) {
         case KernelType::SelectionKernelLiteral:
         case KernelType::SelectionKernelColumns:
@@ -212,22 +382,24 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
             I.cmp = (uint8_t)k.cmp;
             I.logic = (uint8_t)k.logic;
             I.lit = k.literal;
-            I.fin = (uint8_t)freg(k.in_flags, false);
+            I.fin = freg(k.in_flags, false);
             src(k.a, I.va, I.a);
             if (I.kind == FK_SEL_COL) src(k.b, I.vb, I.b);
-            I.fout = (uint8_t)freg(k.out_flags, true);
+            I.fout = freg(k.out_flags, true);
+            D.cost = I.kind == FK_SEL_COL ? 2.0f : 1.0f;
             break;
         case KernelType::LogicalKernel:
             I.kind = FK_LOGICAL;
             I.logic = (uint8_t)k.logic;
-            I.fin = (uint8_t)freg(k.in_flags, false);
-            I.floc = (uint8_t)freg(k.local_flags, false);
-            I.fout = (uint8_t)freg(k.out_flags, true);
+            I.fin = freg(k.in_flags, false);
+            I.floc = freg(k.local_flags, false);
+            I.fout = freg(k.out_flags, true);
+            D.cost = 0.1f;
             break;
         case KernelType::FillKernel:
             I.kind = FK_FILL;
             I.lit = k.literal;
-            dst(k.result, I);
+            dst(k.result, I, D.store);
             break;
         case KernelType::PerformOperationKernelColumns:
         case KernelType::PerformOperationKernelLiteralFirst:
@@ -237,34 +409,41 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
                                                                                  : FK_BINOP_CL;
             I.op = (uint8_t)k.op;
             I.lit = k.literal;
-            I.fin = (uint8_t)freg(k.in_flags, false);
+            I.fin = freg(k.in_flags, false);
             if (I.kind != FK_BINOP_LC) src(k.a, I.va, I.a);
             if (I.kind != FK_BINOP_CL) src(k.b, I.vb, I.b);
-            dst(k.result, I);
+            dst(k.result, I, D.store);
             break;
         case KernelType::FilterJoinKernel:
-            I.kind = FK_PROBE_SEMI;
-            I.fin = (uint8_t)freg(k.in_flags, false);
+        case KernelType::FullJoinKernel: {
+            const bool full = k.type == KernelType::FullJoinKernel;
+            I.fin = freg(k.in_flags, false);
             src(k.a, I.va, I.a);
-            I.ht = k.ht;
             I.ht_min = k.ht_min;
             I.ht_max = k.ht_max;
-            I.fout = (uint8_t)freg(k.out_flags, true);
+            if (k.bits) {
+                I.kind = FK_PROBE_BITS;
+                I.ht = k.bits;
+                I.vkind = full ? (int8_t)k.vkind : (int8_t)FV_NONE;
+                I.vals = full ? k.vals : nullptr;
+                I.vmin = k.vmin;
+                if (full && k.vkind == FV_NONE) return false;
+                D.cost = 1.5f;
+            } else {
+                I.kind = full ? FK_PROBE_FULL : FK_PROBE_SEMI;
+                I.ht = k.ht;
+                D.cost = full ? 3.0f : 2.0f;
+            }
+            if (full) {
+                if (!aligned(k.ht, 8)) return false;
+                dst(k.result, I, D.store);
+            }
+            I.fout = freg(k.out_flags, true);
             break;
-        case KernelType::FullJoinKernel:
-            I.kind = FK_PROBE_FULL;
-            I.fin = (uint8_t)freg(k.in_flags, false);
-            src(k.a, I.va, I.a);
-            I.ht = k.ht;
-            I.ht_min = k.ht_min;
-            I.ht_max = k.ht_max;
-            if (!aligned(k.ht, 8)) return false;
-            dst(k.result, I);
-            I.fout = (uint8_t)freg(k.out_flags, true);
-            break;
+        }
         case KernelType::BuildKeysHTKernel:
             I.kind = FK_BUILD_KEYS;
-            I.fin = (uint8_t)freg(k.in_flags, false);
+            I.fin = freg(k.in_flags, false);
             src(k.a, I.va, I.a);
             I.ht = k.ht;
             I.ht_min = k.ht_min;
@@ -272,7 +451,7 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
             break;
         case KernelType::BuildKeyValsHTKernel:
             I.kind = FK_BUILD_KEYVALS;
-            I.fin = (uint8_t)freg(k.in_flags, false);
+            I.fin = freg(k.in_flags, false);
             src(k.a, I.va, I.a);
             src(k.b, I.vb, I.b);
             I.ht = k.ht;
@@ -282,7 +461,7 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
             break;
         case KernelType::AggregateOperationKernel:
             I.kind = FK_SUM;
-            I.fin = (uint8_t)freg(k.in_flags, false);
+            I.fin = freg(k.in_flags, false);
             src(k.a, I.va, I.a);
             p.sum_acc = k.agg_res;
             if (!aligned(k.agg_res, 8)) return false;
@@ -291,9 +470,9 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
         case KernelType::GroupByAggregateKernel: {
             const GroupBySpec &g = *k.gb;
             const int nk = (int)g.keys.size();
-            if (nk < 1 || nk > F_MAX_KEYS) return false;
+            if (nk < 1 || nk > F_MAX_KEYS || g.prod == 0 || !aligned(g.sums, 8)) return false;
             I.kind = FK_GROUPBY;
-            I.fin = (uint8_t)freg(k.in_flags, false);
+            I.fin = freg(k.in_flags, false);
             FGroup &G = p.gb;
             G.k = nk;
             uint64_t exact = 1;
@@ -312,23 +491,213 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
             G.sums = g.sums;
             G.seen = g.seen;
             G.prod = g.prod;
-            if (g.prod == 0) return false;
-            G.lds = fits && exact == g.prod && g.prod <= fused_groupby_lds_max_slots();
-            if (!aligned(g.sums, 8)) return false;
+            // in-range keys map one-to-one onto slots below 2^32: LDS copies
+            // (dense for small ranges, hashed otherwise) with an exact flush
+            fits = fits && exact == g.prod && g.prod < 0xFFFFFFFFull;
+            G.lds = !fits ? FG_GLOBAL : (g.prod <= F_DENSE_MAX ? FG_DENSE_LDS : FG_HASH_LDS);
             terminal_seen = true;
             break;
         }
         default: return false;
         }
         if (!ok) return false;
-        ni++;
+        d.push_back(D);
     }
-    if (ni == 0) return false;
-    p.ninstr = ni;
-    // write back every flag buffer the bundle changed (or still logically
-    // all-true) that is referenced afterwards
+    if (d.empty() || (int)d.size() > F_MAX_INSTR + 8) return false;
+
+    // --------------------------------------------------------- optimisation
+    fold_search(d);
+    // NONE on the untouched all-true initial flags is AND
+    {
+        std::vector<bool> written(fptr.size(), false);
+        for (Draft &D : d) {
+            FInstr &I = D.I;
+            if (is_predicate(I.kind) && I.kind != FK_PROBE_SEMI && I.kind != FK_PROBE_FULL && I.kind != FK_PROBE_BITS &&
+                I.logic == SDB_NONE && finit[I.fin] == FF_TRUE && !written[I.fin]) {
+                I.logic = SDB_AND;
+            }
+            if (writes_flag(I.kind)) written[I.fout] = true;
+        }
+    }
+    // reorder conjunctive runs
+    {
+        const size_t m = d.size();
+        // producer of each flag read: index of the last earlier writer of that register, -1 initial
+        std::vector<std::array<int, 2>> prod_of(m, {-2, -2});
+        std::vector<std::vector<int>> readers(m); // readers of the version instruction i writes
+        std::vector<int> last(fptr.size(), -1);
+        for (size_t i = 0; i < m; i++) {
+            int r[2];
+            const int nr = flag_reads(d[i].I, r);
+            for (int q = 0; q < nr; q++) {
+                const int w = last[r[q]];
+                prod_of[i][q] = w;
+                if (w >= 0) readers[w].push_back((int)i);
+            }
+            if (writes_flag(d[i].I.kind)) last[d[i].I.fout] = (int)i;
+        }
+        auto live_at_end = [&](int i) {
+            const int reg = d[i].I.fout;
+            return last[reg] == i && ctx.live->contains(fptr[reg]);
+        };
+        size_t s0 = 0;
+        while (s0 < m) {
+            if (!is_conjunctive(d[s0].I)) {
+                s0++;
+                continue;
+            }
+            size_t e = s0;
+            while (e + 1 < m && is_conjunctive(d[e + 1].I) && prod_of[e + 1][0] == (int)e &&
+                   readers[e].size() == 1 && !live_at_end((int)e))
+                e++;
+            const size_t len = e - s0 + 1;
+            bool can = len >= 2;
+            for (size_t i = s0; i <= e && can; i++)
+                if (d[i].I.kind == FK_LOGICAL && prod_of[i][1] >= (int)s0) can = false;
+            for (size_t i = s0; i <= e && can; i++) can = !d[i].store;
+            // values produced in the run: readers after the run must use its final flag
+            for (size_t i = s0; i <= e && can; i++) {
+                if (!has_vout(d[i].I)) continue;
+                std::vector<int> vr;
+                for (size_t k = e + 1; k < m && can; k++) {
+                    value_reads(d[k].I, p.gb, vr);
+                    bool reads = false;
+                    for (int r : vr) reads |= r == d[i].I.vout;
+                    if (reads && prod_of[k][0] != (int)e) can = false;
+                    if (has_vout(d[k].I) && d[k].I.vout == d[i].I.vout) break;
+                }
+            }
+            if (can) {
+                // greedy order by cost / (1 - sel), respecting value dependencies
+                std::vector<Draft> run(d.begin() + s0, d.begin() + e + 1);
+                std::vector<bool> placed(len, false);
+                std::vector<Draft> ordered;
+                const uint8_t in0 = run[0].I.fin, out_final = run[len - 1].I.fout;
+                for (size_t step = 0; step < len; step++) {
+                    int best = -1;
+                    float best_rank = 0;
+                    for (size_t c = 0; c < len; c++) {
+                        if (placed[c]) continue;
+                        // ready when every value it reads that the run produces is placed
+                        std::vector<int> vr;
+                        value_reads(run[c].I, p.gb, vr);
+                        bool ready = true;
+                        for (size_t q = 0; q < len && ready; q++) {
+                            if (placed[q] || q == c || !has_vout(run[q].I)) continue;
+                            for (int r : vr) ready &= r != run[q].I.vout;
+                        }
+                        if (!ready) continue;
+                        const float keep = std::min(run[c].sel, 0.9999f);
+                        const float rank = run[c].cost / (1.0f - keep);
+                        if (best < 0 || rank < best_rank) {
+                            best = (int)c;
+                            best_rank = rank;
+                        }
+                    }
+                    if (best < 0) {
+                        can = false;
+                        break;
+                    }
+                    placed[best] = true;
+                    ordered.push_back(run[best]);
+                }
+                if (can) {
+                    for (size_t q = 0; q < len; q++) {
+                        ordered[q].I.fin = q == 0 ? in0 : out_final;
+                        ordered[q].I.fout = out_final;
+                        d[s0 + q] = ordered[q];
+                    }
+                }
+            }
+            s0 = e + 1;
+        }
+    }
+
+    // --------------------------------------------------------------- layout
+    if ((int)d.size() > F_MAX_INSTR) return false;
+    bool vec = true;
+    for (size_t i = 0; i < d.size(); i++) {
+        const FInstr &I = d[i].I;
+        for (const int32_t *mem : {I.a, I.b})
+            if (mem) vec &= aligned(mem, 16);
+        if (I.store) vec &= aligned(I.store, 16);
+        p.ins[i] = I;
+    }
+    p.ninstr = (int)d.size();
+    for (int j = 0; j < p.gb.k; j++)
+        if (p.gb.key_mem[j]) vec &= aligned(p.gb.key_mem[j], 16);
+    if (p.gb.agg_mem) vec &= aligned(p.gb.agg_mem, 16);
+
+    uint32_t off = 0;
+    auto take = [&](uint32_t bytes) {
+        const uint32_t at = off;
+        off += (bytes + 15u) & ~15u;
+        return at;
+    };
+    p.nvals = (int)vptr.size();
+    p.vreg_off = take((uint32_t)p.nvals * 256u * 16u);
+    // stage the smallest bitmaps in LDS (each <= 32 KiB, together <= 48 KiB)
+    {
+        std::vector<std::pair<uint32_t, int>> cand; // bytes, instruction
+        for (int i = 0; i < p.ninstr; i++) {
+            const FInstr &I = p.ins[i];
+            if (I.kind != FK_PROBE_BITS) continue;
+            const uint64_t len = (uint64_t)((int64_t)I.ht_max - I.ht_min + 1);
+            const uint32_t bytes = (uint32_t)(((len + 31) / 32 + 3) / 4 * 16);
+            if (len <= 32u * 1024u * 8u) cand.emplace_back(bytes, i);
+        }
+        std::sort(cand.begin(), cand.end());
+        uint32_t staged = 0;
+        for (auto &c : cand) {
+            FInstr &I = p.ins[c.second];
+            int blob = -1;
+            for (int b = 0; b < p.nblobs; b++)
+                if (p.blob_src[b] == I.ht) blob = b;
+            if (blob < 0) {
+                if (p.nblobs >= F_MAX_BLOBS || staged + c.first > 48u * 1024u) continue;
+                blob = p.nblobs++;
+                p.blob_src[blob] = I.ht;
+                p.blob_bytes[blob] = c.first;
+                p.blob_off[blob] = take(c.first);
+                staged += c.first;
+            }
+            I.in_lds = 1;
+            I.lds_off = p.blob_off[blob];
+        }
+    }
+    if (p.gb.lds == FG_DENSE_LDS) {
+        p.gb.lds_off = take((uint32_t)(p.gb.prod * 8 + (p.gb.prod + 31) / 32 * 4));
+    } else if (p.gb.lds == FG_HASH_LDS) {
+        p.gb.hash_cap = F_HASH_CAP;
+        p.gb.lds_off = take(F_HASH_CAP * 12u);
+    }
+    p.lds_bytes = off;
+    if (p.lds_bytes > F_LDS_BUDGET) return false;
+
+    // flags: initial values and write-back of changed buffers still referenced
+    std::vector<bool> written(fptr.size(), false);
+    std::vector<int> first(fptr.size(), 0); // 1: first access reads, 2: first access writes
+    for (int i = 0; i < p.ninstr; i++) {
+        int r[2];
+        const int nr = flag_reads(p.ins[i], r);
+        for (int q = 0; q < nr; q++)
+            if (!first[r[q]]) first[r[q]] = 1;
+        if (writes_flag(p.ins[i].kind)) {
+            if (!first[p.ins[i].fout]) first[p.ins[i].fout] = 2;
+            written[p.ins[i].fout] = true;
+        }
+    }
     for (size_t j = 0; j < fptr.size(); j++) {
-        const bool changed = fwritten[j] || p.flag_init[j] == FF_TRUE;
+        // a register whose initial value nobody reads is not loaded (an
+        // all-true buffer still referenced afterwards is written back true)
+        const bool keep_true = finit[j] == FF_TRUE && !written[j] && ctx.live->contains(fptr[j]);
+        if (first[j] != 1 && !keep_true) finit[j] = FF_UNDEF;
+        p.flag_init[j] = finit[j];
+        if (finit[j] == FF_LOAD) {
+            p.flag_load[j] = fptr[j];
+            vec &= aligned(fptr[j], 4);
+        }
+        const bool changed = written[j] || finit[j] == FF_TRUE;
         if (changed && ctx.live->contains(fptr[j])) {
             p.flag_store[j] = const_cast<uint8_t *>(fptr[j]);
             vec &= aligned(fptr[j], 4);

@@ -47,6 +47,12 @@ void launch_groupby_dense(const int32_t *const *keys, int k, const int32_t *mins
 void launch_seen_to_flags(const uint32_t *seen, uint8_t *flags, uint64_t prod, hipStream_t s);
 void minmax_i32(const int32_t *col, int64_t n, int32_t *mn, int32_t *mx, hipStream_t s);
 uint64_t groupby_lds_max_slots();
+// Compact probe tables (ops.hip): bitmap from a uint8 key table, bitmap and
+// uint16 value offsets from a key-value pair table; counts set bits into
+// *count and sets *bad when a value does not fit 16 bits.
+void launch_bits_from_keys(const uint8_t *ht, int32_t len, uint32_t *bits, uint32_t *count, hipStream_t s);
+void launch_bits_from_pairs(const int32_t *ht, int32_t len, uint32_t *bits, uint16_t *vals, int32_t vmin,
+                            uint32_t *count, uint32_t *bad, hipStream_t s);
 
 // --------------------------------------------------------- allocation map
 
@@ -168,6 +174,14 @@ struct KernelData {
     int32_t ht_len = 0, ht_min = 0, ht_max = 0;
     uint64_t *agg_res = nullptr;
     std::shared_ptr<GroupBySpec> gb;
+    // Compact probe tables derived from `ht` once the build side ran (used by
+    // fused bundles only): presence bitmap over [ht_min, ht_max] and, for a
+    // full join, the value per slot (uint16 offset from vmin, or the pairs).
+    const uint32_t *bits = nullptr;
+    const void *vals = nullptr;
+    int vkind = 0;       // FValKind
+    int32_t vmin = 0;
+    float sel = 1.0f;    // estimated fraction of rows a predicate keeps
 };
 
 struct Engine;

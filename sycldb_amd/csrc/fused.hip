@@ -3,25 +3,27 @@
 //
 // Each thread executes the program on chunks of R rows (R = 4: 16-byte column
 // loads; R = 1: any alignment / the tail).  Flag registers of a chunk live in
-// one 32-bit word (4 bits per register), value registers in VGPRs.  The
-// instruction stream is uniform across the grid, so the dispatch on the
-// instruction kind is a scalar branch; per-row work follows the reference
-// functors exactly.  A column operand is only loaded for chunks in which some
-// row still needs it.  Terminal reductions (sum / group-by) are privatised
-// per thread (sum) or per block in LDS (dense group-by) and flushed once.
+// one 32-bit word (4 bits per register); value registers live in an LDS
+// register file (one int4 per register and thread), addressed by the uniform
+// register number.  The instruction stream is uniform across the grid, so the
+// dispatch on the instruction kind is a scalar branch; per-row work follows the
+// reference functors exactly.  A column operand is only loaded for chunks in
+// which some row still needs it, so a selective prefix of the program cuts the
+// HBM lines later columns pull.  Small probe bitmaps are staged in LDS at kernel
+// start.  Terminal reductions are privatised per thread (sum) or per block in
+// LDS (dense slots or an open-addressed hash of slots) and flushed once.
 #include "capi_util.hpp"
 #include "fused.hpp"
 
 #include <algorithm>
 
 namespace sdb {
-
-// from ops.hip
-uint64_t groupby_lds_max_slots();
-
 namespace {
 
 constexpr int BLOCK = 256;
+constexpr uint32_t HASH_EMPTY = 0xFFFFFFFFu;
+
+typedef int v4i __attribute__((ext_vector_type(4)));
 
 template <int R> __device__ __forceinline__ uint32_t flags_in(const uint8_t *p)
 {
@@ -43,13 +45,11 @@ template <int R> __device__ __forceinline__ void flags_out(uint8_t *p, uint32_t 
     }
 }
 
-// Value registers live in LDS: register j of thread t is vregs[j * BLOCK + t]
-// (one int4 = the 4 rows of a chunk).  The register number is uniform, so an
-// access is a single ds_read/ds_write_b128 at a computed address; keeping them
-// in VGPRs would need a dynamically indexed array, which the compiler places
-// in scratch memory.
+// Value registers: register j of thread t is regs[j * BLOCK + t] (an int4 =
+// the rows of one chunk).  The register number is uniform, so an access is
+// one ds_read/ds_write_b128 at a computed address.
 struct VRegs {
-    int4 *base; // &vregs[threadIdx.x]
+    int4 *base; // &regs[threadIdx.x]
 };
 
 template <int R> __device__ __forceinline__ void vget(const VRegs &V, int r, int32_t (&out)[R])
@@ -82,19 +82,21 @@ template <int R> __device__ __forceinline__ void vset(const VRegs &V, int r, con
     }
 }
 
+// Column operand: a value register, or R rows of a column.  Column reads are
+// streaming (non-temporal) so they do not push probe tables out of the caches.
 template <int R>
 __device__ __forceinline__ void operand(int reg, const int32_t *mem, int64_t row0, const VRegs &V, int32_t (&x)[R])
 {
     if (reg >= 0) {
         vget<R>(V, reg, x);
     } else if constexpr (R == 4) {
-        const int4 t = *reinterpret_cast<const int4 *>(mem + row0);
+        const v4i t = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(mem + row0));
         x[0] = t.x;
         x[1] = t.y;
         x[2] = t.z;
         x[3] = t.w;
     } else {
-        x[0] = mem[row0];
+        x[0] = __builtin_nontemporal_load(mem + row0);
     }
 }
 
@@ -112,12 +114,31 @@ template <int R> __device__ __forceinline__ void store_masked(int32_t *dst, int6
         if ((m >> k) & 1u) dst[row0 + k] = x[k];
 }
 
-struct GbState {
-    unsigned long long *lsum;
-    uint32_t *lbits;
+__device__ __forceinline__ uint32_t logic_combine(uint32_t logic, uint32_t in, uint32_t pred)
+{
+    return logic == SDB_NONE ? pred : (logic == SDB_AND ? (in & pred) : (in | pred));
+}
+
+struct Lds {
+    uint8_t *base; // dynamic LDS
 };
 
-__device__ __forceinline__ void gb_row(const FGroup &g, const GbState &st, const int32_t *kv, int32_t agg)
+__device__ __forceinline__ void gb_flush_slot(const FGroup &g, uint32_t s, unsigned long long sum)
+{
+    atomicAdd(reinterpret_cast<unsigned long long *>(&g.sums[s]), sum);
+    if (atomicExch(&g.seen[s], 1u) == 0u) {
+        // slot -> key values (first key varies fastest); exact for in-range keys
+        uint32_t rest = s;
+#pragma unroll
+        for (int j = 0; j < F_MAX_KEYS; j++) {
+            if (j >= g.k) break;
+            g.out_keys[j][s] = (int32_t)((uint32_t)g.mins[j] + rest % g.range[j]);
+            rest /= g.range[j];
+        }
+    }
+}
+
+__device__ __forceinline__ void gb_row(const FGroup &g, const Lds &L, const int32_t *kv, int32_t agg)
 {
     bool in_range = true;
     uint32_t hash = 0, mult = 1;
@@ -128,22 +149,46 @@ __device__ __forceinline__ void gb_row(const FGroup &g, const GbState &st, const
         hash += ((uint32_t)kv[j] - (uint32_t)g.mins[j]) * mult;
         mult *= g.range[j];
     }
-    const uint32_t s = (uint32_t)((uint64_t)(int64_t)(int32_t)hash % g.prod);
-    if (g.lds && in_range) {
-        atomicAdd(&st.lsum[s], (unsigned long long)(int64_t)agg);
-        atomicOr(&st.lbits[s >> 5], 1u << (s & 31));
+    const uint64_t s64 = (uint64_t)(int64_t)(int32_t)hash % g.prod;
+    const unsigned long long v = (unsigned long long)(int64_t)agg;
+    if (in_range && g.lds == FG_DENSE_LDS) {
+        const uint32_t s = (uint32_t)s64;
+        unsigned long long *lsum = reinterpret_cast<unsigned long long *>(L.base + g.lds_off);
+        uint32_t *lbits = reinterpret_cast<uint32_t *>(lsum + g.prod);
+        atomicAdd(&lsum[s], v);
+        atomicOr(&lbits[s >> 5], 1u << (s & 31));
         return;
     }
-    if (atomicExch(&g.seen[s], 1u) == 0u) {
+    if (in_range && g.lds == FG_HASH_LDS) {
+        const uint32_t s = (uint32_t)s64;
+        uint32_t *hk = reinterpret_cast<uint32_t *>(L.base + g.lds_off);
+        unsigned long long *hv = reinterpret_cast<unsigned long long *>(hk + g.hash_cap);
+        const uint32_t mask = g.hash_cap - 1;
+        uint32_t h = (s * 2654435761u) & mask;
+        for (int probe = 0; probe < 32; probe++) {
+            uint32_t cur = hk[h];
+            if (cur == HASH_EMPTY) {
+                cur = atomicCAS(&hk[h], HASH_EMPTY, s);
+                if (cur == HASH_EMPTY) cur = s;
+            }
+            if (cur == s) {
+                atomicAdd(&hv[h], v);
+                return;
+            }
+            h = (h + 1) & mask;
+        }
+        // table crowded: fall through to the global update
+    }
+    if (atomicExch(&g.seen[s64], 1u) == 0u) {
 #pragma unroll
         for (int j = 0; j < F_MAX_KEYS; j++)
-            if (j < g.k) g.out_keys[j][s] = kv[j];
+            if (j < g.k) g.out_keys[j][s64] = kv[j];
     }
-    atomicAdd(reinterpret_cast<unsigned long long *>(&g.sums[s]), (unsigned long long)(int64_t)agg);
+    atomicAdd(reinterpret_cast<unsigned long long *>(&g.sums[s64]), v);
 }
 
 template <int R>
-__device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64_t &acc, const GbState &gst,
+__device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64_t &acc, const Lds &L,
                                           const VRegs &V)
 {
     constexpr uint32_t M = (1u << R) - 1;
@@ -160,7 +205,9 @@ __device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64
         const uint32_t in = (F >> (4 * I.fin)) & M;
         switch (I.kind) {
         case FK_SEL_LIT:
-        case FK_SEL_COL: {
+        case FK_SEL_COL:
+        case FK_SEL_RANGE:
+        case FK_SEL_IN2: {
             const uint32_t need = I.logic == SDB_NONE ? M : (I.logic == SDB_AND ? in : (~in & M));
             uint32_t cm = 0;
             if (need) {
@@ -168,17 +215,20 @@ __device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64
                 operand<R>(I.va, I.a, row0, V, x);
                 if (I.kind == FK_SEL_COL) operand<R>(I.vb, I.b, row0, V, y);
 #pragma unroll
-                for (int k = 0; k < R; k++)
-                    cm |= (uint32_t)compare(I.cmp, x[k], I.kind == FK_SEL_COL ? y[k] : I.lit) << k;
+                for (int k = 0; k < R; k++) {
+                    bool c;
+                    if (I.kind == FK_SEL_RANGE) c = x[k] >= I.lit && x[k] <= I.lit2;
+                    else if (I.kind == FK_SEL_IN2) c = x[k] == I.lit || x[k] == I.lit2;
+                    else c = compare(I.cmp, x[k], I.kind == FK_SEL_COL ? y[k] : I.lit);
+                    cm |= (uint32_t)c << k;
+                }
             }
-            const uint32_t out = I.logic == SDB_NONE ? cm : (I.logic == SDB_AND ? (in & cm) : (in | cm));
-            F = (F & ~(0xFu << (4 * I.fout))) | (out << (4 * I.fout));
+            F = (F & ~(0xFu << (4 * I.fout))) | (logic_combine(I.logic, in, cm) << (4 * I.fout));
             break;
         }
         case FK_LOGICAL: {
             const uint32_t loc = (F >> (4 * I.floc)) & M;
-            const uint32_t out = I.logic == SDB_NONE ? loc : (I.logic == SDB_AND ? (in & loc) : (in | loc));
-            F = (F & ~(0xFu << (4 * I.fout))) | (out << (4 * I.fout));
+            F = (F & ~(0xFu << (4 * I.fout))) | (logic_combine(I.logic, in, loc) << (4 * I.fout));
             break;
         }
         case FK_FILL: {
@@ -236,6 +286,42 @@ __device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64
                 }
                 vset<R>(V, I.vout, r, out);
                 if (I.store && out) store_masked<R>(I.store, row0, r, out);
+            }
+            F = (F & ~(0xFu << (4 * I.fout))) | (out << (4 * I.fout));
+            break;
+        }
+        case FK_PROBE_BITS: {
+            // presence bit of a slot = the reference table's "match"; the value,
+            // when wanted, comes from the compact table or the reference pairs
+            uint32_t out = 0;
+            if (in) {
+                int32_t x[R];
+                uint32_t slot[R];
+                operand<R>(I.va, I.a, row0, V, x);
+                const uint32_t *bits = I.in_lds ? reinterpret_cast<const uint32_t *>(L.base + I.lds_off)
+                                                : reinterpret_cast<const uint32_t *>(I.ht);
+#pragma unroll
+                for (int k = 0; k < R; k++) {
+                    slot[k] = slot_of(x[k], I.ht_min);
+                    if (((in >> k) & 1u) && x[k] >= I.ht_min && x[k] <= I.ht_max &&
+                        ((bits[slot[k] >> 5] >> (slot[k] & 31)) & 1u))
+                        out |= 1u << k;
+                }
+                if (I.vkind != FV_NONE && out) {
+                    int32_t r[R];
+#pragma unroll
+                    for (int k = 0; k < R; k++) {
+                        r[k] = 0;
+                        if ((out >> k) & 1u) {
+                            if (I.vkind == FV_U16)
+                                r[k] = (int32_t)((uint32_t)I.vmin + reinterpret_cast<const uint16_t *>(I.vals)[slot[k]]);
+                            else
+                                r[k] = reinterpret_cast<const int2 *>(I.vals)[slot[k]].y;
+                        }
+                    }
+                    vset<R>(V, I.vout, r, out);
+                    if (I.store) store_masked<R>(I.store, row0, r, out);
+                }
             }
             F = (F & ~(0xFu << (4 * I.fout))) | (out << (4 * I.fout));
             break;
@@ -298,7 +384,7 @@ __device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64
                 int32_t kv[F_MAX_KEYS];
 #pragma unroll
                 for (int j = 0; j < F_MAX_KEYS; j++) kv[j] = keys[j][k];
-                gb_row(g, gst, kv, agg[k]);
+                gb_row(g, L, kv, agg[k]);
             }
             break;
         }
@@ -322,29 +408,42 @@ __global__ void __launch_bounds__(BLOCK) k_fused(const FProgram P_unused, int64_
     const FProgram &P = P_unused;
 #endif
     if (P.magic != F_MAGIC) return;
-    extern __shared__ unsigned long long lds[];
-    __shared__ int4 vregs[F_MAX_VALS * BLOCK];
-    const VRegs V{&vregs[threadIdx.x]};
-    GbState gst{nullptr, nullptr};
-    const uint32_t prod = (uint32_t)P.gb.prod;
-    const bool use_lds = P.gb.lds != 0;
-    if (use_lds) {
-        gst.lsum = lds;
-        gst.lbits = reinterpret_cast<uint32_t *>(lds + prod);
-        for (uint32_t s = threadIdx.x; s < prod; s += BLOCK) gst.lsum[s] = 0;
-        for (uint32_t w = threadIdx.x; w < (prod + 31) / 32; w += BLOCK) gst.lbits[w] = 0;
-        __syncthreads();
+    extern __shared__ uint4 dyn_lds[];
+    const Lds L{reinterpret_cast<uint8_t *>(dyn_lds)};
+    const VRegs V{reinterpret_cast<int4 *>(L.base + P.vreg_off) + threadIdx.x};
+
+    // stage probe bitmaps, clear the aggregation area
+    for (int b = 0; b < P.nblobs; b++) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(P.blob_src[b]);
+        uint4 *dst = reinterpret_cast<uint4 *>(L.base + P.blob_off[b]);
+        for (uint32_t w = threadIdx.x; w < P.blob_bytes[b] / 16; w += BLOCK) dst[w] = src[w];
     }
+    const FGroup &g = P.gb;
+    if (g.lds == FG_DENSE_LDS) {
+        unsigned long long *lsum = reinterpret_cast<unsigned long long *>(L.base + g.lds_off);
+        uint32_t *lbits = reinterpret_cast<uint32_t *>(lsum + g.prod);
+        for (uint32_t s = threadIdx.x; s < (uint32_t)g.prod; s += BLOCK) lsum[s] = 0;
+        for (uint32_t w = threadIdx.x; w < ((uint32_t)g.prod + 31) / 32; w += BLOCK) lbits[w] = 0;
+    } else if (g.lds == FG_HASH_LDS) {
+        uint32_t *hk = reinterpret_cast<uint32_t *>(L.base + g.lds_off);
+        unsigned long long *hv = reinterpret_cast<unsigned long long *>(hk + g.hash_cap);
+        for (uint32_t h = threadIdx.x; h < g.hash_cap; h += BLOCK) {
+            hk[h] = HASH_EMPTY;
+            hv[h] = 0;
+        }
+    }
+    __syncthreads();
+
     int64_t acc = 0;
     if (P.vec) {
         const int64_t nchunks = n >> 2;
         for (int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x; c < nchunks; c += (int64_t)gridDim.x * BLOCK)
-            run_chunk<4>(P, c << 2, acc, gst, V);
+            run_chunk<4>(P, c << 2, acc, L, V);
         if (blockIdx.x == gridDim.x - 1 && threadIdx.x < (n & 3))
-            run_chunk<1>(P, (nchunks << 2) + threadIdx.x, acc, gst, V);
+            run_chunk<1>(P, (nchunks << 2) + threadIdx.x, acc, L, V);
     } else {
         for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
-            run_chunk<1>(P, i, acc, gst, V);
+            run_chunk<1>(P, i, acc, L, V);
     }
     if (P.sum_acc) {
         __shared__ int64_t part[BLOCK / 64];
@@ -359,43 +458,32 @@ __global__ void __launch_bounds__(BLOCK) k_fused(const FProgram P_unused, int64_
             if (s) atomicAdd(reinterpret_cast<unsigned long long *>(P.sum_acc), (unsigned long long)s);
         }
     }
-    if (use_lds) {
+    if (g.lds == FG_DENSE_LDS) {
         __syncthreads();
-        const FGroup &g = P.gb;
-        for (uint32_t s = threadIdx.x; s < prod; s += BLOCK) {
-            if (!((gst.lbits[s >> 5] >> (s & 31)) & 1u)) continue;
-            atomicAdd(reinterpret_cast<unsigned long long *>(&g.sums[s]), gst.lsum[s]);
-            if (atomicExch(&g.seen[s], 1u) == 0u) {
-                uint32_t rest = s;
-                for (int j = 0; j < g.k; j++) {
-                    g.out_keys[j][s] = (int32_t)((uint32_t)g.mins[j] + rest % g.range[j]);
-                    rest /= g.range[j];
-                }
-            }
-        }
+        const unsigned long long *lsum = reinterpret_cast<const unsigned long long *>(L.base + g.lds_off);
+        const uint32_t *lbits = reinterpret_cast<const uint32_t *>(lsum + g.prod);
+        for (uint32_t s = threadIdx.x; s < (uint32_t)g.prod; s += BLOCK)
+            if ((lbits[s >> 5] >> (s & 31)) & 1u) gb_flush_slot(g, s, lsum[s]);
+    } else if (g.lds == FG_HASH_LDS) {
+        __syncthreads();
+        const uint32_t *hk = reinterpret_cast<const uint32_t *>(L.base + g.lds_off);
+        const unsigned long long *hv = reinterpret_cast<const unsigned long long *>(hk + g.hash_cap);
+        for (uint32_t h = threadIdx.x; h < g.hash_cap; h += BLOCK)
+            if (hk[h] != HASH_EMPTY) gb_flush_slot(g, hk[h], hv[h]);
     }
 }
 
 } // namespace
 
-// The fused kernel also holds the value registers in LDS; the dense group-by
-// gets what is left of a 64 KiB workgroup allocation.
-uint64_t fused_groupby_lds_max_slots()
-{
-    constexpr uint64_t total = 64 * 1024;
-    constexpr uint64_t regs = (uint64_t)F_MAX_VALS * BLOCK * 16 + 64;
-    return std::min<uint64_t>(groupby_lds_max_slots(), ((total - regs) * 8) / 65);
-}
-
 void launch_fused(const FProgram &p, int64_t n, hipStream_t s)
 {
     if (n <= 0) return;
     if (p.magic != F_MAGIC) fail(SDB_ERR_RUNTIME, "fused program without magic");
-    size_t lds = 0;
-    if (p.gb.lds) lds = p.gb.prod * 8 + ((p.gb.prod + 31) / 32) * 4;
+    if (p.lds_bytes > F_LDS_BUDGET) fail(SDB_ERR_RUNTIME, "fused program exceeds its LDS budget");
     const int64_t units = p.vec ? (n + 3) / 4 : n;
-    const int per_cu = lds > 32 * 1024 ? 2 : 8;
-    hipLaunchKernelGGL(k_fused, dim3(stream_grid(units, BLOCK, per_cu)), dim3(BLOCK), lds, s, p, n);
+    // blocks per CU from the LDS footprint (160 KiB per CU on gfx950)
+    const int per_cu = std::max(1, std::min(8, (int)((160u * 1024u) / std::max<uint32_t>(p.lds_bytes + 64, 1))));
+    hipLaunchKernelGGL(k_fused, dim3(stream_grid(units, BLOCK, per_cu)), dim3(BLOCK), p.lds_bytes, s, p, n);
     hip_check(hipGetLastError(), "fused bundle");
 }
 

@@ -36,24 +36,35 @@ enum FKind : uint8_t {
     FK_BUILD_KEYS,   // BuildKeysHTKernel
     FK_BUILD_KEYVALS,// BuildKeyValsHTKernel
     FK_SUM,          // AggregateOperationKernel (terminal)
-    FK_GROUPBY       // GroupByAggregateKernel (terminal)
+    FK_GROUPBY,      // GroupByAggregateKernel (terminal)
+    // forms produced by the program optimiser (same per-row results):
+    FK_SEL_RANGE,    // SEARCH over one range: logic(in, lit <= x <= lit2)
+    FK_SEL_IN2,      // SEARCH over two points: logic(in, x == lit || x == lit2)
+    FK_PROBE_BITS    // semi/full probe through a presence bitmap (+ value table)
 };
+
+// value table of FK_PROBE_BITS
+enum FValKind : uint8_t { FV_NONE = 0, FV_U16 = 1, FV_PAIR = 2 };
 
 struct FInstr {
     uint8_t kind, cmp, logic, op;
-    uint8_t fin, fout, floc, pad0;
-    int8_t va, vb, vout, pad1; // value registers; -1 = memory operand a / b
-    int32_t lit;
+    uint8_t fin, fout, floc, in_lds; // in_lds: FK_PROBE_BITS bitmap staged in LDS
+    int8_t va, vb, vout, vkind;      // value registers; -1 = memory operand a / b
+    int32_t lit, lit2;
     const int32_t *a;          // memory operand (row-indexed) when va < 0
     const int32_t *b;          // memory operand when vb < 0
     int32_t *store;            // write-back of vout (masked as the functor writes), or null
-    const void *ht;
-    int32_t ht_min, ht_max, ht_len, pad2;
+    const void *ht;            // reference-layout table; FK_PROBE_BITS: bitmap
+    const void *vals;          // FK_PROBE_BITS value table (FV_U16: offsets from vmin; FV_PAIR: ht)
+    int32_t ht_min, ht_max, ht_len, vmin;
+    uint32_t lds_off, pad;     // byte offset of the staged bitmap in dynamic LDS
 };
+
+enum FGroupMode : int { FG_GLOBAL = 0, FG_DENSE_LDS = 1, FG_HASH_LDS = 2 };
 
 struct FGroup {
     int k;
-    int lds;                   // 1: LDS-privatised dense slots
+    int lds;                   // FGroupMode
     int8_t key_reg[F_MAX_KEYS];
     int8_t agg_reg;
     int8_t pad[3];
@@ -66,6 +77,8 @@ struct FGroup {
     uint64_t *sums;
     uint32_t *seen;
     uint64_t prod;
+    uint32_t lds_off;          // byte offset of the LDS aggregation area
+    uint32_t hash_cap;         // FG_HASH_LDS: slots (power of two)
 };
 
 // Initial value of a flag register.  FF_UNDEF is 0 so that registers a
@@ -74,11 +87,20 @@ enum FFlagInit : uint8_t { FF_UNDEF = 0, FF_LOAD = 1, FF_TRUE = 2 };
 
 constexpr uint32_t F_MAGIC = 0x5DB0F05Eu;
 
+constexpr int F_MAX_BLOBS = 4;
+
 struct FProgram {
     uint32_t magic;            // F_MAGIC; the kernel reads the program in place
                                // from its kernel-argument segment and checks it
     int ninstr;
     int vec;                   // 1: 4-row chunks with 16-byte column loads
+    int nvals;                 // value registers used (LDS register file size)
+    int nblobs;                // tables staged into LDS at kernel start
+    uint32_t lds_bytes;        // dynamic LDS: registers, blobs, aggregation
+    uint32_t vreg_off;         // byte offset of the register file
+    const void *blob_src[F_MAX_BLOBS];
+    uint32_t blob_bytes[F_MAX_BLOBS]; // multiples of 16
+    uint32_t blob_off[F_MAX_BLOBS];
     uint8_t flag_init[F_MAX_FLAGS];
     const uint8_t *flag_load[F_MAX_FLAGS];
     uint8_t *flag_store[F_MAX_FLAGS];
@@ -87,9 +109,14 @@ struct FProgram {
     FInstr ins[F_MAX_INSTR];
 };
 
+// Dynamic LDS one fused block may use (registers, staged bitmaps, group-by).
+constexpr uint32_t F_LDS_BUDGET = 96 * 1024;
+// LDS aggregation sizes: dense slots up to F_DENSE_MAX, else a hash of
+// F_HASH_CAP slots (power of two).
+constexpr uint64_t F_DENSE_MAX = 1024;
+constexpr uint32_t F_HASH_CAP = 1024;
+
 // Launches the program over rows [0, n) of the segment the pointers address.
 void launch_fused(const FProgram &p, int64_t n, hipStream_t s);
-// Slots an LDS-privatised dense group-by can hold in the fused kernel.
-uint64_t fused_groupby_lds_max_slots();
 
 } // namespace sdb

@@ -561,6 +561,51 @@ __global__ void __launch_bounds__(BLOCK) k_seen_to_flags(const uint32_t *seen, u
         flags[i] = seen[i] != 0;
 }
 
+// ------------------------------------------------------ compact probe tables
+// One thread per 32-slot bitmap word; deterministic (no atomics on the
+// tables), one atomic per word for the population count.
+
+__global__ void __launch_bounds__(BLOCK) k_bits_from_keys(const uint8_t *ht, int32_t len, uint32_t *bits,
+                                                          uint32_t *count)
+{
+    const int64_t words = ((int64_t)len + 31) / 32;
+    for (int64_t w = (int64_t)blockIdx.x * BLOCK + threadIdx.x; w < words; w += (int64_t)gridDim.x * BLOCK) {
+        uint32_t m = 0;
+        for (int b = 0; b < 32; b++) {
+            const int64_t s = w * 32 + b;
+            if (s < len && ht[s] != 0) m |= 1u << b;
+        }
+        bits[w] = m;
+        if (m) atomicAdd(count, (uint32_t)__popc(m));
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_bits_from_pairs(const int2 *ht, int32_t len, uint32_t *bits,
+                                                           uint16_t *vals, int32_t vmin, uint32_t *count,
+                                                           uint32_t *bad)
+{
+    const int64_t words = ((int64_t)len + 31) / 32;
+    bool ok = true;
+    for (int64_t w = (int64_t)blockIdx.x * BLOCK + threadIdx.x; w < words; w += (int64_t)gridDim.x * BLOCK) {
+        uint32_t m = 0;
+        for (int b = 0; b < 32; b++) {
+            const int64_t s = w * 32 + b;
+            if (s >= len) break;
+            const int2 e = ht[s];
+            uint32_t d = 0;
+            if (e.x == 1) {
+                m |= 1u << b;
+                d = (uint32_t)e.y - (uint32_t)vmin;
+                if (d > 0xFFFFu) ok = false;
+            }
+            if (vals) vals[s] = (uint16_t)d;
+        }
+        bits[w] = m;
+        if (m) atomicAdd(count, (uint32_t)__popc(m));
+    }
+    if (!ok) atomicOr(bad, 1u);
+}
+
 // ------------------------------------------------------------ compaction
 // Three passes over tiles of COMPACT_TILE rows: count per tile, exclusive
 // scan of the counts (one block), then write ids in row order.
@@ -913,6 +958,25 @@ void launch_seen_to_flags(const uint32_t *seen, uint8_t *flags, uint64_t prod, h
     if (prod == 0) return;
     hipLaunchKernelGGL(k_seen_to_flags, dim3(grid_one((int64_t)prod)), dim3(BLOCK), 0, s, seen, flags, prod);
     check_launch("seen_to_flags");
+}
+
+void launch_bits_from_keys(const uint8_t *ht, int32_t len, uint32_t *bits, uint32_t *count, hipStream_t s)
+{
+    if (len <= 0) return;
+    const int64_t words = ((int64_t)len + 31) / 32;
+    hipLaunchKernelGGL(k_bits_from_keys, dim3(grid_one(words)), dim3(BLOCK), 0, s, ht, len, bits, count);
+    check_launch("bits_from_keys");
+}
+
+void launch_bits_from_pairs(const int32_t *ht, int32_t len, uint32_t *bits, uint16_t *vals, int32_t vmin,
+                            uint32_t *count, uint32_t *bad, hipStream_t s)
+{
+    if (len <= 0) return;
+    if (!aligned(ht, 8)) fail(SDB_ERR_INVALID_ARG, "key-value hash table must be 8-byte aligned");
+    const int64_t words = ((int64_t)len + 31) / 32;
+    hipLaunchKernelGGL(k_bits_from_pairs, dim3(grid_one(words)), dim3(BLOCK), 0, s,
+                       reinterpret_cast<const int2 *>(ht), len, bits, vals, vmin, count, bad);
+    check_launch("bits_from_pairs");
 }
 
 void launch_compact(const uint8_t *flags, int64_t n, int32_t *row_ids, uint32_t *count, hipStream_t s)

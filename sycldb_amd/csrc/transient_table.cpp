@@ -35,6 +35,23 @@ bool is_filter_logical(const std::string &op)
 // (int) of an int64 literal
 int32_t to_int32(int64_t v) { return (int32_t)(uint32_t)(uint64_t)v; }
 
+float clamp01(double x) { return (float)(x < 0 ? 0 : (x > 1 ? 1 : x)); }
+
+// Fraction of a uniformly distributed column in [mn, mx] a comparison keeps
+// (used only to order fused predicates, never for results).
+float estimate_cmp(int cmp, int32_t lit, std::pair<int32_t, int32_t> mm)
+{
+    const double lo = mm.first, hi = mm.second, range = hi - lo + 1, v = lit;
+    switch (cmp) {
+    case SDB_EQ: return (v >= lo && v <= hi) ? clamp01(1.0 / range) : 0.0f;
+    case SDB_NE: return (v >= lo && v <= hi) ? clamp01(1.0 - 1.0 / range) : 1.0f;
+    case SDB_LT: return clamp01((v - lo) / range);
+    case SDB_LE: return clamp01((v - lo + 1) / range);
+    case SDB_GT: return clamp01((hi - v) / range);
+    default: return clamp01((hi - v + 1) / range);
+    }
+}
+
 } // namespace
 
 std::pair<int32_t, int32_t> Column::get_min_max() const
@@ -201,6 +218,7 @@ void TransientTable::apply_filter(const ExprType &expr, const std::string &paren
         k.a = cols[0]->i32(s);
         if (literal) k.literal = literal_value;
         else k.b = cols[1]->i32(s);
+        k.sel = literal ? estimate_cmp(k.cmp, literal_value, cols[0]->get_min_max()) : 0.5f;
         ops[s].add_kernel(k);
     }
     add_pending_kernels(std::move(ops));
@@ -228,6 +246,13 @@ void TransientTable::apply_search(const ExprType &expr, const std::string &paren
     }
     ensure_secondary_flags();
     const int logic = get_logical_op(parent_op);
+    float sel;
+    {
+        const auto mm = c->get_min_max();
+        const double width = (double)mm.second - mm.first + 1;
+        sel = range ? clamp01(((double)std::min(v1, mm.second) - std::max(v0, mm.first) + 1) / width)
+                    : clamp01(2.0 / width);
+    }
     std::vector<KernelBundle> ops(segment_num());
     for (uint64_t s = 0; s < segment_num(); s++) {
         const uint64_t off = s * eng_.segment_size;
@@ -248,6 +273,7 @@ void TransientTable::apply_search(const ExprType &expr, const std::string &paren
         }
         k3.type = KernelType::LogicalKernel;
         k3.logic = logic;
+        k3.sel = sel;
         k3.in_flags = flags_ + off;
         k3.local_flags = local;
         k3.out_flags = flags_secondary_ + off;
@@ -483,6 +509,53 @@ std::tuple<int32_t *, int32_t, int32_t> TransientTable::build_key_vals_hash_tabl
     return {ht, mm.first, mm.second};
 }
 
+// Compact probe tables for fused bundles, derived from the reference-layout
+// table once the build side ran: a presence bitmap (and 16-bit value offsets
+// for a full join when the values fit), plus the fraction of set slots, read
+// back to order predicates.  Results never depend on them: a bit is set
+// exactly when the reference table reports a match.
+struct CompactTable {
+    const uint32_t *bits = nullptr;
+    const void *vals = nullptr;
+    int vkind = FV_NONE;
+    float sel = 1.0f;
+};
+
+static CompactTable build_compact(Engine &eng, const void *ht, bool pairs, int32_t ht_min, int32_t ht_max,
+                                  int32_t vmin, int32_t vmax, uint64_t build_rows)
+{
+    CompactTable c;
+    const int64_t len = (int64_t)ht_max - ht_min + 1;
+    const uint64_t words = (uint64_t)(len + 31) / 32;
+    uint32_t *bits = eng.mm->alloc<uint32_t>((words + 3) / 4 * 4);
+    uint32_t *stats = eng.mm->alloc_zero<uint32_t>(2); // set bits, value overflow
+    const bool narrow = pairs && (int64_t)vmax - vmin < 0xFFFF;
+    uint16_t *vals = narrow ? eng.mm->alloc<uint16_t>((uint64_t)len) : nullptr;
+    if (pairs)
+        launch_bits_from_pairs(static_cast<const int32_t *>(ht), (int32_t)len, bits, vals, vmin, stats, stats + 1,
+                               eng.stream);
+    else
+        launch_bits_from_keys(static_cast<const uint8_t *>(ht), (int32_t)len, bits, stats, eng.stream);
+    eng.launches++;
+    uint32_t host[2] = {0, 0};
+    SDB_HIP(hipMemcpyAsync(host, stats, sizeof(host), hipMemcpyDeviceToHost, eng.stream));
+    SDB_HIP(hipStreamSynchronize(eng.stream));
+    c.bits = bits;
+    // fraction of the build rows selected: the share of uniformly drawn
+    // foreign keys that find a match
+    c.sel = std::min(1.0f, (float)host[0] / (float)std::max<uint64_t>(build_rows, 1));
+    if (pairs) {
+        if (narrow && host[1] == 0) {
+            c.vals = vals;
+            c.vkind = FV_U16;
+        } else {
+            c.vals = ht;
+            c.vkind = FV_PAIR;
+        }
+    }
+    return c;
+}
+
 void TransientTable::apply_join(TransientTable &right, const RelNode &rel)
 {
     if (rel.condition.operands.size() < 2) throw std::invalid_argument("join condition needs two operands");
@@ -497,6 +570,9 @@ void TransientTable::apply_join(TransientTable &right, const RelNode &rel)
         auto ht = right.build_keys_hash_table((int)right_column);
         ensure_secondary_flags();
         right.execute_pending_kernels(false);
+        CompactTable ct;
+        if (eng_.fusion)
+            ct = build_compact(eng_, std::get<0>(ht), false, std::get<1>(ht), std::get<2>(ht), 0, 0, right.nrows_);
         std::vector<KernelBundle> ops(segment_num());
         for (uint64_t s = 0; s < segment_num(); s++) {
             const uint64_t off = s * eng_.segment_size;
@@ -509,6 +585,8 @@ void TransientTable::apply_join(TransientTable &right, const RelNode &rel)
             k.ht = std::get<0>(ht);
             k.ht_min = std::get<1>(ht);
             k.ht_max = std::get<2>(ht);
+            k.bits = ct.bits;
+            k.sel = ct.sel;
             ops[s].add_kernel(k);
         }
         add_pending_kernels(std::move(ops));
@@ -528,6 +606,10 @@ void TransientTable::apply_join(TransientTable &right, const RelNode &rel)
     auto ht = right.build_key_vals_hash_table((int)right_column);
     ensure_secondary_flags();
     right.execute_pending_kernels(false);
+    CompactTable ct;
+    if (eng_.fusion)
+        ct = build_compact(eng_, std::get<0>(ht), true, std::get<1>(ht), std::get<2>(ht), gb_mm.first, gb_mm.second,
+                           right.nrows_);
     std::vector<KernelBundle> ops(segment_num());
     for (uint64_t s = 0; s < segment_num(); s++) {
         const uint64_t off = s * eng_.segment_size;
@@ -541,6 +623,11 @@ void TransientTable::apply_join(TransientTable &right, const RelNode &rel)
         k.ht = std::get<0>(ht);
         k.ht_min = std::get<1>(ht);
         k.ht_max = std::get<2>(ht);
+        k.bits = ct.bits;
+        k.vals = ct.vals;
+        k.vkind = ct.vkind;
+        k.vmin = gb_mm.first;
+        k.sel = ct.sel;
         nc.seg_min[s] = gb_mm.first; // models/models.hpp:1609-1610
         nc.seg_max[s] = gb_mm.second;
         ops[s].add_kernel(k);

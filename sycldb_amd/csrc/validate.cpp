@@ -120,9 +120,20 @@ void validate_program(const FProgram &p, int64_t n_rows, const AllocRegistry &re
 {
     if (p.magic != F_MAGIC) fail(SDB_ERR_RUNTIME, "launch check: program magic");
     if (p.ninstr < 1 || p.ninstr > F_MAX_INSTR) fail(SDB_ERR_RUNTIME, "launch check: instruction count");
+    if (p.lds_bytes > F_LDS_BUDGET) fail(SDB_ERR_RUNTIME, "launch check: LDS budget");
+    if (p.nvals < 0 || p.nvals > F_MAX_VALS) fail(SDB_ERR_RUNTIME, "launch check: register count");
+    auto lds_area = [&](uint64_t off, uint64_t bytes, const char *what) {
+        if (off + bytes > p.lds_bytes) fail(SDB_ERR_RUNTIME, std::string("launch check: LDS area of ") + what);
+    };
+    lds_area(p.vreg_off, (uint64_t)p.nvals * 256 * 16, "registers");
+    if (p.nblobs < 0 || p.nblobs > F_MAX_BLOBS) fail(SDB_ERR_RUNTIME, "launch check: blob count");
+    for (int b = 0; b < p.nblobs; b++) {
+        if (p.blob_bytes[b] % 16) fail(SDB_ERR_RUNTIME, "launch check: blob size");
+        reg.check(p.blob_src[b], p.blob_bytes[b], "staged table");
+        lds_area(p.blob_off[b], p.blob_bytes[b], "a staged table");
+    }
     if (n_rows <= 0) return;
     const uint64_t n = (uint64_t)n_rows, col = 4 * n;
-    bool flag_used[F_MAX_FLAGS] = {};
     for (int j = 0; j < F_MAX_FLAGS; j++) {
         if (p.flag_init[j] > FF_TRUE) fail(SDB_ERR_RUNTIME, "launch check: flag init");
         if (p.flag_init[j] == FF_LOAD) reg.check(p.flag_load[j], n, "loaded flags");
@@ -130,11 +141,13 @@ void validate_program(const FProgram &p, int64_t n_rows, const AllocRegistry &re
     }
     auto fl = [&](uint8_t r) {
         if (r >= F_MAX_FLAGS) fail(SDB_ERR_RUNTIME, "launch check: flag register");
-        flag_used[r] = true;
+    };
+    auto vreg = [&](int8_t r) {
+        if (r < 0 || r >= p.nvals) fail(SDB_ERR_RUNTIME, "launch check: value register");
     };
     auto operand = [&](int8_t r, const int32_t *mem, const char *what) {
-        if (r >= F_MAX_VALS) fail(SDB_ERR_RUNTIME, "launch check: value register");
-        if (r < 0) reg.check(mem, col, what);
+        if (r >= 0) vreg(r);
+        else reg.check(mem, col, what);
     };
     bool has_sum = false, has_gb = false;
     for (int i = 0; i < p.ninstr; i++) {
@@ -143,6 +156,8 @@ void validate_program(const FProgram &p, int64_t n_rows, const AllocRegistry &re
         switch (I.kind) {
         case FK_SEL_LIT:
         case FK_SEL_COL:
+        case FK_SEL_RANGE:
+        case FK_SEL_IN2:
             if (I.cmp > SDB_GE || I.logic > SDB_OR) fail(SDB_ERR_RUNTIME, "launch check: selection op");
             operand(I.va, I.a, "selection column");
             if (I.kind == FK_SEL_COL) operand(I.vb, I.b, "selection column");
@@ -154,7 +169,7 @@ void validate_program(const FProgram &p, int64_t n_rows, const AllocRegistry &re
             fl(I.fout);
             break;
         case FK_FILL:
-            if (I.vout < 0 || I.vout >= F_MAX_VALS) fail(SDB_ERR_RUNTIME, "launch check: fill register");
+            vreg(I.vout);
             if (I.store) reg.check(I.store, col, "fill store");
             break;
         case FK_BINOP_CC:
@@ -163,7 +178,7 @@ void validate_program(const FProgram &p, int64_t n_rows, const AllocRegistry &re
             if (I.op > SDB_SUB) fail(SDB_ERR_RUNTIME, "launch check: arithmetic op");
             if (I.kind != FK_BINOP_LC) operand(I.va, I.a, "arithmetic operand");
             if (I.kind != FK_BINOP_CL) operand(I.vb, I.b, "arithmetic operand");
-            if (I.vout < 0 || I.vout >= F_MAX_VALS) fail(SDB_ERR_RUNTIME, "launch check: result register");
+            vreg(I.vout);
             if (I.store) reg.check(I.store, col, "arithmetic store");
             break;
         case FK_PROBE_SEMI:
@@ -174,10 +189,24 @@ void validate_program(const FProgram &p, int64_t n_rows, const AllocRegistry &re
         case FK_PROBE_FULL:
             operand(I.va, I.a, "probe keys");
             reg.check(I.ht, 8 * range_len(I.ht_min, I.ht_max), "join table");
-            if (I.vout < 0 || I.vout >= F_MAX_VALS) fail(SDB_ERR_RUNTIME, "launch check: join register");
+            vreg(I.vout);
             if (I.store) reg.check(I.store, col, "join value store");
             fl(I.fout);
             break;
+        case FK_PROBE_BITS: {
+            operand(I.va, I.a, "probe keys");
+            const uint64_t len = range_len(I.ht_min, I.ht_max);
+            const uint64_t bytes = (len + 31) / 32 * 4;
+            reg.check(I.ht, bytes, "probe bitmap");
+            if (I.in_lds) lds_area(I.lds_off, bytes, "a probe bitmap");
+            if (I.vkind == FV_U16) reg.check(I.vals, 2 * len, "probe value table");
+            else if (I.vkind == FV_PAIR) reg.check(I.vals, 8 * len, "probe value pairs");
+            else if (I.vkind != FV_NONE) fail(SDB_ERR_RUNTIME, "launch check: value table kind");
+            if (I.vkind != FV_NONE) vreg(I.vout);
+            if (I.store) reg.check(I.store, col, "join value store");
+            fl(I.fout);
+            break;
+        }
         case FK_BUILD_KEYS:
             operand(I.va, I.a, "build keys");
             if (I.ht_len <= 0) fail(SDB_ERR_RUNTIME, "launch check: table length");
@@ -197,7 +226,6 @@ void validate_program(const FProgram &p, int64_t n_rows, const AllocRegistry &re
         case FK_GROUPBY: {
             const FGroup &g = p.gb;
             if (g.k < 1 || g.k > F_MAX_KEYS || g.prod == 0) fail(SDB_ERR_RUNTIME, "launch check: group-by shape");
-            if (g.lds && g.prod > fused_groupby_lds_max_slots()) fail(SDB_ERR_RUNTIME, "launch check: group-by LDS");
             for (int j = 0; j < g.k; j++) {
                 operand(g.key_reg[j], g.key_mem[j], "group key");
                 reg.check(g.out_keys[j], 4 * g.prod, "group key output");
@@ -205,6 +233,16 @@ void validate_program(const FProgram &p, int64_t n_rows, const AllocRegistry &re
             operand(g.agg_reg, g.agg_mem, "aggregated column");
             reg.check(g.sums, 8 * g.prod, "group sums");
             reg.check(g.seen, 4 * g.prod, "group seen flags");
+            if (g.lds == FG_DENSE_LDS) {
+                if (g.prod > F_DENSE_MAX) fail(SDB_ERR_RUNTIME, "launch check: dense LDS slots");
+                lds_area(g.lds_off, g.prod * 8 + (g.prod + 31) / 32 * 4, "the dense aggregation");
+            } else if (g.lds == FG_HASH_LDS) {
+                if (g.hash_cap == 0 || (g.hash_cap & (g.hash_cap - 1)) || g.prod >= 0xFFFFFFFFull)
+                    fail(SDB_ERR_RUNTIME, "launch check: hash aggregation shape");
+                lds_area(g.lds_off, (uint64_t)g.hash_cap * 12, "the hash aggregation");
+            } else if (g.lds != FG_GLOBAL) {
+                fail(SDB_ERR_RUNTIME, "launch check: aggregation mode");
+            }
             has_gb = true;
             break;
         }
@@ -213,7 +251,6 @@ void validate_program(const FProgram &p, int64_t n_rows, const AllocRegistry &re
     }
     if (p.sum_acc && !has_sum) fail(SDB_ERR_RUNTIME, "launch check: accumulator without sum");
     if (p.gb.prod && !has_gb) fail(SDB_ERR_RUNTIME, "launch check: group spec without group-by");
-    (void)flag_used;
 }
 
 } // namespace sdb

@@ -22,6 +22,8 @@ CASES = [
             "group": "city"}),
     ("t4", {}),
     ("t4", {"c_region": 3, "s_region": 0, "mfgr": (2, 4)}),
+    # ~15,000 live groups: overflows the fused kernel's LDS hash into the global path
+    ("t3", {"cust": ("region", 1), "supp": ("region", 1), "date": ("year_range", 1992, 1997), "group": "city"}),
 ]
 
 
