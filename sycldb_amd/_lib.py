@@ -14,6 +14,34 @@ if not os.path.exists(LIB_PATH):
         "libsycldb_hip.so is not built (expected at %s); run __graft_entry__.build() "
         "or `python -m sycldb_amd.build`" % LIB_PATH)
 
+
+
+def _share_hip_runtime():
+    """Keeps one HIP runtime per process.
+
+    PyTorch for ROCm ships its own libamdhip64 (SONAME libamdhip64.so.7) and
+    HSA runtime, which its libraries request as "libamdhip64.so".  If this
+    library were loaded first it would bring in the system ROCm runtime under
+    the .so.7 name, and a later `import torch` would then load a second HIP and
+    HSA runtime whose device initialisation fails.  Loading torch's runtime
+    first, globally, makes this library bind to it (same SONAME) and torch
+    reuse it (same file), whichever of the two is imported first.  Without
+    torch installed the system runtime is used.
+    """
+    import importlib.util
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        spec = None
+    if spec is None or not spec.origin:
+        return None
+    path = os.path.join(os.path.dirname(spec.origin), "lib", "libamdhip64.so")
+    if not os.path.exists(path):
+        return None
+    return ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+
+
+_hip_runtime = _share_hip_runtime()
 lib = ctypes.CDLL(LIB_PATH)
 
 P = ctypes.c_void_p

@@ -6,6 +6,7 @@
 #include <array>
 #include <cstdio>
 #include <cstdlib>
+#include <cmath>
 #include <map>
 
 #include "engine.hpp"
@@ -60,12 +61,12 @@ static void dump_program(const FProgram &p, int64_t n, const std::string &table)
         return e && *e && *e != '0';
     }();
     if (!on) return;
-    fprintf(stderr, "[fused %s] rows=%lld vec=%d instr=%d vals=%d blobs=%d lds=%u group=%d\n", table.c_str(),
-            (long long)n, p.vec, p.ninstr, p.nvals, p.nblobs, p.lds_bytes, p.gb.lds);
+    fprintf(stderr, "[fused %s] rows=%lld vec=%d instr=%d vals=%d prefetch=%d blobs=%d lds=%u group=%d\n",
+            table.c_str(), (long long)n, p.vec, p.ninstr, p.nvals, p.npf, p.nblobs, p.lds_bytes, p.gb.lds);
     for (int i = 0; i < p.ninstr; i++) {
         const FInstr &I = p.ins[i];
-        fprintf(stderr, "  %2d %-12s in=%d out=%d logic=%d va=%d vout=%d lds=%d vkind=%d store=%d\n", i,
-                fkind_name(I.kind), I.fin, I.fout, I.logic, I.va, I.vout, I.in_lds, I.vkind, I.store != nullptr);
+        fprintf(stderr, "  %2d %-12s in=%d out=%d logic=%d va=%d vb=%d vout=%d lds=%d vkind=%d store=%d\n", i,
+                fkind_name(I.kind), I.fin, I.fout, I.logic, I.va, I.vb, I.vout, I.in_lds, I.vkind, I.store != nullptr);
     }
     for (int j = 0; j < F_MAX_FLAGS; j++)
         if (p.flag_init[j] || p.flag_store[j]) fprintf(stderr, "  flag %d init=%d store=%d\n", j, p.flag_init[j], p.flag_store[j] != nullptr);
@@ -309,6 +310,118 @@ void fold_search(std::vector<Draft> &d)
     }
 }
 
+// Bitmaps of FK_PROBE_BITS staged into LDS at kernel start: the smallest
+// first, each at most 32 KiB, together at most 48 KiB.  Decided before the
+// instructions are ordered, since a probe into LDS costs far less than one
+// through the caches.
+std::vector<const void *> choose_staged(const std::vector<Draft> &d)
+{
+    std::vector<std::pair<uint32_t, const void *>> cand; // bytes, bitmap
+    for (const Draft &D : d) {
+        const FInstr &I = D.I;
+        if (I.kind != FK_PROBE_BITS) continue;
+        const uint64_t len = (uint64_t)((int64_t)I.ht_max - I.ht_min + 1);
+        if (len > 32u * 1024u * 8u) continue;
+        const uint32_t bytes = (uint32_t)(((len + 31) / 32 + 3) / 4 * 16);
+        bool dup = false;
+        for (auto &c : cand) dup |= c.second == I.ht;
+        if (!dup) cand.emplace_back(bytes, I.ht);
+    }
+    std::stable_sort(cand.begin(), cand.end(),
+                     [](const std::pair<uint32_t, const void *> &a, const std::pair<uint32_t, const void *> &b) {
+                         return a.first < b.first;
+                     });
+    std::vector<const void *> out;
+    uint32_t staged = 0;
+    for (auto &c : cand) {
+        if ((int)out.size() >= F_MAX_BLOBS || staged + c.first > 48u * 1024u) continue;
+        out.push_back(c.second);
+        staged += c.first;
+    }
+    return out;
+}
+
+// Columns the kernel prefetches.  A lazily loaded column costs the whole
+// wave a dependent memory round trip whenever any of its lanes needs it, which
+// for 64 lanes x 4 rows is nearly always, so every column read is prefetched
+// unless its 128-byte lines (32 rows) would be touched with probability below
+// `min_touch` (estimated from the fraction of rows still needed where it is
+// first read).  Past F_MAX_PF columns the most touched win; later reads of a
+// prefetched column use its register too.
+double prefetch_min_touch()
+{
+    static const double v = [] {
+        const char *e = getenv("SDB_PREFETCH_MIN_TOUCH");
+        return e && *e ? atof(e) : 0.05;
+    }();
+    return v;
+}
+
+void assign_prefetch(std::vector<Draft> &d, FGroup &g, FProgram &p)
+{
+    std::vector<std::pair<const int32_t *, double>> seen; // column, touch probability (first read)
+    auto consider = [&](const int32_t *mem, double frac) {
+        if (!mem) return;
+        for (auto &c : seen)
+            if (c.first == mem) return;
+        seen.emplace_back(mem, 1.0 - std::pow(1.0 - std::min(1.0, std::max(0.0, frac)), 32.0));
+    };
+    double live = 1.0;
+    for (const Draft &D : d) {
+        const FInstr &I = D.I;
+        const bool and_input = I.logic == SDB_AND || I.kind == FK_PROBE_SEMI || I.kind == FK_PROBE_FULL ||
+                               I.kind == FK_PROBE_BITS || I.kind == FK_BINOP_CC || I.kind == FK_BINOP_LC ||
+                               I.kind == FK_BINOP_CL || I.kind == FK_SUM || I.kind == FK_GROUPBY;
+        const double frac = and_input ? live : 1.0;
+        switch (I.kind) {
+        case FK_SEL_COL:
+        case FK_BINOP_CC:
+        case FK_BUILD_KEYVALS:
+            consider(I.a, I.kind == FK_BUILD_KEYVALS ? 1.0 : frac);
+            consider(I.b, frac);
+            break;
+        case FK_BINOP_LC: consider(I.b, frac); break;
+        case FK_BUILD_KEYS: consider(I.a, 1.0); break;
+        case FK_GROUPBY:
+            for (int j = 0; j < g.k; j++) consider(g.key_mem[j], frac);
+            consider(g.agg_mem, frac);
+            break;
+        case FK_LOGICAL:
+        case FK_FILL: break;
+        default: consider(I.a, frac); break;
+        }
+        if (is_conjunctive(I)) live *= std::max(0.0f, std::min(1.0f, D.sel));
+        else if (is_predicate(I.kind)) live = 1.0;
+    }
+    std::stable_sort(seen.begin(), seen.end(),
+                     [](const std::pair<const int32_t *, double> &a, const std::pair<const int32_t *, double> &b) {
+                         return a.second > b.second;
+                     });
+    std::vector<const int32_t *> pf;
+    for (auto &c : seen)
+        if (c.second >= prefetch_min_touch() && (int)pf.size() < F_MAX_PF) pf.push_back(c.first);
+    p.npf = (int)pf.size();
+    for (int j = 0; j < p.npf; j++) p.pf_mem[j] = pf[j];
+    auto rewrite = [&](int8_t &reg, const int32_t *&mem) {
+        if (reg >= 0 || !mem) return;
+        for (int j = 0; j < p.npf; j++)
+            if (pf[j] == mem) {
+                reg = (int8_t)(F_PF_BASE + j);
+                mem = nullptr;
+            }
+    };
+    for (Draft &D : d) {
+        FInstr &I = D.I;
+        if (I.kind != FK_FILL && I.kind != FK_LOGICAL) {
+            if (I.kind != FK_BINOP_LC) rewrite(I.va, I.a);
+            if (I.kind == FK_SEL_COL || I.kind == FK_BINOP_CC || I.kind == FK_BINOP_LC || I.kind == FK_BUILD_KEYVALS)
+                rewrite(I.vb, I.b);
+        }
+    }
+    for (int j = 0; j < g.k; j++) rewrite(g.key_reg[j], g.key_mem[j]);
+    rewrite(g.agg_reg, g.agg_mem);
+}
+
 } // namespace
 
 bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
@@ -519,6 +632,11 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
             if (writes_flag(I.kind)) written[I.fout] = true;
         }
     }
+    // probe costs follow where the bitmap will live
+    const std::vector<const void *> staged = choose_staged(d);
+    for (Draft &D : d)
+        if (D.I.kind == FK_PROBE_BITS)
+            D.cost = std::find(staged.begin(), staged.end(), D.I.ht) != staged.end() ? 0.6f : 2.0f;
     // reorder conjunctive runs
     {
         const size_t m = d.size();
@@ -613,9 +731,12 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
         }
     }
 
+    assign_prefetch(d, p.gb, p);
+
     // --------------------------------------------------------------- layout
     if ((int)d.size() > F_MAX_INSTR) return false;
     bool vec = true;
+    for (int j = 0; j < p.npf; j++) vec &= aligned(p.pf_mem[j], 16);
     for (size_t i = 0; i < d.size(); i++) {
         const FInstr &I = d[i].I;
         for (const int32_t *mem : {I.a, I.b})
@@ -636,34 +757,24 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
     };
     p.nvals = (int)vptr.size();
     p.vreg_off = take((uint32_t)p.nvals * 256u * 16u);
-    // stage the smallest bitmaps in LDS (each <= 32 KiB, together <= 48 KiB)
-    {
-        std::vector<std::pair<uint32_t, int>> cand; // bytes, instruction
-        for (int i = 0; i < p.ninstr; i++) {
-            const FInstr &I = p.ins[i];
-            if (I.kind != FK_PROBE_BITS) continue;
+    // staged bitmaps (chosen above)
+    for (int i = 0; i < p.ninstr; i++) {
+        FInstr &I = p.ins[i];
+        if (I.kind != FK_PROBE_BITS || std::find(staged.begin(), staged.end(), I.ht) == staged.end()) continue;
+        int blob = -1;
+        for (int b = 0; b < p.nblobs; b++)
+            if (p.blob_src[b] == I.ht) blob = b;
+        if (blob < 0) {
+            if (p.nblobs >= F_MAX_BLOBS) return false;
             const uint64_t len = (uint64_t)((int64_t)I.ht_max - I.ht_min + 1);
             const uint32_t bytes = (uint32_t)(((len + 31) / 32 + 3) / 4 * 16);
-            if (len <= 32u * 1024u * 8u) cand.emplace_back(bytes, i);
+            blob = p.nblobs++;
+            p.blob_src[blob] = I.ht;
+            p.blob_bytes[blob] = bytes;
+            p.blob_off[blob] = take(bytes);
         }
-        std::sort(cand.begin(), cand.end());
-        uint32_t staged = 0;
-        for (auto &c : cand) {
-            FInstr &I = p.ins[c.second];
-            int blob = -1;
-            for (int b = 0; b < p.nblobs; b++)
-                if (p.blob_src[b] == I.ht) blob = b;
-            if (blob < 0) {
-                if (p.nblobs >= F_MAX_BLOBS || staged + c.first > 48u * 1024u) continue;
-                blob = p.nblobs++;
-                p.blob_src[blob] = I.ht;
-                p.blob_bytes[blob] = c.first;
-                p.blob_off[blob] = take(c.first);
-                staged += c.first;
-            }
-            I.in_lds = 1;
-            I.lds_off = p.blob_off[blob];
-        }
+        I.in_lds = 1;
+        I.lds_off = p.blob_off[blob];
     }
     if (p.gb.lds == FG_DENSE_LDS) {
         p.gb.lds_off = take((uint32_t)(p.gb.prod * 8 + (p.gb.prod + 31) / 32 * 4));

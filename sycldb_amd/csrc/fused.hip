@@ -16,6 +16,8 @@
 #include "fused.hpp"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 namespace sdb {
 namespace {
@@ -82,12 +84,54 @@ template <int R> __device__ __forceinline__ void vset(const VRegs &V, int r, con
     }
 }
 
-// Column operand: a value register, or R rows of a column.  Column reads are
-// streaming (non-temporal) so they do not push probe tables out of the caches.
-template <int R>
-__device__ __forceinline__ void operand(int reg, const int32_t *mem, int64_t row0, const VRegs &V, int32_t (&x)[R])
+// Prefetched column values of the chunk being run: row k of c<j> is
+// pf_mem[j][row0 + k].  Four named vector values, not an array: a register
+// picked by a uniform number out of an array would become a dynamically
+// indexed scratch access.
+template <int R> struct PfType { typedef v4i type; };
+template <> struct PfType<1> { typedef int32_t type; };
+template <int R> struct Pf {
+    typename PfType<R>::type c0, c1, c2, c3;
+    __device__ __forceinline__ typename PfType<R>::type pick(int j) const
+    {
+        return j == 0 ? c0 : j == 1 ? c1 : j == 2 ? c2 : c3;
+    }
+};
+static_assert(F_MAX_PF == 4, "Pf holds four columns");
+
+template <int R> __device__ __forceinline__ typename PfType<R>::type pf_col(const int32_t *mem, int64_t row0)
 {
-    if (reg >= 0) {
+    if constexpr (R == 4) return __builtin_nontemporal_load(reinterpret_cast<const v4i *>(mem + row0));
+    else return __builtin_nontemporal_load(mem + row0);
+}
+
+template <int R> __device__ __forceinline__ void pf_load(const FProgram &P, int64_t row0, Pf<R> &pf)
+{
+    if (P.npf > 0) pf.c0 = pf_col<R>(P.pf_mem[0], row0);
+    if (P.npf > 1) pf.c1 = pf_col<R>(P.pf_mem[1], row0);
+    if (P.npf > 2) pf.c2 = pf_col<R>(P.pf_mem[2], row0);
+    if (P.npf > 3) pf.c3 = pf_col<R>(P.pf_mem[3], row0);
+}
+
+// Column operand: a prefetched column, a value register, or R rows of a
+// column loaded now.  Column reads are streaming (non-temporal) so they do not
+// push probe tables out of the caches.  Register numbers are uniform, so the
+// prefetch slot is picked by scalar compares.
+template <int R>
+__device__ __forceinline__ void operand(int reg, const int32_t *mem, int64_t row0, const VRegs &V, const Pf<R> &pf,
+                                        int32_t (&x)[R])
+{
+    if (reg >= F_PF_BASE) {
+        const auto t = pf.pick(reg - F_PF_BASE);
+        if constexpr (R == 4) {
+            x[0] = t.x;
+            x[1] = t.y;
+            x[2] = t.z;
+            x[3] = t.w;
+        } else {
+            x[0] = t;
+        }
+    } else if (reg >= 0) {
         vget<R>(V, reg, x);
     } else if constexpr (R == 4) {
         const v4i t = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(mem + row0));
@@ -98,6 +142,20 @@ __device__ __forceinline__ void operand(int reg, const int32_t *mem, int64_t row
     } else {
         x[0] = __builtin_nontemporal_load(mem + row0);
     }
+}
+
+// Row k of an operand (see operand()).
+template <int R>
+__device__ __forceinline__ int32_t operand_row(int reg, const int32_t *mem, int64_t row0, int k, const VRegs &V,
+                                               const Pf<R> &pf)
+{
+    if (reg >= F_PF_BASE) {
+        const auto t = pf.pick(reg - F_PF_BASE);
+        if constexpr (R == 4) return t[k];
+        else return t;
+    }
+    if (reg >= 0) return reinterpret_cast<const int32_t *>(&V.base[reg * BLOCK])[k];
+    return __builtin_nontemporal_load(mem + row0 + k);
 }
 
 template <int R> __device__ __forceinline__ void store_masked(int32_t *dst, int64_t row0, const int32_t (&x)[R], uint32_t m)
@@ -189,7 +247,7 @@ __device__ __forceinline__ void gb_row(const FGroup &g, const Lds &L, const int3
 
 template <int R>
 __device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64_t &acc, const Lds &L,
-                                          const VRegs &V)
+                                          const VRegs &V, const Pf<R> &pf)
 {
     constexpr uint32_t M = (1u << R) - 1;
     uint32_t F = 0;
@@ -212,8 +270,8 @@ __device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64
             uint32_t cm = 0;
             if (need) {
                 int32_t x[R], y[R];
-                operand<R>(I.va, I.a, row0, V, x);
-                if (I.kind == FK_SEL_COL) operand<R>(I.vb, I.b, row0, V, y);
+                operand<R>(I.va, I.a, row0, V, pf, x);
+                if (I.kind == FK_SEL_COL) operand<R>(I.vb, I.b, row0, V, pf, y);
 #pragma unroll
                 for (int k = 0; k < R; k++) {
                     bool c;
@@ -244,8 +302,8 @@ __device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64
         case FK_BINOP_CL: {
             if (!in) break;
             int32_t x[R], y[R], r[R];
-            if (I.kind != FK_BINOP_LC) operand<R>(I.va, I.a, row0, V, x);
-            if (I.kind != FK_BINOP_CL) operand<R>(I.vb, I.b, row0, V, y);
+            if (I.kind != FK_BINOP_LC) operand<R>(I.va, I.a, row0, V, pf, x);
+            if (I.kind != FK_BINOP_CL) operand<R>(I.vb, I.b, row0, V, pf, y);
 #pragma unroll
             for (int k = 0; k < R; k++)
                 r[k] = element_op(I.op, I.kind == FK_BINOP_LC ? I.lit : x[k], I.kind == FK_BINOP_CL ? I.lit : y[k]);
@@ -257,7 +315,7 @@ __device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64
             uint32_t out = 0;
             if (in) {
                 int32_t x[R];
-                operand<R>(I.va, I.a, row0, V, x);
+                operand<R>(I.va, I.a, row0, V, pf, x);
                 const uint8_t *ht = reinterpret_cast<const uint8_t *>(I.ht);
 #pragma unroll
                 for (int k = 0; k < R; k++)
@@ -271,7 +329,7 @@ __device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64
             uint32_t out = 0;
             if (in) {
                 int32_t x[R], r[R];
-                operand<R>(I.va, I.a, row0, V, x);
+                operand<R>(I.va, I.a, row0, V, pf, x);
                 const int2 *ht = reinterpret_cast<const int2 *>(I.ht);
 #pragma unroll
                 for (int k = 0; k < R; k++) {
@@ -297,7 +355,7 @@ __device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64
             if (in) {
                 int32_t x[R];
                 uint32_t slot[R];
-                operand<R>(I.va, I.a, row0, V, x);
+                operand<R>(I.va, I.a, row0, V, pf, x);
                 const uint32_t *bits = I.in_lds ? reinterpret_cast<const uint32_t *>(L.base + I.lds_off)
                                                 : reinterpret_cast<const uint32_t *>(I.ht);
 #pragma unroll
@@ -328,7 +386,7 @@ __device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64
         }
         case FK_BUILD_KEYS: {
             int32_t x[R];
-            operand<R>(I.va, I.a, row0, V, x);
+            operand<R>(I.va, I.a, row0, V, pf, x);
             uint8_t *ht = reinterpret_cast<uint8_t *>(const_cast<void *>(I.ht));
 #pragma unroll
             for (int k = 0; k < R; k++) {
@@ -339,8 +397,8 @@ __device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64
         }
         case FK_BUILD_KEYVALS: {
             int32_t x[R], y[R];
-            operand<R>(I.va, I.a, row0, V, x);
-            if (in) operand<R>(I.vb, I.b, row0, V, y);
+            operand<R>(I.va, I.a, row0, V, pf, x);
+            if (in) operand<R>(I.vb, I.b, row0, V, pf, y);
             int32_t *ht = reinterpret_cast<int32_t *>(const_cast<void *>(I.ht));
 #pragma unroll
             for (int k = 0; k < R; k++) {
@@ -356,7 +414,7 @@ __device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64
         case FK_SUM: {
             if (!in) break;
             int32_t x[R];
-            operand<R>(I.va, I.a, row0, V, x);
+            operand<R>(I.va, I.a, row0, V, pf, x);
 #pragma unroll
             for (int k = 0; k < R; k++)
                 if ((in >> k) & 1u) acc += x[k];
@@ -365,26 +423,15 @@ __device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64
         case FK_GROUPBY: {
             if (!in) break;
             const FGroup &g = P.gb;
-            int32_t keys[F_MAX_KEYS][R];
-            int32_t agg[R];
-#pragma unroll
-            for (int j = 0; j < F_MAX_KEYS; j++) {
-                int32_t t[R];
-                if (j < g.k) operand<R>(g.key_reg[j], g.key_mem[j], row0, V, t);
-                else
-#pragma unroll
-                    for (int k = 0; k < R; k++) t[k] = 0;
-#pragma unroll
-                for (int k = 0; k < R; k++) keys[j][k] = t[k];
-            }
-            operand<R>(g.agg_reg, g.agg_mem, row0, V, agg);
+            // row by row: only one row's keys are live at a time
 #pragma unroll
             for (int k = 0; k < R; k++) {
                 if (!((in >> k) & 1u)) continue;
                 int32_t kv[F_MAX_KEYS];
 #pragma unroll
-                for (int j = 0; j < F_MAX_KEYS; j++) kv[j] = keys[j][k];
-                gb_row(g, L, kv, agg[k]);
+                for (int j = 0; j < F_MAX_KEYS; j++)
+                    kv[j] = j < g.k ? operand_row<R>(g.key_reg[j], g.key_mem[j], row0, k, V, pf) : 0;
+                gb_row(g, L, kv, operand_row<R>(g.agg_reg, g.agg_mem, row0, k, V, pf));
             }
             break;
         }
@@ -394,6 +441,24 @@ __device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64
 #pragma unroll
     for (int j = 0; j < F_MAX_FLAGS; j++)
         if (P.flag_store[j]) flags_out<R>(P.flag_store[j] + row0, (F >> (4 * j)) & M);
+}
+
+// Runs chunks c, c + stride, ... below nchunks (chunk c = rows [c*R, c*R+R)).
+// The prefetched columns of the next chunk are loaded before the current one
+// runs, so their latency overlaps the current chunk's work.
+template <int R>
+__device__ __forceinline__ void stream_chunks(const FProgram &P, int64_t c, int64_t nchunks, int64_t stride,
+                                              int64_t &acc, const Lds &L, const VRegs &V)
+{
+    Pf<R> next;
+    if (c < nchunks) pf_load<R>(P, c * R, next);
+    while (c < nchunks) {
+        const Pf<R> cur = next;
+        const int64_t cn = c + stride;
+        if (cn < nchunks) pf_load<R>(P, cn * R, next);
+        run_chunk<R>(P, c * R, acc, L, V, cur);
+        c = cn;
+    }
 }
 
 // The program is the kernel's first argument; it is read in place from the
@@ -437,13 +502,15 @@ __global__ void __launch_bounds__(BLOCK) k_fused(const FProgram P_unused, int64_
     int64_t acc = 0;
     if (P.vec) {
         const int64_t nchunks = n >> 2;
-        for (int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x; c < nchunks; c += (int64_t)gridDim.x * BLOCK)
-            run_chunk<4>(P, c << 2, acc, L, V);
-        if (blockIdx.x == gridDim.x - 1 && threadIdx.x < (n & 3))
-            run_chunk<1>(P, (nchunks << 2) + threadIdx.x, acc, L, V);
+        stream_chunks<4>(P, (int64_t)blockIdx.x * BLOCK + threadIdx.x, nchunks, (int64_t)gridDim.x * BLOCK, acc, L, V);
+        if (blockIdx.x == gridDim.x - 1 && threadIdx.x < (n & 3)) {
+            Pf<1> pf;
+            const int64_t row0 = (nchunks << 2) + threadIdx.x;
+            pf_load<1>(P, row0, pf);
+            run_chunk<1>(P, row0, acc, L, V, pf);
+        }
     } else {
-        for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
-            run_chunk<1>(P, i, acc, L, V);
+        stream_chunks<1>(P, (int64_t)blockIdx.x * BLOCK + threadIdx.x, n, (int64_t)gridDim.x * BLOCK, acc, L, V);
     }
     if (P.sum_acc) {
         __shared__ int64_t part[BLOCK / 64];
@@ -475,15 +542,35 @@ __global__ void __launch_bounds__(BLOCK) k_fused(const FProgram P_unused, int64_
 
 } // namespace
 
+// Resident blocks of k_fused per CU with `lds` bytes of dynamic LDS, from the
+// runtime's occupancy calculator (per device, cached per LDS size).
+static int fused_blocks_per_cu(uint32_t lds)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, uint32_t>, int> cache;
+    int dev = 0;
+    SDB_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find({dev, lds});
+    if (it != cache.end()) return it->second;
+    int blocks = 0;
+    SDB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_fused, BLOCK, lds));
+    blocks = std::max(1, blocks);
+    cache[{dev, lds}] = blocks;
+    return blocks;
+}
+
 void launch_fused(const FProgram &p, int64_t n, hipStream_t s)
 {
     if (n <= 0) return;
     if (p.magic != F_MAGIC) fail(SDB_ERR_RUNTIME, "fused program without magic");
     if (p.lds_bytes > F_LDS_BUDGET) fail(SDB_ERR_RUNTIME, "fused program exceeds its LDS budget");
     const int64_t units = p.vec ? (n + 3) / 4 : n;
-    // blocks per CU from the LDS footprint (160 KiB per CU on gfx950)
-    const int per_cu = std::max(1, std::min(8, (int)((160u * 1024u) / std::max<uint32_t>(p.lds_bytes + 64, 1))));
-    hipLaunchKernelGGL(k_fused, dim3(stream_grid(units, BLOCK, per_cu)), dim3(BLOCK), p.lds_bytes, s, p, n);
+    // exactly the blocks that are resident at once (registers and LDS): the
+    // grid-stride loop gives every block the same share, so a grid larger
+    // than one resident wave of blocks would run its tail as a second wave
+    hipLaunchKernelGGL(k_fused, dim3(stream_grid(units, BLOCK, fused_blocks_per_cu(p.lds_bytes))), dim3(BLOCK),
+                       p.lds_bytes, s, p, n);
     hip_check(hipGetLastError(), "fused bundle");
 }
 

@@ -22,6 +22,10 @@ constexpr int F_MAX_INSTR = 24;
 constexpr int F_MAX_FLAGS = 8;
 constexpr int F_MAX_VALS = 4;
 constexpr int F_MAX_KEYS = 4;
+// Prefetched columns: operand register F_PF_BASE + j names column pf_mem[j],
+// which the kernel loads for the next chunk while it runs the current one.
+constexpr int F_MAX_PF = 4;
+constexpr int F_PF_BASE = 16;
 
 enum FKind : uint8_t {
     FK_SEL_LIT,      // SelectionKernelLiteral
@@ -49,7 +53,8 @@ enum FValKind : uint8_t { FV_NONE = 0, FV_U16 = 1, FV_PAIR = 2 };
 struct FInstr {
     uint8_t kind, cmp, logic, op;
     uint8_t fin, fout, floc, in_lds; // in_lds: FK_PROBE_BITS bitmap staged in LDS
-    int8_t va, vb, vout, vkind;      // value registers; -1 = memory operand a / b
+    int8_t va, vb, vout, vkind;      // value registers (F_PF_BASE + j: prefetched
+                                     // column j); -1 = memory operand a / b
     int32_t lit, lit2;
     const int32_t *a;          // memory operand (row-indexed) when va < 0
     const int32_t *b;          // memory operand when vb < 0
@@ -98,6 +103,8 @@ struct FProgram {
     int nblobs;                // tables staged into LDS at kernel start
     uint32_t lds_bytes;        // dynamic LDS: registers, blobs, aggregation
     uint32_t vreg_off;         // byte offset of the register file
+    int npf;                   // prefetched columns
+    const int32_t *pf_mem[F_MAX_PF];
     const void *blob_src[F_MAX_BLOBS];
     uint32_t blob_bytes[F_MAX_BLOBS]; // multiples of 16
     uint32_t blob_off[F_MAX_BLOBS];

@@ -145,9 +145,19 @@ void validate_program(const FProgram &p, int64_t n_rows, const AllocRegistry &re
     auto vreg = [&](int8_t r) {
         if (r < 0 || r >= p.nvals) fail(SDB_ERR_RUNTIME, "launch check: value register");
     };
+    if (p.npf < 0 || p.npf > F_MAX_PF) fail(SDB_ERR_RUNTIME, "launch check: prefetch count");
+    for (int j = 0; j < p.npf; j++) {
+        reg.check(p.pf_mem[j], col, "prefetched column");
+        if (p.vec && (uintptr_t)p.pf_mem[j] % 16) fail(SDB_ERR_RUNTIME, "launch check: prefetched column alignment");
+    }
     auto operand = [&](int8_t r, const int32_t *mem, const char *what) {
-        if (r >= 0) vreg(r);
-        else reg.check(mem, col, what);
+        if (r >= F_PF_BASE) {
+            if (r - F_PF_BASE >= p.npf) fail(SDB_ERR_RUNTIME, "launch check: prefetch register");
+        } else if (r >= 0) {
+            vreg(r);
+        } else {
+            reg.check(mem, col, what);
+        }
     };
     bool has_sum = false, has_gb = false;
     for (int i = 0; i < p.ninstr; i++) {

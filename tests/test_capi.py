@@ -67,3 +67,23 @@ def test_date_dimension_matches_oracle_calendar():
     dk, yr, ym = E.date_columns(2556)
     odk, oyr, oym = O.gen_dates(2556)
     assert (dk == odk).all() and (yr == oyr).all() and (ym == oym).all()
+
+
+@pytest.mark.parametrize("order", ["library_first", "torch_first"])
+def test_one_hip_runtime_per_process(order):
+    """Loading the library and torch in either order maps a single HIP and a
+    single HSA runtime (two runtimes in one process cannot both open the GPU)."""
+    import subprocess
+    import sys
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "%s\n"
+        "maps = open('/proc/self/maps').read().splitlines()\n"
+        "hip = {l.split()[-1] for l in maps if 'libamdhip64' in l}\n"
+        "hsa = {l.split()[-1] for l in maps if 'libhsa-runtime64' in l}\n"
+        "print(len(hip), len(hsa))\n"
+    ) % (ROOT, "from sycldb_amd import _lib\nimport torch" if order == "library_first"
+         else "import torch\nfrom sycldb_amd import _lib")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr[-2000:]
+    assert out.stdout.split() == ["1", "1"], out.stdout
