@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--cpu-sample-rows", type=int, default=60_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-unfused", action="store_true")
+    ap.add_argument("--codegen", default="auto", choices=["auto", "interpret", "compile"])
+    ap.add_argument("--unroll", type=int, default=0, choices=[0, 1, 2, 4])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the fact table would be "
@@ -85,12 +87,12 @@ def main():
 
     import torch
     from sycldb_amd import plans as PL
-    from sycldb_amd.engine import Engine
+    from sycldb_amd.engine import Engine, codegen_kernels_compiled
 
     rows, builder, fact_cols = WORKLOADS[args.workload]
     rows = args.fact_rows or rows
     plan = getattr(PL, builder)()
-    eng = Engine()
+    eng = Engine(codegen=args.codegen, unroll=args.unroll)
     t0 = time.time()
     eng.generate(rows, args.seed, fact_columns=fact_cols)
     torch.cuda.synchronize()
@@ -159,7 +161,8 @@ def main():
         "data": "synthetic: seeded star schema generated on the device (SplitMix64 counter hash)",
         "config": {"workload": DESCRIPTIONS[args.workload], "fact_rows": rows, "plan": args.workload,
                    "device_ms_per_step_median": sorted(dev_ms)[len(dev_ms) // 2], "groups": groups,
-                   "fusion": True, "unfused": unfused},
+                   "fusion": True, "codegen": args.codegen, "unroll": args.unroll,
+                   "kernels_compiled": codegen_kernels_compiled(), "unfused": unfused},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

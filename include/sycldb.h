@@ -253,6 +253,26 @@ int sdb_engine_last_launches(sdb_engine *eng, int64_t *launches);
 /* Switches bundle fusion on (1) or off (0) for later executions. */
 int sdb_engine_set_fusion(sdb_engine *eng, int on);
 
+/* How a fused bundle's kernel is produced (no counterpart in the reference,
+ * whose JIT fusion is switched off: models/execution.hpp:546-615).
+ *   SDB_CODEGEN_AUTO       compiled at run time for the program's shape when
+ *                          the hipRTC library loads, else interpreted
+ *   SDB_CODEGEN_INTERPRET  the compiled-in program interpreter
+ *   SDB_CODEGEN_COMPILE    compiled at run time; an error when unavailable
+ * `unroll` = 4-row chunks per thread per loop iteration of a compiled kernel
+ * (1, 2 or 4; 0 = the tuned default). */
+enum sdb_codegen { SDB_CODEGEN_AUTO = 0, SDB_CODEGEN_INTERPRET = 1, SDB_CODEGEN_COMPILE = 2 };
+int sdb_engine_set_codegen(sdb_engine *eng, int mode, int unroll);
+/* 1 when runtime compilation is available, else 0 with the reason in `why`
+ * (may be NULL). */
+int sdb_codegen_available(char *why, int why_cap);
+/* Distinct fused kernels compiled at run time in this process. */
+int sdb_codegen_kernels_compiled(void);
+/* Compiles, without launching (no device needed), the runtime-generated
+ * kernels of programs covering every instruction form for GPU architecture
+ * `arch` (e.g. "gfx950"); *count = kernels compiled. */
+int sdb_codegen_check(const char *arch, int *count);
+
 /* Per-launch timing: with profiling on, every kernel launch of a bundle is
  * bracketed by HIP events on the engine stream and accumulated per name
  * ("fused:<table>" for a fused bundle, "op:<KernelType>:<table>" for an

@@ -56,6 +56,7 @@ SDB_OK, SDB_ERR_INVALID_ARG, SDB_ERR_HIP, SDB_ERR_OOM, SDB_ERR_UNSUPPORTED, SDB_
 EQ, NE, LT, LE, GT, GE = range(6)
 NONE, AND, OR = range(3)
 MUL, DIV, ADD, SUB = range(4)
+CODEGEN_AUTO, CODEGEN_INTERPRET, CODEGEN_COMPILE = range(3)
 
 
 class EngineOptions(ctypes.Structure):
@@ -108,6 +109,10 @@ SIGNATURES = {
     "sdb_engine_reset": (INT, [P]),
     "sdb_engine_last_launches": (INT, [P, ctypes.POINTER(I64)]),
     "sdb_engine_set_fusion": (INT, [P, INT]),
+    "sdb_engine_set_codegen": (INT, [P, INT, INT]),
+    "sdb_codegen_available": (INT, [CSTR, INT]),
+    "sdb_codegen_kernels_compiled": (INT, []),
+    "sdb_codegen_check": (INT, [CSTR, ctypes.POINTER(INT)]),
     "sdb_engine_set_profiling": (INT, [P, INT]),
     "sdb_engine_profile_reset": (INT, [P]),
     "sdb_engine_profile_count": (INT, [P]),

@@ -83,7 +83,9 @@ void KernelBundle::execute(const ExecContext &ctx) const
             dump_program(p, kernels_.front().n, ctx.table ? *ctx.table : std::string("?"));
             Engine::ProfRec rec;
             if (eng.profile) eng.prof_begin("fused:" + (ctx.table ? *ctx.table : std::string("?")), kernels_.front().n, rec);
-            launch_fused(p, kernels_.front().n, ctx.stream);
+            const bool jit = eng.codegen == SDB_CODEGEN_COMPILE || (eng.codegen == SDB_CODEGEN_AUTO && jit_available());
+            if (jit) launch_fused_jit(p, kernels_.front().n, ctx.stream, eng.unroll ? eng.unroll : F_DEFAULT_UNROLL);
+            else launch_fused(p, kernels_.front().n, ctx.stream);
             if (eng.profile) eng.prof_end(rec);
             eng.launches++;
             return;
@@ -756,7 +758,6 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
         return at;
     };
     p.nvals = (int)vptr.size();
-    p.vreg_off = take((uint32_t)p.nvals * 256u * 16u);
     // staged bitmaps (chosen above)
     for (int i = 0; i < p.ninstr; i++) {
         FInstr &I = p.ins[i];
@@ -782,6 +783,10 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
         p.gb.hash_cap = F_HASH_CAP;
         p.gb.lds_off = take(F_HASH_CAP * 12u);
     }
+    // the interpreter's value-register file comes last: a kernel compiled
+    // for the program keeps its registers in VGPRs and needs only the LDS
+    // below vreg_off
+    p.vreg_off = take((uint32_t)p.nvals * 256u * 16u);
     p.lds_bytes = off;
     if (p.lds_bytes > F_LDS_BUDGET) return false;
 

@@ -235,6 +235,8 @@ struct Engine {
     hipStream_t stream = nullptr;
     uint64_t segment_size = 1ull << 28; // common.hpp:12
     bool fusion = true;
+    int codegen = SDB_CODEGEN_AUTO; // how fused kernels are produced
+    int unroll = 0;                 // chunks per thread per iteration (0: default)
     AllocRegistry registry;
     std::unique_ptr<MemoryManager> mm;
     std::map<std::string, std::unique_ptr<Table>> tables;

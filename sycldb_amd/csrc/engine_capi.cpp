@@ -6,6 +6,7 @@
 
 #include "capi_util.hpp"
 #include "engine.hpp"
+#include "fused.hpp"
 
 using namespace sdb;
 
@@ -334,6 +335,40 @@ int sdb_engine_set_fusion(sdb_engine *e, int on)
     return guarded([&] {
         require(e, "null engine");
         e->eng.fusion = on != 0;
+    });
+}
+
+int sdb_engine_set_codegen(sdb_engine *e, int mode, int unroll)
+{
+    return guarded([&] {
+        require(e, "null engine");
+        require(mode == SDB_CODEGEN_AUTO || mode == SDB_CODEGEN_INTERPRET || mode == SDB_CODEGEN_COMPILE,
+                "unknown codegen mode");
+        require(unroll == 0 || unroll == 1 || unroll == 2 || unroll == 4, "unroll must be 0, 1, 2 or 4");
+        if (mode == SDB_CODEGEN_COMPILE && !jit_available())
+            fail(SDB_ERR_UNSUPPORTED, "runtime compilation unavailable: " + jit_unavailable_reason());
+        e->eng.codegen = mode;
+        e->eng.unroll = unroll;
+    });
+}
+
+int sdb_codegen_available(char *why, int why_cap)
+{
+    const bool ok = jit_available();
+    if (why && why_cap > 0) {
+        const std::string r = ok ? std::string() : jit_unavailable_reason();
+        snprintf(why, (size_t)why_cap, "%s", r.c_str());
+    }
+    return ok ? 1 : 0;
+}
+
+int sdb_codegen_kernels_compiled(void) { return jit_kernels_compiled(); }
+
+int sdb_codegen_check(const char *arch, int *count)
+{
+    return guarded([&] {
+        require(arch && count, "null argument");
+        *count = jit_check(arch);
     });
 }
 

@@ -14,6 +14,7 @@
 // LDS (dense slots or an open-addressed hash of slots) and flushed once.
 #include "capi_util.hpp"
 #include "fused.hpp"
+#include "fused_device.hpp"
 
 #include <algorithm>
 #include <map>
@@ -22,30 +23,7 @@
 namespace sdb {
 namespace {
 
-constexpr int BLOCK = 256;
-constexpr uint32_t HASH_EMPTY = 0xFFFFFFFFu;
-
-typedef int v4i __attribute__((ext_vector_type(4)));
-
-template <int R> __device__ __forceinline__ uint32_t flags_in(const uint8_t *p)
-{
-    if constexpr (R == 4) {
-        const uint32_t w = *reinterpret_cast<const uint32_t *>(p);
-        return ((w & 0xffu) != 0) | (((w >> 8) & 0xffu) != 0) << 1 | (((w >> 16) & 0xffu) != 0) << 2 |
-               (((w >> 24) & 0xffu) != 0) << 3;
-    } else {
-        return p[0] != 0;
-    }
-}
-
-template <int R> __device__ __forceinline__ void flags_out(uint8_t *p, uint32_t m)
-{
-    if constexpr (R == 4) {
-        *reinterpret_cast<uint32_t *>(p) = (m & 1u) | ((m >> 1) & 1u) << 8 | ((m >> 2) & 1u) << 16 | ((m >> 3) & 1u) << 24;
-    } else {
-        p[0] = (uint8_t)(m & 1u);
-    }
-}
+using namespace fdev;
 
 // Value registers: register j of thread t is regs[j * BLOCK + t] (an int4 =
 // the rows of one chunk).  The register number is uniform, so an access is
@@ -172,78 +150,9 @@ template <int R> __device__ __forceinline__ void store_masked(int32_t *dst, int6
         if ((m >> k) & 1u) dst[row0 + k] = x[k];
 }
 
-__device__ __forceinline__ uint32_t logic_combine(uint32_t logic, uint32_t in, uint32_t pred)
-{
-    return logic == SDB_NONE ? pred : (logic == SDB_AND ? (in & pred) : (in | pred));
-}
-
 struct Lds {
     uint8_t *base; // dynamic LDS
 };
-
-__device__ __forceinline__ void gb_flush_slot(const FGroup &g, uint32_t s, unsigned long long sum)
-{
-    atomicAdd(reinterpret_cast<unsigned long long *>(&g.sums[s]), sum);
-    if (atomicExch(&g.seen[s], 1u) == 0u) {
-        // slot -> key values (first key varies fastest); exact for in-range keys
-        uint32_t rest = s;
-#pragma unroll
-        for (int j = 0; j < F_MAX_KEYS; j++) {
-            if (j >= g.k) break;
-            g.out_keys[j][s] = (int32_t)((uint32_t)g.mins[j] + rest % g.range[j]);
-            rest /= g.range[j];
-        }
-    }
-}
-
-__device__ __forceinline__ void gb_row(const FGroup &g, const Lds &L, const int32_t *kv, int32_t agg)
-{
-    bool in_range = true;
-    uint32_t hash = 0, mult = 1;
-#pragma unroll
-    for (int j = 0; j < F_MAX_KEYS; j++) {
-        if (j >= g.k) break;
-        in_range &= kv[j] >= g.mins[j] && kv[j] <= g.maxs[j];
-        hash += ((uint32_t)kv[j] - (uint32_t)g.mins[j]) * mult;
-        mult *= g.range[j];
-    }
-    const uint64_t s64 = (uint64_t)(int64_t)(int32_t)hash % g.prod;
-    const unsigned long long v = (unsigned long long)(int64_t)agg;
-    if (in_range && g.lds == FG_DENSE_LDS) {
-        const uint32_t s = (uint32_t)s64;
-        unsigned long long *lsum = reinterpret_cast<unsigned long long *>(L.base + g.lds_off);
-        uint32_t *lbits = reinterpret_cast<uint32_t *>(lsum + g.prod);
-        atomicAdd(&lsum[s], v);
-        atomicOr(&lbits[s >> 5], 1u << (s & 31));
-        return;
-    }
-    if (in_range && g.lds == FG_HASH_LDS) {
-        const uint32_t s = (uint32_t)s64;
-        uint32_t *hk = reinterpret_cast<uint32_t *>(L.base + g.lds_off);
-        unsigned long long *hv = reinterpret_cast<unsigned long long *>(hk + g.hash_cap);
-        const uint32_t mask = g.hash_cap - 1;
-        uint32_t h = (s * 2654435761u) & mask;
-        for (int probe = 0; probe < 32; probe++) {
-            uint32_t cur = hk[h];
-            if (cur == HASH_EMPTY) {
-                cur = atomicCAS(&hk[h], HASH_EMPTY, s);
-                if (cur == HASH_EMPTY) cur = s;
-            }
-            if (cur == s) {
-                atomicAdd(&hv[h], v);
-                return;
-            }
-            h = (h + 1) & mask;
-        }
-        // table crowded: fall through to the global update
-    }
-    if (atomicExch(&g.seen[s64], 1u) == 0u) {
-#pragma unroll
-        for (int j = 0; j < F_MAX_KEYS; j++)
-            if (j < g.k) g.out_keys[j][s64] = kv[j];
-    }
-    atomicAdd(reinterpret_cast<unsigned long long *>(&g.sums[s64]), v);
-}
 
 template <int R>
 __device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64_t &acc, const Lds &L,
@@ -431,7 +340,7 @@ __device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64
 #pragma unroll
                 for (int j = 0; j < F_MAX_KEYS; j++)
                     kv[j] = j < g.k ? operand_row<R>(g.key_reg[j], g.key_mem[j], row0, k, V, pf) : 0;
-                gb_row(g, L, kv, operand_row<R>(g.agg_reg, g.agg_mem, row0, k, V, pf));
+                gb_row(g, L.base, kv, operand_row<R>(g.agg_reg, g.agg_mem, row0, k, V, pf));
             }
             break;
         }
@@ -477,27 +386,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused(const FProgram P_unused, int64_
     const Lds L{reinterpret_cast<uint8_t *>(dyn_lds)};
     const VRegs V{reinterpret_cast<int4 *>(L.base + P.vreg_off) + threadIdx.x};
 
-    // stage probe bitmaps, clear the aggregation area
-    for (int b = 0; b < P.nblobs; b++) {
-        const uint4 *src = reinterpret_cast<const uint4 *>(P.blob_src[b]);
-        uint4 *dst = reinterpret_cast<uint4 *>(L.base + P.blob_off[b]);
-        for (uint32_t w = threadIdx.x; w < P.blob_bytes[b] / 16; w += BLOCK) dst[w] = src[w];
-    }
-    const FGroup &g = P.gb;
-    if (g.lds == FG_DENSE_LDS) {
-        unsigned long long *lsum = reinterpret_cast<unsigned long long *>(L.base + g.lds_off);
-        uint32_t *lbits = reinterpret_cast<uint32_t *>(lsum + g.prod);
-        for (uint32_t s = threadIdx.x; s < (uint32_t)g.prod; s += BLOCK) lsum[s] = 0;
-        for (uint32_t w = threadIdx.x; w < ((uint32_t)g.prod + 31) / 32; w += BLOCK) lbits[w] = 0;
-    } else if (g.lds == FG_HASH_LDS) {
-        uint32_t *hk = reinterpret_cast<uint32_t *>(L.base + g.lds_off);
-        unsigned long long *hv = reinterpret_cast<unsigned long long *>(hk + g.hash_cap);
-        for (uint32_t h = threadIdx.x; h < g.hash_cap; h += BLOCK) {
-            hk[h] = HASH_EMPTY;
-            hv[h] = 0;
-        }
-    }
-    __syncthreads();
+    block_prologue(P, L.base);
 
     int64_t acc = 0;
     if (P.vec) {
@@ -512,32 +401,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused(const FProgram P_unused, int64_
     } else {
         stream_chunks<1>(P, (int64_t)blockIdx.x * BLOCK + threadIdx.x, n, (int64_t)gridDim.x * BLOCK, acc, L, V);
     }
-    if (P.sum_acc) {
-        __shared__ int64_t part[BLOCK / 64];
-        uint64_t v = (uint64_t)acc;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = (int64_t)v;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint64_t s = 0;
-            for (int w = 0; w < BLOCK / 64; w++) s += (uint64_t)part[w];
-            if (s) atomicAdd(reinterpret_cast<unsigned long long *>(P.sum_acc), (unsigned long long)s);
-        }
-    }
-    if (g.lds == FG_DENSE_LDS) {
-        __syncthreads();
-        const unsigned long long *lsum = reinterpret_cast<const unsigned long long *>(L.base + g.lds_off);
-        const uint32_t *lbits = reinterpret_cast<const uint32_t *>(lsum + g.prod);
-        for (uint32_t s = threadIdx.x; s < (uint32_t)g.prod; s += BLOCK)
-            if ((lbits[s >> 5] >> (s & 31)) & 1u) gb_flush_slot(g, s, lsum[s]);
-    } else if (g.lds == FG_HASH_LDS) {
-        __syncthreads();
-        const uint32_t *hk = reinterpret_cast<const uint32_t *>(L.base + g.lds_off);
-        const unsigned long long *hv = reinterpret_cast<const unsigned long long *>(hk + g.hash_cap);
-        for (uint32_t h = threadIdx.x; h < g.hash_cap; h += BLOCK)
-            if (hk[h] != HASH_EMPTY) gb_flush_slot(g, hk[h], hv[h]);
-    }
+    block_epilogue(P, L.base, acc);
 }
 
 } // namespace

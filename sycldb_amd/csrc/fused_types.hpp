@@ -1,0 +1,132 @@
+// Program format of the fused execution of one KernelBundle (the per-segment
+// operator sequence the reference queues in models/execution.hpp:500-651) as
+// a single kernel.
+//
+// The reference designed this fusion (the JIT call sequence of
+// kernels/jit_kernels.hpp:12-39 and models/execution.hpp:546-615) but runs
+// with it disabled.  Here the host translates the bundle's operator list into
+// a small program: every distinct flag buffer the bundle touches becomes a
+// flag register, every column the bundle itself materialises becomes a value
+// register, and each operator becomes one instruction with the reference
+// functor's exact per-row semantics.  Per row the program therefore computes
+// what the unfused sequence computes; only the stores differ: a buffer is
+// written back only when it is still referenced after the bundle.
+//
+// Shared by the host, the compiled interpreter kernel and the kernels compiled
+// at run time for one program shape (which see this file as an in-memory
+// header), so all three agree on the layout.
+#pragma once
+
+#ifndef __HIPCC_RTC__
+#include <hip/hip_runtime.h>
+#endif
+#include <stdint.h>
+
+namespace sdb {
+
+constexpr int F_MAX_INSTR = 24;
+constexpr int F_MAX_FLAGS = 8;
+constexpr int F_MAX_VALS = 4;
+constexpr int F_MAX_KEYS = 4;
+// Prefetched columns: operand register F_PF_BASE + j names column pf_mem[j],
+// which the kernel loads for the next chunk while it runs the current one.
+constexpr int F_MAX_PF = 4;
+constexpr int F_PF_BASE = 16;
+
+enum FKind : uint8_t {
+    FK_SEL_LIT,      // SelectionKernelLiteral
+    FK_SEL_COL,      // SelectionKernelColumns
+    FK_LOGICAL,      // LogicalKernel
+    FK_FILL,         // FillKernel
+    FK_BINOP_CC,     // PerformOperationKernelColumns
+    FK_BINOP_LC,     // PerformOperationKernelLiteralFirst
+    FK_BINOP_CL,     // PerformOperationKernelLiteralSecond
+    FK_PROBE_SEMI,   // FilterJoinKernel
+    FK_PROBE_FULL,   // FullJoinKernel
+    FK_BUILD_KEYS,   // BuildKeysHTKernel
+    FK_BUILD_KEYVALS,// BuildKeyValsHTKernel
+    FK_SUM,          // AggregateOperationKernel (terminal)
+    FK_GROUPBY,      // GroupByAggregateKernel (terminal)
+    // forms produced by the program optimiser (same per-row results):
+    FK_SEL_RANGE,    // SEARCH over one range: logic(in, lit <= x <= lit2)
+    FK_SEL_IN2,      // SEARCH over two points: logic(in, x == lit || x == lit2)
+    FK_PROBE_BITS    // semi/full probe through a presence bitmap (+ value table)
+};
+
+// value table of FK_PROBE_BITS
+enum FValKind : uint8_t { FV_NONE = 0, FV_U16 = 1, FV_PAIR = 2 };
+
+struct FInstr {
+    uint8_t kind, cmp, logic, op;
+    uint8_t fin, fout, floc, in_lds; // in_lds: FK_PROBE_BITS bitmap staged in LDS
+    int8_t va, vb, vout, vkind;      // value registers (F_PF_BASE + j: prefetched
+                                     // column j); -1 = memory operand a / b
+    int32_t lit, lit2;
+    const int32_t *a;          // memory operand (row-indexed) when va < 0
+    const int32_t *b;          // memory operand when vb < 0
+    int32_t *store;            // write-back of vout (masked as the functor writes), or null
+    const void *ht;            // reference-layout table; FK_PROBE_BITS: bitmap
+    const void *vals;          // FK_PROBE_BITS value table (FV_U16: offsets from vmin; FV_PAIR: ht)
+    int32_t ht_min, ht_max, ht_len, vmin;
+    uint32_t lds_off, pad;     // byte offset of the staged bitmap in dynamic LDS
+};
+
+enum FGroupMode : int { FG_GLOBAL = 0, FG_DENSE_LDS = 1, FG_HASH_LDS = 2 };
+
+struct FGroup {
+    int k;
+    int lds;                   // FGroupMode
+    int8_t key_reg[F_MAX_KEYS];
+    int8_t agg_reg;
+    int8_t pad[3];
+    const int32_t *key_mem[F_MAX_KEYS];
+    const int32_t *agg_mem;
+    int32_t mins[F_MAX_KEYS];
+    int32_t maxs[F_MAX_KEYS];
+    uint32_t range[F_MAX_KEYS];
+    int32_t *out_keys[F_MAX_KEYS];
+    uint64_t *sums;
+    uint32_t *seen;
+    uint64_t prod;
+    uint32_t lds_off;          // byte offset of the LDS aggregation area
+    uint32_t hash_cap;         // FG_HASH_LDS: slots (power of two)
+};
+
+// Initial value of a flag register.  FF_UNDEF is 0 so that registers a
+// program does not use (zero-initialised) are never loaded.
+enum FFlagInit : uint8_t { FF_UNDEF = 0, FF_LOAD = 1, FF_TRUE = 2 };
+
+constexpr uint32_t F_MAGIC = 0x5DB0F05Eu;
+
+constexpr int F_MAX_BLOBS = 4;
+
+struct FProgram {
+    uint32_t magic;            // F_MAGIC; the kernel reads the program in place
+                               // from its kernel-argument segment and checks it
+    int ninstr;
+    int vec;                   // 1: 4-row chunks with 16-byte column loads
+    int nvals;                 // value registers used (LDS register file size)
+    int nblobs;                // tables staged into LDS at kernel start
+    uint32_t lds_bytes;        // dynamic LDS: registers, blobs, aggregation
+    uint32_t vreg_off;         // byte offset of the register file
+    int npf;                   // prefetched columns
+    const int32_t *pf_mem[F_MAX_PF];
+    const void *blob_src[F_MAX_BLOBS];
+    uint32_t blob_bytes[F_MAX_BLOBS]; // multiples of 16
+    uint32_t blob_off[F_MAX_BLOBS];
+    uint8_t flag_init[F_MAX_FLAGS];
+    const uint8_t *flag_load[F_MAX_FLAGS];
+    uint8_t *flag_store[F_MAX_FLAGS];
+    uint64_t *sum_acc;         // FK_SUM target
+    FGroup gb;                 // FK_GROUPBY description
+    FInstr ins[F_MAX_INSTR];
+};
+
+// Dynamic LDS one fused block may use (registers, staged bitmaps, group-by).
+constexpr uint32_t F_LDS_BUDGET = 96 * 1024;
+// LDS aggregation sizes: dense slots up to F_DENSE_MAX, else a hash of
+// F_HASH_CAP slots (power of two).
+constexpr uint64_t F_DENSE_MAX = 1024;
+constexpr uint32_t F_HASH_CAP = 1024;
+
+} // namespace sdb

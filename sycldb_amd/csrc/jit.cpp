@@ -1,0 +1,673 @@
+// Fused bundle kernels compiled at run time for one program shape.
+//
+// The reference designed its fusion around a JIT: the SYCLDBContext slot
+// operations of kernels/jit_kernels_impl.hpp:23-90 specialised per bundle
+// (models/execution.hpp:546-615), switched off in its runs.  Here a fused
+// program (fused_types.hpp) is turned into HIP source in which every
+// instruction is a call of a device template from fused_ops.hpp with its
+// code-selecting parameters (comparison, logical and arithmetic op, operand
+// kind, table placement, value-table kind) fixed, and every flag and value
+// register is a plain variable.  The remaining parameters -- literals,
+// pointers, table bounds -- are read from the program itself, which is the
+// kernel's argument exactly as for the interpreter kernel, so one compiled
+// kernel serves every program of the same shape.
+//
+// Source text -> hipRTC (gfx950 code object, cached per architecture and
+// source) -> hipModuleLoadData per device (cached) -> hipModuleLaunchKernel.
+// The hipRTC library is loaded at first use, preferring the one beside the
+// process's HIP runtime so that the compiler and the runtime come from the
+// same ROCm.
+#include <dlfcn.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <vector>
+
+#include "capi_util.hpp"
+#include "fused.hpp"
+
+namespace sdb {
+
+namespace {
+
+// the in-memory headers the generated source includes
+#include "jit_headers.inc"
+
+constexpr int JIT_BLOCK = 256;
+
+struct Rtc {
+    bool ok = false;
+    std::string why, path;
+    hiprtcResult (*create)(hiprtcProgram *, const char *, const char *, int, const char *const *,
+                           const char *const *) = nullptr;
+    hiprtcResult (*compile)(hiprtcProgram, int, const char *const *) = nullptr;
+    hiprtcResult (*log_size)(hiprtcProgram, size_t *) = nullptr;
+    hiprtcResult (*log)(hiprtcProgram, char *) = nullptr;
+    hiprtcResult (*code_size)(hiprtcProgram, size_t *) = nullptr;
+    hiprtcResult (*code)(hiprtcProgram, char *) = nullptr;
+    hiprtcResult (*destroy)(hiprtcProgram *) = nullptr;
+};
+
+Rtc load_rtc()
+{
+    Rtc r;
+    std::vector<std::string> cand;
+    Dl_info info{};
+    if (dladdr(reinterpret_cast<void *>(&hipModuleLoadData), &info) && info.dli_fname) {
+        std::string dir(info.dli_fname);
+        const size_t slash = dir.rfind('/');
+        if (slash != std::string::npos) {
+            dir.resize(slash);
+            cand.push_back(dir + "/libhiprtc.so");
+            cand.push_back(dir + "/libhiprtc.so.7");
+        }
+    }
+    cand.push_back("libhiprtc.so.7");
+    cand.push_back("/opt/rocm/lib/libhiprtc.so.7");
+    void *h = nullptr;
+    for (const std::string &c : cand) {
+        h = dlopen(c.c_str(), RTLD_NOW | RTLD_LOCAL);
+        if (h) {
+            r.path = c;
+            break;
+        }
+    }
+    if (!h) {
+        r.why = "hipRTC library not found";
+        return r;
+    }
+    auto sym = [&](const char *name) {
+        void *p = dlsym(h, name);
+        if (!p && r.why.empty()) r.why = std::string("hipRTC symbol missing: ") + name;
+        return p;
+    };
+    r.create = reinterpret_cast<decltype(r.create)>(sym("hiprtcCreateProgram"));
+    r.compile = reinterpret_cast<decltype(r.compile)>(sym("hiprtcCompileProgram"));
+    r.log_size = reinterpret_cast<decltype(r.log_size)>(sym("hiprtcGetProgramLogSize"));
+    r.log = reinterpret_cast<decltype(r.log)>(sym("hiprtcGetProgramLog"));
+    r.code_size = reinterpret_cast<decltype(r.code_size)>(sym("hiprtcGetCodeSize"));
+    r.code = reinterpret_cast<decltype(r.code)>(sym("hiprtcGetCode"));
+    r.destroy = reinterpret_cast<decltype(r.destroy)>(sym("hiprtcDestroyProgram"));
+    r.ok = r.why.empty();
+    return r;
+}
+
+Rtc &rtc()
+{
+    static Rtc r = load_rtc();
+    return r;
+}
+
+struct Compiled {
+    hipModule_t module = nullptr;
+    hipFunction_t fn = nullptr;
+    std::map<uint32_t, int> blocks_per_cu; // by dynamic LDS bytes
+};
+
+std::mutex g_mu;
+std::map<std::string, std::vector<char>> g_code;          // arch + source -> code object
+std::map<std::pair<int, std::string>, Compiled> g_loaded; // (device, arch + source)
+int g_compiled = 0;
+
+// ------------------------------------------------------------ generation
+
+struct Gen {
+    const FProgram &p;
+    std::ostringstream o;
+    explicit Gen(const FProgram &prog) : p(prog) {}
+
+    static std::string freg(int j) { return "f" + std::to_string(j); }
+    static std::string vreg(int r) { return "v" + std::to_string(r); }
+
+    bool cheap(int reg) const { return reg >= 0; }
+
+    // statement filling int32_t dst[R] with the operand's rows of chunk u
+    std::string operand(int reg, const std::string &mem, const std::string &dst) const
+    {
+        if (reg >= F_PF_BASE) return "vec_rows<R>(pf.c[" + std::to_string(reg - F_PF_BASE) + "][u], " + dst + ");";
+        if (reg >= 0) return "copy_rows<R>(" + vreg(reg) + "[u], " + dst + ");";
+        return "load_rows<R>(" + mem + ", row0[u], " + dst + ");";
+    }
+
+    void instr(int i)
+    {
+        const FInstr &I = p.ins[i];
+        const std::string P = "P.ins[" + std::to_string(i) + "]";
+        const std::string fin = freg(I.fin) + "[u]", fout = freg(I.fout) + "[u]";
+        const std::string L = std::to_string((int)I.logic);
+        o << "    { // " << i << "\n";
+        switch (I.kind) {
+        case FK_SEL_LIT:
+        case FK_SEL_COL:
+        case FK_SEL_RANGE:
+        case FK_SEL_IN2: {
+            const bool two = I.kind == FK_SEL_COL;
+            std::string pred;
+            const std::string C = std::to_string((int)I.cmp);
+            if (I.kind == FK_SEL_LIT) pred = "pred_lit<R, " + C + ">(x[u], " + P + ".lit)";
+            else if (I.kind == FK_SEL_COL) pred = "pred_col<R, " + C + ">(x[u], y[u])";
+            else if (I.kind == FK_SEL_RANGE) pred = "pred_range<R>(x[u], " + P + ".lit, " + P + ".lit2)";
+            else pred = "pred_in2<R>(x[u], " + P + ".lit, " + P + ".lit2)";
+            o << "        int32_t x[UU][R];" << (two ? " int32_t y[UU][R];" : "") << "\n";
+            const bool lazy = !cheap(I.va) || (two && !cheap(I.vb));
+            if (lazy) {
+                o << "        uint32_t need[UU];\n";
+                o << "#pragma unroll\n        for (int u = 0; u < UU; u++) {\n";
+                o << "            need[u] = sel_need<" << L << ", R>(" << fin << ");\n";
+                o << "            if (need[u]) { " << operand(I.va, P + ".a", "x[u]");
+                if (two) o << " " << operand(I.vb, P + ".b", "y[u]");
+                o << " }\n        }\n";
+                o << "#pragma unroll\n        for (int u = 0; u < UU; u++) " << fout << " = combine<" << L << ">(" << fin
+                  << ", need[u] ? " << pred << " : 0u);\n";
+            } else {
+                o << "#pragma unroll\n        for (int u = 0; u < UU; u++) { " << operand(I.va, P + ".a", "x[u]");
+                if (two) o << " " << operand(I.vb, P + ".b", "y[u]");
+                o << " }\n";
+                o << "#pragma unroll\n        for (int u = 0; u < UU; u++) " << fout << " = combine<" << L << ">(" << fin
+                  << ", " << pred << ");\n";
+            }
+            break;
+        }
+        case FK_LOGICAL:
+            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) " << fout << " = combine<" << L << ">(" << fin
+              << ", " << freg(I.floc) << "[u]);\n";
+            break;
+        case FK_FILL:
+            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) {\n";
+            o << "            fill_rows<R>(" << P << ".lit, " << vreg(I.vout) << "[u]);\n";
+            if (I.store) o << "            store_rows<R>(" << P << ".store, row0[u], " << vreg(I.vout) << "[u], M);\n";
+            o << "        }\n";
+            break;
+        case FK_BINOP_CC:
+        case FK_BINOP_LC:
+        case FK_BINOP_CL: {
+            o << "        int32_t x[UU][R], y[UU][R];\n";
+            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) {\n            if (!" << fin << ") continue;\n";
+            o << "            "
+              << (I.kind == FK_BINOP_LC ? "fill_rows<R>(" + P + ".lit, x[u]);" : operand(I.va, P + ".a", "x[u]"))
+              << "\n";
+            o << "            "
+              << (I.kind == FK_BINOP_CL ? "fill_rows<R>(" + P + ".lit, y[u]);" : operand(I.vb, P + ".b", "y[u]"))
+              << "\n        }\n";
+            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) {\n            if (!" << fin << ") continue;\n";
+            o << "            int32_t r[R];\n            binop<R, " << (int)I.op << ">(x[u], y[u], r);\n";
+            o << "            set_rows<R>(" << vreg(I.vout) << "[u], r, " << fin << ");\n";
+            if (I.store) o << "            store_rows<R>(" << P << ".store, row0[u], r, " << fin << ");\n";
+            o << "        }\n";
+            break;
+        }
+        case FK_PROBE_SEMI:
+        case FK_PROBE_FULL:
+        case FK_PROBE_BITS: {
+            o << "        int32_t x[UU][R];\n";
+            if (cheap(I.va))
+                o << "#pragma unroll\n        for (int u = 0; u < UU; u++) { " << operand(I.va, P + ".a", "x[u]")
+                  << " }\n";
+            else
+                o << "#pragma unroll\n        for (int u = 0; u < UU; u++) if (" << fin << ") { "
+                  << operand(I.va, P + ".a", "x[u]") << " }\n";
+            const bool vout = I.kind == FK_PROBE_FULL || (I.kind == FK_PROBE_BITS && I.vkind != FV_NONE);
+            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) {\n            uint32_t o = 0;\n";
+            if (!vout) o << "            int32_t none[R];\n";
+            const std::string vdst = vout ? vreg(I.vout) + "[u]" : "none";
+            o << "            if (" << fin << ") {\n";
+            if (I.kind == FK_PROBE_SEMI) {
+                o << "                o = probe_semi<R>(x[u], " << fin << ", (const uint8_t *)" << P << ".ht, " << P
+                  << ".ht_min, " << P << ".ht_max);\n";
+            } else if (I.kind == FK_PROBE_FULL) {
+                o << "                o = probe_full<R>(x[u], " << fin << ", (const int2 *)" << P << ".ht, " << P
+                  << ".ht_min, " << P << ".ht_max, " << vdst << ");\n";
+            } else {
+                const std::string bits = I.in_lds ? "(const uint32_t *)(lds + " + P + ".lds_off)"
+                                                  : "(const uint32_t *)" + P + ".ht";
+                o << "                o = probe_bits<R, " << (int)I.vkind << ">(x[u], " << fin << ", " << bits << ", "
+                  << P << ".ht_min, " << P << ".ht_max, " << P << ".vals, " << P << ".vmin, " << vdst << ");\n";
+            }
+            if (vout && I.store)
+                o << "                if (o) store_rows<R>(" << P << ".store, row0[u], " << vdst << ", o);\n";
+            o << "            }\n            " << fout << " = o;\n        }\n";
+            break;
+        }
+        case FK_BUILD_KEYS:
+            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) {\n            int32_t x[R];\n            "
+              << operand(I.va, P + ".a", "x") << "\n";
+            o << "            build_keys<R>(x, " << fin << ", (uint8_t *)" << P << ".ht, " << P << ".ht_min, " << P
+              << ".ht_len);\n        }\n";
+            break;
+        case FK_BUILD_KEYVALS:
+            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) {\n            int32_t x[R], y[R];\n            "
+              << operand(I.va, P + ".a", "x") << "\n";
+            o << "            if (" << fin << ") { " << operand(I.vb, P + ".b", "y") << " } else fill_rows<R>(0, y);\n";
+            o << "            build_keyvals<R>(x, y, " << fin << ", (int32_t *)" << P << ".ht, " << P << ".ht_min, " << P
+              << ".ht_len);\n        }\n";
+            break;
+        case FK_SUM:
+            o << "        int32_t x[UU][R];\n";
+            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) if (" << fin << ") { "
+              << operand(I.va, P + ".a", "x[u]") << " }\n";
+            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) if (" << fin << ") sum_rows<R>(x[u], " << fin
+              << ", acc);\n";
+            break;
+        case FK_GROUPBY: {
+            const FGroup &g = p.gb;
+            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) {\n            if (!" << fin << ") continue;\n";
+            o << "            int32_t kx[" << g.k << "][R], ax[R];\n";
+            for (int j = 0; j < g.k; j++)
+                o << "            "
+                  << operand(g.key_reg[j], "P.gb.key_mem[" + std::to_string(j) + "]", "kx[" + std::to_string(j) + "]")
+                  << "\n";
+            o << "            " << operand(g.agg_reg, "P.gb.agg_mem", "ax") << "\n";
+            o << "#pragma unroll\n            for (int k = 0; k < R; k++) {\n";
+            o << "                if (!((" << fin << " >> k) & 1u)) continue;\n";
+            o << "                const int32_t kv[F_MAX_KEYS] = {";
+            for (int j = 0; j < F_MAX_KEYS; j++) o << (j ? ", " : "") << (j < g.k ? "kx[" + std::to_string(j) + "][k]" : "0");
+            o << "};\n                fdev::gb_row(P.gb, lds, kv, ax[k]);\n            }\n        }\n";
+            break;
+        }
+        default: fail(SDB_ERR_RUNTIME, "jit: unknown instruction kind");
+        }
+        o << "    }\n";
+    }
+
+    std::string body()
+    {
+        // registers the program touches
+        std::vector<bool> fused(F_MAX_FLAGS, false);
+        for (int j = 0; j < F_MAX_FLAGS; j++) fused[j] = p.flag_init[j] != FF_UNDEF || p.flag_store[j];
+        for (int i = 0; i < p.ninstr; i++) {
+            const FInstr &I = p.ins[i];
+            fused[I.fin] = true;
+            if (I.kind == FK_LOGICAL) fused[I.floc] = true;
+            if (I.kind == FK_SEL_LIT || I.kind == FK_SEL_COL || I.kind == FK_SEL_RANGE || I.kind == FK_SEL_IN2 ||
+                I.kind == FK_LOGICAL || I.kind == FK_PROBE_SEMI || I.kind == FK_PROBE_FULL || I.kind == FK_PROBE_BITS)
+                fused[I.fout] = true;
+        }
+        o << "    constexpr uint32_t M = (1u << R) - 1;\n";
+        for (int j = 0; j < F_MAX_FLAGS; j++) {
+            if (!fused[j]) continue;
+            o << "    uint32_t " << freg(j) << "[UU];\n";
+            o << "#pragma unroll\n    for (int u = 0; u < UU; u++) " << freg(j) << "[u] = ";
+            if (p.flag_init[j] == FF_TRUE) o << "M;\n";
+            else if (p.flag_init[j] == FF_LOAD && p.flag_load[j])
+                o << "fdev::flags_in<R>(P.flag_load[" << j << "] + row0[u]);\n";
+            else o << "0u;\n";
+        }
+        for (int r = 0; r < p.nvals; r++) o << "    int32_t " << vreg(r) << "[UU][R] = {};\n";
+        for (int i = 0; i < p.ninstr; i++) instr(i);
+        for (int j = 0; j < F_MAX_FLAGS; j++)
+            if (p.flag_store[j])
+                o << "#pragma unroll\n    for (int u = 0; u < UU; u++) fdev::flags_out<R>(P.flag_store[" << j
+                  << "] + row0[u], " << freg(j) << "[u]);\n";
+        return o.str();
+    }
+};
+
+const char *KERNEL_TEMPLATE_INCLUDES = R"(#include "fused_ops.hpp"
+using namespace sdb;
+using namespace sdb::fops;
+namespace {
+)";
+
+const char *KERNEL_TEMPLATE_HEAD = R"(constexpr int BLOCK = fdev::BLOCK;
+template <int R, int UU> struct Pf {
+    typename Vec<R>::type c[NPF > 0 ? NPF : 1][UU];
+};
+template <int R, int UU>
+__device__ __forceinline__ void pf_load(const FProgram &P, const int64_t (&row0)[UU], Pf<R, UU> &pf)
+{
+#pragma unroll
+    for (int j = 0; j < NPF; j++)
+#pragma unroll
+        for (int u = 0; u < UU; u++) pf.c[j][u] = load_vec<R>(P.pf_mem[j], row0[u]);
+}
+template <int R, int UU>
+__device__ __forceinline__ void body(const FProgram &P, const int64_t (&row0)[UU], const Pf<R, UU> &pf, int64_t &acc,
+                                     uint8_t *lds)
+{
+)";
+
+const char *KERNEL_TEMPLATE_TAIL = R"(}
+} // namespace
+
+extern "C" __global__ void __launch_bounds__(256) sdb_fused_program(const FProgram P_arg, int64_t n)
+{
+    // the program is read in place from the kernel-argument segment
+    const FProgram &P = *(const FProgram __attribute__((address_space(4))) *)__builtin_amdgcn_kernarg_segment_ptr();
+    if (P.magic != F_MAGIC) return;
+    extern __shared__ uint4 dyn_lds[];
+    uint8_t *lds = reinterpret_cast<uint8_t *>(dyn_lds);
+    fdev::block_prologue(P, lds);
+    int64_t acc = 0;
+    if constexpr (VEC) {
+    // tiles of U * BLOCK 4-row chunks; chunk u of a thread in tile t starts
+    // at row 4 * (t * U * BLOCK + u * BLOCK + thread); the prefetched columns
+    // of the next tile load while this one runs
+    const int64_t nch = n >> 2, tile = (int64_t)U * BLOCK, ntiles = nch / tile;
+    auto rows_of = [&](int64_t t, int64_t (&r)[U]) {
+#pragma unroll
+        for (int u = 0; u < U; u++) r[u] = (t * tile + u * BLOCK + threadIdx.x) << 2;
+    };
+    int64_t t = blockIdx.x;
+    Pf<4, U> next;
+    if (t < ntiles) {
+        int64_t r[U];
+        rows_of(t, r);
+        pf_load<4, U>(P, r, next);
+    }
+    while (t < ntiles) {
+        const Pf<4, U> cur = next;
+        int64_t r[U];
+        rows_of(t, r);
+        const int64_t tn = t + gridDim.x;
+        if (tn < ntiles) {
+            int64_t rn[U];
+            rows_of(tn, rn);
+            pf_load<4, U>(P, rn, next);
+        }
+        body<4, U>(P, r, cur, acc, lds);
+        t = tn;
+    }
+    for (int64_t c = ntiles * tile + (int64_t)blockIdx.x * BLOCK + threadIdx.x; c < nch; c += (int64_t)gridDim.x * BLOCK) {
+        const int64_t r[1] = {c << 2};
+        Pf<4, 1> pf;
+        pf_load<4, 1>(P, r, pf);
+        body<4, 1>(P, r, pf, acc, lds);
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x < (n & 3)) {
+        const int64_t r[1] = {(nch << 2) + threadIdx.x};
+        Pf<1, 1> pf;
+        pf_load<1, 1>(P, r, pf);
+        body<1, 1>(P, r, pf, acc, lds);
+    }
+    } else {
+    const int64_t stride = (int64_t)gridDim.x * BLOCK;
+    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    Pf<1, 1> next;
+    if (i < n) {
+        const int64_t r[1] = {i};
+        pf_load<1, 1>(P, r, next);
+    }
+    while (i < n) {
+        const Pf<1, 1> cur = next;
+        const int64_t r[1] = {i};
+        const int64_t in = i + stride;
+        if (in < n) {
+            const int64_t rn[1] = {in};
+            pf_load<1, 1>(P, rn, next);
+        }
+        body<1, 1>(P, r, cur, acc, lds);
+        i = in;
+    }
+    }
+    fdev::block_epilogue(P, lds, acc);
+}
+)";
+
+std::string device_arch()
+{
+    int dev = 0;
+    SDB_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    SDB_HIP(hipGetDeviceProperties(&prop, dev));
+    std::string a(prop.gcnArchName);
+    const size_t colon = a.find(':');
+    if (colon != std::string::npos) a.resize(colon);
+    return a.empty() ? std::string("gfx950") : a;
+}
+
+// SDB_JIT_DUMP_DIR=<dir> writes every source compiled to <dir>/sdb_fused_<n>.hip.
+void dump_source(const std::string &src)
+{
+    static const char *dir = getenv("SDB_JIT_DUMP_DIR");
+    static int seq = 0;
+    if (!dir || !*dir) return;
+    const std::string path = std::string(dir) + "/sdb_fused_" + std::to_string(seq++) + ".hip";
+    if (FILE *f = fopen(path.c_str(), "w")) {
+        fwrite(src.data(), 1, src.size(), f);
+        fclose(f);
+    }
+}
+
+std::vector<char> compile_source(const std::string &src, const std::string &arch)
+{
+    Rtc &r = rtc();
+    dump_source(src);
+    if (!r.ok) fail(SDB_ERR_UNSUPPORTED, "runtime compilation unavailable: " + r.why);
+    std::vector<const char *> names, texts;
+    for (const JitHeader &h : JIT_HEADERS) {
+        names.push_back(h.name);
+        texts.push_back(h.text);
+    }
+    hiprtcProgram prog = nullptr;
+    if (r.create(&prog, src.c_str(), "sdb_fused_program.hip", (int)names.size(), texts.data(), names.data()) !=
+        HIPRTC_SUCCESS)
+        fail(SDB_ERR_RUNTIME, "hiprtcCreateProgram failed");
+    const std::string arch_opt = "--offload-arch=" + arch;
+    // the same code-generation flags as the library's own kernels
+    const char *opts[] = {arch_opt.c_str(), "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                          "-packed-fp32-ops"};
+    const hiprtcResult cr = r.compile(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+    if (cr != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        r.log_size(prog, &n);
+        std::string log(n, '\0');
+        if (n) r.log(prog, &log[0]);
+        r.destroy(&prog);
+        fail(SDB_ERR_RUNTIME, "fused kernel compilation failed:\n" + log);
+    }
+    size_t n = 0;
+    r.code_size(prog, &n);
+    std::vector<char> code(n);
+    r.code(prog, code.data());
+    r.destroy(&prog);
+    return code;
+}
+
+// Programs that together use every instruction form, operand kind, value
+// table kind and both chunk widths (pointers are left null: they are only
+// compiled, never launched).
+std::vector<std::pair<FProgram, int>> coverage_programs()
+{
+    std::vector<std::pair<FProgram, int>> out;
+    auto base = [](int vec, int npf) {
+        FProgram p{};
+        p.magic = F_MAGIC;
+        p.vec = vec;
+        p.npf = npf;
+        return p;
+    };
+    auto ins = [](FProgram &p, FInstr I) {
+        if (I.vout == 0 && I.kind != FK_FILL && I.kind != FK_BINOP_CC && I.kind != FK_BINOP_LC &&
+            I.kind != FK_BINOP_CL && I.kind != FK_PROBE_FULL && I.kind != FK_PROBE_BITS)
+            I.vout = -1;
+        p.ins[p.ninstr++] = I;
+    };
+    auto mk = [](uint8_t kind, uint8_t fin, uint8_t fout, int8_t va, int8_t vb = -1, int8_t vout = -1) {
+        FInstr I{};
+        I.kind = kind;
+        I.fin = fin;
+        I.fout = fout;
+        I.va = va;
+        I.vb = vb;
+        I.vout = vout;
+        return I;
+    };
+    static uint8_t flag_dummy[4];
+    {   // selections, logical, sum; flags loaded and stored
+        FProgram p = base(1, 2);
+        p.flag_init[0] = FF_TRUE;
+        p.flag_init[1] = FF_LOAD;
+        p.flag_load[1] = flag_dummy;
+        p.flag_store[2] = flag_dummy;
+        FInstr a = mk(FK_SEL_LIT, 0, 2, F_PF_BASE);
+        a.cmp = SDB_LT, a.logic = SDB_AND;
+        ins(p, a);
+        FInstr b = mk(FK_SEL_RANGE, 2, 2, -1);
+        b.logic = SDB_AND;
+        ins(p, b);
+        FInstr c = mk(FK_SEL_IN2, 2, 3, F_PF_BASE + 1);
+        c.logic = SDB_OR;
+        ins(p, c);
+        FInstr d = mk(FK_SEL_COL, 3, 3, F_PF_BASE, -1);
+        d.cmp = SDB_NE, d.logic = SDB_NONE;
+        ins(p, d);
+        FInstr e = mk(FK_LOGICAL, 1, 2, -1);
+        e.floc = 3, e.logic = SDB_AND;
+        ins(p, e);
+        ins(p, mk(FK_SUM, 2, 0, -1));
+        p.sum_acc = reinterpret_cast<uint64_t *>(flag_dummy);
+        out.emplace_back(p, 2);
+    }
+    {   // projection: fill and the three arithmetic forms, stored
+        FProgram p = base(1, 1);
+        p.nvals = 4;
+        p.flag_init[0] = FF_TRUE;
+        FInstr f = mk(FK_FILL, 0, 0, -1, -1, 0);
+        f.store = reinterpret_cast<int32_t *>(flag_dummy);
+        ins(p, f);
+        FInstr m = mk(FK_BINOP_CC, 0, 0, 0, -1, 1);
+        m.op = SDB_MUL;
+        m.store = reinterpret_cast<int32_t *>(flag_dummy);
+        ins(p, m);
+        FInstr l = mk(FK_BINOP_LC, 0, 0, -1, F_PF_BASE, 2);
+        l.op = SDB_SUB;
+        ins(p, l);
+        FInstr r = mk(FK_BINOP_CL, 0, 0, 1, -1, 3);
+        r.op = SDB_DIV;
+        ins(p, r);
+        ins(p, mk(FK_SUM, 0, 0, 3));
+        p.sum_acc = reinterpret_cast<uint64_t *>(flag_dummy);
+        out.emplace_back(p, 4);
+    }
+    {   // probes of every kind and a three-key group-by, row at a time
+        FProgram p = base(0, 2);
+        p.nvals = 3;
+        p.flag_init[0] = FF_TRUE;
+        ins(p, mk(FK_PROBE_SEMI, 0, 1, -1));
+        FInstr full = mk(FK_PROBE_FULL, 1, 1, F_PF_BASE, -1, 0);
+        full.store = reinterpret_cast<int32_t *>(flag_dummy);
+        ins(p, full);
+        FInstr b0 = mk(FK_PROBE_BITS, 1, 1, -1);
+        b0.in_lds = 1;
+        b0.vkind = FV_NONE;
+        ins(p, b0);
+        FInstr b1 = mk(FK_PROBE_BITS, 1, 1, F_PF_BASE + 1, -1, 1);
+        b1.vkind = FV_U16;
+        ins(p, b1);
+        FInstr b2 = mk(FK_PROBE_BITS, 1, 1, 0, -1, 2);
+        b2.vkind = FV_PAIR;
+        b2.store = reinterpret_cast<int32_t *>(flag_dummy);
+        ins(p, b2);
+        ins(p, mk(FK_GROUPBY, 1, 0, -1));
+        p.gb.k = 3;
+        p.gb.key_reg[0] = 1, p.gb.key_reg[1] = 2, p.gb.key_reg[2] = -1;
+        p.gb.agg_reg = F_PF_BASE + 1;
+        out.emplace_back(p, 1);
+    }
+    {   // builds
+        FProgram p = base(1, 1);
+        p.flag_init[0] = FF_LOAD;
+        p.flag_load[0] = flag_dummy;
+        ins(p, mk(FK_BUILD_KEYS, 0, 0, -1));
+        ins(p, mk(FK_BUILD_KEYVALS, 0, 0, F_PF_BASE, -1));
+        out.emplace_back(p, 1);
+    }
+    {   // one-key group-by over a prefetched key, memory aggregate
+        FProgram p = base(1, 1);
+        p.flag_init[0] = FF_TRUE;
+        ins(p, mk(FK_GROUPBY, 0, 0, -1));
+        p.gb.k = 1;
+        p.gb.key_reg[0] = F_PF_BASE;
+        p.gb.agg_reg = -1;
+        out.emplace_back(p, 4);
+    }
+    return out;
+}
+
+Compiled &get_kernel(const std::string &src)
+{
+    int dev = 0;
+    SDB_HIP(hipGetDevice(&dev));
+    const std::string key = device_arch() + "\n" + src;
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_loaded.find({dev, key});
+    if (it != g_loaded.end()) return it->second;
+    auto ct = g_code.find(key);
+    if (ct == g_code.end()) {
+        ct = g_code.emplace(key, compile_source(src, device_arch())).first;
+        g_compiled++;
+    }
+    Compiled c;
+    SDB_HIP(hipModuleLoadData(&c.module, ct->second.data()));
+    SDB_HIP(hipModuleGetFunction(&c.fn, c.module, "sdb_fused_program"));
+    return g_loaded.emplace(std::make_pair(dev, key), c).first->second;
+}
+
+} // namespace
+
+bool jit_available() { return rtc().ok; }
+
+std::string jit_unavailable_reason() { return rtc().ok ? std::string() : rtc().why; }
+
+int jit_check(const std::string &arch)
+{
+    int n = 0;
+    for (auto &pu : coverage_programs()) {
+        compile_source(jit_source(pu.first, pu.second), arch);
+        n++;
+    }
+    return n;
+}
+
+int jit_kernels_compiled()
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_compiled;
+}
+
+std::string jit_source(const FProgram &p, int unroll)
+{
+    if (unroll != 1 && unroll != 2 && unroll != 4) fail(SDB_ERR_INVALID_ARG, "jit: unroll must be 1, 2 or 4");
+    Gen g(p);
+    std::ostringstream s;
+    s << KERNEL_TEMPLATE_INCLUDES << "constexpr int NPF = " << p.npf << ";\nconstexpr int U = " << unroll
+      << ";\nconstexpr bool VEC = " << (p.vec ? "true" : "false") << ";\n";
+    s << KERNEL_TEMPLATE_HEAD << g.body() << KERNEL_TEMPLATE_TAIL;
+    return s.str();
+}
+
+void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll)
+{
+    if (n <= 0) return;
+    if (p.magic != F_MAGIC) fail(SDB_ERR_RUNTIME, "fused program without magic");
+    Compiled &k = get_kernel(jit_source(p, unroll));
+    // the value-register file (the last LDS area) belongs to the interpreter
+    const uint32_t lds = p.vreg_off;
+    int per_cu;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = k.blocks_per_cu.find(lds);
+        if (it == k.blocks_per_cu.end()) {
+            int b = 0;
+            SDB_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&b, k.fn, JIT_BLOCK, lds));
+            it = k.blocks_per_cu.emplace(lds, std::max(1, b)).first;
+        }
+        per_cu = it->second;
+    }
+    const int64_t units = p.vec ? (n + 3) / 4 : n;
+    const unsigned grid = stream_grid(units, (int64_t)JIT_BLOCK * (p.vec ? unroll : 1), per_cu);
+    struct Args {
+        FProgram p;
+        int64_t n;
+    } args{p, n};
+    size_t size = sizeof(args);
+    void *config[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &args, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size, HIP_LAUNCH_PARAM_END};
+    SDB_HIP(hipModuleLaunchKernel(k.fn, grid, 1, 1, JIT_BLOCK, 1, 1, lds, s, nullptr, config));
+}
+
+} // namespace sdb

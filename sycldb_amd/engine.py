@@ -46,14 +46,50 @@ class Result:
         return "".join(" ".join(str(v) for v in row) + "\n" for row in self.rows())
 
 
+CODEGEN_MODES = {"auto": L.CODEGEN_AUTO, "interpret": L.CODEGEN_INTERPRET, "compile": L.CODEGEN_COMPILE}
+
+
+def codegen_available():
+    """(available, reason): whether fused kernels can be compiled at run time."""
+    why = ctypes.create_string_buffer(512)
+    ok = L.lib.sdb_codegen_available(why, 512) == 1
+    return ok, why.value.decode()
+
+
+def codegen_kernels_compiled():
+    return int(L.lib.sdb_codegen_kernels_compiled())
+
+
 class Engine:
-    def __init__(self, device_budget_bytes=0, segment_size=0, fusion=True, device=0):
+    """One device's tables and executor.
+
+    fusion:  each KernelBundle runs as one fused kernel (True) or one kernel
+             per operator (False, the reference's own mode).
+    codegen: how a fused kernel is produced -- "auto" (compiled at run time
+             for the program's shape when hipRTC loads, else interpreted),
+             "interpret" or "compile".
+    unroll:  4-row chunks per thread per iteration of a compiled kernel
+             (1, 2, 4; 0 = default).
+    """
+
+    def __init__(self, device_budget_bytes=0, segment_size=0, fusion=True, device=0, codegen="auto", unroll=0):
         opts = L.EngineOptions(device_budget_bytes, segment_size, 0 if fusion else 1, device)
         h = ctypes.c_void_p()
         L.check(L.lib.sdb_engine_create(ctypes.byref(opts), ctypes.byref(h)))
         self._h = h
         self.segment_size = segment_size or (1 << 28)
         self.fusion = fusion
+        try:
+            self.set_codegen(codegen, unroll)
+        except Exception:
+            self.close()
+            raise
+
+    def set_codegen(self, mode="auto", unroll=0):
+        if mode not in CODEGEN_MODES:
+            raise ValueError("codegen must be one of %s" % sorted(CODEGEN_MODES))
+        L.check(L.lib.sdb_engine_set_codegen(self._h, CODEGEN_MODES[mode], int(unroll)))
+        self.codegen, self.unroll = mode, unroll
 
     def close(self):
         if self._h:

@@ -87,3 +87,19 @@ def test_one_hip_runtime_per_process(order):
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr[-2000:]
     assert out.stdout.split() == ["1", "1"], out.stdout
+
+
+def test_codegen_compiles_every_instruction_form():
+    """The generator's source for programs covering every instruction form,
+    operand kind and chunk width compiles for gfx950 (hipRTC, no device)."""
+    ok, why = E.codegen_available()
+    if not ok:
+        pytest.skip("runtime compilation unavailable: " + why)
+    n = ctypes.c_int(0)
+    L.check(L.lib.sdb_codegen_check(b"gfx950", ctypes.byref(n)))
+    assert n.value == 5
+
+
+def test_codegen_mode_validation():
+    with pytest.raises(ValueError):
+        E.Engine.set_codegen(E.Engine.__new__(E.Engine), "bogus")

@@ -70,7 +70,9 @@ def test_fused_matches_oracle(fused, expected, idx):
 
 
 @pytest.mark.parametrize("kw", [dict(fusion=False), dict(segment_size=50_000), dict(segment_size=77_777),
-                                dict(segment_size=77_777, fusion=False)])
+                                dict(segment_size=77_777, fusion=False), dict(codegen="interpret"),
+                                dict(codegen="interpret", segment_size=77_777), dict(unroll=1), dict(unroll=4),
+                                dict(unroll=4, segment_size=77_777)])
 def test_unfused_and_segmented_match_oracle(expected, kw):
     print("engine", kw, flush=True)
     eng = make_engine(**kw)
@@ -80,6 +82,17 @@ def test_unfused_and_segmented_match_oracle(expected, kw):
             assert res.rows() == expected[idx], (kw, idx)
     finally:
         eng.close()
+
+
+def test_default_engine_compiles_fused_kernels(fused):
+    """With hipRTC present the default engine runs runtime-compiled kernels."""
+    from sycldb_amd import engine as E
+    ok, why = E.codegen_available()
+    assert ok, why
+    before = E.codegen_kernels_compiled()
+    fused.execute(PL.build("t4", mfgr=(1, 5)))
+    assert E.codegen_kernels_compiled() >= before
+    assert E.codegen_kernels_compiled() > 0
 
 
 def test_fusion_launches_fewer_kernels(fused):
