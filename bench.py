@@ -70,7 +70,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--fact-rows", type=int, default=0)
     ap.add_argument("--seed", type=int, default=45)
@@ -87,7 +87,7 @@ def main():
 
     import torch
     from sycldb_amd import plans as PL
-    from sycldb_amd.engine import Engine, codegen_kernels_compiled
+    from sycldb_amd.engine import Engine, codegen_kernels_compiled, codegen_tuning_report
 
     rows, builder, fact_cols = WORKLOADS[args.workload]
     rows = args.fact_rows or rows
@@ -101,6 +101,9 @@ def main():
     for _ in range(args.warmup):
         res = eng.execute(plan)
     groups = len(res.rows())
+    tuning = codegen_tuning_report()
+    if tuning:
+        log("autotuner:\n" + tuning)
     eng.set_profiling(True)
     eng.profile_reset()
     torch.cuda.synchronize()
@@ -162,7 +165,8 @@ def main():
         "config": {"workload": DESCRIPTIONS[args.workload], "fact_rows": rows, "plan": args.workload,
                    "device_ms_per_step_median": sorted(dev_ms)[len(dev_ms) // 2], "groups": groups,
                    "fusion": True, "codegen": args.codegen, "unroll": args.unroll,
-                   "kernels_compiled": codegen_kernels_compiled(), "unfused": unfused},
+                   "kernels_compiled": codegen_kernels_compiled(), "tuning": tuning.strip().splitlines(),
+                   "unfused": unfused},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

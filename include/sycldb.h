@@ -260,7 +260,9 @@ int sdb_engine_set_fusion(sdb_engine *eng, int on);
  *   SDB_CODEGEN_INTERPRET  the compiled-in program interpreter
  *   SDB_CODEGEN_COMPILE    compiled at run time; an error when unavailable
  * `unroll` = 4-row chunks per thread per loop iteration of a compiled kernel
- * (1, 2 or 4; 0 = the tuned default). */
+ * (1, 2 or 4), or 0: the autotuner times the candidate launch configurations
+ * (chunks per thread x blocks per CU) on the first large launches of each
+ * program shape and keeps the fastest. */
 enum sdb_codegen { SDB_CODEGEN_AUTO = 0, SDB_CODEGEN_INTERPRET = 1, SDB_CODEGEN_COMPILE = 2 };
 int sdb_engine_set_codegen(sdb_engine *eng, int mode, int unroll);
 /* 1 when runtime compilation is available, else 0 with the reason in `why`
@@ -268,6 +270,9 @@ int sdb_engine_set_codegen(sdb_engine *eng, int mode, int unroll);
 int sdb_codegen_available(char *why, int why_cap);
 /* Distinct fused kernels compiled at run time in this process. */
 int sdb_codegen_kernels_compiled(void);
+/* Autotuner state, one text line per program shape; returns the length
+ * (the text is truncated to cap - 1 bytes). */
+int sdb_codegen_tuning_report(char *buf, int cap);
 /* Compiles, without launching (no device needed), the runtime-generated
  * kernels of programs covering every instruction form for GPU architecture
  * `arch` (e.g. "gfx950"); *count = kernels compiled. */

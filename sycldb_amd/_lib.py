@@ -112,6 +112,7 @@ SIGNATURES = {
     "sdb_engine_set_codegen": (INT, [P, INT, INT]),
     "sdb_codegen_available": (INT, [CSTR, INT]),
     "sdb_codegen_kernels_compiled": (INT, []),
+    "sdb_codegen_tuning_report": (INT, [ctypes.c_char_p, INT]),
     "sdb_codegen_check": (INT, [CSTR, ctypes.POINTER(INT)]),
     "sdb_engine_set_profiling": (INT, [P, INT]),
     "sdb_engine_profile_reset": (INT, [P]),

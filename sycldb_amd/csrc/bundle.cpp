@@ -84,7 +84,7 @@ void KernelBundle::execute(const ExecContext &ctx) const
             Engine::ProfRec rec;
             if (eng.profile) eng.prof_begin("fused:" + (ctx.table ? *ctx.table : std::string("?")), kernels_.front().n, rec);
             const bool jit = eng.codegen == SDB_CODEGEN_COMPILE || (eng.codegen == SDB_CODEGEN_AUTO && jit_available());
-            if (jit) launch_fused_jit(p, kernels_.front().n, ctx.stream, eng.unroll ? eng.unroll : F_DEFAULT_UNROLL);
+            if (jit) launch_fused_jit(p, kernels_.front().n, ctx.stream, eng.unroll);
             else launch_fused(p, kernels_.front().n, ctx.stream);
             if (eng.profile) eng.prof_end(rec);
             eng.launches++;
@@ -188,7 +188,23 @@ struct Draft {
     float sel = 1.0f;   // fraction of rows a predicate keeps
     float cost = 1.0f;  // relative per-row cost
     bool store = false; // writes a value back to memory
+    const void *rank = nullptr; // FK_PROBE_BITS: rank dictionary (bits + values) that may be staged
+    uint32_t rank_bytes = 0;
 };
+
+// What an FK_PROBE_BITS instruction stages in LDS, if anything: its rank
+// dictionary (values in LDS too) or only its presence bitmap.
+struct Staging {
+    const void *src = nullptr;
+    uint32_t bytes = 0;
+    bool dict = false;
+};
+
+uint32_t bitmap_stage_bytes(const FInstr &I)
+{
+    const uint64_t len = (uint64_t)((int64_t)I.ht_max - I.ht_min + 1);
+    return (uint32_t)(((len + 31) / 32 + 3) / 4 * 16);
+}
 
 bool is_predicate(uint8_t k)
 {
@@ -312,35 +328,62 @@ void fold_search(std::vector<Draft> &d)
     }
 }
 
-// Bitmaps of FK_PROBE_BITS staged into LDS at kernel start: the smallest
-// first, each at most 32 KiB, together at most 48 KiB.  Decided before the
-// instructions are ordered, since a probe into LDS costs far less than one
-// through the caches.
-std::vector<const void *> choose_staged(const std::vector<Draft> &d)
+// Tables of FK_PROBE_BITS staged into LDS at kernel start, smallest first,
+// each at most 32 KiB, together at most 48 KiB: a probe's rank dictionary when
+// it has one (its values then come from LDS too), else its presence bitmap;
+// a probe whose dictionary does not fit may still stage its bitmap.  Decided
+// before the instructions are ordered, since a probe into LDS costs far less
+// than one through the caches.  Returns the staging of every instruction.
+std::vector<Staging> choose_staged(const std::vector<Draft> &d)
 {
-    std::vector<std::pair<uint32_t, const void *>> cand; // bytes, bitmap
+    struct Cand {
+        uint32_t bytes;
+        const void *src;
+        bool dict;
+    };
+    std::vector<Cand> cand;
+    auto add = [&](const void *src, uint32_t bytes, bool dict) {
+        for (const Cand &c : cand)
+            if (c.src == src) return;
+        cand.push_back({bytes, src, dict});
+    };
     for (const Draft &D : d) {
         const FInstr &I = D.I;
         if (I.kind != FK_PROBE_BITS) continue;
+        if (D.rank && I.vkind == FV_U16) add(D.rank, D.rank_bytes, true);
         const uint64_t len = (uint64_t)((int64_t)I.ht_max - I.ht_min + 1);
-        if (len > 32u * 1024u * 8u) continue;
-        const uint32_t bytes = (uint32_t)(((len + 31) / 32 + 3) / 4 * 16);
-        bool dup = false;
-        for (auto &c : cand) dup |= c.second == I.ht;
-        if (!dup) cand.emplace_back(bytes, I.ht);
+        if (len <= 32u * 1024u * 8u) add(I.ht, bitmap_stage_bytes(I), false);
     }
-    std::stable_sort(cand.begin(), cand.end(),
-                     [](const std::pair<uint32_t, const void *> &a, const std::pair<uint32_t, const void *> &b) {
-                         return a.first < b.first;
-                     });
-    std::vector<const void *> out;
+    std::stable_sort(cand.begin(), cand.end(), [](const Cand &a, const Cand &b) { return a.bytes < b.bytes; });
+    std::vector<const void *> chosen;
     uint32_t staged = 0;
-    for (auto &c : cand) {
-        if ((int)out.size() >= F_MAX_BLOBS || staged + c.first > 48u * 1024u) continue;
-        out.push_back(c.second);
-        staged += c.first;
+    int nblobs = 0;
+    for (const Cand &c : cand) {
+        if (c.bytes > 32u * 1024u || nblobs >= F_MAX_BLOBS || staged + c.bytes > 48u * 1024u) continue;
+        chosen.push_back(c.src);
+        staged += c.bytes;
+        nblobs++;
+    }
+    auto has = [&](const void *p) { return p && std::find(chosen.begin(), chosen.end(), p) != chosen.end(); };
+    std::vector<Staging> out(d.size());
+    for (size_t i = 0; i < d.size(); i++) {
+        const Draft &D = d[i];
+        if (D.I.kind != FK_PROBE_BITS) continue;
+        if (D.rank && D.I.vkind == FV_U16 && has(D.rank)) out[i] = {D.rank, D.rank_bytes, true};
+        else if (has(D.I.ht)) out[i] = {D.I.ht, bitmap_stage_bytes(D.I), false};
     }
     return out;
+}
+
+// SDB_GROUPBY_HASH=0 sends group-by rows whose dense slot range is too large
+// for an LDS array straight to the global arrays instead of an LDS hash.
+bool hash_groupby_enabled()
+{
+    static const bool v = [] {
+        const char *e = getenv("SDB_GROUPBY_HASH");
+        return !(e && *e == '0');
+    }();
+    return v;
 }
 
 // Columns the kernel prefetches.  A lazily loaded column costs the whole
@@ -542,6 +585,10 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
                 I.vkind = full ? (int8_t)k.vkind : (int8_t)FV_NONE;
                 I.vals = full ? k.vals : nullptr;
                 I.vmin = k.vmin;
+                if (full && k.rank) {
+                    D.rank = k.rank;
+                    D.rank_bytes = k.rank_bytes;
+                }
                 if (full && k.vkind == FV_NONE) return false;
                 D.cost = 1.5f;
             } else {
@@ -610,6 +657,7 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
             // (dense for small ranges, hashed otherwise) with an exact flush
             fits = fits && exact == g.prod && g.prod < 0xFFFFFFFFull;
             G.lds = !fits ? FG_GLOBAL : (g.prod <= F_DENSE_MAX ? FG_DENSE_LDS : FG_HASH_LDS);
+            if (G.lds == FG_HASH_LDS && !hash_groupby_enabled()) G.lds = FG_GLOBAL;
             terminal_seen = true;
             break;
         }
@@ -634,11 +682,17 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
             if (writes_flag(I.kind)) written[I.fout] = true;
         }
     }
-    // probe costs follow where the bitmap will live
-    const std::vector<const void *> staged = choose_staged(d);
-    for (Draft &D : d)
-        if (D.I.kind == FK_PROBE_BITS)
-            D.cost = std::find(staged.begin(), staged.end(), D.I.ht) != staged.end() ? 0.6f : 2.0f;
+    // probe costs follow where the tables will live
+    {
+        const std::vector<Staging> st = choose_staged(d);
+        for (size_t i = 0; i < d.size(); i++) {
+            Draft &D = d[i];
+            if (D.I.kind != FK_PROBE_BITS) continue;
+            const bool lds_bits = st[i].src != nullptr;
+            const bool lds_vals = D.I.vkind == FV_NONE || st[i].dict;
+            D.cost = lds_bits ? (lds_vals ? 0.6f : 1.5f) : 2.0f;
+        }
+    }
     // reorder conjunctive runs
     {
         const size_t m = d.size();
@@ -758,24 +812,32 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
         return at;
     };
     p.nvals = (int)vptr.size();
-    // staged bitmaps (chosen above)
-    for (int i = 0; i < p.ninstr; i++) {
-        FInstr &I = p.ins[i];
-        if (I.kind != FK_PROBE_BITS || std::find(staged.begin(), staged.end(), I.ht) == staged.end()) continue;
-        int blob = -1;
-        for (int b = 0; b < p.nblobs; b++)
-            if (p.blob_src[b] == I.ht) blob = b;
-        if (blob < 0) {
-            if (p.nblobs >= F_MAX_BLOBS) return false;
-            const uint64_t len = (uint64_t)((int64_t)I.ht_max - I.ht_min + 1);
-            const uint32_t bytes = (uint32_t)(((len + 31) / 32 + 3) / 4 * 16);
-            blob = p.nblobs++;
-            p.blob_src[blob] = I.ht;
-            p.blob_bytes[blob] = bytes;
-            p.blob_off[blob] = take(bytes);
+    // staged tables (chosen again on the final order: the choice depends only
+    // on the set of probes, not on their order)
+    {
+        const std::vector<Staging> st = choose_staged(d);
+        for (int i = 0; i < p.ninstr; i++) {
+            FInstr &I = p.ins[i];
+            if (!st[i].src) continue;
+            int blob = -1;
+            for (int b = 0; b < p.nblobs; b++)
+                if (p.blob_src[b] == st[i].src) blob = b;
+            if (blob < 0) {
+                if (p.nblobs >= F_MAX_BLOBS) return false;
+                blob = p.nblobs++;
+                p.blob_src[blob] = st[i].src;
+                p.blob_bytes[blob] = st[i].bytes;
+                p.blob_off[blob] = take(st[i].bytes);
+            }
+            I.in_lds = 1;
+            I.lds_off = p.blob_off[blob];
+            I.lds_bytes = st[i].bytes;
+            if (st[i].dict) {
+                I.vkind = FV_RANK16;
+                I.vals = nullptr;
+                I.ht_len = (int32_t)((((int64_t)I.ht_max - I.ht_min + 1) + 31) / 32);
+            }
         }
-        I.in_lds = 1;
-        I.lds_off = p.blob_off[blob];
     }
     if (p.gb.lds == FG_DENSE_LDS) {
         p.gb.lds_off = take((uint32_t)(p.gb.prod * 8 + (p.gb.prod + 31) / 32 * 4));

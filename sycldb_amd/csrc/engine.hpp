@@ -51,6 +51,9 @@ uint64_t groupby_lds_max_slots();
 // uint16 value offsets from a key-value pair table; counts set bits into
 // *count and sets *bad when a value does not fit 16 bits.
 void launch_bits_from_keys(const uint8_t *ht, int32_t len, uint32_t *bits, uint32_t *count, hipStream_t s);
+// Rank dictionary (FV_RANK16 layout) of `words` presence words and their
+// per-slot 16-bit values.
+void launch_rank_dict(const uint32_t *bits, int32_t words, const uint16_t *vals, uint8_t *dict, hipStream_t s);
 void launch_bits_from_pairs(const int32_t *ht, int32_t len, uint32_t *bits, uint16_t *vals, int32_t vmin,
                             uint32_t *count, uint32_t *bad, hipStream_t s);
 
@@ -180,6 +183,8 @@ struct KernelData {
     const uint32_t *bits = nullptr;
     const void *vals = nullptr;
     int vkind = 0;       // FValKind
+    const void *rank = nullptr; // rank dictionary of bits + vals (FV_RANK16 layout), if small
+    uint32_t rank_bytes = 0;
     int32_t vmin = 0;
     float sel = 1.0f;    // estimated fraction of rows a predicate keeps
 };

@@ -364,6 +364,13 @@ int sdb_codegen_available(char *why, int why_cap)
 
 int sdb_codegen_kernels_compiled(void) { return jit_kernels_compiled(); }
 
+int sdb_codegen_tuning_report(char *buf, int cap)
+{
+    const std::string r = jit_tuning_report();
+    if (buf && cap > 0) snprintf(buf, (size_t)cap, "%s", r.c_str());
+    return (int)r.size();
+}
+
 int sdb_codegen_check(const char *arch, int *count)
 {
     return guarded([&] {

@@ -280,10 +280,20 @@ __device__ __forceinline__ void run_chunk(const FProgram &P, int64_t row0, int64
                     for (int k = 0; k < R; k++) {
                         r[k] = 0;
                         if ((out >> k) & 1u) {
-                            if (I.vkind == FV_U16)
+                            if (I.vkind == FV_U16) {
                                 r[k] = (int32_t)((uint32_t)I.vmin + reinterpret_cast<const uint16_t *>(I.vals)[slot[k]]);
-                            else
+                            } else if (I.vkind == FV_RANK16) {
+                                // rank dictionary in LDS: prefix count of the word + set bits below
+                                const uint8_t *dict = L.base + I.lds_off;
+                                const uint32_t w = slot[k] >> 5;
+                                const uint32_t rank =
+                                    reinterpret_cast<const uint16_t *>(dict + rank_prefix_off((uint32_t)I.ht_len))[w] +
+                                    __popc(bits[w] & ((1u << (slot[k] & 31)) - 1u));
+                                r[k] = (int32_t)((uint32_t)I.vmin + reinterpret_cast<const uint16_t *>(
+                                                                        dict + rank_values_off((uint32_t)I.ht_len))[rank]);
+                            } else {
                                 r[k] = reinterpret_cast<const int2 *>(I.vals)[slot[k]].y;
+                            }
                         }
                     }
                     vset<R>(V, I.vout, r, out);

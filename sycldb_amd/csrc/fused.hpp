@@ -13,8 +13,8 @@ namespace sdb {
 void launch_fused(const FProgram &p, int64_t n, hipStream_t s);
 
 
-// 4-row chunks per thread per loop iteration of a compiled kernel when the
-// engine does not choose.
+// 4-row chunks per thread per loop iteration of a compiled kernel for
+// launches too small to tune.
 constexpr int F_DEFAULT_UNROLL = 2;
 
 // Runtime-compiled fused kernels.  jit_available() is false when the hipRTC
@@ -23,8 +23,12 @@ bool jit_available();
 std::string jit_unavailable_reason();
 // Launches the program through a kernel compiled for its shape (compiled on
 // first use and cached per device); `unroll` = 4-row chunks per thread per
-// iteration (1, 2 or 4).
+// iteration (1, 2 or 4), or 0 to let the autotuner choose the launch
+// configuration per program shape.
 void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll);
+// One line per tuned program shape: the chosen configuration and the best
+// measured ns/row of every candidate.
+std::string jit_tuning_report();
 // Generated source for the program shape (exposed for tests and debugging).
 std::string jit_source(const FProgram &p, int unroll);
 // Number of distinct kernels compiled so far in this process.

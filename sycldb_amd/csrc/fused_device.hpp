@@ -64,8 +64,10 @@ __device__ __forceinline__ void gb_flush_slot(const FGroup &g, uint32_t s, unsig
 // One flagged row: slot = (sum_j (key_j - min_j) * prod_{l<j} range_l) as
 // int32, then % prod in uint64.  Rows whose keys are all in range go to the
 // block's LDS copy (dense, or an open-addressed hash of slots); others, and
-// rows meeting a crowded hash, update the global arrays directly.
-__device__ __forceinline__ void gb_row(const FGroup &g, uint8_t *lds, const int32_t *kv, int32_t agg)
+// rows meeting a crowded hash, update the global arrays directly.  MODE is
+// the FGroupMode (fixed per program, so only its path is compiled).
+template <int MODE>
+__device__ __forceinline__ void gb_row_mode(const FGroup &g, uint8_t *lds, const int32_t *kv, int32_t agg)
 {
     bool in_range = true;
     uint32_t hash = 0, mult = 1;
@@ -78,40 +80,56 @@ __device__ __forceinline__ void gb_row(const FGroup &g, uint8_t *lds, const int3
     }
     const uint64_t s64 = (uint64_t)(int64_t)(int32_t)hash % g.prod;
     const unsigned long long v = (unsigned long long)(int64_t)agg;
-    if (in_range && g.lds == FG_DENSE_LDS) {
-        const uint32_t s = (uint32_t)s64;
-        unsigned long long *lsum = reinterpret_cast<unsigned long long *>(lds + g.lds_off);
-        uint32_t *lbits = reinterpret_cast<uint32_t *>(lsum + g.prod);
-        atomicAdd(&lsum[s], v);
-        atomicOr(&lbits[s >> 5], 1u << (s & 31));
-        return;
-    }
-    if (in_range && g.lds == FG_HASH_LDS) {
-        const uint32_t s = (uint32_t)s64;
-        uint32_t *hk = reinterpret_cast<uint32_t *>(lds + g.lds_off);
-        unsigned long long *hv = reinterpret_cast<unsigned long long *>(hk + g.hash_cap);
-        const uint32_t mask = g.hash_cap - 1;
-        uint32_t h = (s * 2654435761u) & mask;
-        for (int probe = 0; probe < 32; probe++) {
-            uint32_t cur = hk[h];
-            if (cur == HASH_EMPTY) {
-                cur = atomicCAS(&hk[h], HASH_EMPTY, s);
-                if (cur == HASH_EMPTY) cur = s;
-            }
-            if (cur == s) {
-                atomicAdd(&hv[h], v);
-                return;
-            }
-            h = (h + 1) & mask;
+    if constexpr (MODE == FG_DENSE_LDS) {
+        if (in_range) {
+            const uint32_t s = (uint32_t)s64;
+            unsigned long long *lsum = reinterpret_cast<unsigned long long *>(lds + g.lds_off);
+            uint32_t *lbits = reinterpret_cast<uint32_t *>(lsum + g.prod);
+            atomicAdd(&lsum[s], v);
+            atomicOr(&lbits[s >> 5], 1u << (s & 31));
+            return;
         }
-        // table crowded: fall through to the global update
     }
-    if (atomicExch(&g.seen[s64], 1u) == 0u) {
+    if constexpr (MODE == FG_HASH_LDS) {
+        if (in_range) {
+            const uint32_t s = (uint32_t)s64;
+            uint32_t *hk = reinterpret_cast<uint32_t *>(lds + g.lds_off);
+            unsigned long long *hv = reinterpret_cast<unsigned long long *>(hk + g.hash_cap);
+            const uint32_t mask = g.hash_cap - 1;
+            uint32_t h = (s * 2654435761u) & mask;
+#pragma unroll 1
+            for (int probe = 0; probe < 32; probe++) {
+                uint32_t cur = hk[h];
+                if (cur == HASH_EMPTY) {
+                    cur = atomicCAS(&hk[h], HASH_EMPTY, s);
+                    if (cur == HASH_EMPTY) cur = s;
+                }
+                if (cur == s) {
+                    atomicAdd(&hv[h], v);
+                    return;
+                }
+                h = (h + 1) & mask;
+            }
+            // table crowded: fall through to the global update
+        }
+    }
+    // a slot already marked needs no exchange (a plain read; a stale 0 just
+    // leads to the exchange, which decides)
+    if (__hip_atomic_load(&g.seen[s64], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u &&
+        atomicExch(&g.seen[s64], 1u) == 0u) {
 #pragma unroll
         for (int j = 0; j < F_MAX_KEYS; j++)
             if (j < g.k) g.out_keys[j][s64] = kv[j];
     }
     atomicAdd(reinterpret_cast<unsigned long long *>(&g.sums[s64]), v);
+}
+
+// The same with the mode read from the program.
+__device__ __forceinline__ void gb_row(const FGroup &g, uint8_t *lds, const int32_t *kv, int32_t agg)
+{
+    if (g.lds == FG_DENSE_LDS) gb_row_mode<FG_DENSE_LDS>(g, lds, kv, agg);
+    else if (g.lds == FG_HASH_LDS) gb_row_mode<FG_HASH_LDS>(g, lds, kv, agg);
+    else gb_row_mode<FG_GLOBAL>(g, lds, kv, agg);
 }
 
 // Block start: stage probe bitmaps into LDS and clear the aggregation area.

@@ -78,6 +78,13 @@ template <int R> __device__ __forceinline__ void store_rows(int32_t *dst, int64_
         if ((m >> k) & 1u) dst[row0 + k] = x[k];
 }
 
+// N row values held as one vector value: element b (b per lane, not
+// necessarily uniform) is then read with a register extract -- an indexed
+// access to an array would put the array in scratch memory.
+template <int N> struct Lanes {
+    typedef int32_t type __attribute__((ext_vector_type(N)));
+};
+
 // rows a selection with logical op LOGIC has to evaluate
 template <int LOGIC, int R> __device__ __forceinline__ uint32_t sel_need(uint32_t in)
 {
@@ -134,62 +141,116 @@ __device__ __forceinline__ void binop(const int32_t (&x)[R], const int32_t (&y)[
     for (int k = 0; k < R; k++) r[k] = element_op(OP, x[k], y[k]);
 }
 
-// FilterJoinKernel (kernels/join.hpp:129-160): 1-byte table
-template <int R>
-__device__ __forceinline__ uint32_t probe_semi(const int32_t (&x)[R], uint32_t in, const uint8_t *ht, int32_t mn,
-                                               int32_t mx)
+// Two-phase probes.  *_issue reads, for every flagged row whose key is in
+// range, what the probe needs from the table (the presence word or table
+// entry and, for value tables, the value, loaded together rather than one
+// after the other); *_finish forms the result from it.  Code between the two
+// phases (the next tile's column loads) is issued before the probe's loads
+// are waited for.
+template <int R> struct ProbeTmp {
+    uint32_t word[R];
+    int32_t val[R];
+};
+
+// A semi or full probe through the presence bitmap of the reference table
+// (bit set = the table's match flag; FilterJoinKernel / FullJoinKernel,
+// kernels/join.hpp:129-160, 256-300), with the matched value, when wanted,
+// from a 16-bit offset table (FV_U16), the reference pairs (FV_PAIR) or a
+// rank dictionary in LDS (FV_RANK16: `bits` is the dictionary, `words` its
+// word count, and the issue phase reads the word's prefix count).
+template <int R, int VKIND>
+__device__ __forceinline__ void probe_bits_issue(const int32_t (&x)[R], uint32_t in, const uint32_t *bits, int32_t mn,
+                                                 int32_t mx, const void *vals, int32_t vmin, int32_t words,
+                                                 ProbeTmp<R> &t)
 {
-    uint32_t out = 0;
 #pragma unroll
-    for (int k = 0; k < R; k++)
-        if (((in >> k) & 1u) && x[k] >= mn && x[k] <= mx && ht[slot_of(x[k], mn)] != 0) out |= 1u << k;
-    return out;
+    for (int k = 0; k < R; k++) {
+        t.word[k] = 0;
+        t.val[k] = 0;
+        if (((in >> k) & 1u) && x[k] >= mn && x[k] <= mx) {
+            const uint32_t s = slot_of(x[k], mn);
+            t.word[k] = bits[s >> 5];
+            if constexpr (VKIND == FV_U16)
+                t.val[k] = (int32_t)((uint32_t)vmin + reinterpret_cast<const uint16_t *>(vals)[s]);
+            else if constexpr (VKIND == FV_PAIR)
+                t.val[k] = reinterpret_cast<const int2 *>(vals)[s].y;
+            else if constexpr (VKIND == FV_RANK16)
+                t.val[k] = reinterpret_cast<const uint16_t *>(reinterpret_cast<const uint8_t *>(bits) +
+                                                              rank_prefix_off((uint32_t)words))[s >> 5];
+        }
+    }
 }
 
-// FullJoinKernel (kernels/join.hpp:256-300): (flag, value) pairs; matched
-// rows take the value
-template <int R>
-__device__ __forceinline__ uint32_t probe_full(const int32_t (&x)[R], uint32_t in, const int2 *ht, int32_t mn,
-                                               int32_t mx, int32_t (&v)[R])
+template <int R, int VKIND>
+__device__ __forceinline__ uint32_t probe_bits_finish(const int32_t (&x)[R], uint32_t in, int32_t mn, int32_t mx,
+                                                      const ProbeTmp<R> &t, const uint32_t *bits, int32_t vmin,
+                                                      int32_t words, int32_t (&v)[R])
 {
     uint32_t out = 0;
 #pragma unroll
     for (int k = 0; k < R; k++) {
-        if (((in >> k) & 1u) && x[k] >= mn && x[k] <= mx) {
-            const int2 e = ht[slot_of(x[k], mn)];
-            if (e.x == 1) {
-                v[k] = e.y;
-                out |= 1u << k;
+        const uint32_t s = slot_of(x[k], mn);
+        if (((in >> k) & 1u) && x[k] >= mn && x[k] <= mx && ((t.word[k] >> (s & 31)) & 1u)) {
+            out |= 1u << k;
+            if constexpr (VKIND == FV_RANK16) {
+                const uint32_t rank = (uint32_t)t.val[k] + __popc(t.word[k] & ((1u << (s & 31)) - 1u));
+                v[k] = (int32_t)((uint32_t)vmin + reinterpret_cast<const uint16_t *>(
+                                                      reinterpret_cast<const uint8_t *>(bits) +
+                                                      rank_values_off((uint32_t)words))[rank]);
+            } else if constexpr (VKIND != FV_NONE) {
+                v[k] = t.val[k];
             }
         }
     }
     return out;
 }
 
-// A semi or full probe through the presence bitmap of the reference table
-// (bit set = the table's match flag), with the matched value, when wanted,
-// from a 16-bit offset table (FV_U16) or the reference pairs (FV_PAIR).
-template <int R, int VKIND>
-__device__ __forceinline__ uint32_t probe_bits(const int32_t (&x)[R], uint32_t in, const uint32_t *bits,
-                                               int32_t mn, int32_t mx, const void *vals, int32_t vmin,
-                                               int32_t (&v)[R])
+template <int R>
+__device__ __forceinline__ void probe_semi_issue(const int32_t (&x)[R], uint32_t in, const uint8_t *ht, int32_t mn,
+                                                 int32_t mx, ProbeTmp<R> &t)
 {
-    uint32_t out = 0, slot[R];
 #pragma unroll
     for (int k = 0; k < R; k++) {
-        slot[k] = slot_of(x[k], mn);
-        if (((in >> k) & 1u) && x[k] >= mn && x[k] <= mx && ((bits[slot[k] >> 5] >> (slot[k] & 31)) & 1u))
-            out |= 1u << k;
+        t.word[k] = 0;
+        if (((in >> k) & 1u) && x[k] >= mn && x[k] <= mx) t.word[k] = ht[slot_of(x[k], mn)] != 0;
     }
-    if constexpr (VKIND != FV_NONE) {
+}
+
+template <int R>
+__device__ __forceinline__ uint32_t probe_semi_finish(uint32_t in, const ProbeTmp<R> &t)
+{
+    uint32_t out = 0;
 #pragma unroll
-        for (int k = 0; k < R; k++) {
-            if ((out >> k) & 1u) {
-                if constexpr (VKIND == FV_U16)
-                    v[k] = (int32_t)((uint32_t)vmin + reinterpret_cast<const uint16_t *>(vals)[slot[k]]);
-                else
-                    v[k] = reinterpret_cast<const int2 *>(vals)[slot[k]].y;
-            }
+    for (int k = 0; k < R; k++)
+        if (((in >> k) & 1u) && t.word[k]) out |= 1u << k;
+    return out;
+}
+
+template <int R>
+__device__ __forceinline__ void probe_full_issue(const int32_t (&x)[R], uint32_t in, const int2 *ht, int32_t mn,
+                                                 int32_t mx, ProbeTmp<R> &t)
+{
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        t.word[k] = 0;
+        t.val[k] = 0;
+        if (((in >> k) & 1u) && x[k] >= mn && x[k] <= mx) {
+            const int2 e = ht[slot_of(x[k], mn)];
+            t.word[k] = e.x == 1;
+            t.val[k] = e.y;
+        }
+    }
+}
+
+template <int R>
+__device__ __forceinline__ uint32_t probe_full_finish(uint32_t in, const ProbeTmp<R> &t, int32_t (&v)[R])
+{
+    uint32_t out = 0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        if (((in >> k) & 1u) && t.word[k]) {
+            out |= 1u << k;
+            v[k] = t.val[k];
         }
     }
     return out;

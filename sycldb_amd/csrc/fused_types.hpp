@@ -54,7 +54,18 @@ enum FKind : uint8_t {
 };
 
 // value table of FK_PROBE_BITS
-enum FValKind : uint8_t { FV_NONE = 0, FV_U16 = 1, FV_PAIR = 2 };
+//   FV_U16     16-bit offsets from vmin, one per slot (global memory)
+//   FV_PAIR    the reference (flag, value) pairs (global memory)
+//   FV_RANK16  a rank dictionary staged in LDS at lds_off: the presence words
+//              (u32 x ht_len words), the number of set bits before each word
+//              (u16 x ht_len, padded to 4 bytes) and the values of the set
+//              slots in slot order (u16 offsets from vmin); the value of a set
+//              slot is values[prefix[w] + popcount(word & lower bits)]
+enum FValKind : uint8_t { FV_NONE = 0, FV_U16 = 1, FV_PAIR = 2, FV_RANK16 = 3 };
+
+// byte offsets of the parts of a rank dictionary of `words` presence words
+__host__ __device__ constexpr uint32_t rank_prefix_off(uint32_t words) { return 4u * words; }
+__host__ __device__ constexpr uint32_t rank_values_off(uint32_t words) { return 4u * words + (2u * words + 3u) / 4u * 4u; }
 
 struct FInstr {
     uint8_t kind, cmp, logic, op;
@@ -67,8 +78,9 @@ struct FInstr {
     int32_t *store;            // write-back of vout (masked as the functor writes), or null
     const void *ht;            // reference-layout table; FK_PROBE_BITS: bitmap
     const void *vals;          // FK_PROBE_BITS value table (FV_U16: offsets from vmin; FV_PAIR: ht)
-    int32_t ht_min, ht_max, ht_len, vmin;
-    uint32_t lds_off, pad;     // byte offset of the staged bitmap in dynamic LDS
+    int32_t ht_min, ht_max, ht_len, vmin; // FV_RANK16: ht_len = presence words
+    uint32_t lds_off;          // byte offset of the staged bitmap / dictionary in dynamic LDS
+    uint32_t lds_bytes;        // its size
 };
 
 enum FGroupMode : int { FG_GLOBAL = 0, FG_DENSE_LDS = 1, FG_HASH_LDS = 2 };
@@ -121,6 +133,9 @@ struct FProgram {
     FGroup gb;                 // FK_GROUPBY description
     FInstr ins[F_MAX_INSTR];
 };
+
+// Largest rank dictionary built for LDS staging.
+constexpr uint32_t F_RANK_DICT_MAX = 32 * 1024;
 
 // Dynamic LDS one fused block may use (registers, staged bitmaps, group-by).
 constexpr uint32_t F_LDS_BUDGET = 96 * 1024;

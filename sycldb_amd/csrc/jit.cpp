@@ -134,13 +134,108 @@ struct Gen {
         return "load_rows<R>(" + mem + ", row0[u], " + dst + ");";
     }
 
-    void instr(int i)
+    std::vector<bool> hoisted;
+    std::string hoist_code;
+
+    // Load phase of probe i (operand rows into x<i>, table reads into t<i>)
+    // for the rows of flag mask `gate`.
+    std::string probe_issue(int i, const std::string &gate) const
+    {
+        const FInstr &I = p.ins[i];
+        const std::string P = "P.ins[" + std::to_string(i) + "]";
+        const std::string x = "x" + std::to_string(i) + "[u]", t = "t" + std::to_string(i) + "[u]";
+        const std::string U = "#pragma unroll\n        for (int u = 0; u < UU; u++) ";
+        std::ostringstream a;
+        if (cheap(I.va)) a << U << "{ " << operand(I.va, P + ".a", x) << " }\n";
+        else a << U << "if (" << gate << ") { " << operand(I.va, P + ".a", x) << " }\n";
+        a << U << "if (" << gate << ") ";
+        if (I.kind == FK_PROBE_SEMI)
+            a << "probe_semi_issue<R>(" << x << ", " << gate << ", (const uint8_t *)" << P << ".ht, " << P << ".ht_min, "
+              << P << ".ht_max, " << t << ");\n";
+        else if (I.kind == FK_PROBE_FULL)
+            a << "probe_full_issue<R>(" << x << ", " << gate << ", (const int2 *)" << P << ".ht, " << P << ".ht_min, "
+              << P << ".ht_max, " << t << ");\n";
+        else {
+            const std::string bits =
+                I.in_lds ? "(const uint32_t *)(lds + " + P + ".lds_off)" : "(const uint32_t *)" + P + ".ht";
+            a << "probe_bits_issue<R, " << (int)I.vkind << ">(" << x << ", " << gate << ", " << bits << ", " << P
+              << ".ht_min, " << P << ".ht_max, " << P << ".vals, " << P << ".vmin, " << P << ".ht_len, " << t
+              << ");\n";
+        }
+        return a.str();
+    }
+
+    static bool is_probe(uint8_t k) { return k == FK_PROBE_SEMI || k == FK_PROBE_FULL || k == FK_PROBE_BITS; }
+
+    static bool writes_flags(uint8_t k)
+    {
+        return k == FK_SEL_LIT || k == FK_SEL_COL || k == FK_SEL_RANGE || k == FK_SEL_IN2 || k == FK_LOGICAL ||
+               is_probe(k);
+    }
+
+    static bool conjunctive(const FInstr &I)
+    {
+        return is_probe(I.kind) || (writes_flags(I.kind) && I.logic == SDB_AND);
+    }
+
+    // Probes after the first instruction that waits for global memory (fw)
+    // whose global reads can be issued together with fw's: keyed by a
+    // prefetched column, and reached from fw only through conjunctive
+    // instructions on fw's flag register F (so the rows they will probe are
+    // a subset of F's rows at fw, which gates their early reads).
+    void plan_hoisting(int fw)
+    {
+        hoisted.assign(p.ninstr, false);
+        if (fw < 0) return;
+        const FInstr &W = p.ins[fw];
+        const int F = W.fin;
+        if (!(conjunctive(W) && W.fout == F)) return;
+        for (int i = fw + 1; i < p.ninstr; i++) {
+            const FInstr &I = p.ins[i];
+            if (is_probe(I.kind) && I.fin == F && I.va >= F_PF_BASE && waits(i)) {
+                hoisted[i] = true;
+                hoist_code += probe_issue(i, freg(F) + "[u]");
+            }
+            if (writes_flags(I.kind) && !(conjunctive(I) && I.fin == F && I.fout == F)) break;
+            if (I.kind == FK_LOGICAL && I.floc == F) break;
+        }
+    }
+
+    // Whether instruction i waits for global memory: a column operand loaded
+    // now, or a table outside LDS.
+    bool waits(int i) const
+    {
+        const FInstr &I = p.ins[i];
+        switch (I.kind) {
+        case FK_LOGICAL:
+        case FK_FILL: return false;
+        case FK_PROBE_SEMI:
+        case FK_PROBE_FULL: return true;
+        case FK_PROBE_BITS:
+            if (!I.in_lds || I.vkind != FV_NONE) return true;
+            break;
+        case FK_GROUPBY: {
+            for (int j = 0; j < p.gb.k; j++)
+                if (p.gb.key_reg[j] < 0) return true;
+            return p.gb.agg_reg < 0;
+        }
+        default: break;
+        }
+        const bool has_b = I.kind == FK_SEL_COL || I.kind == FK_BINOP_CC || I.kind == FK_BINOP_LC ||
+                           I.kind == FK_BUILD_KEYVALS;
+        const bool has_a = I.kind != FK_BINOP_LC;
+        return (has_a && I.va < 0) || (has_b && I.vb < 0);
+    }
+
+    // Emits instruction i as { declarations; load phase; [split]; compute }.
+    void instr(int i, const std::string &split)
     {
         const FInstr &I = p.ins[i];
         const std::string P = "P.ins[" + std::to_string(i) + "]";
         const std::string fin = freg(I.fin) + "[u]", fout = freg(I.fout) + "[u]";
         const std::string L = std::to_string((int)I.logic);
-        o << "    { // " << i << "\n";
+        const std::string U = "#pragma unroll\n        for (int u = 0; u < UU; u++) ";
+        std::ostringstream d, a, c; // declarations, load phase, compute phase
         switch (I.kind) {
         case FK_SEL_LIT:
         case FK_SEL_COL:
@@ -153,125 +248,119 @@ struct Gen {
             else if (I.kind == FK_SEL_COL) pred = "pred_col<R, " + C + ">(x[u], y[u])";
             else if (I.kind == FK_SEL_RANGE) pred = "pred_range<R>(x[u], " + P + ".lit, " + P + ".lit2)";
             else pred = "pred_in2<R>(x[u], " + P + ".lit, " + P + ".lit2)";
-            o << "        int32_t x[UU][R];" << (two ? " int32_t y[UU][R];" : "") << "\n";
+            d << "        int32_t x[UU][R];" << (two ? " int32_t y[UU][R];" : "") << "\n";
             const bool lazy = !cheap(I.va) || (two && !cheap(I.vb));
             if (lazy) {
-                o << "        uint32_t need[UU];\n";
-                o << "#pragma unroll\n        for (int u = 0; u < UU; u++) {\n";
-                o << "            need[u] = sel_need<" << L << ", R>(" << fin << ");\n";
-                o << "            if (need[u]) { " << operand(I.va, P + ".a", "x[u]");
-                if (two) o << " " << operand(I.vb, P + ".b", "y[u]");
-                o << " }\n        }\n";
-                o << "#pragma unroll\n        for (int u = 0; u < UU; u++) " << fout << " = combine<" << L << ">(" << fin
-                  << ", need[u] ? " << pred << " : 0u);\n";
+                d << "        uint32_t need[UU];\n";
+                a << U << "{\n            need[u] = sel_need<" << L << ", R>(" << fin << ");\n";
+                a << "            if (need[u]) { " << operand(I.va, P + ".a", "x[u]");
+                if (two) a << " " << operand(I.vb, P + ".b", "y[u]");
+                a << " }\n        }\n";
+                c << U << fout << " = combine<" << L << ">(" << fin << ", need[u] ? " << pred << " : 0u);\n";
             } else {
-                o << "#pragma unroll\n        for (int u = 0; u < UU; u++) { " << operand(I.va, P + ".a", "x[u]");
-                if (two) o << " " << operand(I.vb, P + ".b", "y[u]");
-                o << " }\n";
-                o << "#pragma unroll\n        for (int u = 0; u < UU; u++) " << fout << " = combine<" << L << ">(" << fin
-                  << ", " << pred << ");\n";
+                a << U << "{ " << operand(I.va, P + ".a", "x[u]");
+                if (two) a << " " << operand(I.vb, P + ".b", "y[u]");
+                a << " }\n";
+                c << U << fout << " = combine<" << L << ">(" << fin << ", " << pred << ");\n";
             }
             break;
         }
         case FK_LOGICAL:
-            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) " << fout << " = combine<" << L << ">(" << fin
-              << ", " << freg(I.floc) << "[u]);\n";
+            c << U << fout << " = combine<" << L << ">(" << fin << ", " << freg(I.floc) << "[u]);\n";
             break;
         case FK_FILL:
-            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) {\n";
-            o << "            fill_rows<R>(" << P << ".lit, " << vreg(I.vout) << "[u]);\n";
-            if (I.store) o << "            store_rows<R>(" << P << ".store, row0[u], " << vreg(I.vout) << "[u], M);\n";
-            o << "        }\n";
+            c << U << "{\n            fill_rows<R>(" << P << ".lit, " << vreg(I.vout) << "[u]);\n";
+            if (I.store) c << "            store_rows<R>(" << P << ".store, row0[u], " << vreg(I.vout) << "[u], M);\n";
+            c << "        }\n";
             break;
         case FK_BINOP_CC:
         case FK_BINOP_LC:
         case FK_BINOP_CL: {
-            o << "        int32_t x[UU][R], y[UU][R];\n";
-            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) {\n            if (!" << fin << ") continue;\n";
-            o << "            "
+            d << "        int32_t x[UU][R], y[UU][R];\n";
+            a << U << "{\n            if (!" << fin << ") continue;\n";
+            a << "            "
               << (I.kind == FK_BINOP_LC ? "fill_rows<R>(" + P + ".lit, x[u]);" : operand(I.va, P + ".a", "x[u]"))
               << "\n";
-            o << "            "
+            a << "            "
               << (I.kind == FK_BINOP_CL ? "fill_rows<R>(" + P + ".lit, y[u]);" : operand(I.vb, P + ".b", "y[u]"))
               << "\n        }\n";
-            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) {\n            if (!" << fin << ") continue;\n";
-            o << "            int32_t r[R];\n            binop<R, " << (int)I.op << ">(x[u], y[u], r);\n";
-            o << "            set_rows<R>(" << vreg(I.vout) << "[u], r, " << fin << ");\n";
-            if (I.store) o << "            store_rows<R>(" << P << ".store, row0[u], r, " << fin << ");\n";
-            o << "        }\n";
+            c << U << "{\n            if (!" << fin << ") continue;\n";
+            c << "            int32_t r[R];\n            binop<R, " << (int)I.op << ">(x[u], y[u], r);\n";
+            c << "            set_rows<R>(" << vreg(I.vout) << "[u], r, " << fin << ");\n";
+            if (I.store) c << "            store_rows<R>(" << P << ".store, row0[u], r, " << fin << ");\n";
+            c << "        }\n";
             break;
         }
         case FK_PROBE_SEMI:
         case FK_PROBE_FULL:
         case FK_PROBE_BITS: {
-            o << "        int32_t x[UU][R];\n";
-            if (cheap(I.va))
-                o << "#pragma unroll\n        for (int u = 0; u < UU; u++) { " << operand(I.va, P + ".a", "x[u]")
-                  << " }\n";
-            else
-                o << "#pragma unroll\n        for (int u = 0; u < UU; u++) if (" << fin << ") { "
-                  << operand(I.va, P + ".a", "x[u]") << " }\n";
+            if (!hoisted[i]) a << probe_issue(i, fin);
+            const std::string x = "x" + std::to_string(i) + "[u]", t = "t" + std::to_string(i) + "[u]";
             const bool vout = I.kind == FK_PROBE_FULL || (I.kind == FK_PROBE_BITS && I.vkind != FV_NONE);
-            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) {\n            uint32_t o = 0;\n";
-            if (!vout) o << "            int32_t none[R];\n";
             const std::string vdst = vout ? vreg(I.vout) + "[u]" : "none";
-            o << "            if (" << fin << ") {\n";
-            if (I.kind == FK_PROBE_SEMI) {
-                o << "                o = probe_semi<R>(x[u], " << fin << ", (const uint8_t *)" << P << ".ht, " << P
-                  << ".ht_min, " << P << ".ht_max);\n";
-            } else if (I.kind == FK_PROBE_FULL) {
-                o << "                o = probe_full<R>(x[u], " << fin << ", (const int2 *)" << P << ".ht, " << P
-                  << ".ht_min, " << P << ".ht_max, " << vdst << ");\n";
-            } else {
-                const std::string bits = I.in_lds ? "(const uint32_t *)(lds + " + P + ".lds_off)"
-                                                  : "(const uint32_t *)" + P + ".ht";
-                o << "                o = probe_bits<R, " << (int)I.vkind << ">(x[u], " << fin << ", " << bits << ", "
-                  << P << ".ht_min, " << P << ".ht_max, " << P << ".vals, " << P << ".vmin, " << vdst << ");\n";
+            c << U << "{\n            uint32_t o = 0;\n";
+            if (!vout) c << "            int32_t none[R];\n";
+            c << "            if (" << fin << ") {\n";
+            if (I.kind == FK_PROBE_SEMI) c << "                o = probe_semi_finish<R>(" << fin << ", " << t << ");\n";
+            else if (I.kind == FK_PROBE_FULL)
+                c << "                o = probe_full_finish<R>(" << fin << ", " << t << ", " << vdst << ");\n";
+            else {
+                const std::string bits =
+                    I.in_lds ? "(const uint32_t *)(lds + " + P + ".lds_off)" : "(const uint32_t *)" + P + ".ht";
+                c << "                o = probe_bits_finish<R, " << (int)I.vkind << ">(" << x << ", " << fin << ", " << P
+                  << ".ht_min, " << P << ".ht_max, " << t << ", " << bits << ", " << P << ".vmin, " << P
+                  << ".ht_len, " << vdst << ");\n";
             }
             if (vout && I.store)
-                o << "                if (o) store_rows<R>(" << P << ".store, row0[u], " << vdst << ", o);\n";
-            o << "            }\n            " << fout << " = o;\n        }\n";
+                c << "                if (o) store_rows<R>(" << P << ".store, row0[u], " << vdst << ", o);\n";
+            c << "            }\n            " << fout << " = o;\n        }\n";
             break;
         }
         case FK_BUILD_KEYS:
-            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) {\n            int32_t x[R];\n            "
-              << operand(I.va, P + ".a", "x") << "\n";
-            o << "            build_keys<R>(x, " << fin << ", (uint8_t *)" << P << ".ht, " << P << ".ht_min, " << P
-              << ".ht_len);\n        }\n";
+            d << "        int32_t x[UU][R];\n";
+            a << U << "{ " << operand(I.va, P + ".a", "x[u]") << " }\n";
+            c << U << "build_keys<R>(x[u], " << fin << ", (uint8_t *)" << P << ".ht, " << P << ".ht_min, " << P
+              << ".ht_len);\n";
             break;
         case FK_BUILD_KEYVALS:
-            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) {\n            int32_t x[R], y[R];\n            "
-              << operand(I.va, P + ".a", "x") << "\n";
-            o << "            if (" << fin << ") { " << operand(I.vb, P + ".b", "y") << " } else fill_rows<R>(0, y);\n";
-            o << "            build_keyvals<R>(x, y, " << fin << ", (int32_t *)" << P << ".ht, " << P << ".ht_min, " << P
-              << ".ht_len);\n        }\n";
+            d << "        int32_t x[UU][R], y[UU][R];\n";
+            a << U << "{\n            " << operand(I.va, P + ".a", "x[u]") << "\n";
+            a << "            if (" << fin << ") { " << operand(I.vb, P + ".b", "y[u]")
+              << " } else fill_rows<R>(0, y[u]);\n        }\n";
+            c << U << "build_keyvals<R>(x[u], y[u], " << fin << ", (int32_t *)" << P << ".ht, " << P << ".ht_min, "
+              << P << ".ht_len);\n";
             break;
         case FK_SUM:
-            o << "        int32_t x[UU][R];\n";
-            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) if (" << fin << ") { "
-              << operand(I.va, P + ".a", "x[u]") << " }\n";
-            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) if (" << fin << ") sum_rows<R>(x[u], " << fin
-              << ", acc);\n";
+            d << "        int32_t x[UU][R];\n";
+            a << U << "if (" << fin << ") { " << operand(I.va, P + ".a", "x[u]") << " }\n";
+            c << U << "if (" << fin << ") sum_rows<R>(x[u], " << fin << ", acc);\n";
             break;
         case FK_GROUPBY: {
+            // operands of every chunk first; then one loop over the flagged
+            // rows of all chunks (bit u * R + k), so the row update is
+            // compiled once
             const FGroup &g = p.gb;
-            o << "#pragma unroll\n        for (int u = 0; u < UU; u++) {\n            if (!" << fin << ") continue;\n";
-            o << "            int32_t kx[" << g.k << "][R], ax[R];\n";
+            d << "        typename Lanes<UU * R>::type ax = {}";
+            for (int j = 0; j < g.k; j++) d << ", kx" << j << " = {}";
+            d << ";\n        uint32_t rows = 0;\n";
+            a << U << "{\n            rows |= " << fin << " << (u * R);\n            if (!" << fin << ") continue;\n";
+            a << "            int32_t t[R];\n";
             for (int j = 0; j < g.k; j++)
-                o << "            "
-                  << operand(g.key_reg[j], "P.gb.key_mem[" + std::to_string(j) + "]", "kx[" + std::to_string(j) + "]")
-                  << "\n";
-            o << "            " << operand(g.agg_reg, "P.gb.agg_mem", "ax") << "\n";
-            o << "#pragma unroll\n            for (int k = 0; k < R; k++) {\n";
-            o << "                if (!((" << fin << " >> k) & 1u)) continue;\n";
-            o << "                const int32_t kv[F_MAX_KEYS] = {";
-            for (int j = 0; j < F_MAX_KEYS; j++) o << (j ? ", " : "") << (j < g.k ? "kx[" + std::to_string(j) + "][k]" : "0");
-            o << "};\n                fdev::gb_row(P.gb, lds, kv, ax[k]);\n            }\n        }\n";
+                a << "            " << operand(g.key_reg[j], "P.gb.key_mem[" + std::to_string(j) + "]", "t")
+                  << "\n#pragma unroll\n            for (int k = 0; k < R; k++) kx" << j << "[u * R + k] = t[k];\n";
+            a << "            " << operand(g.agg_reg, "P.gb.agg_mem", "t")
+              << "\n#pragma unroll\n            for (int k = 0; k < R; k++) ax[u * R + k] = t[k];\n        }\n";
+            c << "#pragma unroll 1\n        while (rows) {\n            const int b = __builtin_ctz(rows);\n";
+            c << "            rows &= rows - 1;\n            const int32_t kv[F_MAX_KEYS] = {";
+            for (int j = 0; j < F_MAX_KEYS; j++) c << (j ? ", " : "") << (j < g.k ? "kx" + std::to_string(j) + "[b]" : "0");
+            c << "};\n";
+            c << "            fdev::gb_row_mode<" << g.lds << ">(P.gb, lds, kv, ax[b]);\n        }\n";
             break;
         }
         default: fail(SDB_ERR_RUNTIME, "jit: unknown instruction kind");
         }
-        o << "    }\n";
+        o << "    { // " << i << "\n" << d.str() << a.str() << (split.empty() ? "" : hoist_code + split) << c.str()
+          << "    }\n";
     }
 
     std::string body()
@@ -298,7 +387,26 @@ struct Gen {
             else o << "0u;\n";
         }
         for (int r = 0; r < p.nvals; r++) o << "    int32_t " << vreg(r) << "[UU][R] = {};\n";
-        for (int i = 0; i < p.ninstr; i++) instr(i);
+        for (int i = 0; i < p.ninstr; i++)
+            if (is_probe(p.ins[i].kind))
+                o << "    int32_t x" << i << "[UU][R];\n    ProbeTmp<R> t" << i << "[UU];\n";
+        // the next tile's column loads go out after the loads of the first
+        // instruction that waits for global memory and before that wait, so
+        // the in-order completion counter does not make it wait for them too
+        const std::string prefetch = "        if (has_next) pf_load<R, UU>(P, nrow0, next);\n";
+        int first_wait = -1;
+        for (int i = 0; i < p.ninstr && first_wait < 0; i++)
+            if (waits(i)) first_wait = i;
+        // early issue of chained probes trades extra table reads for one
+        // fewer wait; measured slower on the star joins, so opt-in
+        // (SDB_JIT_HOIST=1)
+        static const bool hoist = [] {
+            const char *e = getenv("SDB_JIT_HOIST");
+            return e && *e == '1';
+        }();
+        plan_hoisting(hoist ? first_wait : -1);
+        if (first_wait < 0) o << prefetch;
+        for (int i = 0; i < p.ninstr; i++) instr(i, i == first_wait ? prefetch : std::string());
         for (int j = 0; j < F_MAX_FLAGS; j++)
             if (p.flag_store[j])
                 o << "#pragma unroll\n    for (int u = 0; u < UU; u++) fdev::flags_out<R>(P.flag_store[" << j
@@ -327,7 +435,7 @@ __device__ __forceinline__ void pf_load(const FProgram &P, const int64_t (&row0)
 }
 template <int R, int UU>
 __device__ __forceinline__ void body(const FProgram &P, const int64_t (&row0)[UU], const Pf<R, UU> &pf, int64_t &acc,
-                                     uint8_t *lds)
+                                     uint8_t *lds, bool has_next, const int64_t (&nrow0)[UU], Pf<R, UU> &next)
 {
 )";
 
@@ -361,28 +469,24 @@ extern "C" __global__ void __launch_bounds__(256) sdb_fused_program(const FProgr
     }
     while (t < ntiles) {
         const Pf<4, U> cur = next;
-        int64_t r[U];
+        int64_t r[U], rn[U];
         rows_of(t, r);
         const int64_t tn = t + gridDim.x;
-        if (tn < ntiles) {
-            int64_t rn[U];
-            rows_of(tn, rn);
-            pf_load<4, U>(P, rn, next);
-        }
-        body<4, U>(P, r, cur, acc, lds);
+        rows_of(tn, rn);
+        body<4, U>(P, r, cur, acc, lds, tn < ntiles, rn, next);
         t = tn;
     }
     for (int64_t c = ntiles * tile + (int64_t)blockIdx.x * BLOCK + threadIdx.x; c < nch; c += (int64_t)gridDim.x * BLOCK) {
         const int64_t r[1] = {c << 2};
-        Pf<4, 1> pf;
+        Pf<4, 1> pf, unused;
         pf_load<4, 1>(P, r, pf);
-        body<4, 1>(P, r, pf, acc, lds);
+        body<4, 1>(P, r, pf, acc, lds, false, r, unused);
     }
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x < (n & 3)) {
         const int64_t r[1] = {(nch << 2) + threadIdx.x};
-        Pf<1, 1> pf;
+        Pf<1, 1> pf, unused;
         pf_load<1, 1>(P, r, pf);
-        body<1, 1>(P, r, pf, acc, lds);
+        body<1, 1>(P, r, pf, acc, lds, false, r, unused);
     }
     } else {
     const int64_t stride = (int64_t)gridDim.x * BLOCK;
@@ -396,11 +500,8 @@ extern "C" __global__ void __launch_bounds__(256) sdb_fused_program(const FProgr
         const Pf<1, 1> cur = next;
         const int64_t r[1] = {i};
         const int64_t in = i + stride;
-        if (in < n) {
-            const int64_t rn[1] = {in};
-            pf_load<1, 1>(P, rn, next);
-        }
-        body<1, 1>(P, r, cur, acc, lds);
+        const int64_t rn[1] = {in};
+        body<1, 1>(P, r, cur, acc, lds, in < n, rn, next);
         i = in;
     }
     }
@@ -577,6 +678,28 @@ std::vector<std::pair<FProgram, int>> coverage_programs()
         ins(p, mk(FK_BUILD_KEYVALS, 0, 0, F_PF_BASE, -1));
         out.emplace_back(p, 1);
     }
+    {   // star join: LDS-bitmap semi probe, global and LDS probes with 16-bit
+        // value tables, two-key group-by hashed in LDS
+        FProgram p = base(1, 4);
+        p.nvals = 2;
+        p.flag_init[0] = FF_TRUE;
+        FInstr a = mk(FK_PROBE_BITS, 0, 1, F_PF_BASE);
+        a.in_lds = 1;
+        ins(p, a);
+        FInstr b = mk(FK_PROBE_BITS, 1, 1, F_PF_BASE + 1, -1, 0);
+        b.vkind = FV_U16;
+        ins(p, b);
+        FInstr c = mk(FK_PROBE_BITS, 1, 1, F_PF_BASE + 2, -1, 1);
+        c.in_lds = 1;
+        c.vkind = FV_RANK16;
+        ins(p, c);
+        ins(p, mk(FK_GROUPBY, 1, 0, -1));
+        p.gb.k = 2;
+        p.gb.lds = FG_HASH_LDS;
+        p.gb.key_reg[0] = 1, p.gb.key_reg[1] = 0;
+        p.gb.agg_reg = F_PF_BASE + 3;
+        out.emplace_back(p, 2);
+    }
     {   // one-key group-by over a prefetched key, memory aggregate
         FProgram p = base(1, 1);
         p.flag_init[0] = FF_TRUE;
@@ -641,11 +764,13 @@ std::string jit_source(const FProgram &p, int unroll)
     return s.str();
 }
 
-void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll)
+namespace {
+
+// Launches the compiled kernel of `src` with `per_cu_mult` times the blocks
+// per CU the occupancy calculator allows.
+void launch_compiled(const FProgram &p, int64_t n, hipStream_t s, int unroll, int per_cu_mult, const std::string &src)
 {
-    if (n <= 0) return;
-    if (p.magic != F_MAGIC) fail(SDB_ERR_RUNTIME, "fused program without magic");
-    Compiled &k = get_kernel(jit_source(p, unroll));
+    Compiled &k = get_kernel(src);
     // the value-register file (the last LDS area) belongs to the interpreter
     const uint32_t lds = p.vreg_off;
     int per_cu;
@@ -657,7 +782,7 @@ void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll)
             SDB_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&b, k.fn, JIT_BLOCK, lds));
             it = k.blocks_per_cu.emplace(lds, std::max(1, b)).first;
         }
-        per_cu = it->second;
+        per_cu = it->second * per_cu_mult;
     }
     const int64_t units = p.vec ? (n + 3) / 4 : n;
     const unsigned grid = stream_grid(units, (int64_t)JIT_BLOCK * (p.vec ? unroll : 1), per_cu);
@@ -668,6 +793,116 @@ void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll)
     size_t size = sizeof(args);
     void *config[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &args, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size, HIP_LAUNCH_PARAM_END};
     SDB_HIP(hipModuleLaunchKernel(k.fn, grid, 1, 1, JIT_BLOCK, 1, 1, lds, s, nullptr, config));
+}
+
+// ------------------------------------------------------------- autotuner
+//
+// Per program shape (and device), the first launches over at least
+// TUNE_MIN_ROWS rows each try one launch configuration -- chunks per thread
+// per iteration x blocks per CU -- timed with HIP events; after TUNE_ROUNDS
+// timings of every candidate the fastest per row is used from then on.
+// Every configuration computes the same result; only the time differs.
+constexpr int64_t TUNE_MIN_ROWS = 4 << 20;
+constexpr int TUNE_ROUNDS = 2;
+
+struct TuneCfg {
+    int unroll, per_cu_mult;
+};
+const TuneCfg TUNE_CANDIDATES[] = {{2, 1}, {1, 1}, {4, 1}, {2, 2}, {1, 2}};
+constexpr int N_CAND = (int)(sizeof(TUNE_CANDIDATES) / sizeof(TUNE_CANDIDATES[0]));
+
+struct Tune {
+    double best_ns_per_row[N_CAND];
+    int trials = 0;
+    int chosen = -1;
+    Tune()
+    {
+        for (double &b : best_ns_per_row) b = 1e300;
+    }
+};
+
+std::mutex g_tune_mu;
+std::map<std::pair<int, std::string>, Tune> g_tune;
+
+} // namespace
+
+void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll)
+{
+    if (n <= 0) return;
+    if (p.magic != F_MAGIC) fail(SDB_ERR_RUNTIME, "fused program without magic");
+    if (unroll != 0) {
+        launch_compiled(p, n, s, unroll, 1, jit_source(p, unroll));
+        return;
+    }
+    if (n < TUNE_MIN_ROWS) {
+        launch_compiled(p, n, s, F_DEFAULT_UNROLL, 1, jit_source(p, F_DEFAULT_UNROLL));
+        return;
+    }
+    int dev = 0;
+    SDB_HIP(hipGetDevice(&dev));
+    const std::pair<int, std::string> key{dev, jit_source(p, 1)};
+    int cand;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        Tune &t = g_tune[key];
+        cand = t.chosen >= 0 ? -1 - t.chosen : t.trials % N_CAND;
+    }
+    if (cand < 0) {
+        const TuneCfg &c = TUNE_CANDIDATES[-1 - cand];
+        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, jit_source(p, c.unroll));
+        return;
+    }
+    const TuneCfg &c = TUNE_CANDIDATES[cand];
+    const std::string src = jit_source(p, c.unroll);
+    get_kernel(src); // compile outside the timed region
+    hipEvent_t a, b;
+    SDB_HIP(hipEventCreate(&a));
+    SDB_HIP(hipEventCreate(&b));
+    float ms = 0;
+    hipError_t e = hipEventRecord(a, s);
+    if (e == hipSuccess) {
+        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, src);
+        e = hipEventRecord(b, s);
+    }
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    hip_check(e, "fused kernel tuning");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    Tune &t = g_tune[key];
+    if (t.chosen >= 0) return;
+    t.best_ns_per_row[cand] = std::min(t.best_ns_per_row[cand], ms * 1e6 / (double)n);
+    if (++t.trials >= N_CAND * TUNE_ROUNDS) {
+        int best = 0;
+        for (int i = 1; i < N_CAND; i++)
+            if (t.best_ns_per_row[i] < t.best_ns_per_row[best]) best = i;
+        t.chosen = best;
+    }
+}
+
+std::string jit_tuning_report()
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    std::ostringstream o;
+    int shape = 0;
+    for (auto &kv : g_tune) {
+        const Tune &t = kv.second;
+        o << "shape " << shape++ << " device " << kv.first.first << ": ";
+        if (t.chosen >= 0)
+            o << "unroll " << TUNE_CANDIDATES[t.chosen].unroll << ", blocks/CU x"
+              << TUNE_CANDIDATES[t.chosen].per_cu_mult;
+        else
+            o << "tuning (" << t.trials << " trials)";
+        o << "; ns/row";
+        for (int i = 0; i < N_CAND; i++) {
+            o << " [u" << TUNE_CANDIDATES[i].unroll << " x" << TUNE_CANDIDATES[i].per_cu_mult << "] ";
+            if (t.best_ns_per_row[i] < 1e299) o << t.best_ns_per_row[i];
+            else o << "-";
+        }
+        o << "\n";
+    }
+    return o.str();
 }
 
 } // namespace sdb

@@ -10,6 +10,7 @@
 // Pointers that are not 16-byte (columns) or 4-byte (flags) aligned take a
 // one-row-per-thread path instead.
 #include "capi_util.hpp"
+#include "fused_types.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -606,6 +607,35 @@ __global__ void __launch_bounds__(BLOCK) k_bits_from_pairs(const int2 *ht, int32
     if (!ok) atomicOr(bad, 1u);
 }
 
+// Rank dictionary of a presence bitmap with per-slot 16-bit values (the
+// layout FV_RANK16 describes in fused_types.hpp): one block; each thread owns
+// a run of consecutive words, the block scans the run totals, then every
+// thread writes its words, their prefix counts and the values of their set
+// slots in slot order.
+__global__ void __launch_bounds__(BLOCK) k_rank_dict(const uint32_t *bits, int32_t words, const uint16_t *vals,
+                                                     uint8_t *dict)
+{
+    __shared__ uint32_t run_total[BLOCK];
+    const int32_t per = (words + BLOCK - 1) / BLOCK;
+    const int32_t w0 = min(words, (int32_t)threadIdx.x * per), w1 = min(words, w0 + per);
+    uint32_t local = 0;
+    for (int32_t w = w0; w < w1; w++) local += __popc(bits[w]);
+    run_total[threadIdx.x] = local;
+    __syncthreads();
+    // exclusive scan of BLOCK run totals (serial per thread over a small array)
+    uint32_t base = 0;
+    for (int t = 0; t < (int)threadIdx.x; t++) base += run_total[t];
+    uint32_t *out_bits = reinterpret_cast<uint32_t *>(dict);
+    uint16_t *out_prefix = reinterpret_cast<uint16_t *>(dict + rank_prefix_off((uint32_t)words));
+    uint16_t *out_vals = reinterpret_cast<uint16_t *>(dict + rank_values_off((uint32_t)words));
+    for (int32_t w = w0; w < w1; w++) {
+        const uint32_t m = bits[w];
+        out_bits[w] = m;
+        out_prefix[w] = (uint16_t)base;
+        for (uint32_t r = m; r; r &= r - 1) out_vals[base++] = vals[(int64_t)w * 32 + __builtin_ctz(r)];
+    }
+}
+
 // ------------------------------------------------------------ compaction
 // Three passes over tiles of COMPACT_TILE rows: count per tile, exclusive
 // scan of the counts (one block), then write ids in row order.
@@ -977,6 +1007,13 @@ void launch_bits_from_pairs(const int32_t *ht, int32_t len, uint32_t *bits, uint
     hipLaunchKernelGGL(k_bits_from_pairs, dim3(grid_one(words)), dim3(BLOCK), 0, s,
                        reinterpret_cast<const int2 *>(ht), len, bits, vals, vmin, count, bad);
     check_launch("bits_from_pairs");
+}
+
+void launch_rank_dict(const uint32_t *bits, int32_t words, const uint16_t *vals, uint8_t *dict, hipStream_t s)
+{
+    if (words <= 0) return;
+    hipLaunchKernelGGL(k_rank_dict, dim3(1), dim3(BLOCK), 0, s, bits, words, vals, dict);
+    check_launch("rank_dict");
 }
 
 void launch_compact(const uint8_t *flags, int64_t n, int32_t *row_ids, uint32_t *count, hipStream_t s)

@@ -518,6 +518,8 @@ struct CompactTable {
     const uint32_t *bits = nullptr;
     const void *vals = nullptr;
     int vkind = FV_NONE;
+    const void *rank = nullptr; // FV_RANK16 dictionary, when small enough for LDS
+    uint32_t rank_bytes = 0;
     float sel = 1.0f;
 };
 
@@ -548,6 +550,15 @@ static CompactTable build_compact(Engine &eng, const void *ht, bool pairs, int32
         if (narrow && host[1] == 0) {
             c.vals = vals;
             c.vkind = FV_U16;
+            // a rank dictionary small enough to stage in LDS
+            const uint32_t dict_bytes = (rank_values_off((uint32_t)words) + 2u * host[0] + 15u) / 16u * 16u;
+            if (host[0] <= 0xFFFFu && words <= 0xFFFFu && dict_bytes <= F_RANK_DICT_MAX) {
+                uint8_t *dict = eng.mm->alloc<uint8_t>(dict_bytes);
+                launch_rank_dict(bits, (int32_t)words, vals, dict, eng.stream);
+                eng.launches++;
+                c.rank = dict;
+                c.rank_bytes = dict_bytes;
+            }
         } else {
             c.vals = ht;
             c.vkind = FV_PAIR;
@@ -626,6 +637,8 @@ void TransientTable::apply_join(TransientTable &right, const RelNode &rel)
         k.bits = ct.bits;
         k.vals = ct.vals;
         k.vkind = ct.vkind;
+        k.rank = ct.rank;
+        k.rank_bytes = ct.rank_bytes;
         k.vmin = gb_mm.first;
         k.sel = ct.sel;
         nc.seg_min[s] = gb_mm.first; // models/models.hpp:1609-1610

@@ -209,9 +209,21 @@ void validate_program(const FProgram &p, int64_t n_rows, const AllocRegistry &re
             const uint64_t bytes = (len + 31) / 32 * 4;
             reg.check(I.ht, bytes, "probe bitmap");
             if (I.in_lds) lds_area(I.lds_off, bytes, "a probe bitmap");
-            if (I.vkind == FV_U16) reg.check(I.vals, 2 * len, "probe value table");
-            else if (I.vkind == FV_PAIR) reg.check(I.vals, 8 * len, "probe value pairs");
-            else if (I.vkind != FV_NONE) fail(SDB_ERR_RUNTIME, "launch check: value table kind");
+            if (I.vkind == FV_U16) {
+                reg.check(I.vals, 2 * len, "probe value table");
+            } else if (I.vkind == FV_PAIR) {
+                reg.check(I.vals, 8 * len, "probe value pairs");
+            } else if (I.vkind == FV_RANK16) {
+                // the dictionary: words, prefix counts and at least one value
+                // per set slot lie in its staged LDS area
+                if (!I.in_lds || (uint64_t)I.ht_len != (len + 31) / 32)
+                    fail(SDB_ERR_RUNTIME, "launch check: rank dictionary shape");
+                if (I.lds_bytes < rank_values_off((uint32_t)I.ht_len))
+                    fail(SDB_ERR_RUNTIME, "launch check: rank dictionary size");
+                lds_area(I.lds_off, I.lds_bytes, "a rank dictionary");
+            } else if (I.vkind != FV_NONE) {
+                fail(SDB_ERR_RUNTIME, "launch check: value table kind");
+            }
             if (I.vkind != FV_NONE) vreg(I.vout);
             if (I.store) reg.check(I.store, col, "join value store");
             fl(I.fout);

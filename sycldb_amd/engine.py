@@ -60,6 +60,14 @@ def codegen_kernels_compiled():
     return int(L.lib.sdb_codegen_kernels_compiled())
 
 
+def codegen_tuning_report():
+    """The autotuner's choice per program shape (text, one line each)."""
+    n = L.lib.sdb_codegen_tuning_report(None, 0)
+    buf = ctypes.create_string_buffer(n + 1)
+    L.lib.sdb_codegen_tuning_report(buf, n + 1)
+    return buf.value.decode()
+
+
 class Engine:
     """One device's tables and executor.
 
@@ -69,7 +77,7 @@ class Engine:
              for the program's shape when hipRTC loads, else interpreted),
              "interpret" or "compile".
     unroll:  4-row chunks per thread per iteration of a compiled kernel
-             (1, 2, 4; 0 = default).
+             (1, 2, 4), or 0: chosen per program shape by the autotuner.
     """
 
     def __init__(self, device_budget_bytes=0, segment_size=0, fusion=True, device=0, codegen="auto", unroll=0):

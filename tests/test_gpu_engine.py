@@ -141,3 +141,21 @@ def test_small_budget_raises_memory_error():
         e.generate(FACT_ROWS, SEED)
         with pytest.raises(MemoryError):
             e.execute(PL.build("t3"))
+
+
+def test_autotuned_launches_match_oracle():
+    """Launches over >= 4M rows go through the autotuner, which tries each
+    candidate configuration on successive executions: every one of them must
+    return the oracle's result, and a configuration is chosen at the end."""
+    from sycldb_amd import engine as E
+    rows = 4_500_000
+    tables = datagen.generate_tables(S, rows, seed=SEED + 1)
+    plan = PL.build("t3", cust=("region", 1), supp=("region", 1), date=("year_range", 1992, 1997), group="city")
+    want = PO.execute_plan(plan, tables).rows()
+    with E.Engine(unroll=0) as eng:
+        eng.generate(rows, SEED + 1)
+        for rep in range(12):
+            assert eng.execute(plan).rows() == want, rep
+    report = E.codegen_tuning_report()
+    print(report)
+    assert "unroll" in report
