@@ -23,6 +23,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+COPY_CAL_BYTES = 1 << 30
+# HBM traffic of the fused fact kernel per launch, from the committed
+# rocprofv3 counter summary of this workload (FETCH_SIZE + WRITE_SIZE scaled by
+# the copy-kernel calibration); reported as roofline.traffic
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "c3_q2x_600M_fused_fact_kernel.txt")
 
 WORKLOADS = {
     # name: (fact rows, plan builder, fact columns the plan reads)
@@ -42,6 +47,22 @@ DESCRIPTIONS = {
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def measured_traffic(workload, rows):
+    """HBM bytes per fused fact-kernel launch from the committed counter
+    summary, when it covers this workload at this size."""
+    if workload != "c3" or rows != WORKLOADS["c3"][0] or not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    vals = {}
+    for line in open(TRAFFIC_SUMMARY):
+        if ":" in line and line.split(":")[0].strip() in ("traffic_bytes_per_launch", "launch_rows"):
+            k, v = line.split(":", 1)
+            vals[k.strip()] = float(v.split()[0])
+    if "traffic_bytes_per_launch" not in vals:
+        return None
+    return {"bytes_per_launch": vals["traffic_bytes_per_launch"],
+            "source": os.path.relpath(TRAFFIC_SUMMARY, ROOT)}
 
 
 def cpu_baseline(plan, fact_cols, sample_rows, seed):
@@ -79,6 +100,9 @@ def main():
     ap.add_argument("--no-unfused", action="store_true")
     ap.add_argument("--codegen", default="auto", choices=["auto", "interpret", "compile"])
     ap.add_argument("--unroll", type=int, default=0, choices=[0, 1, 2, 4])
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 1 GiB three times (reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the fact table would be "
@@ -86,6 +110,13 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        x = torch.ones(COPY_CAL_BYTES // 4, dtype=torch.int32, device="cuda")
+        for _ in range(3):
+            y = x.clone()
+        torch.cuda.synchronize()
+        log("copied %d bytes three times" % (y.numel() * 4))
+        return
     from sycldb_amd import plans as PL
     from sycldb_amd.engine import Engine, codegen_kernels_compiled, codegen_tuning_report
 
@@ -123,9 +154,12 @@ def main():
     launches, krows, kms = prof.get("fused:lineorder", (0, 0, 0.0))
     bytes_per_row = 4 * len(fact_cols)
     achieved = (bytes_per_row * krows / (kms / 1000.0)) / 1e9 if kms > 0 else None
+    traffic = measured_traffic(args.workload, rows)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                "kernel": "k_fused (fused:lineorder bundle)", "launches_per_step": launches // max(args.steps, 1),
+                "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                "traffic": traffic["bytes_per_launch"] if traffic else None,
+                "traffic_source": traffic["source"] if traffic else None,
+                "kernel": "sdb_fused_program (fused lineorder bundle, compiled for its program shape)" if args.codegen != "interpret" else "k_fused (fused lineorder bundle, interpreted)", "launches_per_step": launches // max(args.steps, 1),
                 "avg_launch_ms": kms / launches if launches else None,
                 "algorithmic_bytes_per_row": bytes_per_row}
 

@@ -205,8 +205,10 @@ typedef struct sdb_engine_options {
     uint64_t device_budget_bytes; /* scratch budget of the device allocator
                                      (memory_manager; default 8 GiB,
                                      common.hpp:7) */
-    uint64_t segment_size;        /* rows per segment (default 2^28,
-                                     common.hpp:12) */
+    uint64_t segment_size;        /* rows per segment (default 2^30; the
+                                     reference's SEGMENT_SIZE is 2^28,
+                                     common.hpp:12; results do not depend on
+                                     it) */
     int disable_fusion;           /* 0 (default): each bundle runs as one
                                      fused kernel; 1: one kernel per operator,
                                      as the reference runs with

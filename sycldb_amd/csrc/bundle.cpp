@@ -653,9 +653,11 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
             G.sums = g.sums;
             G.seen = g.seen;
             G.prod = g.prod;
-            // in-range keys map one-to-one onto slots below 2^32: LDS copies
-            // (dense for small ranges, hashed otherwise) with an exact flush
-            fits = fits && exact == g.prod && g.prod < 0xFFFFFFFFull;
+            // in-range keys map one-to-one onto slots below prod <= 2^31 (so
+            // the reference's int32 slot is non-negative and needs no
+            // modulo): LDS copies (dense for small ranges, hashed otherwise)
+            // with an exact flush
+            fits = fits && exact == g.prod && g.prod <= 0x80000000ull;
             G.lds = !fits ? FG_GLOBAL : (g.prod <= F_DENSE_MAX ? FG_DENSE_LDS : FG_HASH_LDS);
             if (G.lds == FG_HASH_LDS && !hash_groupby_enabled()) G.lds = FG_GLOBAL;
             terminal_seen = true;

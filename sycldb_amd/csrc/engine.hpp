@@ -238,7 +238,10 @@ class KernelBundle {
 struct Engine {
     int device = 0;
     hipStream_t stream = nullptr;
-    uint64_t segment_size = 1ull << 28; // common.hpp:12
+    // rows per segment: the reference uses 2^28 (common.hpp:12); with 288 GB
+    // of HBM a segment here defaults to 2^30 rows, so a fact table of up to
+    // ~1.07e9 rows is one launch per bundle (results do not depend on it)
+    uint64_t segment_size = 1ull << 30;
     bool fusion = true;
     int codegen = SDB_CODEGEN_AUTO; // how fused kernels are produced
     int unroll = 0;                 // chunks per thread per iteration (0: default)

@@ -78,11 +78,12 @@ __device__ __forceinline__ void gb_row_mode(const FGroup &g, uint8_t *lds, const
         hash += ((uint32_t)kv[j] - (uint32_t)g.mins[j]) * mult;
         mult *= g.range[j];
     }
-    const uint64_t s64 = (uint64_t)(int64_t)(int32_t)hash % g.prod;
     const unsigned long long v = (unsigned long long)(int64_t)agg;
+    // LDS modes are chosen only when the key ranges multiply to exactly prod
+    // < 2^32, so an in-range row's slot is `hash` itself (no modulo needed)
     if constexpr (MODE == FG_DENSE_LDS) {
         if (in_range) {
-            const uint32_t s = (uint32_t)s64;
+            const uint32_t s = hash;
             unsigned long long *lsum = reinterpret_cast<unsigned long long *>(lds + g.lds_off);
             uint32_t *lbits = reinterpret_cast<uint32_t *>(lsum + g.prod);
             atomicAdd(&lsum[s], v);
@@ -92,7 +93,7 @@ __device__ __forceinline__ void gb_row_mode(const FGroup &g, uint8_t *lds, const
     }
     if constexpr (MODE == FG_HASH_LDS) {
         if (in_range) {
-            const uint32_t s = (uint32_t)s64;
+            const uint32_t s = hash;
             uint32_t *hk = reinterpret_cast<uint32_t *>(lds + g.lds_off);
             unsigned long long *hv = reinterpret_cast<unsigned long long *>(hk + g.hash_cap);
             const uint32_t mask = g.hash_cap - 1;
@@ -113,6 +114,7 @@ __device__ __forceinline__ void gb_row_mode(const FGroup &g, uint8_t *lds, const
             // table crowded: fall through to the global update
         }
     }
+    const uint64_t s64 = (uint64_t)(int64_t)(int32_t)hash % g.prod;
     // a slot already marked needs no exchange (a plain read; a stale 0 just
     // leads to the exchange, which decides)
     if (__hip_atomic_load(&g.seen[s64], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u &&

@@ -803,7 +803,7 @@ void launch_compiled(const FProgram &p, int64_t n, hipStream_t s, int unroll, in
 // timings of every candidate the fastest per row is used from then on.
 // Every configuration computes the same result; only the time differs.
 constexpr int64_t TUNE_MIN_ROWS = 4 << 20;
-constexpr int TUNE_ROUNDS = 2;
+constexpr int TUNE_ROUNDS = 1;
 
 struct TuneCfg {
     int unroll, per_cu_mult;

@@ -257,9 +257,10 @@ void validate_program(const FProgram &p, int64_t n_rows, const AllocRegistry &re
             reg.check(g.seen, 4 * g.prod, "group seen flags");
             if (g.lds == FG_DENSE_LDS) {
                 if (g.prod > F_DENSE_MAX) fail(SDB_ERR_RUNTIME, "launch check: dense LDS slots");
+                if (g.k > F_MAX_KEYS) fail(SDB_ERR_RUNTIME, "launch check: dense LDS keys");
                 lds_area(g.lds_off, g.prod * 8 + (g.prod + 31) / 32 * 4, "the dense aggregation");
             } else if (g.lds == FG_HASH_LDS) {
-                if (g.hash_cap == 0 || (g.hash_cap & (g.hash_cap - 1)) || g.prod >= 0xFFFFFFFFull)
+                if (g.hash_cap == 0 || (g.hash_cap & (g.hash_cap - 1)) || g.prod > 0x80000000ull)
                     fail(SDB_ERR_RUNTIME, "launch check: hash aggregation shape");
                 lds_area(g.lds_off, (uint64_t)g.hash_cap * 12, "the hash aggregation");
             } else if (g.lds != FG_GLOBAL) {

@@ -85,7 +85,7 @@ class Engine:
         h = ctypes.c_void_p()
         L.check(L.lib.sdb_engine_create(ctypes.byref(opts), ctypes.byref(h)))
         self._h = h
-        self.segment_size = segment_size or (1 << 28)
+        self.segment_size = segment_size or (1 << 30)
         self.fusion = fusion
         try:
             self.set_codegen(codegen, unroll)
