@@ -176,6 +176,19 @@ int sdb_gather_i32(const int32_t *src, const int32_t *row_ids, int64_t count, in
 int sdb_minmax_i32(const int32_t *col, int64_t n, int32_t *min_out, int32_t *max_out,
                    sdb_stream stream);
 
+/* ------------------------------------------- device memory and streams */
+
+/* Conveniences for callers without the HIP headers (an FFI, the C++ layer
+ * in sycldb.hpp); the reference's own allocations come from its
+ * memory_manager / sycl::malloc_* instead.  Copies are ordered on `stream`
+ * and complete before the call returns. */
+int sdb_device_alloc(void **ptr, uint64_t bytes);
+int sdb_device_free(void *ptr);
+int sdb_copy_to_device(void *dst, const void *host_src, uint64_t bytes, sdb_stream stream);
+int sdb_copy_to_host(void *host_dst, const void *src, uint64_t bytes, sdb_stream stream);
+/* Waits for the work queued on `stream` (sycl::event::wait_and_throw). */
+int sdb_stream_synchronize(sdb_stream stream);
+
 /* ------------------------------------------------ synthetic data (seeded) */
 
 /* out[i] = lo + U(seed, stream_id, first_row + i) over [lo, hi]

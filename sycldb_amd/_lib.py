@@ -108,6 +108,11 @@ SIGNATURES = {
     "sdb_engine_last_timing": (INT, [P, ctypes.POINTER(DBL), ctypes.POINTER(DBL)]),
     "sdb_engine_reset": (INT, [P]),
     "sdb_engine_last_launches": (INT, [P, ctypes.POINTER(I64)]),
+    "sdb_device_alloc": (INT, [ctypes.POINTER(P), U64]),
+    "sdb_device_free": (INT, [P]),
+    "sdb_copy_to_device": (INT, [P, P, U64, P]),
+    "sdb_copy_to_host": (INT, [P, P, U64, P]),
+    "sdb_stream_synchronize": (INT, [P]),
     "sdb_engine_set_fusion": (INT, [P, INT]),
     "sdb_engine_set_codegen": (INT, [P, INT, INT]),
     "sdb_codegen_available": (INT, [CSTR, INT]),

@@ -169,6 +169,53 @@ int sdb_gen_lookup(int32_t *out, int64_t n, uint64_t seed, uint64_t stream_id, i
     });
 }
 
+int sdb_device_alloc(void **ptr, uint64_t bytes)
+{
+    return guarded([&] {
+        require(ptr != nullptr, "null output pointer");
+        *ptr = nullptr;
+        if (bytes == 0) return;
+        const hipError_t e = hipMalloc(ptr, bytes);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            fail(SDB_ERR_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+        }
+        hip_check(e, "hipMalloc");
+    });
+}
+
+int sdb_device_free(void *ptr)
+{
+    return guarded([&] {
+        if (ptr) SDB_HIP(hipFree(ptr));
+    });
+}
+
+int sdb_copy_to_device(void *dst, const void *host_src, uint64_t bytes, sdb_stream stream)
+{
+    return guarded([&] {
+        if (bytes == 0) return;
+        require(dst && host_src, "null pointer");
+        SDB_HIP(hipMemcpyAsync(dst, host_src, bytes, hipMemcpyHostToDevice, as_stream(stream)));
+        SDB_HIP(hipStreamSynchronize(as_stream(stream)));
+    });
+}
+
+int sdb_copy_to_host(void *host_dst, const void *src, uint64_t bytes, sdb_stream stream)
+{
+    return guarded([&] {
+        if (bytes == 0) return;
+        require(host_dst && src, "null pointer");
+        SDB_HIP(hipMemcpyAsync(host_dst, src, bytes, hipMemcpyDeviceToHost, as_stream(stream)));
+        SDB_HIP(hipStreamSynchronize(as_stream(stream)));
+    });
+}
+
+int sdb_stream_synchronize(sdb_stream stream)
+{
+    return guarded([&] { SDB_HIP(hipStreamSynchronize(as_stream(stream))); });
+}
+
 int sdb_gen_rowid(int32_t *out, int64_t n, int64_t first_row, sdb_stream stream)
 {
     return guarded([&] {

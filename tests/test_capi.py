@@ -103,3 +103,30 @@ def test_codegen_compiles_every_instruction_form():
 def test_codegen_mode_validation():
     with pytest.raises(ValueError):
         E.Engine.set_codegen(E.Engine.__new__(E.Engine), "bogus")
+
+
+API_CHECK = os.path.join(ROOT, "sycldb_amd", "sdb_api_check")
+
+
+def test_cxx_layer_host_checks():
+    """include/sycldb.hpp (the reference-named C++ layer) compiles, links and
+    maps op strings and errors as the reference does (tests/cxx/api_check.cpp)."""
+    import subprocess
+    if not os.path.exists(API_CHECK):
+        pytest.skip("api check program not built")
+    out = subprocess.run([API_CHECK, "host"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip() == "ok"
+
+
+@pytest.mark.gpu
+def test_cxx_layer_device_pipeline():
+    """The reference-named C++ layer runs a selection / join / projection /
+    aggregation / group-by pipeline on the GPU and matches values computed on
+    the host."""
+    import subprocess
+    assert os.path.exists(API_CHECK), "api check program not built"
+    print("cxx layer device pipeline", flush=True)
+    out = subprocess.run([API_CHECK, "device"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip() == "ok"
