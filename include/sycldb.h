@@ -256,6 +256,14 @@ int sdb_engine_table_finalize(sdb_engine *eng, const char *name);
 /* Runs a plan (JSON mirror of calciteserver.thrift PlanResult) to
  * completion and returns its result table. */
 int sdb_engine_execute(sdb_engine *eng, const char *plan_json, sdb_result **out);
+/* Runs n plans as one batch (no counterpart in the reference): the
+ * dimension-side work of every plan runs as in sdb_engine_execute; the
+ * fact-table bundles of the plans that fuse run together in one kernel per
+ * segment that scans the union of their fact columns once; the other plans
+ * run alone afterwards.  out[i] receives plan i's result, identical to
+ * running it alone; *batched (may be NULL) = how many shared the scan. */
+int sdb_engine_execute_batch(sdb_engine *eng, const char *const *plans, int n, sdb_result **out, int *batched);
+
 /* Device time (ms, hipEvents) and host wall time (ms) of the last execute. */
 int sdb_engine_last_timing(sdb_engine *eng, double *device_ms, double *wall_ms);
 /* Resets the scratch allocator (memory_manager::reset, re-zeroing the

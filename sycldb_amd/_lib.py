@@ -113,6 +113,8 @@ SIGNATURES = {
     "sdb_copy_to_device": (INT, [P, P, U64, P]),
     "sdb_copy_to_host": (INT, [P, P, U64, P]),
     "sdb_stream_synchronize": (INT, [P]),
+    "sdb_engine_execute_batch": (INT, [P, ctypes.POINTER(ctypes.c_char_p), INT, ctypes.POINTER(P),
+                                       ctypes.POINTER(INT)]),
     "sdb_engine_set_fusion": (INT, [P, INT]),
     "sdb_engine_set_codegen": (INT, [P, INT, INT]),
     "sdb_codegen_available": (INT, [CSTR, INT]),

@@ -76,6 +76,19 @@ void KernelBundle::execute(const ExecContext &ctx) const
 {
     if (kernels_.empty()) return;
     Engine &eng = *ctx.eng;
+    if (eng.capture.active && ctx.table && *ctx.table == eng.capture.table) {
+        // batched execution: record the fused program of the fact-table bundle
+        FProgram p;
+        if (!build_program(ctx, p)) throw NotBatchable("a fact-table bundle does not fuse");
+        try {
+            validate_program(p, kernels_.front().n, eng.registry);
+        } catch (const Error &e) {
+            throw NotBatchable(std::string("a fact-table bundle touches a row-sized intermediate: ") + e.what());
+        }
+        dump_program(p, kernels_.front().n, *ctx.table + " (captured)");
+        eng.capture.bundles.push_back({p, kernels_.front().n, ctx.segment});
+        return;
+    }
     if (eng.fusion) {
         FProgram p;
         if (build_program(ctx, p)) {
@@ -334,7 +347,7 @@ void fold_search(std::vector<Draft> &d)
 // a probe whose dictionary does not fit may still stage its bitmap.  Decided
 // before the instructions are ordered, since a probe into LDS costs far less
 // than one through the caches.  Returns the staging of every instruction.
-std::vector<Staging> choose_staged(const std::vector<Draft> &d)
+std::vector<Staging> choose_staged(const std::vector<Draft> &d, uint32_t total_cap)
 {
     struct Cand {
         uint32_t bytes;
@@ -359,7 +372,7 @@ std::vector<Staging> choose_staged(const std::vector<Draft> &d)
     uint32_t staged = 0;
     int nblobs = 0;
     for (const Cand &c : cand) {
-        if (c.bytes > 32u * 1024u || nblobs >= F_MAX_BLOBS || staged + c.bytes > 48u * 1024u) continue;
+        if (c.bytes > 32u * 1024u || nblobs >= F_MAX_BLOBS || staged + c.bytes > total_cap) continue;
         chosen.push_back(c.src);
         staged += c.bytes;
         nblobs++;
@@ -473,6 +486,12 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
 {
     p = FProgram{};
     p.magic = F_MAGIC;
+    // a program captured for a batched launch shares the block's LDS with the
+    // other pipelines' programs: it gets the capture's budget
+    const Engine &eng = *ctx.eng;
+    const bool captured = eng.capture.active && ctx.table && *ctx.table == eng.capture.table;
+    const uint32_t lds_budget = captured ? eng.capture.lds_budget : F_LDS_BUDGET;
+    uint32_t group_area = 0;
     const int64_t n = kernels_.front().n;
     for (const KernelData &k : kernels_)
         if (k.n != n) return false;
@@ -660,6 +679,13 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
             fits = fits && exact == g.prod && g.prod <= 0x80000000ull;
             G.lds = !fits ? FG_GLOBAL : (g.prod <= F_DENSE_MAX ? FG_DENSE_LDS : FG_HASH_LDS);
             if (G.lds == FG_HASH_LDS && !hash_groupby_enabled()) G.lds = FG_GLOBAL;
+            group_area = G.lds == FG_DENSE_LDS ? (uint32_t)(g.prod * 8 + (g.prod + 31) / 32 * 4 + 16)
+                         : G.lds == FG_HASH_LDS ? F_HASH_CAP * 12u + 16u
+                                                : 0u;
+            if (group_area > lds_budget) {
+                G.lds = FG_GLOBAL;
+                group_area = 0;
+            }
             terminal_seen = true;
             break;
         }
@@ -684,9 +710,11 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
             if (writes_flag(I.kind)) written[I.fout] = true;
         }
     }
+    // LDS left for staged probe tables
+    const uint32_t stage_cap = std::min<uint32_t>(48u * 1024u, lds_budget - std::min(lds_budget, group_area));
     // probe costs follow where the tables will live
     {
-        const std::vector<Staging> st = choose_staged(d);
+        const std::vector<Staging> st = choose_staged(d, stage_cap);
         for (size_t i = 0; i < d.size(); i++) {
             Draft &D = d[i];
             if (D.I.kind != FK_PROBE_BITS) continue;
@@ -817,7 +845,7 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
     // staged tables (chosen again on the final order: the choice depends only
     // on the set of probes, not on their order)
     {
-        const std::vector<Staging> st = choose_staged(d);
+        const std::vector<Staging> st = choose_staged(d, stage_cap);
         for (int i = 0; i < p.ninstr; i++) {
             FInstr &I = p.ins[i];
             if (!st[i].src) continue;
@@ -853,6 +881,7 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
     p.vreg_off = take((uint32_t)p.nvals * 256u * 16u);
     p.lds_bytes = off;
     if (p.lds_bytes > F_LDS_BUDGET) return false;
+    if (captured && p.vreg_off > lds_budget) return false;
 
     // flags: initial values and write-back of changed buffers still referenced
     std::vector<bool> written(fptr.size(), false);

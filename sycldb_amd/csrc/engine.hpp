@@ -8,6 +8,7 @@
 
 #include <cstdint>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -213,6 +214,7 @@ struct ExecContext {
     hipStream_t stream;
     const LiveSet *live;
     const std::string *table = nullptr; // base table the bundle runs over
+    mutable uint64_t segment = 0;       // segment the bundle covers
     // flags buffer the table has not written yet: logically all true
     const uint8_t *virgin_begin = nullptr, *virgin_end = nullptr;
     mutable bool virgin_filled = false; // set once an unfused bundle filled it
@@ -234,6 +236,33 @@ class KernelBundle {
 };
 
 // ------------------------------------------------------------------- engine
+
+// A pipeline that cannot take part in a batched fact scan (it runs through
+// the plain path instead).
+struct NotBatchable : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// Capture of one pipeline's fact-table bundles for a batched execution
+// (sdb_engine_execute_batch): while active, the fused program of every
+// bundle over `table` is recorded instead of launched, the table's row-sized
+// intermediates get placeholder addresses (never registered, so a launch
+// that would touch one fails its bounds check), and device work that must
+// follow the fact pass is deferred.
+struct CapturedBundle {
+    FProgram p;
+    int64_t n;
+    uint64_t segment;
+};
+
+struct Capture {
+    bool active = false;
+    std::string table;
+    uint32_t lds_budget = F_LDS_BUDGET;           // LDS one captured program may use
+    std::vector<CapturedBundle> bundles;
+    std::vector<std::function<void()>> deferred;  // run after the batched fact pass
+    uint64_t phantom_next = 0;
+};
 
 struct Engine {
     int device = 0;
@@ -271,6 +300,16 @@ struct Engine {
     void prof_collect(); // after the stream is synchronised
 
     Table *find_table(const std::string &name);
+
+    Capture capture;
+    // Buffer of `count` T for a row-sized intermediate of `table` (a
+    // placeholder address while that table's bundles are being captured).
+    template <class T> T *row_buffer(const std::string &table, uint64_t count)
+    {
+        if (capture.active && table == capture.table) return static_cast<T *>(phantom(count * sizeof(T)));
+        return mm->alloc<T>(count);
+    }
+    void *phantom(uint64_t bytes);
 };
 
 // -------------------------------------------------------- transient table
@@ -347,7 +386,23 @@ struct ResultTable {
     std::vector<Col> cols;
 };
 
-// execute_ddor_plan, executor/ddor_executor.cpp:22-245
+// The final table of a plan, on the device (valid until the allocator resets).
+struct FinalTable {
+    uint64_t nrows = 0;
+    const uint8_t *flags = nullptr;
+    struct Col {
+        const void *data;
+        bool is_u64;
+        uint64_t rows;
+    };
+    std::vector<Col> cols;
+};
+
+// execute_ddor_plan, executor/ddor_executor.cpp:22-245, in two steps: the
+// plan walk (device work queued on the engine stream) and the host copy of
+// the final table (result_writer.cpp's materialize_host_view).
+FinalTable run_ddor_plan(Engine &eng, const PlanResult &plan);
+ResultTable copy_result(Engine &eng, const FinalTable &t);
 ResultTable execute_ddor_plan(Engine &eng, const PlanResult &plan);
 
 extern const std::map<std::string, int> TABLE_COLUMN_NUMBERS;   // kernels/types.cpp:13-21

@@ -163,7 +163,7 @@ ExecutionInfo parse_execution_info(const PlanResult &plan)
     return info;
 }
 
-ResultTable execute_ddor_plan(Engine &eng, const PlanResult &plan)
+FinalTable run_ddor_plan(Engine &eng, const PlanResult &plan)
 {
     const ExecutionInfo info = parse_execution_info(plan);
     if (info.dag_order.empty()) throw std::runtime_error("DAG topological sort failed");
@@ -211,33 +211,44 @@ ResultTable execute_ddor_plan(Engine &eng, const PlanResult &plan)
     TransientTable &final_table = input_of((int64_t)plan.rels.size() - 1);
     final_table.execute_pending_kernels(false);
 
+    FinalTable t;
+    t.nrows = final_table.get_nrows();
+    t.flags = final_table.flags_device();
+    for (Column *c : final_table.get_columns()) {
+        if (!c || !c->loaded || !c->data) continue;
+        t.cols.push_back({c->data, c->is_aggregate, std::min(t.nrows, c->nrows)});
+    }
+    return t;
+}
+
+ResultTable copy_result(Engine &eng, const FinalTable &t)
+{
     // host copy of the result (the materialize_host_view + save_result path,
     // executor/result_writer.cpp:28-67)
     ResultTable res;
-    res.nrows = final_table.get_nrows();
+    res.nrows = t.nrows;
     res.flags.resize(res.nrows);
-    const uint8_t *flags = final_table.flags_device();
     if (res.nrows) {
-        eng.registry.check(flags, res.nrows, "result flags");
-        SDB_HIP(hipMemcpyAsync(res.flags.data(), flags, res.nrows, hipMemcpyDeviceToHost, eng.stream));
+        eng.registry.check(t.flags, res.nrows, "result flags");
+        SDB_HIP(hipMemcpyAsync(res.flags.data(), t.flags, res.nrows, hipMemcpyDeviceToHost, eng.stream));
     }
-    for (Column *c : final_table.get_columns()) {
-        if (!c || !c->loaded || !c->data) continue;
+    for (const FinalTable::Col &c : t.cols) {
         ResultTable::Col rc;
-        rc.is_u64 = c->is_aggregate;
-        const uint64_t rows = std::min(res.nrows, c->nrows);
+        rc.is_u64 = c.is_u64;
         const uint64_t elem = rc.is_u64 ? 8 : 4;
         if (rc.is_u64) rc.u64.assign(res.nrows, 0);
         else rc.i32.assign(res.nrows, 0);
-        if (rows) {
-            eng.registry.check(c->data, rows * elem, "result column");
+        if (c.rows) {
+            eng.registry.check(c.data, c.rows * elem, "result column");
             void *dst = rc.is_u64 ? (void *)rc.u64.data() : (void *)rc.i32.data();
-            SDB_HIP(hipMemcpyAsync(dst, c->data, rows * elem, hipMemcpyDeviceToHost, eng.stream));
+            SDB_HIP(hipMemcpyAsync(dst, c.data, c.rows * elem, hipMemcpyDeviceToHost, eng.stream));
         }
         res.cols.push_back(std::move(rc));
     }
     SDB_HIP(hipStreamSynchronize(eng.stream));
     return res;
 }
+
+ResultTable execute_ddor_plan(Engine &eng, const PlanResult &plan) { return copy_result(eng, run_ddor_plan(eng, plan)); }
 
 } // namespace sdb

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <string>
+#include <vector>
 
 #include "fused_types.hpp"
 
@@ -33,6 +34,12 @@ std::string jit_tuning_report();
 std::string jit_source(const FProgram &p, int unroll);
 // Number of distinct kernels compiled so far in this process.
 int jit_kernels_compiled();
+// Runs the fact-table programs of several pipelines over rows [0, n) in one
+// launch (one scan of their columns).  dev_progs: device room for
+// in.size() programs.  unroll: 1, 2 or 4 (0: default).
+void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t s, FProgram *dev_progs, int unroll);
+// Compiles (no device needed) a batched kernel over the coverage programs.
+int jit_batch_check(const std::string &arch);
 // Compiles (without launching; no device needed) the kernels of programs
 // covering every instruction form for `arch`; returns how many compiled.
 int jit_check(const std::string &arch);

@@ -121,6 +121,7 @@ struct FProgram {
     int nblobs;                // tables staged into LDS at kernel start
     uint32_t lds_bytes;        // dynamic LDS: registers, blobs, aggregation
     uint32_t vreg_off;         // byte offset of the register file
+    uint32_t lds_base;         // batched launch: start of this program's LDS region
     int npf;                   // prefetched columns
     const int32_t *pf_mem[F_MAX_PF];
     const void *blob_src[F_MAX_BLOBS];
@@ -133,6 +134,9 @@ struct FProgram {
     FGroup gb;                 // FK_GROUPBY description
     FInstr ins[F_MAX_INSTR];
 };
+
+// LDS shared by the programs of one batched launch.
+constexpr uint32_t F_BATCH_LDS = 64 * 1024;
 
 // Largest rank dictionary built for LDS staging.
 constexpr uint32_t F_RANK_DICT_MAX = 32 * 1024;

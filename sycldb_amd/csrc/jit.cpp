@@ -119,7 +119,8 @@ int g_compiled = 0;
 struct Gen {
     const FProgram &p;
     std::ostringstream o;
-    explicit Gen(const FProgram &prog) : p(prog) {}
+    bool prefetch_in_body; // the body issues the next tile's column loads
+    explicit Gen(const FProgram &prog, bool prefetch = true) : p(prog), prefetch_in_body(prefetch) {}
 
     static std::string freg(int j) { return "f" + std::to_string(j); }
     static std::string vreg(int r) { return "v" + std::to_string(r); }
@@ -393,7 +394,8 @@ struct Gen {
         // the next tile's column loads go out after the loads of the first
         // instruction that waits for global memory and before that wait, so
         // the in-order completion counter does not make it wait for them too
-        const std::string prefetch = "        if (has_next) pf_load<R, UU>(P, nrow0, next);\n";
+        const std::string prefetch =
+            prefetch_in_body ? "        if (has_next) pf_load<R, UU>(P, nrow0, next);\n" : std::string();
         int first_wait = -1;
         for (int i = 0; i < p.ninstr && first_wait < 0; i++)
             if (waits(i)) first_wait = i;
@@ -404,8 +406,9 @@ struct Gen {
             const char *e = getenv("SDB_JIT_HOIST");
             return e && *e == '1';
         }();
-        plan_hoisting(hoist ? first_wait : -1);
+        plan_hoisting(hoist && prefetch_in_body ? first_wait : -1);
         if (first_wait < 0) o << prefetch;
+        if (!prefetch_in_body) first_wait = -1;
         for (int i = 0; i < p.ninstr; i++) instr(i, i == first_wait ? prefetch : std::string());
         for (int j = 0; j < F_MAX_FLAGS; j++)
             if (p.flag_store[j])
@@ -712,7 +715,7 @@ std::vector<std::pair<FProgram, int>> coverage_programs()
     return out;
 }
 
-Compiled &get_kernel(const std::string &src)
+Compiled &get_kernel(const std::string &src, const char *name = "sdb_fused_program")
 {
     int dev = 0;
     SDB_HIP(hipGetDevice(&dev));
@@ -727,7 +730,7 @@ Compiled &get_kernel(const std::string &src)
     }
     Compiled c;
     SDB_HIP(hipModuleLoadData(&c.module, ct->second.data()));
-    SDB_HIP(hipModuleGetFunction(&c.fn, c.module, "sdb_fused_program"));
+    SDB_HIP(hipModuleGetFunction(&c.fn, c.module, name));
     return g_loaded.emplace(std::make_pair(dev, key), c).first->second;
 }
 
@@ -744,7 +747,7 @@ int jit_check(const std::string &arch)
         compile_source(jit_source(pu.first, pu.second), arch);
         n++;
     }
-    return n;
+    return n + jit_batch_check(arch);
 }
 
 int jit_kernels_compiled()
@@ -879,6 +882,280 @@ void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll)
             if (t.best_ns_per_row[i] < t.best_ns_per_row[best]) best = i;
         t.chosen = best;
     }
+}
+
+// ------------------------------------------------------------ batched kernel
+//
+// Many pipelines' fact-table programs in one launch: one scan of the union of
+// their prefetched columns; per tile, the programs of each shape (identical
+// generated body) run in a loop that reads each program from a device array,
+// so pipelines that differ only in constants and tables share code.  Each
+// program has its own LDS region (lds_base) and its own scalar-sum slot.
+
+namespace {
+
+constexpr int BATCH_MAX_PF = 12;
+constexpr int BATCH_MAX_SHAPES = 32;
+
+const char *BATCH_HEAD = R"(constexpr int BLOCK = fdev::BLOCK;
+struct BatchArgs {
+    const FProgram *progs;
+    int64_t n;
+    int32_t nprog;
+    uint32_t sums_off;
+    int32_t first[NSHAPE + 1];
+    const int32_t *pf[NPF > 0 ? NPF : 1];
+};
+template <int R, int UU> struct Pf {
+    typename Vec<R>::type c[NPF > 0 ? NPF : 1][UU];
+};
+template <int R, int UU>
+__device__ __forceinline__ void pf_load(const BatchArgs &A, const int64_t (&row0)[UU], Pf<R, UU> &pf)
+{
+#pragma unroll
+    for (int j = 0; j < NPF; j++)
+#pragma unroll
+        for (int u = 0; u < UU; u++) pf.c[j][u] = load_vec<R>(A.pf[j], row0[u]);
+}
+)";
+
+const char *BATCH_TAIL = R"(
+extern "C" __global__ void __launch_bounds__(256) sdb_fused_batch(const BatchArgs A_arg)
+{
+    const BatchArgs &A = *(const BatchArgs __attribute__((address_space(4))) *)__builtin_amdgcn_kernarg_segment_ptr();
+    extern __shared__ uint4 dyn_lds[];
+    uint8_t *lds = reinterpret_cast<uint8_t *>(dyn_lds);
+    for (int i = 0; i < A.nprog; i++) {
+        const FProgram &P = A.progs[i];
+        if (P.magic != F_MAGIC) return;
+        fdev::block_prologue(P, lds + P.lds_base);
+    }
+    unsigned long long *sums = reinterpret_cast<unsigned long long *>(lds + A.sums_off);
+    for (int i = threadIdx.x; i < A.nprog; i += BLOCK) sums[i] = 0;
+    __syncthreads();
+    const int64_t n = A.n;
+    if constexpr (VEC) {
+        const int64_t nch = n >> 2, tile = (int64_t)U * BLOCK, ntiles = nch / tile;
+        auto rows_of = [&](int64_t t, int64_t (&r)[U]) {
+#pragma unroll
+            for (int u = 0; u < U; u++) r[u] = (t * tile + u * BLOCK + threadIdx.x) << 2;
+        };
+        int64_t t = blockIdx.x;
+        Pf<4, U> next;
+        if (t < ntiles) {
+            int64_t r[U];
+            rows_of(t, r);
+            pf_load<4, U>(A, r, next);
+        }
+        while (t < ntiles) {
+            const Pf<4, U> cur = next;
+            int64_t r[U];
+            rows_of(t, r);
+            const int64_t tn = t + gridDim.x;
+            if (tn < ntiles) {
+                int64_t rn[U];
+                rows_of(tn, rn);
+                pf_load<4, U>(A, rn, next);
+            }
+            run_all<4, U>(A, r, cur, lds, sums);
+            t = tn;
+        }
+        for (int64_t c = ntiles * tile + (int64_t)blockIdx.x * BLOCK + threadIdx.x; c < nch;
+             c += (int64_t)gridDim.x * BLOCK) {
+            const int64_t r[1] = {c << 2};
+            Pf<4, 1> pf;
+            pf_load<4, 1>(A, r, pf);
+            run_all<4, 1>(A, r, pf, lds, sums);
+        }
+        if (blockIdx.x == gridDim.x - 1 && threadIdx.x < (n & 3)) {
+            const int64_t r[1] = {(nch << 2) + threadIdx.x};
+            Pf<1, 1> pf;
+            pf_load<1, 1>(A, r, pf);
+            run_all<1, 1>(A, r, pf, lds, sums);
+        }
+    } else {
+        for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+            const int64_t r[1] = {i};
+            Pf<1, 1> pf;
+            pf_load<1, 1>(A, r, pf);
+            run_all<1, 1>(A, r, pf, lds, sums);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < A.nprog; i += BLOCK)
+        if (sums[i] && A.progs[i].sum_acc)
+            atomicAdd(reinterpret_cast<unsigned long long *>(A.progs[i].sum_acc), sums[i]);
+    for (int i = 0; i < A.nprog; i++) {
+        const FProgram &P = A.progs[i];
+        fdev::block_epilogue(P, lds + P.lds_base, 0);
+    }
+}
+)";
+
+// Rewrites a program's prefetch registers to indices of the union list
+// `pf` (adding its columns); a column beyond BATCH_MAX_PF is read from memory
+// instead.
+void remap_prefetch(FProgram &q, std::vector<const int32_t *> &pf)
+{
+    int map[F_MAX_PF];
+    for (int j = 0; j < q.npf; j++) {
+        auto it = std::find(pf.begin(), pf.end(), q.pf_mem[j]);
+        if (it != pf.end()) map[j] = (int)(it - pf.begin());
+        else if ((int)pf.size() < BATCH_MAX_PF) {
+            map[j] = (int)pf.size();
+            pf.push_back(q.pf_mem[j]);
+        } else map[j] = -1;
+    }
+    auto fix = [&](int8_t &reg, const int32_t *&mem, const int32_t *col) {
+        if (reg < F_PF_BASE) return;
+        const int j = reg - F_PF_BASE;
+        if (map[j] >= 0) reg = (int8_t)(F_PF_BASE + map[j]);
+        else {
+            reg = -1;
+            mem = col;
+        }
+    };
+    for (int i = 0; i < q.ninstr; i++) {
+        FInstr &I = q.ins[i];
+        if (I.va >= F_PF_BASE) fix(I.va, I.a, q.pf_mem[I.va - F_PF_BASE]);
+        if (I.vb >= F_PF_BASE) fix(I.vb, I.b, q.pf_mem[I.vb - F_PF_BASE]);
+    }
+    for (int j = 0; j < q.gb.k; j++)
+        if (q.gb.key_reg[j] >= F_PF_BASE) fix(q.gb.key_reg[j], q.gb.key_mem[j], q.pf_mem[q.gb.key_reg[j] - F_PF_BASE]);
+    if (q.gb.agg_reg >= F_PF_BASE) fix(q.gb.agg_reg, q.gb.agg_mem, q.pf_mem[q.gb.agg_reg - F_PF_BASE]);
+    q.npf = 0; // the batch kernel loads the union list itself
+}
+
+struct BatchPlan {
+    std::vector<FProgram> progs; // in shape order
+    std::vector<int> order;      // progs[k] is input program order[k]
+    std::vector<int> first;      // shape k: progs[first[k], first[k+1])
+    std::vector<const int32_t *> pf;
+    std::string src;
+    uint32_t lds = 0, sums_off = 0;
+    bool vec = true;
+};
+
+BatchPlan plan_batch(const std::vector<FProgram> &in, int unroll)
+{
+    BatchPlan b;
+    std::vector<FProgram> q = in;
+    for (FProgram &p : q) {
+        remap_prefetch(p, b.pf);
+        b.vec = b.vec && p.vec;
+    }
+    // shapes: identical generated bodies
+    std::vector<std::string> bodies;
+    std::vector<int> shape_of(q.size());
+    for (size_t i = 0; i < q.size(); i++) {
+        Gen g(q[i], false);
+        const std::string body = g.body();
+        auto it = std::find(bodies.begin(), bodies.end(), body);
+        shape_of[i] = (int)(it - bodies.begin());
+        if (it == bodies.end()) bodies.push_back(body);
+    }
+    if ((int)bodies.size() > BATCH_MAX_SHAPES) fail(SDB_ERR_UNSUPPORTED, "batched launch: too many program shapes");
+    for (int k = 0; k < (int)bodies.size(); k++) {
+        b.first.push_back((int)b.progs.size());
+        for (size_t i = 0; i < q.size(); i++)
+            if (shape_of[i] == k) {
+                b.progs.push_back(q[i]);
+                b.order.push_back((int)i);
+            }
+    }
+    b.first.push_back((int)b.progs.size());
+    // LDS regions, then the scalar-sum slots
+    uint32_t off = 0;
+    for (FProgram &p : b.progs) {
+        p.lds_base = off;
+        off += (p.vreg_off + 15u) & ~15u;
+    }
+    b.sums_off = off;
+    b.lds = off + 8u * (uint32_t)b.progs.size();
+    std::ostringstream s;
+    s << KERNEL_TEMPLATE_INCLUDES << "constexpr int NPF = " << b.pf.size() << ";\nconstexpr int U = " << unroll
+      << ";\nconstexpr bool VEC = " << (b.vec ? "true" : "false") << ";\nconstexpr int NSHAPE = " << bodies.size()
+      << ";\n" << BATCH_HEAD;
+    for (size_t k = 0; k < bodies.size(); k++) {
+        s << "template <int R, int UU>\n__device__ __forceinline__ void body" << k
+          << "(const FProgram &P, const int64_t (&row0)[UU], const Pf<R, UU> &pf, int64_t &acc, uint8_t *lds)\n{\n"
+          << bodies[k] << "}\n";
+    }
+    s << "template <int R, int UU>\n__device__ __forceinline__ void run_all(const BatchArgs &A, const int64_t "
+         "(&row0)[UU], const Pf<R, UU> &pf, uint8_t *lds, unsigned long long *sums)\n{\n";
+    for (size_t k = 0; k < bodies.size(); k++) {
+        s << "#pragma unroll 1\n    for (int i = A.first[" << k << "]; i < A.first[" << k + 1 << "]; i++) {\n"
+          << "        const FProgram &P = A.progs[i];\n        int64_t acc = 0;\n"
+          << "        body" << k << "<R, UU>(P, row0, pf, acc, lds + P.lds_base);\n"
+          << "        if (acc) atomicAdd(&sums[i], (unsigned long long)acc);\n    }\n";
+    }
+    s << "}\n} // namespace\n" << BATCH_TAIL;
+    b.src = s.str();
+    return b;
+}
+
+} // namespace
+
+int jit_batch_check(const std::string &arch)
+{
+    std::vector<FProgram> progs;
+    for (auto &pu : coverage_programs()) progs.push_back(pu.first);
+    // two copies of each shape, as pipelines with different constants
+    std::vector<FProgram> two = progs;
+    two.insert(two.end(), progs.begin(), progs.end());
+    compile_source(plan_batch(two, 2).src, arch);
+    return 1;
+}
+
+void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t s, FProgram *dev_progs, int unroll)
+{
+    if (n <= 0 || in.empty()) return;
+    if (unroll != 1 && unroll != 2 && unroll != 4) unroll = F_DEFAULT_UNROLL;
+    const BatchPlan b = plan_batch(in, unroll);
+    if (b.lds > F_BATCH_LDS + 8u * 1024u) fail(SDB_ERR_UNSUPPORTED, "batched launch: LDS regions exceed the budget");
+    Compiled &k = get_kernel(b.src, "sdb_fused_batch");
+    SDB_HIP(hipMemcpyAsync(dev_progs, b.progs.data(), b.progs.size() * sizeof(FProgram), hipMemcpyHostToDevice, s));
+    // kernel argument block: progs, n, nprog, sums_off, first[NSHAPE+1], pf[NPF]
+    const int nshape = (int)b.first.size() - 1;
+    std::vector<uint8_t> args;
+    auto put = [&](const void *v, size_t bytes, size_t align) {
+        while (args.size() % align) args.push_back(0);
+        const uint8_t *c = static_cast<const uint8_t *>(v);
+        args.insert(args.end(), c, c + bytes);
+    };
+    const FProgram *pp = dev_progs;
+    const int32_t nprog = (int32_t)b.progs.size();
+    put(&pp, 8, 8);
+    put(&n, 8, 8);
+    put(&nprog, 4, 4);
+    put(&b.sums_off, 4, 4);
+    for (int i = 0; i <= nshape; i++) {
+        const int32_t f = b.first[i];
+        put(&f, 4, 4);
+    }
+    const size_t npf = std::max<size_t>(b.pf.size(), 1);
+    for (size_t j = 0; j < npf; j++) {
+        const int32_t *c = j < b.pf.size() ? b.pf[j] : nullptr;
+        put(&c, 8, 8);
+    }
+    while (args.size() % 8) args.push_back(0);
+    int per_cu;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = k.blocks_per_cu.find(b.lds);
+        if (it == k.blocks_per_cu.end()) {
+            int bl = 0;
+            SDB_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&bl, k.fn, JIT_BLOCK, b.lds));
+            it = k.blocks_per_cu.emplace(b.lds, std::max(1, bl)).first;
+        }
+        per_cu = it->second;
+    }
+    const int64_t units = b.vec ? (n + 3) / 4 : n;
+    const unsigned grid = stream_grid(units, (int64_t)JIT_BLOCK * (b.vec ? unroll : 1), per_cu);
+    size_t size = args.size();
+    void *config[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, args.data(), HIP_LAUNCH_PARAM_BUFFER_SIZE, &size,
+                      HIP_LAUNCH_PARAM_END};
+    SDB_HIP(hipModuleLaunchKernel(k.fn, grid, 1, 1, JIT_BLOCK, 1, 1, b.lds, s, nullptr, config));
 }
 
 std::string jit_tuning_report()

@@ -68,6 +68,14 @@ void MemoryManager::reset(hipStream_t s)
     }
 }
 
+void *Engine::phantom(uint64_t bytes)
+{
+    // addresses above the 57-bit virtual range: never device memory
+    const uint64_t at = 0xF000000000000000ull + capture.phantom_next;
+    capture.phantom_next += (bytes + 255) & ~255ull;
+    return reinterpret_cast<void *>(at);
+}
+
 uint64_t MemoryManager::used_bytes() const
 {
     uint64_t u = 0;

@@ -70,7 +70,7 @@ TransientTable::TransientTable(Engine &eng, Table *base) : eng_(eng), nrows_(bas
 {
     // flags start true (transient_table.hpp:84-106); the fill is deferred:
     // the buffer stays logically all-true until an operator reads it
-    flags_ = eng_.mm->alloc<uint8_t>(std::max<uint64_t>(nrows_, 1));
+    flags_ = eng_.row_buffer<uint8_t>(name_, std::max<uint64_t>(nrows_, 1));
     virgin_ = flags_;
     current_columns_.reserve(base->columns.size() + 64);
     for (Column &c : base->columns) current_columns_.push_back(&c);
@@ -107,7 +107,7 @@ Column &TransientTable::new_column(uint64_t nrows, bool aggregate)
 
 void TransientTable::ensure_secondary_flags()
 {
-    if (!flags_secondary_) flags_secondary_ = eng_.mm->alloc<uint8_t>(std::max<uint64_t>(nrows_, 1));
+    if (!flags_secondary_) flags_secondary_ = eng_.row_buffer<uint8_t>(name_, std::max<uint64_t>(nrows_, 1));
 }
 
 void TransientTable::swap_flag_buffers()
@@ -129,6 +129,14 @@ void TransientTable::add_pending_kernels(std::vector<KernelBundle> ops)
 const uint8_t *TransientTable::flags_device()
 {
     if (virgin_ == flags_) {
+        try {
+            eng_.registry.check(flags_, std::max<uint64_t>(nrows_, 1), "initial flags");
+        } catch (const Error &) {
+            // a row-sized placeholder of a captured fact table
+            if (eng_.capture.active && name_ == eng_.capture.table)
+                throw NotBatchable("the fact table's flags are needed in memory");
+            throw;
+        }
         launch_fill_u8(flags_, 1, (int64_t)nrows_, eng_.stream);
         eng_.launches++;
         virgin_ = nullptr;
@@ -157,8 +165,10 @@ void TransientTable::execute_pending_kernels(bool dead_after)
     const uint64_t nseg = segment_num();
     for (const auto &phase : pending_kernels_)
         if (phase.size() != nseg) throw std::runtime_error("Pending kernels segment number mismatch.");
-    for (uint64_t s = 0; s < nseg; s++)
+    for (uint64_t s = 0; s < nseg; s++) {
+        ctx.segment = s;
         for (const auto &phase : pending_kernels_) phase[s].execute(ctx);
+    }
     pending_kernels_.clear();
     virgin_ = nullptr;
 }
@@ -257,7 +267,7 @@ void TransientTable::apply_search(const ExprType &expr, const std::string &paren
     for (uint64_t s = 0; s < segment_num(); s++) {
         const uint64_t off = s * eng_.segment_size;
         const int64_t n = (int64_t)seg_rows(s);
-        uint8_t *local = eng_.mm->alloc<uint8_t>(n);
+        uint8_t *local = eng_.row_buffer<uint8_t>(name_, n);
         KernelData k1, k2, k3;
         k1.type = k2.type = KernelType::SelectionKernelLiteral;
         k1.n = k2.n = k3.n = n;
@@ -304,7 +314,7 @@ void TransientTable::apply_project(const std::vector<ExprType> &exprs)
         case ExprOption::LITERAL: {
             const int32_t v = to_int32(e.literal.value);
             Column &nc = new_column(nrows_);
-            nc.data = eng_.mm->alloc<int32_t>(std::max<uint64_t>(nrows_, 1));
+            nc.data = eng_.row_buffer<int32_t>(name_, std::max<uint64_t>(nrows_, 1));
             std::vector<KernelBundle> ops(segment_num());
             for (uint64_t s = 0; s < segment_num(); s++) {
                 KernelData k;
@@ -337,7 +347,7 @@ void TransientTable::apply_project(const std::vector<ExprType> &exprs)
             Column *ca = k0 == ExprOption::COLUMN ? col(e.operands[0].input, "project") : nullptr;
             Column *cb = k1 == ExprOption::COLUMN ? col(e.operands[1].input, "project") : nullptr;
             Column &nc = new_column(nrows_);
-            nc.data = eng_.mm->alloc<int32_t>(std::max<uint64_t>(nrows_, 1));
+            nc.data = eng_.row_buffer<int32_t>(name_, std::max<uint64_t>(nrows_, 1));
             std::vector<KernelBundle> ops(segment_num());
             for (uint64_t s = 0; s < segment_num(); s++) {
                 KernelData k;
@@ -441,8 +451,18 @@ void TransientTable::apply_aggregate(const AggType &agg, const std::vector<int64
     execute_pending_kernels(true);
     nrows_ = prod;
     flags_ = eng_.mm->alloc<uint8_t>(prod);
-    launch_seen_to_flags(seen, flags_, prod, eng_.stream);
-    eng_.launches++;
+    if (eng_.capture.active && name_ == eng_.capture.table) {
+        // the fact pass has not run yet: convert after the batched launch
+        uint8_t *f = flags_;
+        Engine *eng = &eng_;
+        eng_.capture.deferred.push_back([=] {
+            launch_seen_to_flags(seen, f, prod, eng->stream);
+            eng->launches++;
+        });
+    } else {
+        launch_seen_to_flags(seen, flags_, prod, eng_.stream);
+        eng_.launches++;
+    }
     virgin_ = nullptr;
     flags_secondary_ = nullptr;
     current_columns_.clear();
@@ -608,7 +628,7 @@ void TransientTable::apply_join(TransientTable &right, const RelNode &rel)
 
     if (!right.group_by_column_) throw std::runtime_error("full join: the build table has no group-by column");
     Column &nc = new_column(nrows_);
-    nc.data = eng_.mm->alloc<int32_t>(std::max<uint64_t>(nrows_, 1));
+    nc.data = eng_.row_buffer<int32_t>(name_, std::max<uint64_t>(nrows_, 1));
     const auto gb_mm = right.group_by_column_->get_min_max();
     const uint64_t gb_index = current_columns_.size() + right.group_by_column_index_;
     for (size_t i = 0; i < right.current_columns_.size(); i++) current_columns_.push_back(nullptr);

@@ -191,6 +191,18 @@ class Engine:
             self.finalize(t)
 
     # ----------------------------------------------------------- execution
+    def execute_batch(self, plans):
+        """Runs several plans as one batch: their fact-table passes share one
+        scan.  Returns (results in plan order, number that shared the scan);
+        each result equals the plan's result when run alone."""
+        texts = [p if isinstance(p, str) else json.dumps(p) for p in plans]
+        n = len(texts)
+        arr = (ctypes.c_char_p * max(n, 1))(*[t.encode() for t in texts])
+        out = (ctypes.c_void_p * max(n, 1))()
+        batched = ctypes.c_int(0)
+        L.check(L.lib.sdb_engine_execute_batch(self._h, arr, n, out, ctypes.byref(batched)))
+        return [Result(out[i]) for i in range(n)], batched.value
+
     def execute(self, plan):
         text = plan if isinstance(plan, str) else json.dumps(plan)
         h = ctypes.c_void_p()

@@ -261,3 +261,67 @@ def config2_plan(region=2):
 def config3_plan(category=1, region=1):
     """600M-row configuration: part category (4 %) and supplier region (20 %)."""
     return t2_plan(("category", category), region)
+
+
+# ----------------------------------------------------------- random pipelines
+
+def random_pipelines(n, seed):
+    """n (template, params) pairs drawn from the four templates with seeded
+    random predicate constants, spanning fact selectivities from about 1e-6
+    (a city pair x a city x one month) to about 0.2 (one region)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        t = ("t1", "t2", "t3", "t4")[i % 4]
+        if t == "t1":
+            kind = rng.integers(0, 3)
+            if kind == 0:
+                date = ("year", int(rng.integers(1992, 1999)))
+            elif kind == 1:
+                y = int(rng.integers(1992, 1999))
+                date = ("yearmonthnum", y * 100 + int(rng.integers(1, 13)))
+            else:
+                a = int(rng.integers(1992, 1998))
+                date = ("year_range_and", a, a + int(rng.integers(0, 3)))
+            lo = int(rng.integers(0, 9))
+            disc = (lo, lo + int(rng.integers(0, 3)))
+            if rng.integers(0, 2):
+                qty = ("lt", int(rng.integers(2, 51)))
+            else:
+                q = int(rng.integers(1, 45))
+                qty = ("range", q, q + int(rng.integers(0, 6)))
+            out.append(("t1", {"date": date, "disc": disc, "qty": qty}))
+        elif t == "t2":
+            kind = rng.integers(0, 3)
+            if kind == 0:
+                part = ("category", int(rng.integers(0, 25)))
+            elif kind == 1:
+                part = ("brand", int(rng.integers(0, 1000)))
+            else:
+                b0 = int(rng.integers(0, 990))
+                part = ("brand_range", b0, b0 + int(rng.integers(1, 9)))
+            out.append(("t2", {"part": part, "s_region": int(rng.integers(0, 5))}))
+        elif t == "t3":
+            def ent():
+                k = rng.integers(0, 4)
+                if k == 0:
+                    return ("region", int(rng.integers(0, 5)))
+                if k == 1:
+                    return ("nation", int(rng.integers(0, 25)))
+                if k == 2:
+                    return ("city", int(rng.integers(0, 250)))
+                c = int(rng.integers(0, 248))
+                return ("city2", c, c + int(rng.integers(1, 3)))
+            if rng.integers(0, 3):
+                a = int(rng.integers(1992, 1998))
+                date = ("year_range", a, a + int(rng.integers(0, 6 - (a - 1992) + 1)))
+            else:
+                date = ("yearmonthnum", int(rng.integers(1992, 1999)) * 100 + int(rng.integers(1, 13)))
+            out.append(("t3", {"cust": ent(), "supp": ent(), "date": date,
+                               "group": "city" if rng.integers(0, 2) else "nation"}))
+        else:
+            m = int(rng.integers(0, 4))
+            out.append(("t4", {"c_region": int(rng.integers(0, 5)), "s_region": int(rng.integers(0, 5)),
+                               "mfgr": (m, m + 1)}))
+    return out

@@ -1,0 +1,92 @@
+"""Batched execution (sdb_engine_execute_batch): many pipelines whose
+fact-table passes share one scan.  Every pipeline's result must equal the
+oracle's and the plain engine's result for that pipeline alone."""
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import datagen, plan_oracle as PO  # noqa: E402
+from sycldb_amd import plans as PL  # noqa: E402
+from sycldb_amd import schema as S  # noqa: E402
+
+FACT_ROWS = 300_007
+SEED = 46
+
+
+@pytest.fixture(scope="module")
+def cpu_tables():
+    return datagen.generate_tables(S, FACT_ROWS, seed=SEED)
+
+
+@pytest.fixture(scope="module")
+def pipelines():
+    return PL.random_pipelines(32, 11)
+
+
+@pytest.fixture(scope="module")
+def expected(cpu_tables, pipelines):
+    return [PO.execute_plan(PL.build(t, **p), cpu_tables).rows() for t, p in pipelines]
+
+
+def _engine(**kw):
+    from sycldb_amd.engine import Engine
+    e = Engine(**kw)
+    e.generate(FACT_ROWS, SEED)
+    return e
+
+
+def test_batch_of_32_matches_oracle(pipelines, expected):
+    eng = _engine()
+    try:
+        plans = [PL.build(t, **p) for t, p in pipelines]
+        print("batch of", len(plans), flush=True)
+        results, batched = eng.execute_batch(plans)
+        assert batched >= 24, batched
+        for i, r in enumerate(results):
+            assert r.rows() == expected[i], (i, pipelines[i])
+        # the same batch again (allocator reset between batches)
+        results2, _ = eng.execute_batch(plans)
+        assert [r.rows() for r in results2] == expected
+        # each plan alone through the plain path agrees
+        for i in range(0, len(plans), 5):
+            assert eng.execute(plans[i]).rows() == expected[i]
+    finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("kw", [dict(segment_size=77_777), dict(unroll=1), dict(codegen="interpret")])
+def test_batch_variants_match_oracle(pipelines, expected, kw):
+    """Several segments per pipeline; another chunking; and an engine whose
+    fused bundles are interpreted (the batch then runs each plan plainly)."""
+    eng = _engine(**kw)
+    try:
+        plans = [PL.build(t, **p) for t, p in pipelines[:12]]
+        print("batch", kw, flush=True)
+        results, batched = eng.execute_batch(plans)
+        for i, r in enumerate(results):
+            assert r.rows() == expected[i], (kw, i, pipelines[i])
+    finally:
+        eng.close()
+
+
+def test_batch_with_unfused_engine_runs_plainly(pipelines, expected):
+    eng = _engine(fusion=False)
+    try:
+        plans = [PL.build(t, **p) for t, p in pipelines[:4]]
+        results, batched = eng.execute_batch(plans)
+        assert batched == 0
+        assert [r.rows() for r in results] == expected[:4]
+    finally:
+        eng.close()
+
+
+def test_batch_of_one_and_empty():
+    eng = _engine()
+    try:
+        results, batched = eng.execute_batch([])
+        assert results == [] and batched == 0
+        plan = PL.build("t2")
+        results, _ = eng.execute_batch([plan])
+        assert results[0].rows() == eng.execute(plan).rows()
+    finally:
+        eng.close()
