@@ -49,6 +49,36 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def batched_measure(eng, n_pipes, seed, reps):
+    """32 (by default) pipelines from the four templates with seeded random
+    constants over the resident tables: run one by one through the plain
+    path, then as one batch sharing the fact scans.  Results must agree."""
+    import torch
+    from sycldb_amd import plans as PL
+    pipes = PL.random_pipelines(n_pipes, seed)
+    plans = [PL.build(t, **p) for t, p in pipes]
+    alone = [eng.execute(p).rows() for p in plans]  # warm, and the reference results
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for p in plans:
+            eng.execute(p)
+    torch.cuda.synchronize()
+    seq_s = (time.perf_counter() - t0) / reps
+    results, nb = eng.execute_batch(plans)  # warm
+    assert [r.rows() for r in results] == alone, "batched results differ from running each plan alone"
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        eng.execute_batch(plans)
+    torch.cuda.synchronize()
+    bat_s = (time.perf_counter() - t0) / reps
+    return {"pipelines": n_pipes, "pipeline_seed": seed, "sharing_the_scan": nb,
+            "sequential_ms": seq_s * 1e3, "batched_ms": bat_s * 1e3,
+            "sequential_pipelines_per_s": n_pipes / seq_s, "batched_pipelines_per_s": n_pipes / bat_s,
+            "speedup": seq_s / bat_s, "results_identical": True}
+
+
 def measured_traffic(workload, rows):
     """HBM bytes per fused fact-kernel launch from the committed counter
     summary, when it covers this workload at this size."""
@@ -98,6 +128,9 @@ def main():
     ap.add_argument("--cpu-sample-rows", type=int, default=60_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-unfused", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-pipelines", type=int, default=32)
+    ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--codegen", default="auto", choices=["auto", "interpret", "compile"])
     ap.add_argument("--unroll", type=int, default=0, choices=[0, 1, 2, 4])
     ap.add_argument("--copy-calibration", action="store_true",
@@ -125,7 +158,8 @@ def main():
     plan = getattr(PL, builder)()
     eng = Engine(codegen=args.codegen, unroll=args.unroll)
     t0 = time.time()
-    eng.generate(rows, args.seed, fact_columns=fact_cols)
+    # all fact columns: the batched and served pipelines read up to nine
+    eng.generate(rows, args.seed)
     torch.cuda.synchronize()
     log("generated %d fact rows on the device in %.1fs" % (rows, time.time() - t0))
 
@@ -178,6 +212,11 @@ def main():
         eng.set_fusion(True)
         log("unfused: %.3f ms/step" % unf_ms)
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_measure(eng, args.batch_pipelines, args.seed + 1, args.batch_reps)
+        log("batched: %s" % batched)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(plan, fact_cols, min(args.cpu_sample_rows, rows), args.seed)
@@ -203,6 +242,7 @@ def main():
                    "unfused": unfused},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
     eng.close()

@@ -303,6 +303,16 @@ int sdb_engine_execute(sdb_engine *e, const char *plan_json, sdb_result **out)
     });
 }
 
+// SDB_DEBUG_BATCH=1 reports on stderr why a plan runs outside the shared scan.
+static bool debug_batch()
+{
+    static const bool on = [] {
+        const char *e = getenv("SDB_DEBUG_BATCH");
+        return e && *e == '1';
+    }();
+    return on;
+}
+
 // Batched execution: every plan's dimension work runs as usual; the
 // fact-table bundles of the plans that fuse are captured and run together in
 // one kernel per segment (one scan of the union of their fact columns); the
@@ -352,7 +362,10 @@ int sdb_engine_execute_batch(sdb_engine *e, const char *const *plans, int n, sdb
                     pipes[i].deferred = std::move(eng.capture.deferred);
                     pipes[i].batched = true;
                 }
-            } catch (const NotBatchable &) {
+                else if (debug_batch())
+                    fprintf(stderr, "[batch] plan %d runs alone: %zu captured bundles\n", i, eng.capture.bundles.size());
+            } catch (const NotBatchable &ex) {
+                if (debug_batch()) fprintf(stderr, "[batch] plan %d runs alone: %s\n", i, ex.what());
             }
             eng.capture = Capture{};
         }
