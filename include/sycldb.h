@@ -264,6 +264,44 @@ int sdb_engine_execute(sdb_engine *eng, const char *plan_json, sdb_result **out)
  * running it alone; *batched (may be NULL) = how many shared the scan. */
 int sdb_engine_execute_batch(sdb_engine *eng, const char *const *plans, int n, sdb_result **out, int *batched);
 
+/* ----------------------------------------------------------- serving */
+
+/* A serving loop over engines with resident tables (no counterpart in the
+ * reference, which runs one query at a time).  Requests are plans for one
+ * engine, queued with their arrival time.  With batching (serial == 0) each
+ * engine has a worker that, whenever it is free, admits waiting requests of
+ * its engine in arrival order while a slot (max_batch) and the memory budget
+ * (the requests' estimated build-table and group-by footprints) allow --
+ * always at least one -- and runs them as one batch sharing the fact scan
+ * (sdb_engine_execute_batch); requests arriving meanwhile join a later batch.
+ * Workers run on their engines' streams, so batches of different engines
+ * overlap on the device.  With serial != 0 one worker runs every request
+ * alone in arrival order (the sequential baseline).  While a server runs,
+ * its engines must not be used directly. */
+typedef struct sdb_server sdb_server;
+typedef struct sdb_server_options {
+    int max_batch;                /* slots per batch (default 32) */
+    uint64_t memory_budget_bytes; /* admission budget per batch (default 2 GiB) */
+    int serial;                   /* 1: one request at a time, arrival order */
+} sdb_server_options;
+typedef struct sdb_request_info {
+    double latency_ms; /* completion - arrival */
+    double queue_ms;   /* start - arrival */
+    double run_ms;     /* completion - start (the whole batch) */
+    int batch_size;    /* requests in the batch that ran it */
+} sdb_request_info;
+
+int sdb_server_create(sdb_engine *const *engines, int n_engines, const sdb_server_options *opts,
+                      sdb_server **out);
+/* Queues plan_json for engine index `engine`; *ticket identifies it. */
+int sdb_server_submit(sdb_server *srv, int engine, const char *plan_json, int64_t *ticket);
+/* Waits for the request and returns its result (and timing, if info). */
+int sdb_server_wait(sdb_server *srv, int64_t ticket, sdb_result **out, sdb_request_info *info);
+/* Batches run so far and the requests they held. */
+int sdb_server_stats(sdb_server *srv, int64_t *batches, int64_t *requests);
+/* Finishes the queued requests, then stops the workers. */
+void sdb_server_destroy(sdb_server *srv);
+
 /* Device time (ms, hipEvents) and host wall time (ms) of the last execute. */
 int sdb_engine_last_timing(sdb_engine *eng, double *device_ms, double *wall_ms);
 /* Resets the scratch allocator (memory_manager::reset, re-zeroing the

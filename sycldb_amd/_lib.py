@@ -66,6 +66,19 @@ class EngineOptions(ctypes.Structure):
                 ("device", ctypes.c_int)]
 
 
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("max_batch", ctypes.c_int),
+                ("memory_budget_bytes", ctypes.c_uint64),
+                ("serial", ctypes.c_int)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("latency_ms", ctypes.c_double),
+                ("queue_ms", ctypes.c_double),
+                ("run_ms", ctypes.c_double),
+                ("batch_size", ctypes.c_int)]
+
+
 # name: (restype, argtypes)
 SIGNATURES = {
     "sdb_last_error": (CSTR, []),
@@ -115,6 +128,11 @@ SIGNATURES = {
     "sdb_stream_synchronize": (INT, [P]),
     "sdb_engine_execute_batch": (INT, [P, ctypes.POINTER(ctypes.c_char_p), INT, ctypes.POINTER(P),
                                        ctypes.POINTER(INT)]),
+    "sdb_server_create": (INT, [ctypes.POINTER(P), INT, ctypes.POINTER(ServerOptions), ctypes.POINTER(P)]),
+    "sdb_server_submit": (INT, [P, INT, CSTR, ctypes.POINTER(I64)]),
+    "sdb_server_wait": (INT, [P, I64, ctypes.POINTER(P), ctypes.POINTER(RequestInfo)]),
+    "sdb_server_stats": (INT, [P, ctypes.POINTER(I64), ctypes.POINTER(I64)]),
+    "sdb_server_destroy": (None, [P]),
     "sdb_engine_set_fusion": (INT, [P, INT]),
     "sdb_engine_set_codegen": (INT, [P, INT, INT]),
     "sdb_codegen_available": (INT, [CSTR, INT]),

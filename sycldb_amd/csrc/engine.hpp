@@ -404,6 +404,9 @@ struct FinalTable {
 FinalTable run_ddor_plan(Engine &eng, const PlanResult &plan);
 ResultTable copy_result(Engine &eng, const FinalTable &t);
 ResultTable execute_ddor_plan(Engine &eng, const PlanResult &plan);
+// Several plans with a shared fact scan (sdb_engine_execute_batch); the
+// results are in plan order.  *batched = plans that shared the scan.
+std::vector<ResultTable> execute_batch(Engine &eng, const std::vector<PlanResult> &plans, int *batched);
 
 extern const std::map<std::string, int> TABLE_COLUMN_NUMBERS;   // kernels/types.cpp:13-21
 extern const std::map<std::string, std::vector<int>> TABLE_COLUMN_INDICES; // types.cpp:3-11

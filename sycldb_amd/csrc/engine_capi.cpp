@@ -8,15 +8,9 @@
 #include "engine.hpp"
 #include "fused.hpp"
 
+#include "capi_handles.hpp"
+
 using namespace sdb;
-
-struct sdb_engine {
-    Engine eng;
-};
-
-struct sdb_result {
-    ResultTable t;
-};
 
 namespace sdb {
 
@@ -303,6 +297,8 @@ int sdb_engine_execute(sdb_engine *e, const char *plan_json, sdb_result **out)
     });
 }
 
+} // extern "C"
+
 // SDB_DEBUG_BATCH=1 reports on stderr why a plan runs outside the shared scan.
 static bool debug_batch()
 {
@@ -318,6 +314,85 @@ static bool debug_batch()
 // one kernel per segment (one scan of the union of their fact columns); the
 // rest run through the plain path.  Results are those of running each plan
 // alone.
+namespace sdb {
+std::vector<ResultTable> execute_batch(Engine &eng, const std::vector<PlanResult> &parsed, int *batched)
+{
+    const int n = (int)parsed.size();
+    if (batched) *batched = 0;
+    eng.mm->reset(eng.stream);
+    struct Pipe {
+        FinalTable fin;
+        std::vector<CapturedBundle> bundles;
+        std::vector<std::function<void()>> deferred;
+        bool batched = false;
+    };
+    std::vector<Pipe> pipes(n);
+    const bool can = eng.fusion && eng.codegen != SDB_CODEGEN_INTERPRET && jit_available() && n > 1;
+    const std::string fact = "lineorder";
+    const uint32_t lds_each = n > 0 ? std::min<uint32_t>(F_LDS_BUDGET, (F_BATCH_LDS / (uint32_t)n) & ~15u) : 0;
+    for (int i = 0; i < n && can; i++) {
+        eng.capture = Capture{};
+        eng.capture.active = true;
+        eng.capture.table = fact;
+        eng.capture.lds_budget = lds_each;
+        try {
+            FinalTable fin = run_ddor_plan(eng, parsed[i]);
+            // one bundle per segment, in segment order
+            bool ok = !eng.capture.bundles.empty();
+            for (size_t b = 0; b < eng.capture.bundles.size() && ok; b++) ok = eng.capture.bundles[b].segment == b;
+            if (ok) {
+                pipes[i].fin = fin;
+                pipes[i].bundles = std::move(eng.capture.bundles);
+                pipes[i].deferred = std::move(eng.capture.deferred);
+                pipes[i].batched = true;
+            } else if (debug_batch()) {
+                fprintf(stderr, "[batch] plan %d runs alone: %zu captured bundles\n", i, eng.capture.bundles.size());
+            }
+        } catch (const NotBatchable &ex) {
+            if (debug_batch()) fprintf(stderr, "[batch] plan %d runs alone: %s\n", i, ex.what());
+        } catch (...) {
+            eng.capture = Capture{};
+            throw;
+        }
+        eng.capture = Capture{};
+    }
+    // the shared fact passes
+    std::vector<int> members;
+    for (int i = 0; i < n; i++)
+        if (pipes[i].batched) members.push_back(i);
+    if (!members.empty()) {
+        const size_t nseg = pipes[members[0]].bundles.size();
+        for (int i : members)
+            if (pipes[i].bundles.size() != nseg) fail(SDB_ERR_RUNTIME, "batched pipelines disagree on segments");
+        FProgram *dev_progs = eng.mm->alloc<FProgram>(members.size());
+        for (size_t sgi = 0; sgi < nseg; sgi++) {
+            std::vector<FProgram> progs;
+            const int64_t rows = pipes[members[0]].bundles[sgi].n;
+            for (int i : members) {
+                if (pipes[i].bundles[sgi].n != rows) fail(SDB_ERR_RUNTIME, "batched pipelines disagree on rows");
+                progs.push_back(pipes[i].bundles[sgi].p);
+            }
+            eng.registry.check(dev_progs, members.size() * sizeof(FProgram), "batched programs");
+            Engine::ProfRec rec;
+            if (eng.profile) eng.prof_begin("batch:" + fact, rows, rec);
+            launch_fused_batch(progs, rows, eng.stream, dev_progs, eng.unroll);
+            if (eng.profile) eng.prof_end(rec);
+            eng.launches++;
+        }
+        for (int i : members)
+            for (auto &f : pipes[i].deferred) f();
+    }
+    for (int i = 0; i < n; i++)
+        if (!pipes[i].batched) pipes[i].fin = run_ddor_plan(eng, parsed[i]);
+    std::vector<ResultTable> out;
+    for (int i = 0; i < n; i++) out.push_back(copy_result(eng, pipes[i].fin));
+    if (batched) *batched = (int)members.size();
+    return out;
+}
+} // namespace sdb
+
+extern "C" {
+
 int sdb_engine_execute_batch(sdb_engine *e, const char *const *plans, int n, sdb_result **out, int *batched)
 {
     return guarded([&] {
@@ -331,77 +406,10 @@ int sdb_engine_execute_batch(sdb_engine *e, const char *const *plans, int n, sdb
             require(plans[i] != nullptr, "null plan");
             parsed.push_back(parse_plan(plans[i]));
         }
-        eng.mm->reset(eng.stream);
         eng.launches = 0;
         const auto w0 = std::chrono::steady_clock::now();
         SDB_HIP(hipEventRecord(eng.ev_start, eng.stream));
-
-        struct Pipe {
-            FinalTable fin;
-            std::vector<CapturedBundle> bundles;
-            std::vector<std::function<void()>> deferred;
-            bool batched = false;
-        };
-        std::vector<Pipe> pipes(n);
-        const bool can = eng.fusion && eng.codegen != SDB_CODEGEN_INTERPRET && jit_available() && n > 1;
-        const std::string fact = "lineorder";
-        const uint32_t lds_each = n > 0 ? std::min<uint32_t>(F_LDS_BUDGET, (F_BATCH_LDS / (uint32_t)n) & ~15u) : 0;
-        for (int i = 0; i < n && can; i++) {
-            eng.capture = Capture{};
-            eng.capture.active = true;
-            eng.capture.table = fact;
-            eng.capture.lds_budget = lds_each;
-            try {
-                FinalTable fin = run_ddor_plan(eng, parsed[i]);
-                // one bundle per segment, in segment order
-                bool ok = !eng.capture.bundles.empty();
-                for (size_t b = 0; b < eng.capture.bundles.size() && ok; b++) ok = eng.capture.bundles[b].segment == b;
-                if (ok) {
-                    pipes[i].fin = fin;
-                    pipes[i].bundles = std::move(eng.capture.bundles);
-                    pipes[i].deferred = std::move(eng.capture.deferred);
-                    pipes[i].batched = true;
-                }
-                else if (debug_batch())
-                    fprintf(stderr, "[batch] plan %d runs alone: %zu captured bundles\n", i, eng.capture.bundles.size());
-            } catch (const NotBatchable &ex) {
-                if (debug_batch()) fprintf(stderr, "[batch] plan %d runs alone: %s\n", i, ex.what());
-            }
-            eng.capture = Capture{};
-        }
-        // the shared fact passes
-        std::vector<int> members;
-        for (int i = 0; i < n; i++)
-            if (pipes[i].batched) members.push_back(i);
-        if (!members.empty()) {
-            const size_t nseg = pipes[members[0]].bundles.size();
-            for (int i : members)
-                if (pipes[i].bundles.size() != nseg) fail(SDB_ERR_RUNTIME, "batched pipelines disagree on segments");
-            FProgram *dev_progs = eng.mm->alloc<FProgram>(members.size());
-            for (size_t sgi = 0; sgi < nseg; sgi++) {
-                std::vector<FProgram> progs;
-                const int64_t rows = pipes[members[0]].bundles[sgi].n;
-                for (int i : members) {
-                    if (pipes[i].bundles[sgi].n != rows) fail(SDB_ERR_RUNTIME, "batched pipelines disagree on rows");
-                    progs.push_back(pipes[i].bundles[sgi].p);
-                }
-                eng.registry.check(dev_progs, members.size() * sizeof(FProgram), "batched programs");
-                Engine::ProfRec rec;
-                if (eng.profile) eng.prof_begin("batch:" + fact, rows, rec);
-                launch_fused_batch(progs, rows, eng.stream, dev_progs, eng.unroll);
-                if (eng.profile) eng.prof_end(rec);
-                eng.launches++;
-            }
-            for (int i : members)
-                for (auto &f : pipes[i].deferred) f();
-        }
-        for (int i = 0; i < n; i++)
-            if (!pipes[i].batched) pipes[i].fin = run_ddor_plan(eng, parsed[i]);
-        for (int i = 0; i < n; i++) {
-            auto r = std::make_unique<sdb_result>();
-            r->t = copy_result(eng, pipes[i].fin);
-            out[i] = r.release();
-        }
+        std::vector<ResultTable> res = execute_batch(eng, parsed, batched);
         SDB_HIP(hipEventRecord(eng.ev_stop, eng.stream));
         SDB_HIP(hipEventSynchronize(eng.ev_stop));
         if (eng.profile) eng.prof_collect();
@@ -409,7 +417,11 @@ int sdb_engine_execute_batch(sdb_engine *e, const char *const *plans, int n, sdb
         SDB_HIP(hipEventElapsedTime(&ms, eng.ev_start, eng.ev_stop));
         eng.last_device_ms = ms;
         eng.last_wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
-        if (batched) *batched = (int)members.size();
+        for (int i = 0; i < n; i++) {
+            auto r = std::make_unique<sdb_result>();
+            r->t = std::move(res[i]);
+            out[i] = r.release();
+        }
     });
 }
 

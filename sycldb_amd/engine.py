@@ -266,3 +266,71 @@ def dag_order(plan):
     count = ctypes.c_int()
     L.check(L.lib.sdb_plan_dag_order(text.encode(), order, 4096, ctypes.byref(count)))
     return list(order[:count.value])
+
+
+class Server:
+    """Serving loop over engines with resident tables (sdb_server_*):
+    requests are plans for one engine; with batching, each engine's worker
+    admits the waiting requests (slots and memory budget permitting) and runs
+    them as one batch sharing the fact scan, while later arrivals queue for
+    the next batch; serial=True runs every request alone in arrival order.
+    The engines must not be used directly while the server runs."""
+
+    def __init__(self, engines, max_batch=32, memory_budget_bytes=0, serial=False):
+        self._engines = list(engines)
+        arr = (ctypes.c_void_p * len(self._engines))(*[e._h for e in self._engines])
+        opts = L.ServerOptions(max_batch, memory_budget_bytes, 1 if serial else 0)
+        h = ctypes.c_void_p()
+        L.check(L.lib.sdb_server_create(arr, len(self._engines), ctypes.byref(opts), ctypes.byref(h)))
+        self._h = h
+
+    def submit(self, engine_index, plan):
+        text = plan if isinstance(plan, str) else json.dumps(plan)
+        t = ctypes.c_int64()
+        L.check(L.lib.sdb_server_submit(self._h, int(engine_index), text.encode(), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        """(Result, info dict with latency_ms, queue_ms, run_ms, batch_size)."""
+        h = ctypes.c_void_p()
+        info = L.RequestInfo()
+        L.check(L.lib.sdb_server_wait(self._h, int(ticket), ctypes.byref(h), ctypes.byref(info)))
+        return Result(h), {"latency_ms": info.latency_ms, "queue_ms": info.queue_ms, "run_ms": info.run_ms,
+                           "batch_size": info.batch_size}
+
+    def stats(self):
+        b, r = ctypes.c_int64(), ctypes.c_int64()
+        L.check(L.lib.sdb_server_stats(self._h, ctypes.byref(b), ctypes.byref(r)))
+        return {"batches": b.value, "requests": r.value}
+
+    def close(self):
+        if self._h:
+            L.lib.sdb_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def replay(server, requests, arrivals_s):
+    """Submits requests [(engine_index, plan)] at the given arrival offsets
+    (seconds from now), then waits for all; returns [(Result, info)] in
+    request order."""
+    import time
+    t0 = time.perf_counter()
+    tickets = []
+    for (e, plan), at in zip(requests, arrivals_s):
+        delay = t0 + at - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        tickets.append(server.submit(e, plan))
+    return [server.wait(t) for t in tickets]

@@ -1,0 +1,91 @@
+"""Serving loop (sdb_server_*): requests with Poisson arrivals over three
+engines with resident tables of different sizes.  With continuous batching
+every request's result must equal the same plan run alone on its engine,
+requests arriving while others run must have been batched together, and the
+serial baseline must give the same results one request at a time."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from sycldb_amd import plans as PL  # noqa: E402
+
+SIZES = (100_003, 200_003, 300_007)
+
+
+@pytest.fixture(scope="module")
+def engines():
+    from sycldb_amd.engine import Engine
+    es = []
+    for i, n in enumerate(SIZES):
+        e = Engine()
+        e.generate(n, 50 + i)
+        es.append(e)
+    yield es
+    for e in es:
+        e.close()
+
+
+@pytest.fixture(scope="module")
+def trace():
+    rng = np.random.default_rng(5)
+    pipes = PL.random_pipelines(48, 21)
+    reqs = [(int(rng.integers(0, len(SIZES))), PL.build(t, **p)) for t, p in pipes]
+    arrivals = np.cumsum(rng.exponential(0.0002, len(reqs)))  # ~5000 requests/s: arrivals overlap running batches
+    return reqs, arrivals
+
+
+@pytest.fixture(scope="module")
+def alone(engines, trace):
+    reqs, _ = trace
+    return [engines[e].execute(plan).rows() for e, plan in reqs]
+
+
+def test_batched_serving_matches_alone(engines, trace, alone):
+    from sycldb_amd.engine import Server, replay
+    reqs, arrivals = trace
+    print("serving", len(reqs), "requests", flush=True)
+    with Server(engines, max_batch=16) as srv:
+        out = replay(srv, reqs, arrivals)
+        stats = srv.stats()
+    for i, (res, info) in enumerate(out):
+        assert res.rows() == alone[i], i
+        assert info["latency_ms"] >= info["run_ms"] >= 0
+    assert stats["requests"] == len(reqs)
+    assert max(info["batch_size"] for _, info in out) > 1  # arrivals during a batch were batched
+    assert stats["batches"] < len(reqs)
+
+
+def test_serial_serving_matches_alone(engines, trace, alone):
+    from sycldb_amd.engine import Server, replay
+    reqs, arrivals = trace
+    with Server(engines, serial=True) as srv:
+        out = replay(srv, reqs[:16], arrivals[:16])
+    for i, (res, info) in enumerate(out):
+        assert res.rows() == alone[i]
+        assert info["batch_size"] == 1
+
+
+def test_tight_memory_budget_still_serves(engines, trace, alone):
+    """A budget below one request's footprint admits one request per batch."""
+    from sycldb_amd.engine import Server, replay
+    reqs, arrivals = trace
+    with Server(engines, max_batch=16, memory_budget_bytes=1) as srv:
+        out = replay(srv, reqs[:12], arrivals[:12] * 0)
+    for i, (res, info) in enumerate(out):
+        assert res.rows() == alone[i]
+        assert info["batch_size"] == 1
+
+
+def test_bad_request_fails_alone(engines):
+    """An invalid plan is rejected at submission; the server keeps serving."""
+    from sycldb_amd.engine import Server
+    with Server(engines) as srv:
+        bad = PL.build("t1")
+        bad["rels"][5]["exprs"][0]["op"] = "%"
+        t_bad = srv.submit(0, bad)
+        t_ok = srv.submit(0, PL.build("t1"))
+        with pytest.raises(ValueError):
+            srv.wait(t_bad)
+        res, _ = srv.wait(t_ok)
+        assert res.rows() == engines[0].execute(PL.build("t1")).rows()
