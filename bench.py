@@ -57,7 +57,12 @@ def batched_measure(eng, n_pipes, seed, reps):
     from sycldb_amd import plans as PL
     pipes = PL.random_pipelines(n_pipes, seed)
     plans = [PL.build(t, **p) for t, p in pipes]
-    alone = [eng.execute(p).rows() for p in plans]  # warm, and the reference results
+    alone = [eng.execute(p).rows() for p in plans]  # the reference results
+    # warm: the autotuner settles every plan shape (and compiles its
+    # candidates) before anything is timed
+    for _ in range(6):
+        for p in plans:
+            eng.execute(p)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -77,6 +82,68 @@ def batched_measure(eng, n_pipes, seed, reps):
             "sequential_ms": seq_s * 1e3, "batched_ms": bat_s * 1e3,
             "sequential_pipelines_per_s": n_pipes / seq_s, "batched_pipelines_per_s": n_pipes / bat_s,
             "speedup": seq_s / bat_s, "results_identical": True}
+
+
+def served_measure(eng_big, seed, n_requests, qps, max_batch):
+    """Serving: three resident fact tables (6M, 60M and the benchmark's
+    600M rows) with their dimensions; Poisson arrivals at a fixed rate of
+    pipelines from the four templates with random constants and table sizes.
+    The same trace is served one request at a time in arrival order (the
+    plain sequential baseline) and with continuous batching; every result of
+    the batched run must equal the sequential one."""
+    import numpy as np
+    from sycldb_amd import plans as PL
+    from sycldb_amd.engine import Engine, Server, replay
+    # a fixed launch configuration: no autotuning trials while serving
+    engines = []
+    for i, n in enumerate((6_000_000, 60_000_000)):
+        e = Engine(unroll=2)
+        e.generate(n, seed + 10 + i)
+        engines.append(e)
+    eng_big.set_codegen("auto", 2)
+    engines.append(eng_big)
+    try:
+        rng = np.random.default_rng(seed)
+        pipes = PL.random_pipelines(n_requests, seed + 2)
+        reqs = [(int(rng.integers(0, 3)), PL.build(t, **p)) for t, p in pipes]
+        arrivals = np.cumsum(rng.exponential(1.0 / qps, n_requests))
+        # compile every kernel the trace needs before measuring: each plan
+        # alone, and each engine's requests as one batch
+        for e in range(3):
+            sub = [plan for ei, plan in reqs if ei == e]
+            for plan in sub:
+                engines[e].execute(plan)
+            for k in range(0, len(sub), max_batch):
+                if len(sub[k:k + max_batch]) > 1:
+                    engines[e].execute_batch(sub[k:k + max_batch])
+
+        def serve(serial):
+            with Server(engines, max_batch=max_batch, serial=serial) as srv:
+                t0 = time.perf_counter()
+                out = replay(srv, reqs, arrivals)
+                wall = time.perf_counter() - t0
+                stats = srv.stats()
+            lat = np.array([info["latency_ms"] for _, info in out])
+            return out, {"pipelines_per_s": n_requests / wall, "p50_ms": float(np.percentile(lat, 50)),
+                         "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
+                         "batches": stats["batches"], "mean_batch": stats["requests"] / max(stats["batches"], 1),
+                         "wall_s": wall}
+
+        out_seq, seq = serve(True)
+        out_bat, bat = serve(False)
+        identical = all(a.rows() == b.rows() for (a, _), (b, _) in zip(out_seq, out_bat))
+        assert identical, "served results differ from the sequential ones"
+        return {"requests": n_requests, "qps": qps, "tables_rows": [6_000_000, 60_000_000, eng_big_rows(eng_big)],
+                "max_batch": max_batch, "sequential": seq, "batched": bat,
+                "throughput_ratio": bat["pipelines_per_s"] / seq["pipelines_per_s"],
+                "results_identical": identical}
+    finally:
+        for e in engines[:2]:
+            e.close()
+
+
+def eng_big_rows(eng):
+    return getattr(eng, "fact_rows", None)
 
 
 def measured_traffic(workload, rows):
@@ -131,6 +198,10 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-pipelines", type=int, default=32)
     ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-qps", type=float, default=200.0)
+    ap.add_argument("--serve-max-batch", type=int, default=32)
     ap.add_argument("--codegen", default="auto", choices=["auto", "interpret", "compile"])
     ap.add_argument("--unroll", type=int, default=0, choices=[0, 1, 2, 4])
     ap.add_argument("--copy-calibration", action="store_true",
@@ -217,6 +288,12 @@ def main():
         batched = batched_measure(eng, args.batch_pipelines, args.seed + 1, args.batch_reps)
         log("batched: %s" % batched)
 
+    served = None
+    if not args.no_served:
+        eng.fact_rows = rows
+        served = served_measure(eng, args.seed + 3, args.serve_requests, args.serve_qps, args.serve_max_batch)
+        log("served: %s" % served)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(plan, fact_cols, min(args.cpu_sample_rows, rows), args.seed)
@@ -243,6 +320,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
     eng.close()

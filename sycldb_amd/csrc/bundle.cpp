@@ -710,8 +710,11 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
             if (writes_flag(I.kind)) written[I.fout] = true;
         }
     }
-    // LDS left for staged probe tables
-    const uint32_t stage_cap = std::min<uint32_t>(48u * 1024u, lds_budget - std::min(lds_budget, group_area));
+    // LDS left for staged probe tables.  A program captured for a batched
+    // launch stages none: the batch decides its probes on fact key columns
+    // through shared masks, and its LDS share goes to the group-by partials.
+    const uint32_t stage_cap =
+        captured ? 0u : std::min<uint32_t>(48u * 1024u, lds_budget - std::min(lds_budget, group_area));
     // probe costs follow where the tables will live
     {
         const std::vector<Staging> st = choose_staged(d, stage_cap);

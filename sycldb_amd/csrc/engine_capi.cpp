@@ -309,6 +309,18 @@ static bool debug_batch()
     return on;
 }
 
+// SDB_BATCH_MASKS=0 keeps every probe inside its program in batched launches
+// (no shared probe masks).
+static bool batch_masks()
+{
+    const char *e = getenv("SDB_BATCH_MASKS");
+    return !(e && *e == '0');
+}
+
+namespace sdb {
+static void *batch_alloc(void *ctx, size_t bytes) { return static_cast<Engine *>(ctx)->mm->alloc<uint8_t>(bytes); }
+} // namespace sdb
+
 // Batched execution: every plan's dimension work runs as usual; the
 // fact-table bundles of the plans that fuse are captured and run together in
 // one kernel per segment (one scan of the union of their fact columns); the
@@ -317,6 +329,20 @@ static bool debug_batch()
 namespace sdb {
 std::vector<ResultTable> execute_batch(Engine &eng, const std::vector<PlanResult> &parsed, int *batched)
 {
+    if (parsed.size() > F_BATCH_MAX) {
+        // in groups of F_BATCH_MAX (each program's LDS share is fixed)
+        std::vector<ResultTable> all;
+        int total = 0;
+        for (size_t k = 0; k < parsed.size(); k += F_BATCH_MAX) {
+            std::vector<PlanResult> part(parsed.begin() + k, parsed.begin() + std::min(parsed.size(), k + F_BATCH_MAX));
+            int b = 0;
+            std::vector<ResultTable> r = execute_batch(eng, part, &b);
+            total += b;
+            for (auto &t : r) all.push_back(std::move(t));
+        }
+        if (batched) *batched = total;
+        return all;
+    }
     const int n = (int)parsed.size();
     if (batched) *batched = 0;
     eng.mm->reset(eng.stream);
@@ -329,7 +355,7 @@ std::vector<ResultTable> execute_batch(Engine &eng, const std::vector<PlanResult
     std::vector<Pipe> pipes(n);
     const bool can = eng.fusion && eng.codegen != SDB_CODEGEN_INTERPRET && jit_available() && n > 1;
     const std::string fact = "lineorder";
-    const uint32_t lds_each = n > 0 ? std::min<uint32_t>(F_LDS_BUDGET, (F_BATCH_LDS / (uint32_t)n) & ~15u) : 0;
+    const uint32_t lds_each = F_BATCH_LDS_EACH;
     for (int i = 0; i < n && can; i++) {
         eng.capture = Capture{};
         eng.capture.active = true;
@@ -365,6 +391,7 @@ std::vector<ResultTable> execute_batch(Engine &eng, const std::vector<PlanResult
         for (int i : members)
             if (pipes[i].bundles.size() != nseg) fail(SDB_ERR_RUNTIME, "batched pipelines disagree on segments");
         FProgram *dev_progs = eng.mm->alloc<FProgram>(members.size());
+        try {
         for (size_t sgi = 0; sgi < nseg; sgi++) {
             std::vector<FProgram> progs;
             const int64_t rows = pipes[members[0]].bundles[sgi].n;
@@ -375,9 +402,18 @@ std::vector<ResultTable> execute_batch(Engine &eng, const std::vector<PlanResult
             eng.registry.check(dev_progs, members.size() * sizeof(FProgram), "batched programs");
             Engine::ProfRec rec;
             if (eng.profile) eng.prof_begin("batch:" + fact, rows, rec);
-            launch_fused_batch(progs, rows, eng.stream, dev_progs, eng.unroll);
+            launch_fused_batch(progs, rows, eng.stream, dev_progs, eng.unroll, eng.find_table(fact), batch_alloc,
+                               &eng, batch_masks());
             if (eng.profile) eng.prof_end(rec);
             eng.launches++;
+        }
+        } catch (const Error &e) {
+            // a combination the batched kernel does not take (found before
+            // its first launch): every member runs through the plain path
+            if (e.code != SDB_ERR_UNSUPPORTED) throw;
+            if (debug_batch()) fprintf(stderr, "[batch] shared pass refused: %s\n", e.what());
+            for (int i : members) pipes[i].batched = false;
+            members.clear();
         }
         for (int i : members)
             for (auto &f : pipes[i].deferred) f();

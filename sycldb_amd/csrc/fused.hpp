@@ -34,10 +34,22 @@ std::string jit_tuning_report();
 std::string jit_source(const FProgram &p, int unroll);
 // Number of distinct kernels compiled so far in this process.
 int jit_kernels_compiled();
+// Device memory for one batched launch (mask tables and their descriptors),
+// valid until the launch has completed.
+typedef void *(*BatchAlloc)(void *ctx, size_t bytes);
+
 // Runs the fact-table programs of several pipelines over rows [0, n) in one
 // launch (one scan of their columns).  dev_progs: device room for
-// in.size() programs.  unroll: 1, 2 or 4 (0: default).
-void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t s, FProgram *dev_progs, int unroll);
+// in.size() programs.  unroll: 1, 2 or 4 (0: default).  scope: identity of
+// the fact table, whose batches share one library of compiled shapes.
+// Probes on fact key columns that every row of a program must pass are
+// evaluated once per row for all programs through shared mask tables
+// (FMaskCol) allocated with `alloc`; masks = false keeps every probe in its
+// program.
+void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t s, FProgram *dev_progs, int unroll,
+                        const void *scope, BatchAlloc alloc, void *alloc_ctx, bool masks = true);
+// Builds the mask tables of a batched launch (ops.hip).
+void launch_batch_masks(const FMaskCol *cols, int ncols, uint32_t max_len, const FMaskEntry *ents, hipStream_t s);
 // Compiles (no device needed) a batched kernel over the coverage programs.
 int jit_batch_check(const std::string &arch);
 // Compiles (without launching; no device needed) the kernels of programs

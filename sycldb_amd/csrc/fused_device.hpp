@@ -134,24 +134,25 @@ __device__ __forceinline__ void gb_row(const FGroup &g, uint8_t *lds, const int3
     else gb_row_mode<FG_GLOBAL>(g, lds, kv, agg);
 }
 
-// Block start: stage probe bitmaps into LDS and clear the aggregation area.
-__device__ __forceinline__ void block_prologue(const FProgram &P, uint8_t *lds)
+// Block start: stage probe bitmaps into LDS and clear the aggregation area
+// (NT: threads per block).
+template <int NT = BLOCK> __device__ __forceinline__ void block_prologue(const FProgram &P, uint8_t *lds)
 {
     for (int b = 0; b < P.nblobs; b++) {
         const uint4 *src = reinterpret_cast<const uint4 *>(P.blob_src[b]);
         uint4 *dst = reinterpret_cast<uint4 *>(lds + P.blob_off[b]);
-        for (uint32_t w = threadIdx.x; w < P.blob_bytes[b] / 16; w += BLOCK) dst[w] = src[w];
+        for (uint32_t w = threadIdx.x; w < P.blob_bytes[b] / 16; w += NT) dst[w] = src[w];
     }
     const FGroup &g = P.gb;
     if (g.lds == FG_DENSE_LDS) {
         unsigned long long *lsum = reinterpret_cast<unsigned long long *>(lds + g.lds_off);
         uint32_t *lbits = reinterpret_cast<uint32_t *>(lsum + g.prod);
-        for (uint32_t s = threadIdx.x; s < (uint32_t)g.prod; s += BLOCK) lsum[s] = 0;
-        for (uint32_t w = threadIdx.x; w < ((uint32_t)g.prod + 31) / 32; w += BLOCK) lbits[w] = 0;
+        for (uint32_t s = threadIdx.x; s < (uint32_t)g.prod; s += NT) lsum[s] = 0;
+        for (uint32_t w = threadIdx.x; w < ((uint32_t)g.prod + 31) / 32; w += NT) lbits[w] = 0;
     } else if (g.lds == FG_HASH_LDS) {
         uint32_t *hk = reinterpret_cast<uint32_t *>(lds + g.lds_off);
         unsigned long long *hv = reinterpret_cast<unsigned long long *>(hk + g.hash_cap);
-        for (uint32_t h = threadIdx.x; h < g.hash_cap; h += BLOCK) {
+        for (uint32_t h = threadIdx.x; h < g.hash_cap; h += NT) {
             hk[h] = HASH_EMPTY;
             hv[h] = 0;
         }
@@ -160,11 +161,11 @@ __device__ __forceinline__ void block_prologue(const FProgram &P, uint8_t *lds)
 }
 
 // Block end: the terminal sum (one atomic per block) and the flush of the
-// block's LDS group-by partials.
-__device__ __forceinline__ void block_epilogue(const FProgram &P, uint8_t *lds, int64_t acc)
+// block's LDS group-by partials (NT: threads per block).
+template <int NT = BLOCK> __device__ __forceinline__ void block_epilogue(const FProgram &P, uint8_t *lds, int64_t acc)
 {
     if (P.sum_acc) {
-        __shared__ int64_t part[BLOCK / 64];
+        __shared__ int64_t part[NT / 64];
         uint64_t v = (uint64_t)acc;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -172,7 +173,7 @@ __device__ __forceinline__ void block_epilogue(const FProgram &P, uint8_t *lds, 
         __syncthreads();
         if (threadIdx.x == 0) {
             uint64_t s = 0;
-            for (int w = 0; w < BLOCK / 64; w++) s += (uint64_t)part[w];
+            for (int w = 0; w < NT / 64; w++) s += (uint64_t)part[w];
             if (s) atomicAdd(reinterpret_cast<unsigned long long *>(P.sum_acc), (unsigned long long)s);
         }
     }
@@ -181,13 +182,13 @@ __device__ __forceinline__ void block_epilogue(const FProgram &P, uint8_t *lds, 
         __syncthreads();
         const unsigned long long *lsum = reinterpret_cast<const unsigned long long *>(lds + g.lds_off);
         const uint32_t *lbits = reinterpret_cast<const uint32_t *>(lsum + g.prod);
-        for (uint32_t s = threadIdx.x; s < (uint32_t)g.prod; s += BLOCK)
+        for (uint32_t s = threadIdx.x; s < (uint32_t)g.prod; s += NT)
             if ((lbits[s >> 5] >> (s & 31)) & 1u) gb_flush_slot(g, s, lsum[s]);
     } else if (g.lds == FG_HASH_LDS) {
         __syncthreads();
         const uint32_t *hk = reinterpret_cast<const uint32_t *>(lds + g.lds_off);
         const unsigned long long *hv = reinterpret_cast<const unsigned long long *>(hk + g.hash_cap);
-        for (uint32_t h = threadIdx.x; h < g.hash_cap; h += BLOCK)
+        for (uint32_t h = threadIdx.x; h < g.hash_cap; h += NT)
             if (hk[h] != HASH_EMPTY) gb_flush_slot(g, hk[h], hv[h]);
     }
 }

@@ -12,6 +12,9 @@
 
 #include "fused_device.hpp"
 
+// OR of a value over the active lanes of the wavefront (ROCm device library)
+extern "C" __device__ __attribute__((const)) unsigned int __ockl_wfred_or_u32(unsigned int);
+
 namespace sdb {
 namespace fops {
 
@@ -203,6 +206,34 @@ __device__ __forceinline__ uint32_t probe_bits_finish(const int32_t (&x)[R], uin
         }
     }
     return out;
+}
+
+// The value of a full probe whose match the shared masks of a batched launch
+// have already established for the rows that matter: the table's value for
+// every row of `in` whose key is in range (FV_U16, FV_PAIR, or the reference
+// pairs of a FullJoinKernel table when VKIND is FV_NONE).
+template <int R, int VKIND>
+__device__ __forceinline__ void probe_value_issue(const int32_t (&x)[R], uint32_t in, const void *vals, int32_t mn,
+                                                  int32_t mx, int32_t vmin, ProbeTmp<R> &t)
+{
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        t.val[k] = 0;
+        if (((in >> k) & 1u) && x[k] >= mn && x[k] <= mx) {
+            const uint32_t s = slot_of(x[k], mn);
+            if constexpr (VKIND == FV_U16)
+                t.val[k] = (int32_t)((uint32_t)vmin + reinterpret_cast<const uint16_t *>(vals)[s]);
+            else
+                t.val[k] = reinterpret_cast<const int2 *>(vals)[s].y;
+        }
+    }
+}
+
+template <int R> __device__ __forceinline__ void probe_value_finish(uint32_t out, const ProbeTmp<R> &t, int32_t (&v)[R])
+{
+#pragma unroll
+    for (int k = 0; k < R; k++)
+        if ((out >> k) & 1u) v[k] = t.val[k];
 }
 
 template <int R>

@@ -135,8 +135,38 @@ struct FProgram {
     FInstr ins[F_MAX_INSTR];
 };
 
-// LDS shared by the programs of one batched launch.
-constexpr uint32_t F_BATCH_LDS = 64 * 1024;
+// LDS shared by the programs of one batched launch, and each program's share
+// (a fixed share, so that a program's generated code does not depend on how
+// many others it is batched with).
+constexpr uint32_t F_BATCH_LDS = 128 * 1024;
+constexpr uint32_t F_BATCH_MAX = 32;
+constexpr uint32_t F_BATCH_LDS_EACH = F_BATCH_LDS / F_BATCH_MAX;
+
+// Shared probe masks of a batched launch.  For one fact key column, a table
+// over keys [lo, lo + len) of words whose bit i is 1 when every probe of
+// program i on that column passes for the key (1 when program i has none);
+// keys outside the table take `dflt` (only the bits of programs without a
+// probe on the column).  A program's row can only reach its terminal where
+// the AND of the row's words over all masked columns has its bit set.
+enum FMaskKind : uint32_t {
+    FM_BITS = 0, // presence bitmap, 32 slots per word
+    FM_SEMI = 1, // the reference's bool table (BuildKeysHTKernel)
+    FM_PAIR = 2  // the reference's (flag, value) pairs (BuildKeyValsHTKernel)
+};
+struct FMaskEntry {
+    const void *ht;
+    int32_t ht_min, ht_max;
+    uint32_t bit, kind;
+};
+struct FMaskCol {
+    uint32_t *tab;
+    int32_t lo;
+    uint32_t len;
+    uint32_t full;         // bits of every program of the launch
+    uint32_t first, count; // its entries
+};
+constexpr int F_MASK_MAX_COLS = 8;
+constexpr int64_t F_MASK_MAX_LEN = int64_t(1) << 24;
 
 // Largest rank dictionary built for LDS staging.
 constexpr uint32_t F_RANK_DICT_MAX = 32 * 1024;

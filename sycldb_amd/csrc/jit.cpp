@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <sstream>
@@ -120,7 +121,21 @@ struct Gen {
     const FProgram &p;
     std::ostringstream o;
     bool prefetch_in_body; // the body issues the next tile's column loads
+    // batched form: the body takes `act` (rows whose shared probe masks keep
+    // this program); flags start from it and the terminal reads only its
+    // rows.  For the probes marked in `decided` the masks have established
+    // the match on every `act` row: a semi probe passes its input flags
+    // through, a full probe only fetches its value.
+    bool batch_act = false;
+    std::vector<bool> decided;
     explicit Gen(const FProgram &prog, bool prefetch = true) : p(prog), prefetch_in_body(prefetch) {}
+
+    bool is_decided(int i) const { return i < (int)decided.size() && decided[i]; }
+    bool dropped(int i) const
+    {
+        const FInstr &I = p.ins[i];
+        return is_decided(i) && (I.kind == FK_PROBE_SEMI || (I.kind == FK_PROBE_BITS && I.vkind == FV_NONE));
+    }
 
     static std::string freg(int j) { return "f" + std::to_string(j); }
     static std::string vreg(int r) { return "v" + std::to_string(r); }
@@ -150,7 +165,13 @@ struct Gen {
         if (cheap(I.va)) a << U << "{ " << operand(I.va, P + ".a", x) << " }\n";
         else a << U << "if (" << gate << ") { " << operand(I.va, P + ".a", x) << " }\n";
         a << U << "if (" << gate << ") ";
-        if (I.kind == FK_PROBE_SEMI)
+        if (is_decided(i)) {
+            // the value alone (FV_U16 / FV_PAIR tables, or the reference pairs)
+            const int vk = I.kind == FK_PROBE_FULL ? (int)FV_NONE : (int)I.vkind;
+            const std::string vals = I.kind == FK_PROBE_FULL ? P + ".ht" : P + ".vals";
+            a << "probe_value_issue<R, " << vk << ">(" << x << ", " << gate << ", " << vals << ", " << P << ".ht_min, "
+              << P << ".ht_max, " << P << ".vmin, " << t << ");\n";
+        } else if (I.kind == FK_PROBE_SEMI)
             a << "probe_semi_issue<R>(" << x << ", " << gate << ", (const uint8_t *)" << P << ".ht, " << P << ".ht_min, "
               << P << ".ht_max, " << t << ");\n";
         else if (I.kind == FK_PROBE_FULL)
@@ -207,6 +228,7 @@ struct Gen {
     bool waits(int i) const
     {
         const FInstr &I = p.ins[i];
+        if (dropped(i)) return false;
         switch (I.kind) {
         case FK_LOGICAL:
         case FK_FILL: return false;
@@ -236,7 +258,15 @@ struct Gen {
         const std::string fin = freg(I.fin) + "[u]", fout = freg(I.fout) + "[u]";
         const std::string L = std::to_string((int)I.logic);
         const std::string U = "#pragma unroll\n        for (int u = 0; u < UU; u++) ";
+        // the rows a terminal instruction acts on
+        const std::string fterm = batch_act ? "(" + fin + " & act[u])" : fin;
         std::ostringstream d, a, c; // declarations, load phase, compute phase
+        if (dropped(i)) {
+            o << "    { // " << i << " (decided by the shared masks)\n";
+            if (I.fin != I.fout) o << U << fout << " = " << fin << ";\n";
+            o << (split.empty() ? "" : hoist_code + split) << "    }\n";
+            return;
+        }
         switch (I.kind) {
         case FK_SEL_LIT:
         case FK_SEL_COL:
@@ -299,6 +329,12 @@ struct Gen {
             const std::string x = "x" + std::to_string(i) + "[u]", t = "t" + std::to_string(i) + "[u]";
             const bool vout = I.kind == FK_PROBE_FULL || (I.kind == FK_PROBE_BITS && I.vkind != FV_NONE);
             const std::string vdst = vout ? vreg(I.vout) + "[u]" : "none";
+            if (is_decided(i)) {
+                c << U << "{\n            const uint32_t o = " << fin << " & act[u];\n";
+                c << "            if (o) probe_value_finish<R>(o, " << t << ", " << vdst << ");\n";
+                c << "            " << fout << " = o;\n        }\n";
+                break;
+            }
             c << U << "{\n            uint32_t o = 0;\n";
             if (!vout) c << "            int32_t none[R];\n";
             c << "            if (" << fin << ") {\n";
@@ -333,8 +369,8 @@ struct Gen {
             break;
         case FK_SUM:
             d << "        int32_t x[UU][R];\n";
-            a << U << "if (" << fin << ") { " << operand(I.va, P + ".a", "x[u]") << " }\n";
-            c << U << "if (" << fin << ") sum_rows<R>(x[u], " << fin << ", acc);\n";
+            a << U << "if (" << fterm << ") { " << operand(I.va, P + ".a", "x[u]") << " }\n";
+            c << U << "if (" << fterm << ") sum_rows<R>(x[u], " << fterm << ", acc);\n";
             break;
         case FK_GROUPBY: {
             // operands of every chunk first; then one loop over the flagged
@@ -344,7 +380,7 @@ struct Gen {
             d << "        typename Lanes<UU * R>::type ax = {}";
             for (int j = 0; j < g.k; j++) d << ", kx" << j << " = {}";
             d << ";\n        uint32_t rows = 0;\n";
-            a << U << "{\n            rows |= " << fin << " << (u * R);\n            if (!" << fin << ") continue;\n";
+            a << U << "{\n            rows |= " << fterm << " << (u * R);\n            if (!" << fterm << ") continue;\n";
             a << "            int32_t t[R];\n";
             for (int j = 0; j < g.k; j++)
                 a << "            " << operand(g.key_reg[j], "P.gb.key_mem[" + std::to_string(j) + "]", "t")
@@ -382,14 +418,14 @@ struct Gen {
             if (!fused[j]) continue;
             o << "    uint32_t " << freg(j) << "[UU];\n";
             o << "#pragma unroll\n    for (int u = 0; u < UU; u++) " << freg(j) << "[u] = ";
-            if (p.flag_init[j] == FF_TRUE) o << "M;\n";
+            if (p.flag_init[j] == FF_TRUE) o << (batch_act ? "act[u];\n" : "M;\n");
             else if (p.flag_init[j] == FF_LOAD && p.flag_load[j])
-                o << "fdev::flags_in<R>(P.flag_load[" << j << "] + row0[u]);\n";
+                o << "fdev::flags_in<R>(P.flag_load[" << j << "] + row0[u])" << (batch_act ? " & act[u]" : "") << ";\n";
             else o << "0u;\n";
         }
         for (int r = 0; r < p.nvals; r++) o << "    int32_t " << vreg(r) << "[UU][R] = {};\n";
         for (int i = 0; i < p.ninstr; i++)
-            if (is_probe(p.ins[i].kind))
+            if (is_probe(p.ins[i].kind) && !dropped(i))
                 o << "    int32_t x" << i << "[UU][R];\n    ProbeTmp<R> t" << i << "[UU];\n";
         // the next tile's column loads go out after the loads of the first
         // instruction that waits for global memory and before that wait, so
@@ -400,16 +436,19 @@ struct Gen {
         for (int i = 0; i < p.ninstr && first_wait < 0; i++)
             if (waits(i)) first_wait = i;
         // early issue of chained probes trades extra table reads for one
-        // fewer wait; measured slower on the star joins, so opt-in
-        // (SDB_JIT_HOIST=1)
+        // fewer wait; measured slower on the star joins of a single
+        // program, so opt-in there (SDB_JIT_HOIST=1); a batched body, which
+        // runs between the other programs' bodies, always issues early
         static const bool hoist = [] {
             const char *e = getenv("SDB_JIT_HOIST");
             return e && *e == '1';
         }();
-        plan_hoisting(hoist && prefetch_in_body ? first_wait : -1);
+        plan_hoisting((hoist && prefetch_in_body) || batch_act ? first_wait : -1);
         if (first_wait < 0) o << prefetch;
-        if (!prefetch_in_body) first_wait = -1;
-        for (int i = 0; i < p.ninstr; i++) instr(i, i == first_wait ? prefetch : std::string());
+        const std::string at_wait =
+            prefetch_in_body ? prefetch : (hoist_code.empty() ? std::string() : "        // issued early above\n");
+        if (at_wait.empty()) first_wait = -1;
+        for (int i = 0; i < p.ninstr; i++) instr(i, i == first_wait ? at_wait : std::string());
         for (int j = 0; j < F_MAX_FLAGS; j++)
             if (p.flag_store[j])
                 o << "#pragma unroll\n    for (int u = 0; u < UU; u++) fdev::flags_out<R>(P.flag_store[" << j
@@ -514,14 +553,21 @@ extern "C" __global__ void __launch_bounds__(256) sdb_fused_program(const FProgr
 
 std::string device_arch()
 {
+    static std::mutex mu;
+    static std::map<int, std::string> cache;
     int dev = 0;
     SDB_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(dev);
+    if (it != cache.end()) return it->second;
     hipDeviceProp_t prop;
     SDB_HIP(hipGetDeviceProperties(&prop, dev));
     std::string a(prop.gcnArchName);
     const size_t colon = a.find(':');
     if (colon != std::string::npos) a.resize(colon);
-    return a.empty() ? std::string("gfx950") : a;
+    if (a.empty()) a = "gfx950";
+    cache[dev] = a;
+    return a;
 }
 
 // SDB_JIT_DUMP_DIR=<dir> writes every source compiled to <dir>/sdb_fused_<n>.hip.
@@ -887,24 +933,42 @@ void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll)
 // ------------------------------------------------------------ batched kernel
 //
 // Many pipelines' fact-table programs in one launch: one scan of the union of
-// their prefetched columns; per tile, the programs of each shape (identical
-// generated body) run in a loop that reads each program from a device array,
-// so pipelines that differ only in constants and tables share code.  Each
-// program has its own LDS region (lds_base) and its own scalar-sum slot.
+// their prefetched columns.  Probes on fact key columns that a program's rows
+// must all pass to reach its terminal are evaluated for every program at once:
+// per masked column one table lookup per row gives a word of program bits
+// (FMaskCol), and the AND of a row's words says which programs the row can
+// still contribute to.  Per tile, the programs of each shape (identical
+// generated body) then run in a loop that reads each program from a device
+// array, skipping programs no row of the wave keeps; a program sees only its
+// kept rows (`act`), and the probes the masks decided are left out of its
+// body.  Each program has its own LDS region (lds_base) and its own
+// scalar-sum slot.
 
 namespace {
 
-constexpr int BATCH_MAX_PF = 12;
+constexpr int BATCH_MAX_PF = 16;
+// threads per block of the batched kernel: one block per CU holds 16 waves
+// and the whole LDS budget of the batch's programs
+constexpr int BATCH_BLOCK = 1024;
+constexpr int64_t BATCH_MIN_TILES = 8;
 constexpr int BATCH_MAX_SHAPES = 32;
+// a probe joins the shared masks only over a key range of at most this many
+// keys (4 bytes of table per key)
+constexpr int64_t BATCH_MASK_PROBE_LEN = int64_t(1) << 22;
 
-const char *BATCH_HEAD = R"(constexpr int BLOCK = fdev::BLOCK;
+const char *BATCH_HEAD = R"(constexpr int BLOCK = 1024;
 struct BatchArgs {
     const FProgram *progs;
     int64_t n;
     int32_t nprog;
     uint32_t sums_off;
+    uint32_t full;
     int32_t first[NSHAPE + 1];
     const int32_t *pf[NPF > 0 ? NPF : 1];
+    const uint32_t *mtab[NMASK > 0 ? NMASK : 1];
+    int32_t mlo[NMASK > 0 ? NMASK : 1];
+    uint32_t mlen[NMASK > 0 ? NMASK : 1];
+    uint32_t mdflt[NMASK > 0 ? NMASK : 1];
 };
 template <int R, int UU> struct Pf {
     typename Vec<R>::type c[NPF > 0 ? NPF : 1][UU];
@@ -914,21 +978,70 @@ __device__ __forceinline__ void pf_load(const BatchArgs &A, const int64_t (&row0
 {
 #pragma unroll
     for (int j = 0; j < NPF; j++)
+        if (A.pf[j]) // columns this batch does not read stay unloaded
 #pragma unroll
-        for (int u = 0; u < UU; u++) pf.c[j][u] = load_vec<R>(A.pf[j], row0[u]);
+            for (int u = 0; u < UU; u++) pf.c[j][u] = load_vec<R>(A.pf[j], row0[u]);
+}
+// Program bits of every row: the AND of the row's words in the mask tables
+// of the masked columns (a column this batch does not mask has a one-word
+// table of all ones).  Indexes are clamped into the table so every load is
+// unconditional; out-of-table keys then take the column's default word.
+template <int R, int UU>
+__device__ __forceinline__ void mask_rows(const BatchArgs &A, const Pf<R, UU> &pf, uint32_t (&mw)[UU][R])
+{
+    uint32_t w[NMASK > 0 ? NMASK : 1][UU][R];
+#pragma unroll
+    for (int m = 0; m < NMASK; m++)
+#pragma unroll
+        for (int u = 0; u < UU; u++) {
+            int32_t x[R];
+            vec_rows<R>(pf.c[MSLOT[m]][u], x);
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                const uint32_t idx = (uint32_t)x[k] - (uint32_t)A.mlo[m];
+                w[m][u][k] = A.mtab[m][idx < A.mlen[m] ? idx : A.mlen[m] - 1u];
+            }
+        }
+#pragma unroll
+    for (int u = 0; u < UU; u++)
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            uint32_t v = A.full;
+#pragma unroll
+            for (int m = 0; m < NMASK; m++) {
+                int32_t x[R];
+                vec_rows<R>(pf.c[MSLOT[m]][u], x);
+                const uint32_t idx = (uint32_t)x[k] - (uint32_t)A.mlo[m];
+                v &= idx < A.mlen[m] ? w[m][u][k] : A.mdflt[m];
+            }
+            mw[u][k] = v;
+        }
+}
+// The programs of the launch, read-only while it runs: reading them through
+// the constant address space lets their fields go through scalar loads.
+typedef const FProgram __attribute__((address_space(4))) *ProgPtr;
+template <int R, int UU>
+__device__ __forceinline__ void rows_of_program(const uint32_t (&mw)[UU][R], int i, uint32_t (&act)[UU])
+{
+#pragma unroll
+    for (int u = 0; u < UU; u++) {
+        act[u] = 0;
+#pragma unroll
+        for (int k = 0; k < R; k++) act[u] |= ((mw[u][k] >> i) & 1u) << k;
+    }
 }
 )";
 
 const char *BATCH_TAIL = R"(
-extern "C" __global__ void __launch_bounds__(256) sdb_fused_batch(const BatchArgs A_arg)
+extern "C" __global__ void __launch_bounds__(BLOCK) sdb_fused_batch(const BatchArgs A_arg)
 {
     const BatchArgs &A = *(const BatchArgs __attribute__((address_space(4))) *)__builtin_amdgcn_kernarg_segment_ptr();
     extern __shared__ uint4 dyn_lds[];
     uint8_t *lds = reinterpret_cast<uint8_t *>(dyn_lds);
     for (int i = 0; i < A.nprog; i++) {
-        const FProgram &P = A.progs[i];
+        const FProgram &P = ((ProgPtr)A.progs)[i];
         if (P.magic != F_MAGIC) return;
-        fdev::block_prologue(P, lds + P.lds_base);
+        fdev::block_prologue<BLOCK>(P, lds + P.lds_base);
     }
     unsigned long long *sums = reinterpret_cast<unsigned long long *>(lds + A.sums_off);
     for (int i = threadIdx.x; i < A.nprog; i += BLOCK) sums[i] = 0;
@@ -941,7 +1054,7 @@ extern "C" __global__ void __launch_bounds__(256) sdb_fused_batch(const BatchArg
             for (int u = 0; u < U; u++) r[u] = (t * tile + u * BLOCK + threadIdx.x) << 2;
         };
         int64_t t = blockIdx.x;
-        Pf<4, U> next;
+        Pf<4, U> next = {};
         if (t < ntiles) {
             int64_t r[U];
             rows_of(t, r);
@@ -963,108 +1076,326 @@ extern "C" __global__ void __launch_bounds__(256) sdb_fused_batch(const BatchArg
         for (int64_t c = ntiles * tile + (int64_t)blockIdx.x * BLOCK + threadIdx.x; c < nch;
              c += (int64_t)gridDim.x * BLOCK) {
             const int64_t r[1] = {c << 2};
-            Pf<4, 1> pf;
+            Pf<4, 1> pf = {};
             pf_load<4, 1>(A, r, pf);
             run_all<4, 1>(A, r, pf, lds, sums);
         }
         if (blockIdx.x == gridDim.x - 1 && threadIdx.x < (n & 3)) {
             const int64_t r[1] = {(nch << 2) + threadIdx.x};
-            Pf<1, 1> pf;
+            Pf<1, 1> pf = {};
             pf_load<1, 1>(A, r, pf);
             run_all<1, 1>(A, r, pf, lds, sums);
         }
     } else {
         for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
             const int64_t r[1] = {i};
-            Pf<1, 1> pf;
+            Pf<1, 1> pf = {};
             pf_load<1, 1>(A, r, pf);
             run_all<1, 1>(A, r, pf, lds, sums);
         }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < A.nprog; i += BLOCK)
-        if (sums[i] && A.progs[i].sum_acc)
-            atomicAdd(reinterpret_cast<unsigned long long *>(A.progs[i].sum_acc), sums[i]);
+        if (sums[i] && ((ProgPtr)A.progs)[i].sum_acc)
+            atomicAdd(reinterpret_cast<unsigned long long *>(((ProgPtr)A.progs)[i].sum_acc), sums[i]);
     for (int i = 0; i < A.nprog; i++) {
-        const FProgram &P = A.progs[i];
-        fdev::block_epilogue(P, lds + P.lds_base, 0);
+        const FProgram &P = ((ProgPtr)A.progs)[i];
+        fdev::block_epilogue<BLOCK>(P, lds + P.lds_base, 0);
     }
 }
 )";
 
-// Rewrites a program's prefetch registers to indices of the union list
-// `pf` (adding its columns); a column beyond BATCH_MAX_PF is read from memory
-// instead.
-void remap_prefetch(FProgram &q, std::vector<const int32_t *> &pf)
+// Shape library of the batched kernels (per device, fact table, chunk width
+// and vector mode): the bodies of every program shape met so far, a stable
+// prefetch slot for every fact column met so far, and the slots masked so
+// far.  A batched kernel contains all the library's bodies (absent shapes run
+// empty loops), so batches of different composition reuse one compiled
+// kernel once their shapes are known.
+struct BatchLibrary {
+    std::vector<std::string> bodies;
+    std::vector<const int32_t *> slots;
+    std::vector<int> mask_slots;
+};
+
+std::mutex g_lib_mu;
+std::map<std::string, BatchLibrary> g_libs;
+
+// Rewrites a program's prefetch registers to library slots (adding its
+// columns); a column beyond BATCH_MAX_PF slots is read from memory instead.
+void remap_prefetch(FProgram &q, std::vector<const int32_t *> &slots)
 {
     int map[F_MAX_PF];
     for (int j = 0; j < q.npf; j++) {
-        auto it = std::find(pf.begin(), pf.end(), q.pf_mem[j]);
-        if (it != pf.end()) map[j] = (int)(it - pf.begin());
-        else if ((int)pf.size() < BATCH_MAX_PF) {
-            map[j] = (int)pf.size();
-            pf.push_back(q.pf_mem[j]);
+        auto it = std::find(slots.begin(), slots.end(), q.pf_mem[j]);
+        if (it != slots.end()) map[j] = (int)(it - slots.begin());
+        else if ((int)slots.size() < BATCH_MAX_PF) {
+            map[j] = (int)slots.size();
+            slots.push_back(q.pf_mem[j]);
         } else map[j] = -1;
     }
-    auto fix = [&](int8_t &reg, const int32_t *&mem, const int32_t *col) {
+    auto fix = [&](int8_t &reg, const int32_t *&mem) {
         if (reg < F_PF_BASE) return;
         const int j = reg - F_PF_BASE;
-        if (map[j] >= 0) reg = (int8_t)(F_PF_BASE + map[j]);
-        else {
+        if (map[j] >= 0) {
+            reg = (int8_t)(F_PF_BASE + map[j]);
+        } else {
+            mem = q.pf_mem[j];
             reg = -1;
-            mem = col;
         }
     };
     for (int i = 0; i < q.ninstr; i++) {
         FInstr &I = q.ins[i];
-        if (I.va >= F_PF_BASE) fix(I.va, I.a, q.pf_mem[I.va - F_PF_BASE]);
-        if (I.vb >= F_PF_BASE) fix(I.vb, I.b, q.pf_mem[I.vb - F_PF_BASE]);
+        fix(I.va, I.a);
+        fix(I.vb, I.b);
     }
-    for (int j = 0; j < q.gb.k; j++)
-        if (q.gb.key_reg[j] >= F_PF_BASE) fix(q.gb.key_reg[j], q.gb.key_mem[j], q.pf_mem[q.gb.key_reg[j] - F_PF_BASE]);
-    if (q.gb.agg_reg >= F_PF_BASE) fix(q.gb.agg_reg, q.gb.agg_mem, q.pf_mem[q.gb.agg_reg - F_PF_BASE]);
-    q.npf = 0; // the batch kernel loads the union list itself
+    for (int j = 0; j < q.gb.k; j++) fix(q.gb.key_reg[j], q.gb.key_mem[j]);
+    fix(q.gb.agg_reg, q.gb.agg_mem);
+    q.npf = 0; // the batch kernel loads the slots itself
+    // columns the program reads from memory (lazily, per flagged row) come
+    // from the shared scan too, while slots last: the scan streams every
+    // slot once for all programs
+    auto lift = [&](int8_t &reg, const int32_t *&mem) {
+        if (reg >= 0 || !mem) return;
+        auto it = std::find(slots.begin(), slots.end(), mem);
+        if (it == slots.end()) {
+            if ((int)slots.size() >= BATCH_MAX_PF) return;
+            slots.push_back(mem);
+            it = slots.end() - 1;
+        }
+        reg = (int8_t)(F_PF_BASE + (it - slots.begin()));
+        mem = nullptr;
+    };
+    for (int i = 0; i < q.ninstr; i++) {
+        FInstr &I = q.ins[i];
+        switch (I.kind) {
+        case FK_SEL_COL:
+        case FK_BINOP_CC:
+            lift(I.va, I.a);
+            lift(I.vb, I.b);
+            break;
+        case FK_BINOP_LC: lift(I.vb, I.b); break;
+        case FK_SEL_LIT:
+        case FK_SEL_RANGE:
+        case FK_SEL_IN2:
+        case FK_BINOP_CL:
+        case FK_PROBE_SEMI:
+        case FK_PROBE_FULL:
+        case FK_PROBE_BITS:
+        case FK_SUM: lift(I.va, I.a); break;
+        default: break;
+        }
+    }
+    for (int j = 0; j < q.gb.k; j++) lift(q.gb.key_reg[j], q.gb.key_mem[j]);
+    lift(q.gb.agg_reg, q.gb.agg_mem);
 }
+
+bool is_probe_kind(uint8_t k) { return k == FK_PROBE_SEMI || k == FK_PROBE_FULL || k == FK_PROBE_BITS; }
+
+// The probes every row reaching the program's terminal has passed, keyed by
+// a prefetched fact column: the flag of each register is followed as the
+// AND of a set of the program's predicates (AND and probes add one, NONE
+// restarts from one, a logical op joins two sets).  Only programs whose sole
+// effect is their terminal (SUM or GROUPBY; no stored buffer or flags, no
+// build) whose terminal flag is such a conjunction take part; for the others
+// the list is empty and the program sees every row.
+std::vector<int> shared_probes(const FProgram &q)
+{
+    std::vector<int> out;
+    if (q.ninstr < 1) return out;
+    const FInstr &T = q.ins[q.ninstr - 1];
+    if (T.kind != FK_SUM && T.kind != FK_GROUPBY) return out;
+    for (int j = 0; j < F_MAX_FLAGS; j++)
+        if (q.flag_store[j]) return out;
+    static_assert(F_MAX_INSTR < 31, "instruction sets are 31-bit masks");
+    constexpr uint32_t BAD = 1u << 31;
+    uint32_t conj[F_MAX_FLAGS];
+    for (int j = 0; j < F_MAX_FLAGS; j++) conj[j] = q.flag_init[j] == FF_UNDEF ? BAD : 0u;
+    for (int i = 0; i + 1 < q.ninstr; i++) {
+        const FInstr &I = q.ins[i];
+        if (I.store) return out;
+        switch (I.kind) {
+        case FK_SEL_LIT:
+        case FK_SEL_COL:
+        case FK_SEL_RANGE:
+        case FK_SEL_IN2:
+            conj[I.fout] = I.logic == SDB_AND ? (conj[I.fin] | (1u << i)) : I.logic == SDB_NONE ? (1u << i) : BAD;
+            break;
+        case FK_LOGICAL:
+            conj[I.fout] = I.logic == SDB_AND ? (conj[I.fin] | conj[I.floc]) : I.logic == SDB_NONE ? conj[I.floc] : BAD;
+            break;
+        case FK_PROBE_SEMI:
+        case FK_PROBE_FULL:
+        case FK_PROBE_BITS: conj[I.fout] = conj[I.fin] | (1u << i); break;
+        case FK_FILL:
+        case FK_BINOP_CC:
+        case FK_BINOP_LC:
+        case FK_BINOP_CL: break;
+        default: return out; // builds, a second terminal
+        }
+    }
+    const uint32_t c = conj[T.fin];
+    if (c & BAD) return out;
+    for (int i = 0; i + 1 < q.ninstr; i++) {
+        const FInstr &I = q.ins[i];
+        // (a rank dictionary's values need its presence word: such probes
+        // stay whole)
+        if (((c >> i) & 1u) && is_probe_kind(I.kind) && I.va >= F_PF_BASE &&
+            !(I.kind == FK_PROBE_BITS && I.vkind == FV_RANK16) &&
+            (int64_t)I.ht_max - I.ht_min + 1 <= BATCH_MASK_PROBE_LEN)
+            out.push_back(i);
+    }
+    return out;
+}
+
+uint32_t mask_kind(const FInstr &I)
+{
+    return I.kind == FK_PROBE_BITS ? FM_BITS : I.kind == FK_PROBE_SEMI ? FM_SEMI : FM_PAIR;
+}
+
+struct MaskPlan {
+    int slot;
+    int64_t lo = 0, hi = -1;
+    std::vector<FMaskEntry> ents;
+};
 
 struct BatchPlan {
     std::vector<FProgram> progs; // in shape order
-    std::vector<int> order;      // progs[k] is input program order[k]
     std::vector<int> first;      // shape k: progs[first[k], first[k+1])
-    std::vector<const int32_t *> pf;
+    std::vector<const int32_t *> pf;  // slot -> column this batch reads (null: unused)
+    std::vector<int> mask_slots;      // the library's masked slots (kernel order)
+    std::vector<MaskPlan> masks;      // this batch's tables, one per masked slot it uses
     std::string src;
-    uint32_t lds = 0, sums_off = 0;
+    uint32_t lds = 0, sums_off = 0, full = 0;
     bool vec = true;
 };
 
-BatchPlan plan_batch(const std::vector<FProgram> &in, int unroll)
+std::string batch_source(const BatchLibrary &lib, int unroll, bool vec)
+{
+    std::ostringstream s;
+    s << KERNEL_TEMPLATE_INCLUDES << "constexpr int NPF = " << std::max<size_t>(lib.slots.size(), 1)
+      << ";\nconstexpr int U = " << unroll << ";\nconstexpr bool VEC = " << (vec ? "true" : "false")
+      << ";\nconstexpr int NSHAPE = " << lib.bodies.size() << ";\nconstexpr int NMASK = " << lib.mask_slots.size()
+      << ";\nconstexpr int MSLOT[" << std::max<size_t>(lib.mask_slots.size(), 1) << "] = {";
+    for (size_t m = 0; m < lib.mask_slots.size(); m++) s << (m ? ", " : "") << lib.mask_slots[m];
+    if (lib.mask_slots.empty()) s << "0";
+    s << "};\n" << BATCH_HEAD;
+    for (size_t k = 0; k < lib.bodies.size(); k++)
+        s << "template <int R, int UU>\n__device__ __forceinline__ void body" << k
+          << "(const FProgram &P, const int64_t (&row0)[UU], const Pf<R, UU> &pf, int64_t &acc, uint8_t *lds, "
+             "const uint32_t (&act)[UU])\n{\n"
+          << lib.bodies[k] << "}\n";
+    s << "template <int R, int UU>\n__device__ __forceinline__ void run_all(const BatchArgs &A, const int64_t "
+         "(&row0)[UU], const Pf<R, UU> &pf, uint8_t *lds, unsigned long long *sums)\n{\n"
+         "    uint32_t mw[UU][R];\n    mask_rows<R, UU>(A, pf, mw);\n    uint32_t any = 0;\n"
+         "#pragma unroll\n    for (int u = 0; u < UU; u++)\n#pragma unroll\n"
+         "        for (int k = 0; k < R; k++) any |= mw[u][k];\n"
+         "    // programs some row of the wave keeps (read from the first lane so\n"
+         "    // the compiler sees one value for the wave)\n"
+         "    const uint32_t wave = __builtin_amdgcn_readfirstlane(__ockl_wfred_or_u32(any));\n";
+    for (size_t k = 0; k < lib.bodies.size(); k++)
+        s << "#pragma unroll 1\n    for (int i = A.first[" << k << "]; i < A.first[" << k + 1 << "]; i++) {\n"
+          << "        if (!((wave >> i) & 1u)) continue;\n"
+          << "        uint32_t act[UU];\n        rows_of_program<R, UU>(mw, i, act);\n"
+          << "        const FProgram &P = ((ProgPtr)A.progs)[i];\n        int64_t acc = 0;\n"
+          << "        body" << k << "<R, UU>(P, row0, pf, acc, lds + P.lds_base, act);\n"
+          << "        if (acc) atomicAdd(&sums[i], (unsigned long long)acc);\n    }\n";
+    s << "}\n} // namespace\n" << BATCH_TAIL;
+    return s.str();
+}
+
+BatchPlan plan_batch(const std::vector<FProgram> &in, int unroll, const std::string &lib_key, bool masks)
 {
     BatchPlan b;
+    if (in.size() > 32) fail(SDB_ERR_UNSUPPORTED, "batched launch: more than 32 programs");
     std::vector<FProgram> q = in;
-    for (FProgram &p : q) {
-        remap_prefetch(p, b.pf);
-        b.vec = b.vec && p.vec;
-    }
-    // shapes: identical generated bodies
-    std::vector<std::string> bodies;
+    for (const FProgram &p : q) b.vec = b.vec && p.vec;
+    std::lock_guard<std::mutex> lk(g_lib_mu);
+    BatchLibrary &lib = g_libs[lib_key + (b.vec ? "/vec" : "/row") + "/u" + std::to_string(unroll)];
     std::vector<int> shape_of(q.size());
+    std::vector<std::vector<int>> shared(q.size());
     for (size_t i = 0; i < q.size(); i++) {
+        remap_prefetch(q[i], lib.slots);
+        if (masks) shared[i] = shared_probes(q[i]);
         Gen g(q[i], false);
+        g.batch_act = true;
+        g.decided.assign(q[i].ninstr, false);
+        for (int j : shared[i]) {
+            const FInstr &I = q[i].ins[j];
+            g.decided[j] = true;
+            const int slot = I.va - F_PF_BASE;
+            if (std::find(lib.mask_slots.begin(), lib.mask_slots.end(), slot) == lib.mask_slots.end())
+                lib.mask_slots.push_back(slot);
+        }
+        // tables staged only for the probes the masks decided stay unloaded
+        for (int bl = 0; bl < q[i].nblobs; bl++) {
+            bool used = false;
+            for (int j = 0; j < q[i].ninstr; j++) {
+                const FInstr &I = q[i].ins[j];
+                used |= !g.decided[j] && I.in_lds && I.lds_off == q[i].blob_off[bl];
+            }
+            if (!used) q[i].blob_bytes[bl] = 0;
+        }
         const std::string body = g.body();
-        auto it = std::find(bodies.begin(), bodies.end(), body);
-        shape_of[i] = (int)(it - bodies.begin());
-        if (it == bodies.end()) bodies.push_back(body);
+        auto it = std::find(lib.bodies.begin(), lib.bodies.end(), body);
+        if (it == lib.bodies.end()) {
+            if ((int)lib.bodies.size() >= BATCH_MAX_SHAPES) fail(SDB_ERR_UNSUPPORTED, "batched launch: too many program shapes");
+            lib.bodies.push_back(body);
+            it = lib.bodies.end() - 1;
+        }
+        shape_of[i] = (int)(it - lib.bodies.begin());
     }
-    if ((int)bodies.size() > BATCH_MAX_SHAPES) fail(SDB_ERR_UNSUPPORTED, "batched launch: too many program shapes");
-    for (int k = 0; k < (int)bodies.size(); k++) {
+    // programs in shape order; a program's bit in the masks is its position
+    std::vector<int> order;
+    for (int k = 0; k < (int)lib.bodies.size(); k++) {
         b.first.push_back((int)b.progs.size());
         for (size_t i = 0; i < q.size(); i++)
             if (shape_of[i] == k) {
                 b.progs.push_back(q[i]);
-                b.order.push_back((int)i);
+                order.push_back((int)i);
             }
     }
     b.first.push_back((int)b.progs.size());
-    // LDS regions, then the scalar-sum slots
+    b.full = b.progs.size() >= 32 ? ~0u : (1u << b.progs.size()) - 1u;
+    b.mask_slots = lib.mask_slots;
+    for (size_t pos = 0; pos < order.size(); pos++)
+        for (int j : shared[order[pos]]) {
+            const FInstr &I = q[order[pos]].ins[j];
+            const int slot = I.va - F_PF_BASE;
+            auto it = std::find_if(b.masks.begin(), b.masks.end(), [&](const MaskPlan &m) { return m.slot == slot; });
+            if (it == b.masks.end()) {
+                b.masks.push_back(MaskPlan{slot});
+                it = b.masks.end() - 1;
+            }
+            if (I.ht_min <= I.ht_max) {
+                if (it->lo > it->hi) {
+                    it->lo = I.ht_min;
+                    it->hi = I.ht_max;
+                } else {
+                    it->lo = std::min<int64_t>(it->lo, I.ht_min);
+                    it->hi = std::max<int64_t>(it->hi, I.ht_max);
+                }
+            }
+            it->ents.push_back(FMaskEntry{I.ht, I.ht_min, I.ht_max, (uint32_t)pos, mask_kind(I)});
+        }
+    for (const MaskPlan &m : b.masks)
+        if (m.hi - m.lo + 1 > F_MASK_MAX_LEN)
+            fail(SDB_ERR_UNSUPPORTED, "batched launch: the probed key ranges of one column span more than 2^24 keys");
+    if ((int)b.masks.size() > F_MASK_MAX_COLS) fail(SDB_ERR_UNSUPPORTED, "batched launch: too many masked columns");
+    // columns this batch reads (other slots stay unloaded)
+    b.pf.assign(std::max<size_t>(lib.slots.size(), 1), nullptr);
+    auto use = [&](int8_t reg) {
+        if (reg >= F_PF_BASE) b.pf[reg - F_PF_BASE] = lib.slots[reg - F_PF_BASE];
+    };
+    for (const FProgram &p : b.progs) {
+        for (int i = 0; i < p.ninstr; i++) {
+            use(p.ins[i].va);
+            use(p.ins[i].vb);
+        }
+        for (int j = 0; j < p.gb.k; j++) use(p.gb.key_reg[j]);
+        use(p.gb.agg_reg);
+    }
+    // a masked slot of the library this batch does not read keeps its
+    // zero-initialised registers; its lookups read the all-ones word
     uint32_t off = 0;
     for (FProgram &p : b.progs) {
         p.lds_base = off;
@@ -1072,25 +1403,7 @@ BatchPlan plan_batch(const std::vector<FProgram> &in, int unroll)
     }
     b.sums_off = off;
     b.lds = off + 8u * (uint32_t)b.progs.size();
-    std::ostringstream s;
-    s << KERNEL_TEMPLATE_INCLUDES << "constexpr int NPF = " << b.pf.size() << ";\nconstexpr int U = " << unroll
-      << ";\nconstexpr bool VEC = " << (b.vec ? "true" : "false") << ";\nconstexpr int NSHAPE = " << bodies.size()
-      << ";\n" << BATCH_HEAD;
-    for (size_t k = 0; k < bodies.size(); k++) {
-        s << "template <int R, int UU>\n__device__ __forceinline__ void body" << k
-          << "(const FProgram &P, const int64_t (&row0)[UU], const Pf<R, UU> &pf, int64_t &acc, uint8_t *lds)\n{\n"
-          << bodies[k] << "}\n";
-    }
-    s << "template <int R, int UU>\n__device__ __forceinline__ void run_all(const BatchArgs &A, const int64_t "
-         "(&row0)[UU], const Pf<R, UU> &pf, uint8_t *lds, unsigned long long *sums)\n{\n";
-    for (size_t k = 0; k < bodies.size(); k++) {
-        s << "#pragma unroll 1\n    for (int i = A.first[" << k << "]; i < A.first[" << k + 1 << "]; i++) {\n"
-          << "        const FProgram &P = A.progs[i];\n        int64_t acc = 0;\n"
-          << "        body" << k << "<R, UU>(P, row0, pf, acc, lds + P.lds_base);\n"
-          << "        if (acc) atomicAdd(&sums[i], (unsigned long long)acc);\n    }\n";
-    }
-    s << "}\n} // namespace\n" << BATCH_TAIL;
-    b.src = s.str();
+    b.src = batch_source(lib, unroll, b.vec);
     return b;
 }
 
@@ -1103,19 +1416,86 @@ int jit_batch_check(const std::string &arch)
     // two copies of each shape, as pipelines with different constants
     std::vector<FProgram> two = progs;
     two.insert(two.end(), progs.begin(), progs.end());
-    compile_source(plan_batch(two, 2).src, arch);
+    compile_source(plan_batch(two, 2, "check:" + arch, true).src, arch);
     return 1;
 }
 
-void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t s, FProgram *dev_progs, int unroll)
+void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t s, FProgram *dev_progs, int unroll,
+                        const void *scope, BatchAlloc alloc, void *alloc_ctx, bool masks)
 {
     if (n <= 0 || in.empty()) return;
-    if (unroll != 1 && unroll != 2 && unroll != 4) unroll = F_DEFAULT_UNROLL;
-    const BatchPlan b = plan_batch(in, unroll);
-    if (b.lds > F_BATCH_LDS + 8u * 1024u) fail(SDB_ERR_UNSUPPORTED, "batched launch: LDS regions exceed the budget");
+    // one 4-row chunk per thread: the shared scan holds up to nine columns
+    // in registers, twice (current and next tile)
+    if (unroll != 1 && unroll != 2 && unroll != 4) unroll = 1;
+    int dev = 0;
+    SDB_HIP(hipGetDevice(&dev));
+    char scope_key[32];
+    snprintf(scope_key, sizeof(scope_key), "%p", scope);
+    const BatchPlan b =
+        plan_batch(in, unroll, device_arch() + "/" + std::to_string(dev) + "/" + scope_key + (masks ? "/m" : ""), masks);
+    if (b.lds > F_BATCH_LDS + 8u * F_BATCH_MAX) fail(SDB_ERR_UNSUPPORTED, "batched launch: LDS regions exceed the budget");
     Compiled &k = get_kernel(b.src, "sdb_fused_batch");
     SDB_HIP(hipMemcpyAsync(dev_progs, b.progs.data(), b.progs.size() * sizeof(FProgram), hipMemcpyHostToDevice, s));
-    // kernel argument block: progs, n, nprog, sums_off, first[NSHAPE+1], pf[NPF]
+
+    // mask tables: descriptors, entries and tables in one allocation, plus
+    // one all-ones word for the library's masked slots this batch leaves out
+    const int nmask = (int)b.mask_slots.size();
+    std::vector<const uint32_t *> mtab(std::max(nmask, 1), nullptr);
+    std::vector<int32_t> mlo(std::max(nmask, 1), 0);
+    std::vector<uint32_t> mlen(std::max(nmask, 1), 1), mdflt(std::max(nmask, 1), ~0u);
+    if (nmask > 0) {
+        size_t nents = 0, words = 1;
+        for (const MaskPlan &m : b.masks) {
+            nents += m.ents.size();
+            words += (size_t)std::max<int64_t>(m.hi - m.lo + 1, 1);
+        }
+        const size_t desc_bytes = (b.masks.size() * sizeof(FMaskCol) + 255) / 256 * 256;
+        const size_t ent_bytes = (nents * sizeof(FMaskEntry) + 255) / 256 * 256;
+        uint8_t *mem = static_cast<uint8_t *>(alloc(alloc_ctx, desc_bytes + ent_bytes + words * 4));
+        FMaskCol *d_cols = reinterpret_cast<FMaskCol *>(mem);
+        FMaskEntry *d_ents = reinterpret_cast<FMaskEntry *>(mem + desc_bytes);
+        uint32_t *tabs = reinterpret_cast<uint32_t *>(mem + desc_bytes + ent_bytes);
+        // tabs[0]: the all-ones word
+        std::vector<uint8_t> host(desc_bytes + ent_bytes + 4, 0);
+        std::vector<FMaskCol> cols;
+        std::vector<FMaskEntry> ents;
+        size_t at = 1;
+        uint32_t max_len = 0;
+        for (const MaskPlan &m : b.masks) {
+            FMaskCol c{};
+            const int64_t len = std::max<int64_t>(m.hi - m.lo + 1, 1);
+            c.tab = tabs + at;
+            c.lo = m.lo <= m.hi ? (int32_t)m.lo : 0;
+            c.len = (uint32_t)len;
+            c.full = b.full;
+            c.first = (uint32_t)ents.size();
+            c.count = (uint32_t)m.ents.size();
+            uint32_t probing = 0;
+            for (const FMaskEntry &e : m.ents) {
+                ents.push_back(e);
+                probing |= 1u << e.bit;
+            }
+            cols.push_back(c);
+            max_len = std::max(max_len, c.len);
+            const int mi = (int)(std::find(b.mask_slots.begin(), b.mask_slots.end(), m.slot) - b.mask_slots.begin());
+            mtab[mi] = c.tab;
+            mlo[mi] = c.lo;
+            mlen[mi] = c.len;
+            mdflt[mi] = b.full & ~probing;
+            at += (size_t)len;
+        }
+        std::memcpy(host.data(), cols.data(), cols.size() * sizeof(FMaskCol));
+        std::memcpy(host.data() + desc_bytes, ents.data(), ents.size() * sizeof(FMaskEntry));
+        const uint32_t ones = ~0u;
+        std::memcpy(host.data() + desc_bytes + ent_bytes, &ones, 4);
+        SDB_HIP(hipMemcpyAsync(mem, host.data(), host.size(), hipMemcpyHostToDevice, s));
+        for (int mi = 0; mi < nmask; mi++)
+            if (!mtab[mi]) mtab[mi] = tabs; // mlen 1, dflt all ones
+        launch_batch_masks(d_cols, (int)cols.size(), max_len, d_ents, s);
+    }
+
+    // kernel argument block: progs, n, nprog, sums_off, full, first[NSHAPE+1],
+    // pf[NPF], mtab[NMASK], mlo[NMASK], mlen[NMASK], mdflt[NMASK]
     const int nshape = (int)b.first.size() - 1;
     std::vector<uint8_t> args;
     auto put = [&](const void *v, size_t bytes, size_t align) {
@@ -1129,15 +1509,16 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     put(&n, 8, 8);
     put(&nprog, 4, 4);
     put(&b.sums_off, 4, 4);
+    put(&b.full, 4, 4);
     for (int i = 0; i <= nshape; i++) {
         const int32_t f = b.first[i];
         put(&f, 4, 4);
     }
-    const size_t npf = std::max<size_t>(b.pf.size(), 1);
-    for (size_t j = 0; j < npf; j++) {
-        const int32_t *c = j < b.pf.size() ? b.pf[j] : nullptr;
-        put(&c, 8, 8);
-    }
+    for (const int32_t *c : b.pf) put(&c, 8, 8);
+    for (const uint32_t *t : mtab) put(&t, 8, 8);
+    for (int32_t v : mlo) put(&v, 4, 4);
+    for (uint32_t v : mlen) put(&v, 4, 4);
+    for (uint32_t v : mdflt) put(&v, 4, 4);
     while (args.size() % 8) args.push_back(0);
     int per_cu;
     {
@@ -1145,17 +1526,22 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
         auto it = k.blocks_per_cu.find(b.lds);
         if (it == k.blocks_per_cu.end()) {
             int bl = 0;
-            SDB_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&bl, k.fn, JIT_BLOCK, b.lds));
+            SDB_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&bl, k.fn, BATCH_BLOCK, b.lds));
             it = k.blocks_per_cu.emplace(b.lds, std::max(1, bl)).first;
         }
         per_cu = it->second;
     }
     const int64_t units = b.vec ? (n + 3) / 4 : n;
-    const unsigned grid = stream_grid(units, (int64_t)JIT_BLOCK * (b.vec ? unroll : 1), per_cu);
+    // at least BATCH_MIN_TILES tiles per block: each block also stages and
+    // flushes every program's LDS state, which small tables must amortise
+    const int64_t tiles = (units + (int64_t)BATCH_BLOCK * (b.vec ? unroll : 1) - 1) / ((int64_t)BATCH_BLOCK * (b.vec ? unroll : 1));
+    const unsigned grid = (unsigned)std::max<int64_t>(
+        1, std::min<int64_t>(stream_grid(units, (int64_t)BATCH_BLOCK * (b.vec ? unroll : 1), per_cu),
+                             (tiles + BATCH_MIN_TILES - 1) / BATCH_MIN_TILES));
     size_t size = args.size();
     void *config[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, args.data(), HIP_LAUNCH_PARAM_BUFFER_SIZE, &size,
                       HIP_LAUNCH_PARAM_END};
-    SDB_HIP(hipModuleLaunchKernel(k.fn, grid, 1, 1, JIT_BLOCK, 1, 1, b.lds, s, nullptr, config));
+    SDB_HIP(hipModuleLaunchKernel(k.fn, grid, 1, 1, BATCH_BLOCK, 1, 1, b.lds, s, nullptr, config));
 }
 
 std::string jit_tuning_report()

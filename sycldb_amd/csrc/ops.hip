@@ -636,6 +636,33 @@ __global__ void __launch_bounds__(BLOCK) k_rank_dict(const uint32_t *bits, int32
     }
 }
 
+// ------------------------------------------------------- batch probe masks
+// One thread per key of a masked fact column (blockIdx.y = column): the key's
+// word starts with every program's bit and loses the bit of each program one
+// of whose probes on the column fails for the key -- the probe's own test
+// (range check, then the table entry: FilterJoinKernel / FullJoinKernel,
+// kernels/join.hpp:129-160, 256-300).
+__global__ void __launch_bounds__(BLOCK) k_batch_masks(const FMaskCol *cols, const FMaskEntry *ents)
+{
+    const FMaskCol c = cols[blockIdx.y];
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < c.len; i += gridDim.x * BLOCK) {
+        const int32_t key = (int32_t)((uint32_t)c.lo + i);
+        uint32_t w = c.full;
+        for (uint32_t e = c.first; e < c.first + c.count; e++) {
+            const FMaskEntry E = ents[e];
+            bool pass = key >= E.ht_min && key <= E.ht_max;
+            if (pass) {
+                const uint32_t s = slot_of(key, E.ht_min);
+                if (E.kind == FM_BITS) pass = (reinterpret_cast<const uint32_t *>(E.ht)[s >> 5] >> (s & 31)) & 1u;
+                else if (E.kind == FM_SEMI) pass = reinterpret_cast<const uint8_t *>(E.ht)[s] != 0;
+                else pass = reinterpret_cast<const int2 *>(E.ht)[s].x == 1;
+            }
+            if (!pass) w &= ~(1u << E.bit);
+        }
+        c.tab[i] = w;
+    }
+}
+
 // ------------------------------------------------------------ compaction
 // Three passes over tiles of COMPACT_TILE rows: count per tile, exclusive
 // scan of the counts (one block), then write ids in row order.
@@ -1014,6 +1041,14 @@ void launch_rank_dict(const uint32_t *bits, int32_t words, const uint16_t *vals,
     if (words <= 0) return;
     hipLaunchKernelGGL(k_rank_dict, dim3(1), dim3(BLOCK), 0, s, bits, words, vals, dict);
     check_launch("rank_dict");
+}
+
+void launch_batch_masks(const FMaskCol *cols, int ncols, uint32_t max_len, const FMaskEntry *ents, hipStream_t s)
+{
+    if (ncols <= 0 || max_len == 0) return;
+    const unsigned gx = (unsigned)std::min<uint64_t>(((uint64_t)max_len + BLOCK - 1) / BLOCK, 4096);
+    hipLaunchKernelGGL(k_batch_masks, dim3(gx, (unsigned)ncols), dim3(BLOCK), 0, s, cols, ents);
+    check_launch("batch_masks");
 }
 
 void launch_compact(const uint8_t *flags, int64_t n, int32_t *row_ids, uint32_t *count, hipStream_t s)

@@ -90,3 +90,47 @@ def test_batch_of_one_and_empty():
         assert results[0].rows() == eng.execute(plan).rows()
     finally:
         eng.close()
+
+
+def test_batch_without_shared_masks(pipelines, expected, monkeypatch):
+    """SDB_BATCH_MASKS=0: every probe stays inside its program."""
+    monkeypatch.setenv("SDB_BATCH_MASKS", "0")
+    eng = _engine()
+    try:
+        plans = [PL.build(t, **p) for t, p in pipelines[:12]]
+        print("batch without masks", flush=True)
+        results, batched = eng.execute_batch(plans)
+        assert batched >= 8
+        for i, r in enumerate(results):
+            assert r.rows() == expected[i], (i, pipelines[i])
+    finally:
+        eng.close()
+
+
+def test_batch_mask_edge_cases(cpu_tables):
+    """Shared probe masks at the extremes: a pipeline no row passes (a year
+    outside the date table), near-empty ones, identical pipelines side by
+    side, and every template's widest predicates."""
+    pipes = [
+        ("t1", {"date": ("year", 2050), "disc": (1, 3), "qty": ("lt", 25)}),
+        ("t2", {"part": ("brand", 999), "s_region": 4}),
+        ("t2", {"part": ("brand", 999), "s_region": 4}),
+        ("t3", {"cust": ("city", 249), "supp": ("city", 0), "date": ("yearmonthnum", 199812),
+                "group": "city"}),
+        ("t3", {"cust": ("region", 0), "supp": ("region", 0), "date": ("year_range", 1992, 1998),
+                "group": "nation"}),
+        ("t4", {"c_region": 0, "s_region": 4, "mfgr": (0, 4)}),
+        ("t1", {"date": ("year_range_and", 1992, 1998), "disc": (0, 10), "qty": ("lt", 51)}),
+        ("t2", {"part": ("brand_range", 0, 999), "s_region": 0}),
+    ]
+    expected = [PO.execute_plan(PL.build(t, **p), cpu_tables).rows() for t, p in pipes]
+    eng = _engine()
+    try:
+        plans = [PL.build(t, **p) for t, p in pipes]
+        print("mask edge batch", flush=True)
+        results, batched = eng.execute_batch(plans)
+        assert batched == len(plans)
+        for i, r in enumerate(results):
+            assert r.rows() == expected[i], (i, pipes[i])
+    finally:
+        eng.close()
