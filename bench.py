@@ -199,8 +199,8 @@ def main():
     ap.add_argument("--batch-pipelines", type=int, default=32)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--no-served", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=300)
-    ap.add_argument("--serve-qps", type=float, default=200.0)
+    ap.add_argument("--serve-requests", type=int, default=600)
+    ap.add_argument("--serve-qps", type=float, default=1000.0)
     ap.add_argument("--serve-max-batch", type=int, default=32)
     ap.add_argument("--codegen", default="auto", choices=["auto", "interpret", "compile"])
     ap.add_argument("--unroll", type=int, default=0, choices=[0, 1, 2, 4])

@@ -229,6 +229,23 @@ __device__ __forceinline__ void probe_value_issue(const int32_t (&x)[R], uint32_
     }
 }
 
+// The value of a probe's matched rows `out`, fetched after the match (the
+// issue phase then reads only the presence words).
+template <int R, int VKIND>
+__device__ __forceinline__ void probe_value_after(const int32_t (&x)[R], uint32_t out, const void *vals, int32_t mn,
+                                                  int32_t vmin, int32_t (&v)[R])
+{
+#pragma unroll
+    for (int k = 0; k < R; k++)
+        if ((out >> k) & 1u) {
+            const uint32_t s = slot_of(x[k], mn);
+            if constexpr (VKIND == FV_U16)
+                v[k] = (int32_t)((uint32_t)vmin + reinterpret_cast<const uint16_t *>(vals)[s]);
+            else
+                v[k] = reinterpret_cast<const int2 *>(vals)[s].y;
+        }
+}
+
 template <int R> __device__ __forceinline__ void probe_value_finish(uint32_t out, const ProbeTmp<R> &t, int32_t (&v)[R])
 {
 #pragma unroll
