@@ -84,13 +84,13 @@ def batched_measure(eng, n_pipes, seed, reps):
             "speedup": seq_s / bat_s, "results_identical": True}
 
 
-def served_measure(eng_big, seed, n_requests, qps, max_batch):
+def served_measure(eng_big, seed, n_requests, rates, max_batch):
     """Serving: three resident fact tables (6M, 60M and the benchmark's
     600M rows) with their dimensions; Poisson arrivals at a fixed rate of
     pipelines from the four templates with random constants and table sizes.
-    The same trace is served one request at a time in arrival order (the
-    plain sequential baseline) and with continuous batching; every result of
-    the batched run must equal the sequential one."""
+    At every rate the same trace is served one request at a time in arrival
+    order (the plain sequential baseline) and with continuous batching; every
+    result of the batched run must equal the sequential one."""
     import numpy as np
     from sycldb_amd import plans as PL
     from sycldb_amd.engine import Engine, Server, replay
@@ -106,7 +106,7 @@ def served_measure(eng_big, seed, n_requests, qps, max_batch):
         rng = np.random.default_rng(seed)
         pipes = PL.random_pipelines(n_requests, seed + 2)
         reqs = [(int(rng.integers(0, 3)), PL.build(t, **p)) for t, p in pipes]
-        arrivals = np.cumsum(rng.exponential(1.0 / qps, n_requests))
+        gaps = rng.exponential(1.0, n_requests)
         # compile every kernel the trace needs before measuring: each plan
         # alone, and each engine's requests as one batch
         for e in range(3):
@@ -117,7 +117,7 @@ def served_measure(eng_big, seed, n_requests, qps, max_batch):
                 if len(sub[k:k + max_batch]) > 1:
                     engines[e].execute_batch(sub[k:k + max_batch])
 
-        def serve(serial):
+        def serve(serial, arrivals):
             with Server(engines, max_batch=max_batch, serial=serial) as srv:
                 t0 = time.perf_counter()
                 out = replay(srv, reqs, arrivals)
@@ -129,14 +129,19 @@ def served_measure(eng_big, seed, n_requests, qps, max_batch):
                          "batches": stats["batches"], "mean_batch": stats["requests"] / max(stats["batches"], 1),
                          "wall_s": wall}
 
-        out_seq, seq = serve(True)
-        out_bat, bat = serve(False)
-        identical = all(a.rows() == b.rows() for (a, _), (b, _) in zip(out_seq, out_bat))
-        assert identical, "served results differ from the sequential ones"
-        return {"requests": n_requests, "qps": qps, "tables_rows": [6_000_000, 60_000_000, eng_big_rows(eng_big)],
-                "max_batch": max_batch, "sequential": seq, "batched": bat,
-                "throughput_ratio": bat["pipelines_per_s"] / seq["pipelines_per_s"],
-                "results_identical": identical}
+        by_rate = {}
+        for qps in rates:
+            arrivals = np.cumsum(gaps / qps)
+            out_seq, seq = serve(True, arrivals)
+            out_bat, bat = serve(False, arrivals)
+            identical = all(a.rows() == b.rows() for (a, _), (b, _) in zip(out_seq, out_bat))
+            assert identical, "served results differ from the sequential ones"
+            by_rate["%g" % qps] = {"sequential": seq, "batched": bat,
+                                   "throughput_ratio": bat["pipelines_per_s"] / seq["pipelines_per_s"],
+                                   "p50_ratio": seq["p50_ms"] / bat["p50_ms"], "p99_ratio": seq["p99_ms"] / bat["p99_ms"],
+                                   "results_identical": identical}
+        return {"requests": n_requests, "tables_rows": [6_000_000, 60_000_000, eng_big_rows(eng_big)],
+                "max_batch": max_batch, "by_qps": by_rate}
     finally:
         for e in engines[:2]:
             e.close()
@@ -200,7 +205,8 @@ def main():
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=600)
-    ap.add_argument("--serve-qps", type=float, default=1000.0)
+    ap.add_argument("--serve-qps", default="1000,3000",
+                    help="arrival rates (requests/s), comma separated")
     ap.add_argument("--serve-max-batch", type=int, default=32)
     ap.add_argument("--codegen", default="auto", choices=["auto", "interpret", "compile"])
     ap.add_argument("--unroll", type=int, default=0, choices=[0, 1, 2, 4])
@@ -291,7 +297,8 @@ def main():
     served = None
     if not args.no_served:
         eng.fact_rows = rows
-        served = served_measure(eng, args.seed + 3, args.serve_requests, args.serve_qps, args.serve_max_batch)
+        served = served_measure(eng, args.seed + 3, args.serve_requests,
+                                [float(q) for q in str(args.serve_qps).split(",")], args.serve_max_batch)
         log("served: %s" % served)
 
     cpu = None

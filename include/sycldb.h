@@ -268,16 +268,17 @@ int sdb_engine_execute_batch(sdb_engine *eng, const char *const *plans, int n, s
 
 /* A serving loop over engines with resident tables (no counterpart in the
  * reference, which runs one query at a time).  Requests are plans for one
- * engine, queued with their arrival time.  With batching (serial == 0) each
- * engine has a worker that, whenever it is free, admits waiting requests of
- * its engine in arrival order while a slot (max_batch) and the memory budget
+ * engine, queued per engine with their arrival time.  With batching
+ * (serial == 0) one worker, whenever it is free, takes the engine whose
+ * oldest waiting request arrived first and admits that engine's waiting
+ * requests in arrival order while a slot (max_batch) and the memory budget
  * (the requests' estimated build-table and group-by footprints) allow --
  * always at least one -- and runs them as one batch sharing the fact scan
  * (sdb_engine_execute_batch); requests arriving meanwhile join a later batch.
- * Workers run on their engines' streams, so batches of different engines
- * overlap on the device.  With serial != 0 one worker runs every request
- * alone in arrival order (the sequential baseline).  While a server runs,
- * its engines must not be used directly. */
+ * One batch runs at a time (a batched launch fills the device).  With
+ * serial != 0 the worker runs every request alone in arrival order (the
+ * sequential baseline).  While a server runs, its engines must not be used
+ * directly. */
 typedef struct sdb_server sdb_server;
 typedef struct sdb_server_options {
     int max_batch;                /* slots per batch (default 32) */

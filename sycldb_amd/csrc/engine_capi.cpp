@@ -345,6 +345,9 @@ std::vector<ResultTable> execute_batch(Engine &eng, const std::vector<PlanResult
     }
     const int n = (int)parsed.size();
     if (batched) *batched = 0;
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    auto ms_since = [](clk::time_point a) { return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
     eng.mm->reset(eng.stream);
     struct Pipe {
         FinalTable fin;
@@ -382,6 +385,7 @@ std::vector<ResultTable> execute_batch(Engine &eng, const std::vector<PlanResult
         }
         eng.capture = Capture{};
     }
+    const double t_capture = ms_since(t0);
     // the shared fact passes
     std::vector<int> members;
     for (int i = 0; i < n; i++)
@@ -402,8 +406,10 @@ std::vector<ResultTable> execute_batch(Engine &eng, const std::vector<PlanResult
             eng.registry.check(dev_progs, members.size() * sizeof(FProgram), "batched programs");
             Engine::ProfRec rec;
             if (eng.profile) eng.prof_begin("batch:" + fact, rows, rec);
-            launch_fused_batch(progs, rows, eng.stream, dev_progs, eng.unroll, eng.find_table(fact), batch_alloc,
-                               &eng, batch_masks());
+            // one 4-row chunk per thread whatever the plain path's unroll: the
+            // shared scan's double-buffered columns fill the registers
+            launch_fused_batch(progs, rows, eng.stream, dev_progs, 1, eng.find_table(fact), batch_alloc, &eng,
+                               batch_masks());
             if (eng.profile) eng.prof_end(rec);
             eng.launches++;
         }
@@ -418,11 +424,16 @@ std::vector<ResultTable> execute_batch(Engine &eng, const std::vector<PlanResult
         for (int i : members)
             for (auto &f : pipes[i].deferred) f();
     }
+    const double t_shared = ms_since(t0);
     for (int i = 0; i < n; i++)
         if (!pipes[i].batched) pipes[i].fin = run_ddor_plan(eng, parsed[i]);
+    const double t_plain = ms_since(t0);
     std::vector<ResultTable> out;
     for (int i = 0; i < n; i++) out.push_back(copy_result(eng, pipes[i].fin));
     if (batched) *batched = (int)members.size();
+    if (debug_batch())
+        fprintf(stderr, "[batch] %d plans, %zu shared: host ms capture %.2f, shared pass %.2f, plain %.2f, results %.2f\n",
+                n, members.size(), t_capture, t_shared - t_capture, t_plain - t_shared, ms_since(t0) - t_plain);
     return out;
 }
 } // namespace sdb

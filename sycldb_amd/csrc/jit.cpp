@@ -1442,8 +1442,8 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
                         const void *scope, BatchAlloc alloc, void *alloc_ctx, bool masks)
 {
     if (n <= 0 || in.empty()) return;
-    // one 4-row chunk per thread: the shared scan holds up to nine columns
-    // in registers, twice (current and next tile)
+    // the shared scan holds up to BATCH_MAX_PF columns in registers, twice
+    // (current and next tile): callers pass 1 (one 4-row chunk per thread)
     if (unroll != 1 && unroll != 2 && unroll != 4) unroll = 1;
     int dev = 0;
     SDB_HIP(hipGetDevice(&dev));

@@ -2,16 +2,24 @@
 // whose application runs one query at a time: app/application.cpp).
 //
 // A server holds several engines (each with its own resident tables, stream
-// and allocator).  Requests -- a plan for one engine -- are queued with their
-// arrival time.  In batching mode each engine has a worker thread that, as
-// soon as it is free, admits the waiting requests of its engine in arrival
-// order while a slot (max_batch) and the memory budget (the sum of the
-// requests' estimated footprints) allow -- always at least one -- and runs
-// them as one batch (execute_batch: one shared fact scan).  Requests that
-// arrive while a batch runs wait for the next one: continuous batching.
-// Workers of different engines run on their own streams, so their batches
-// overlap on the device.  In serial mode one worker serves all requests one
-// at a time in arrival order through the plain path: the sequential baseline.
+// and allocator).  Requests -- a plan for one engine -- are queued per engine
+// with their arrival time.  In batching mode one worker, as soon as it is
+// free, takes the engine whose first waiting request is the oldest and admits
+// that engine's waiting requests in arrival order while a slot (max_batch)
+// and the memory budget (the sum of the requests' estimated footprints) allow
+// -- always at least one -- and runs them as one batch (execute_batch: one
+// shared fact scan).  Requests that arrive while a batch runs wait for the
+// next one: continuous batching.  One batch runs at a time: a batched launch
+// fills every CU, so batches of different engines would only queue behind
+// each other on the device (and a small table's requests behind a large
+// table's scan).  A shared scan has a fixed cost that a few small requests do
+// not repay, so per engine the worker learns the batch size from which
+// batching pays: it keeps a running mean of a request's plain run time and,
+// after every batch, moves the engine's threshold up when the batch took
+// longer than its requests would have taken one by one and down when it took
+// clearly less; admitted requests below the threshold run one by one.  In
+// serial mode the same worker serves all requests one at a time in arrival
+// order through the plain path: the sequential baseline.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -41,6 +49,7 @@ struct Request {
     int status = SDB_OK;
     std::string error;
     ResultTable result;
+    bool ran = false;      // ran through the plain path
     bool finished = false;
 };
 
@@ -80,21 +89,35 @@ struct Server {
     bool serial;
     std::mutex mu;
     std::condition_variable work_cv, done_cv;
-    std::vector<std::deque<std::shared_ptr<Request>>> queues; // per engine (serial: one queue)
+    std::vector<std::deque<std::shared_ptr<Request>>> queues; // per engine
     std::map<int64_t, std::shared_ptr<Request>> requests;
     int64_t next_ticket = 1;
     bool stopping = false;
     int64_t batches = 0, batched_requests = 0;
+    // per engine (worker thread only): running mean of one request's plain
+    // run time (ms; 0 = none seen yet) and the smallest batch worth sharing
+    std::vector<double> plain_ms;
+    std::vector<int> min_batch;
     std::vector<std::thread> workers;
 
-    void worker(int q)
+    // the queue whose first request arrived first (-1: all empty)
+    int oldest_queue() const
+    {
+        int q = -1;
+        for (int i = 0; i < (int)queues.size(); i++)
+            if (!queues[i].empty() && (q < 0 || queues[i].front()->arrival < queues[q].front()->arrival)) q = i;
+        return q;
+    }
+
+    void worker()
     {
         while (true) {
             std::vector<std::shared_ptr<Request>> admitted;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                work_cv.wait(lk, [&] { return stopping || !queues[q].empty(); });
-                if (queues[q].empty()) return; // stopping with nothing left
+                work_cv.wait(lk, [&] { return stopping || oldest_queue() >= 0; });
+                const int q = oldest_queue();
+                if (q < 0) return; // stopping with nothing left
                 uint64_t used = 0;
                 while (!queues[q].empty() && (int)admitted.size() < max_batch) {
                     auto &r = queues[q].front();
@@ -115,6 +138,25 @@ struct Server {
         }
     }
 
+    // one request through the plain path (its failure is its own)
+    void run_alone(Engine &eng, Request &r)
+    {
+        const auto t0 = Clock::now();
+        r.status = guarded([&] {
+            SDB_HIP(hipSetDevice(eng.device));
+            eng.mm->reset(eng.stream);
+            r.result = execute_ddor_plan(eng, r.plan);
+        });
+        r.ran = true;
+        if (r.status != SDB_OK) {
+            r.error = sdb_last_error();
+            return;
+        }
+        const double ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+        double &m = plain_ms[r.engine];
+        m = m > 0 ? 0.8 * m + 0.2 * ms : ms;
+    }
+
     void run(std::vector<std::shared_ptr<Request>> &rs)
     {
         const auto start = Clock::now();
@@ -124,30 +166,33 @@ struct Server {
         }
         // all requests of one worker call share an engine (serial mode: one
         // request per call)
-        Engine &eng = *engines[rs.front()->engine];
+        const int e = rs.front()->engine;
+        Engine &eng = *engines[e];
+        const int n = (int)rs.size();
         bool ok = false;
-        if (rs.size() > 1) {
+        // until a plain run time is known, the first request runs alone
+        size_t first_shared = plain_ms[e] > 0 ? 0 : 1;
+        if (n - (int)first_shared > 1 && n - (int)first_shared >= min_batch[e]) {
             try {
+                for (size_t i = 0; i < first_shared; i++) run_alone(eng, *rs[i]);
+                const auto b0 = Clock::now();
                 SDB_HIP(hipSetDevice(eng.device));
                 std::vector<PlanResult> plans;
-                for (auto &r : rs) plans.push_back(r->plan);
+                for (size_t i = first_shared; i < rs.size(); i++) plans.push_back(rs[i]->plan);
                 std::vector<ResultTable> out = execute_batch(eng, plans, nullptr);
-                for (size_t i = 0; i < rs.size(); i++) rs[i]->result = std::move(out[i]);
+                for (size_t i = first_shared; i < rs.size(); i++) rs[i]->result = std::move(out[i - first_shared]);
                 ok = true;
+                const double batch_ms = std::chrono::duration<double, std::milli>(Clock::now() - b0).count();
+                const double alone_ms = plain_ms[e] * (double)plans.size();
+                if (batch_ms > alone_ms) min_batch[e] = std::min(max_batch, (int)plans.size() + 1);
+                else if (batch_ms < 0.8 * alone_ms) min_batch[e] = std::max(2, min_batch[e] - 1);
             } catch (const std::exception &) {
                 // a failing request must not fail its batch: run them one by one
             }
         }
-        if (!ok) {
-            for (auto &r : rs) {
-                r->status = guarded([&] {
-                    SDB_HIP(hipSetDevice(eng.device));
-                    eng.mm->reset(eng.stream);
-                    r->result = execute_ddor_plan(eng, r->plan);
-                });
-                if (r->status != SDB_OK) r->error = sdb_last_error();
-            }
-        }
+        if (!ok)
+            for (auto &r : rs)
+                if (r->status == SDB_OK && !r->ran) run_alone(eng, *r);
         const auto done = Clock::now();
         for (auto &r : rs) r->done = done;
     }
@@ -179,9 +224,10 @@ int sdb_server_create(sdb_engine *const *engines, int n_engines, const sdb_serve
         s.budget = opts && opts->memory_budget_bytes ? opts->memory_budget_bytes : (2ull << 30);
         s.serial = opts && opts->serial;
         if (s.serial) s.max_batch = 1;
-        const int nq = s.serial ? 1 : n_engines;
-        s.queues.resize(nq);
-        for (int q = 0; q < nq; q++) s.workers.emplace_back([&s, q] { s.worker(q); });
+        s.queues.resize(n_engines);
+        s.plain_ms.assign(n_engines, 0.0);
+        s.min_batch.assign(n_engines, 2);
+        s.workers.emplace_back([&s] { s.worker(); });
         *out = srv.release();
     });
 }
@@ -203,7 +249,7 @@ int sdb_server_submit(sdb_server *srv, int engine, const char *plan_json, int64_
             require(!s.stopping, "server is stopping");
             r->ticket = s.next_ticket++;
             s.requests[r->ticket] = r;
-            s.queues[s.serial ? 0 : engine].push_back(r);
+            s.queues[engine].push_back(r);
             *ticket = r->ticket;
         }
         s.work_cv.notify_all();
