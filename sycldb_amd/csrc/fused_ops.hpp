@@ -161,49 +161,120 @@ template <int R> struct ProbeTmp {
 // from a 16-bit offset table (FV_U16), the reference pairs (FV_PAIR) or a
 // rank dictionary in LDS (FV_RANK16: `bits` is the dictionary, `words` its
 // word count, and the issue phase reads the word's prefix count).
-template <int R, int VKIND>
+//
+// The key range test mn <= x <= mx is the single unsigned test
+// (x - mn) <= (mx - mn), exact whenever mx >= mn (an empty range matches
+// nothing).  A bitmap in LDS (IN_LDS) is read for every row at an index
+// clamped into the bitmap, so the read needs no branch; a bitmap in global
+// memory is read only for the rows that need it.
+template <int R, int VKIND, bool IN_LDS = false>
 __device__ __forceinline__ void probe_bits_issue(const int32_t (&x)[R], uint32_t in, const uint32_t *bits, int32_t mn,
                                                  int32_t mx, const void *vals, int32_t vmin, int32_t words,
                                                  ProbeTmp<R> &t)
 {
+    const uint32_t span = (uint32_t)mx - (uint32_t)mn;
+    const bool nonempty = mx >= mn;
 #pragma unroll
     for (int k = 0; k < R; k++) {
         t.word[k] = 0;
         t.val[k] = 0;
-        if (((in >> k) & 1u) && x[k] >= mn && x[k] <= mx) {
-            const uint32_t s = slot_of(x[k], mn);
-            t.word[k] = bits[s >> 5];
-            if constexpr (VKIND == FV_U16)
-                t.val[k] = (int32_t)((uint32_t)vmin + reinterpret_cast<const uint16_t *>(vals)[s]);
-            else if constexpr (VKIND == FV_PAIR)
-                t.val[k] = reinterpret_cast<const int2 *>(vals)[s].y;
-            else if constexpr (VKIND == FV_RANK16)
-                t.val[k] = reinterpret_cast<const uint16_t *>(reinterpret_cast<const uint8_t *>(bits) +
-                                                              rank_prefix_off((uint32_t)words))[s >> 5];
+        const uint32_t s = (uint32_t)x[k] - (uint32_t)mn;
+        if constexpr (IN_LDS && VKIND == FV_NONE) {
+            if (nonempty) t.word[k] = bits[(s < span ? s : span) >> 5];
+        } else {
+            const bool go = nonempty & (((in >> k) & 1u) != 0) & (s <= span);
+            if (go) {
+                t.word[k] = bits[s >> 5];
+                if constexpr (VKIND == FV_U16)
+                    t.val[k] = (int32_t)((uint32_t)vmin + reinterpret_cast<const uint16_t *>(vals)[s]);
+                else if constexpr (VKIND == FV_PAIR)
+                    t.val[k] = reinterpret_cast<const int2 *>(vals)[s].y;
+                else if constexpr (VKIND == FV_RANK16)
+                    t.val[k] = reinterpret_cast<const uint16_t *>(reinterpret_cast<const uint8_t *>(bits) +
+                                                                  rank_prefix_off((uint32_t)words))[s >> 5];
+            }
         }
     }
 }
 
-template <int R, int VKIND>
+template <int R, int VKIND, bool IN_LDS = false>
 __device__ __forceinline__ uint32_t probe_bits_finish(const int32_t (&x)[R], uint32_t in, int32_t mn, int32_t mx,
                                                       const ProbeTmp<R> &t, const uint32_t *bits, int32_t vmin,
                                                       int32_t words, int32_t (&v)[R])
 {
+    const uint32_t span = (uint32_t)mx - (uint32_t)mn;
+    const bool nonempty = mx >= mn;
     uint32_t out = 0;
 #pragma unroll
     for (int k = 0; k < R; k++) {
-        const uint32_t s = slot_of(x[k], mn);
-        if (((in >> k) & 1u) && x[k] >= mn && x[k] <= mx && ((t.word[k] >> (s & 31)) & 1u)) {
-            out |= 1u << k;
-            if constexpr (VKIND == FV_RANK16) {
+        const uint32_t s = (uint32_t)x[k] - (uint32_t)mn;
+        const bool hit = nonempty & (((in >> k) & 1u) != 0) & (s <= span) & (((t.word[k] >> (s & 31)) & 1u) != 0);
+        out |= (uint32_t)hit << k;
+        if constexpr (VKIND == FV_RANK16) {
+            if (hit) {
                 const uint32_t rank = (uint32_t)t.val[k] + __popc(t.word[k] & ((1u << (s & 31)) - 1u));
                 v[k] = (int32_t)((uint32_t)vmin + reinterpret_cast<const uint16_t *>(
                                                       reinterpret_cast<const uint8_t *>(bits) +
                                                       rank_values_off((uint32_t)words))[rank]);
-            } else if constexpr (VKIND != FV_NONE) {
-                v[k] = t.val[k];
             }
+        } else if constexpr (VKIND != FV_NONE) {
+            v[k] = hit ? t.val[k] : v[k];
         }
+    }
+    return out;
+}
+
+template <int R>
+__device__ __forceinline__ void probe_semi_issue(const int32_t (&x)[R], uint32_t in, const uint8_t *ht, int32_t mn,
+                                                 int32_t mx, ProbeTmp<R> &t)
+{
+    const uint32_t span = (uint32_t)mx - (uint32_t)mn;
+    const bool nonempty = mx >= mn;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        t.word[k] = 0;
+        const uint32_t s = (uint32_t)x[k] - (uint32_t)mn;
+        if (nonempty & (((in >> k) & 1u) != 0) & (s <= span)) t.word[k] = ht[s] != 0;
+    }
+}
+
+template <int R>
+__device__ __forceinline__ uint32_t probe_semi_finish(uint32_t in, const ProbeTmp<R> &t)
+{
+    uint32_t out = 0;
+#pragma unroll
+    for (int k = 0; k < R; k++) out |= (uint32_t)((((in >> k) & 1u) != 0) & (t.word[k] != 0)) << k;
+    return out;
+}
+
+template <int R>
+__device__ __forceinline__ void probe_full_issue(const int32_t (&x)[R], uint32_t in, const int2 *ht, int32_t mn,
+                                                 int32_t mx, ProbeTmp<R> &t)
+{
+    const uint32_t span = (uint32_t)mx - (uint32_t)mn;
+    const bool nonempty = mx >= mn;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        t.word[k] = 0;
+        t.val[k] = 0;
+        const uint32_t s = (uint32_t)x[k] - (uint32_t)mn;
+        if (nonempty & (((in >> k) & 1u) != 0) & (s <= span)) {
+            const int2 e = ht[s];
+            t.word[k] = e.x == 1;
+            t.val[k] = e.y;
+        }
+    }
+}
+
+template <int R>
+__device__ __forceinline__ uint32_t probe_full_finish(uint32_t in, const ProbeTmp<R> &t, int32_t (&v)[R])
+{
+    uint32_t out = 0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const bool hit = (((in >> k) & 1u) != 0) & (t.word[k] != 0);
+        out |= (uint32_t)hit << k;
+        v[k] = hit ? t.val[k] : v[k];
     }
     return out;
 }
@@ -251,57 +322,6 @@ template <int R> __device__ __forceinline__ void probe_value_finish(uint32_t out
 #pragma unroll
     for (int k = 0; k < R; k++)
         if ((out >> k) & 1u) v[k] = t.val[k];
-}
-
-template <int R>
-__device__ __forceinline__ void probe_semi_issue(const int32_t (&x)[R], uint32_t in, const uint8_t *ht, int32_t mn,
-                                                 int32_t mx, ProbeTmp<R> &t)
-{
-#pragma unroll
-    for (int k = 0; k < R; k++) {
-        t.word[k] = 0;
-        if (((in >> k) & 1u) && x[k] >= mn && x[k] <= mx) t.word[k] = ht[slot_of(x[k], mn)] != 0;
-    }
-}
-
-template <int R>
-__device__ __forceinline__ uint32_t probe_semi_finish(uint32_t in, const ProbeTmp<R> &t)
-{
-    uint32_t out = 0;
-#pragma unroll
-    for (int k = 0; k < R; k++)
-        if (((in >> k) & 1u) && t.word[k]) out |= 1u << k;
-    return out;
-}
-
-template <int R>
-__device__ __forceinline__ void probe_full_issue(const int32_t (&x)[R], uint32_t in, const int2 *ht, int32_t mn,
-                                                 int32_t mx, ProbeTmp<R> &t)
-{
-#pragma unroll
-    for (int k = 0; k < R; k++) {
-        t.word[k] = 0;
-        t.val[k] = 0;
-        if (((in >> k) & 1u) && x[k] >= mn && x[k] <= mx) {
-            const int2 e = ht[slot_of(x[k], mn)];
-            t.word[k] = e.x == 1;
-            t.val[k] = e.y;
-        }
-    }
-}
-
-template <int R>
-__device__ __forceinline__ uint32_t probe_full_finish(uint32_t in, const ProbeTmp<R> &t, int32_t (&v)[R])
-{
-    uint32_t out = 0;
-#pragma unroll
-    for (int k = 0; k < R; k++) {
-        if (((in >> k) & 1u) && t.word[k]) {
-            out |= 1u << k;
-            v[k] = t.val[k];
-        }
-    }
-    return out;
 }
 
 // BuildKeysHTKernel (kernels/join.hpp:17-40): every row writes its flag

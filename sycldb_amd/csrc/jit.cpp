@@ -180,7 +180,8 @@ struct Gen {
         else {
             const std::string bits =
                 I.in_lds ? "(const uint32_t *)(lds + " + P + ".lds_off)" : "(const uint32_t *)" + P + ".ht";
-            a << "probe_bits_issue<R, " << (late_value(i) ? 0 : (int)I.vkind) << ">(" << x << ", " << gate << ", "
+            a << "probe_bits_issue<R, " << (late_value(i) ? 0 : (int)I.vkind) << ", " << (I.in_lds ? "true" : "false")
+              << ">(" << x << ", " << gate << ", "
               << bits << ", " << P << ".ht_min, " << P << ".ht_max, " << P << ".vals, " << P << ".vmin, " << P
               << ".ht_len, " << t << ");\n";
         }
@@ -359,7 +360,8 @@ struct Gen {
             else {
                 const std::string bits =
                     I.in_lds ? "(const uint32_t *)(lds + " + P + ".lds_off)" : "(const uint32_t *)" + P + ".ht";
-                c << "                o = probe_bits_finish<R, " << (late_value(i) ? 0 : (int)I.vkind) << ">(" << x
+                c << "                o = probe_bits_finish<R, " << (late_value(i) ? 0 : (int)I.vkind) << ", "
+                  << (I.in_lds ? "true" : "false") << ">(" << x
                   << ", " << fin << ", " << P << ".ht_min, " << P << ".ht_max, " << t << ", " << bits << ", " << P
                   << ".vmin, " << P << ".ht_len, " << (late_value(i) ? std::string("none") : vdst) << ");\n";
                 if (late_value(i))
