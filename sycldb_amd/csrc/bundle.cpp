@@ -388,6 +388,18 @@ std::vector<Staging> choose_staged(const std::vector<Draft> &d, uint32_t total_c
     return out;
 }
 
+// LDS bytes one fused program may give to staged probe tables
+// (SDB_STAGE_MAX_BYTES; default 48 KiB): staging trades occupancy (LDS per
+// block) for probes that never leave the CU.
+uint32_t stage_max_bytes()
+{
+    static const uint32_t v = [] {
+        const char *e = getenv("SDB_STAGE_MAX_BYTES");
+        return e && *e ? (uint32_t)atol(e) : 48u * 1024u;
+    }();
+    return v;
+}
+
 // SDB_GROUPBY_HASH=0 sends group-by rows whose dense slot range is too large
 // for an LDS array straight to the global arrays instead of an LDS hash.
 bool hash_groupby_enabled()
@@ -410,7 +422,7 @@ double prefetch_min_touch()
 {
     static const double v = [] {
         const char *e = getenv("SDB_PREFETCH_MIN_TOUCH");
-        return e && *e ? atof(e) : 0.05;
+        return e && *e ? atof(e) : 0.3;
     }();
     return v;
 }
@@ -714,7 +726,7 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
     // launch stages none: the batch decides its probes on fact key columns
     // through shared masks, and its LDS share goes to the group-by partials.
     const uint32_t stage_cap =
-        captured ? 0u : std::min<uint32_t>(48u * 1024u, lds_budget - std::min(lds_budget, group_area));
+        captured ? 0u : std::min<uint32_t>(stage_max_bytes(), lds_budget - std::min(lds_budget, group_area));
     // probe costs follow where the tables will live
     {
         const std::vector<Staging> st = choose_staged(d, stage_cap);

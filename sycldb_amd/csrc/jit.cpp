@@ -147,7 +147,106 @@ struct Gen {
     {
         if (reg >= F_PF_BASE) return "vec_rows<R>(pf.c[" + std::to_string(reg - F_PF_BASE) + "][u], " + dst + ");";
         if (reg >= 0) return "copy_rows<R>(" + vreg(reg) + "[u], " + dst + ");";
+        auto it = preload.find(mem);
+        if (it != preload.end()) return "copy_rows<R>(" + it->second + "[u], " + dst + ");";
         return "load_rows<R>(" + mem + ", row0[u], " + dst + ");";
+    }
+
+    // Early reads of lazily loaded columns.  A column an instruction reads
+    // from memory for the rows of its input flags F is read, instead, right
+    // after the last probe through a table in global memory that narrowed F,
+    // when only conjunctive instructions on F lie between (so the rows then
+    // in F include every row that will need it): its read then overlaps that
+    // probe's value reads and the reads of the other columns so placed,
+    // instead of waiting in turn.  preload: operand expression -> register
+    // array; after_finish[k]: the reads issued after instruction k.
+    std::map<std::string, std::string> preload;
+    std::map<int, std::string> after_finish;
+    std::string preload_decls;
+
+    bool global_probe(int k) const
+    {
+        const FInstr &I = p.ins[k];
+        if (dropped(k)) return false;
+        return I.kind == FK_PROBE_SEMI || I.kind == FK_PROBE_FULL ||
+               (I.kind == FK_PROBE_BITS && (!I.in_lds || (I.vkind != FV_NONE && I.vkind != FV_RANK16)));
+    }
+
+    int preload_anchor(int j, int F) const
+    {
+        for (int k = j - 1; k >= 0; k--) {
+            const FInstr &I = p.ins[k];
+            const bool on_f = writes_flags(I.kind) && I.fout == F;
+            if (!on_f) continue;
+            if (!(conjunctive(I) && I.fin == F)) return -1;
+            if (global_probe(k)) return k;
+        }
+        return -1;
+    }
+
+    void plan_preloads()
+    {
+        preload.clear();
+        after_finish.clear();
+        preload_decls.clear();
+        static const bool on = [] {
+            const char *e = getenv("SDB_JIT_EARLY_COLUMNS");
+            return !(e && *e == '0');
+        }();
+        if (!on) return;
+        int n = 0;
+        for (int j = 0; j < p.ninstr; j++) {
+            const FInstr &I = p.ins[j];
+            const std::string P = "P.ins[" + std::to_string(j) + "]";
+            std::vector<std::string> mems;
+            switch (I.kind) {
+            case FK_SEL_LIT:
+            case FK_SEL_RANGE:
+            case FK_SEL_IN2:
+                // only an AND selection reads for the rows of its input alone
+                if (I.logic == SDB_AND && I.va < 0) mems.push_back(P + ".a");
+                break;
+            case FK_SEL_COL:
+                if (I.logic == SDB_AND) {
+                    if (I.va < 0) mems.push_back(P + ".a");
+                    if (I.vb < 0) mems.push_back(P + ".b");
+                }
+                break;
+            case FK_BINOP_CC:
+                if (I.va < 0) mems.push_back(P + ".a");
+                if (I.vb < 0) mems.push_back(P + ".b");
+                break;
+            case FK_BINOP_LC:
+                if (I.vb < 0) mems.push_back(P + ".b");
+                break;
+            case FK_BINOP_CL:
+            case FK_SUM:
+            case FK_PROBE_SEMI:
+            case FK_PROBE_FULL:
+            case FK_PROBE_BITS:
+                if (I.va < 0 && !dropped(j)) mems.push_back(P + ".a");
+                break;
+            case FK_GROUPBY:
+                for (int g = 0; g < p.gb.k; g++)
+                    if (p.gb.key_reg[g] < 0) mems.push_back("P.gb.key_mem[" + std::to_string(g) + "]");
+                if (p.gb.agg_reg < 0) mems.push_back("P.gb.agg_mem");
+                break;
+            default: break;
+            }
+            if (mems.empty()) continue;
+            const int k = preload_anchor(j, I.fin);
+            if (k < 0) continue;
+            const std::string F = freg(I.fin) + "[u]";
+            std::string &code = after_finish[k];
+            for (const std::string &m : mems) {
+                if (preload.count(m)) continue;
+                const std::string name = "early" + std::to_string(n++);
+                preload[m] = name;
+                preload_decls += "    int32_t " + name + "[UU][R];\n";
+                code += "#pragma unroll\n    for (int u = 0; u < UU; u++) if (" + F + ") load_rows<R>(" + m +
+                        ", row0[u], " + name + "[u]);\n";
+            }
+        }
     }
 
     std::vector<bool> hoisted;
@@ -255,15 +354,16 @@ struct Gen {
             break;
         case FK_GROUPBY: {
             for (int j = 0; j < p.gb.k; j++)
-                if (p.gb.key_reg[j] < 0) return true;
-            return p.gb.agg_reg < 0;
+                if (p.gb.key_reg[j] < 0 && !preload.count("P.gb.key_mem[" + std::to_string(j) + "]")) return true;
+            return p.gb.agg_reg < 0 && !preload.count("P.gb.agg_mem");
         }
         default: break;
         }
+        const std::string P = "P.ins[" + std::to_string(i) + "]";
         const bool has_b = I.kind == FK_SEL_COL || I.kind == FK_BINOP_CC || I.kind == FK_BINOP_LC ||
                            I.kind == FK_BUILD_KEYVALS;
         const bool has_a = I.kind != FK_BINOP_LC;
-        return (has_a && I.va < 0) || (has_b && I.vb < 0);
+        return (has_a && I.va < 0 && !preload.count(P + ".a")) || (has_b && I.vb < 0 && !preload.count(P + ".b"));
     }
 
     // Emits instruction i as { declarations; load phase; [split]; compute }.
@@ -418,6 +518,8 @@ struct Gen {
         }
         o << "    { // " << i << "\n" << d.str() << a.str() << (split.empty() ? "" : hoist_code + split) << c.str()
           << "    }\n";
+        auto early = after_finish.find(i);
+        if (early != after_finish.end()) o << early->second;
     }
 
     std::string body()
@@ -444,6 +546,8 @@ struct Gen {
             else o << "0u;\n";
         }
         for (int r = 0; r < p.nvals; r++) o << "    int32_t " << vreg(r) << "[UU][R] = {};\n";
+        plan_preloads();
+        o << preload_decls;
         for (int i = 0; i < p.ninstr; i++)
             if (is_probe(p.ins[i].kind) && !dropped(i))
                 o << "    int32_t x" << i << "[UU][R];\n    ProbeTmp<R> t" << i << "[UU];\n";
@@ -767,6 +871,28 @@ std::vector<std::pair<FProgram, int>> coverage_programs()
         p.gb.lds = FG_HASH_LDS;
         p.gb.key_reg[0] = 1, p.gb.key_reg[1] = 0;
         p.gb.agg_reg = F_PF_BASE + 3;
+        out.emplace_back(p, 2);
+    }
+    {   // star join whose later columns are read from memory: the date key
+        // and the aggregate are read early, after the global probe
+        FProgram p = base(1, 2);
+        p.nvals = 2;
+        p.flag_init[0] = FF_TRUE;
+        FInstr a = mk(FK_PROBE_BITS, 0, 0, F_PF_BASE);
+        a.in_lds = 1;
+        ins(p, a);
+        FInstr b = mk(FK_PROBE_BITS, 0, 0, F_PF_BASE + 1, -1, 0);
+        b.vkind = FV_U16;
+        ins(p, b);
+        FInstr c = mk(FK_PROBE_BITS, 0, 0, -1, -1, 1);
+        c.in_lds = 1;
+        c.vkind = FV_RANK16;
+        ins(p, c);
+        ins(p, mk(FK_GROUPBY, 0, 0, -1));
+        p.gb.k = 2;
+        p.gb.lds = FG_HASH_LDS;
+        p.gb.key_reg[0] = 1, p.gb.key_reg[1] = 0;
+        p.gb.agg_reg = -1;
         out.emplace_back(p, 2);
     }
     {   // one-key group-by over a prefetched key, memory aggregate
