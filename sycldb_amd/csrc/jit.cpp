@@ -327,7 +327,8 @@ struct Gen {
         const FInstr &W = p.ins[fw];
         const int F = W.fin;
         if (!(conjunctive(W) && W.fout == F)) return;
-        for (int i = fw + 1; i < p.ninstr; i++) {
+        const int end = tail_start >= 0 ? tail_start : p.ninstr;
+        for (int i = fw + 1; i < end; i++) {
             const FInstr &I = p.ins[i];
             if (is_probe(I.kind) && I.fin == F && I.va >= F_PF_BASE && waits(i)) {
                 hoisted[i] = true;
@@ -522,6 +523,221 @@ struct Gen {
         if (early != after_finish.end()) o << early->second;
     }
 
+    // Row-at-a-time tail.  The instructions after the last probe through a
+    // table in global memory see only the few rows that passed every probe
+    // before them (the star joins: under 1 %), yet in chunk form their code
+    // runs for every row slot some lane of the wave still holds -- nearly all
+    // of them.  They run instead in one loop over each lane's surviving rows
+    // (as the group-by update already does), on the R = 1 forms of the same
+    // templates: the per-row result is the chunk form's.  Taken only when the
+    // tail is a conjunctive chain on one flag register, stores nothing, and
+    // reads memory only through early reads.
+    int tail_start = -1;
+
+    static bool tail_kind(const FInstr &I)
+    {
+        switch (I.kind) {
+        case FK_SEL_LIT:
+        case FK_SEL_COL:
+        case FK_SEL_RANGE:
+        case FK_SEL_IN2: return I.logic == SDB_AND;
+        case FK_PROBE_SEMI:
+        case FK_PROBE_FULL:
+        case FK_PROBE_BITS:
+        case FK_BINOP_CC:
+        case FK_BINOP_LC:
+        case FK_BINOP_CL:
+        case FK_SUM:
+        case FK_GROUPBY: return true;
+        default: return false;
+        }
+    }
+
+    void plan_tail()
+    {
+        tail_start = -1;
+        static const bool on = [] {
+            const char *e = getenv("SDB_JIT_ROW_TAIL");
+            return !(e && *e == '0');
+        }();
+        if (!on || batch_act) return;
+        int last_global = -1;
+        for (int i = 0; i < p.ninstr; i++)
+            if (global_probe(i)) last_global = i;
+        const int T = last_global + 1;
+        // a terminal alone already runs row at a time (group-by) or is cheap
+        if (last_global < 0 || T >= p.ninstr - 1) return;
+        const int F = p.ins[T].fin;
+        if (p.flag_store[F]) return;
+        for (int i = T; i < p.ninstr; i++) {
+            const FInstr &I = p.ins[i];
+            if (!tail_kind(I) || I.fin != F || I.store) return;
+            if (writes_flags(I.kind) && I.fout != F) return;
+            const std::string P = "P.ins[" + std::to_string(i) + "]";
+            const bool has_b = I.kind == FK_SEL_COL || I.kind == FK_BINOP_CC || I.kind == FK_BINOP_LC;
+            const bool has_a = I.kind != FK_BINOP_LC && I.kind != FK_GROUPBY;
+            if (has_a && I.va < 0 && !preload.count(P + ".a")) return;
+            if (has_b && I.vb < 0 && !preload.count(P + ".b")) return;
+        }
+        const FInstr &Z = p.ins[p.ninstr - 1];
+        if (Z.kind == FK_GROUPBY) {
+            for (int g = 0; g < p.gb.k; g++)
+                if (p.gb.key_reg[g] < 0 && !preload.count("P.gb.key_mem[" + std::to_string(g) + "]")) return;
+            if (p.gb.agg_reg < 0 && !preload.count("P.gb.agg_mem")) return;
+        }
+        // a terminal, if any, only last
+        for (int i = T; i + 1 < p.ninstr; i++)
+            if (p.ins[i].kind == FK_SUM || p.ins[i].kind == FK_GROUPBY) return;
+        tail_start = T;
+    }
+
+    // The tail's row vectors (one element per row slot of the tile) and the
+    // per-row expression of an operand.
+    std::map<std::string, std::string> tail_vecs; // source statement key -> vector name
+    std::vector<std::pair<std::string, std::string>> tail_fill; // vector name, statement filling t[R]
+
+    std::string tail_operand(int reg, const std::string &mem)
+    {
+        if (reg >= 0 && reg < F_PF_BASE) return "s" + vreg(reg); // the row's value register
+        std::string key, fill;
+        if (reg >= F_PF_BASE) {
+            key = "pf" + std::to_string(reg - F_PF_BASE);
+            fill = "vec_rows<R>(pf.c[" + std::to_string(reg - F_PF_BASE) + "][u], t);";
+        } else {
+            const std::string &arr = preload.at(mem);
+            key = arr;
+            fill = "copy_rows<R>(" + arr + "[u], t);";
+        }
+        auto it = tail_vecs.find(key);
+        if (it != tail_vecs.end()) return it->second + "[b]";
+        const std::string name = "tv_" + key;
+        tail_vecs[key] = name;
+        tail_fill.push_back({name, fill});
+        return name + "[b]";
+    }
+
+    void tail_code()
+    {
+        const int T = tail_start, F = p.ins[T].fin;
+        tail_vecs.clear();
+        tail_fill.clear();
+        std::ostringstream loop;
+        for (int i = T; i < p.ninstr; i++) {
+            const FInstr &I = p.ins[i];
+            const std::string P = "P.ins[" + std::to_string(i) + "]";
+            const std::string L = std::to_string((int)I.logic), C = std::to_string((int)I.cmp);
+            loop << "            { // " << i << "\n";
+            switch (I.kind) {
+            case FK_SEL_LIT:
+                loop << "                const int32_t x[1] = {" << tail_operand(I.va, P + ".a") << "};\n"
+                     << "                if (!pred_lit<1, " << C << ">(x, " << P << ".lit)) continue;\n";
+                break;
+            case FK_SEL_RANGE:
+                loop << "                const int32_t x[1] = {" << tail_operand(I.va, P + ".a") << "};\n"
+                     << "                if (!pred_range<1>(x, " << P << ".lit, " << P << ".lit2)) continue;\n";
+                break;
+            case FK_SEL_IN2:
+                loop << "                const int32_t x[1] = {" << tail_operand(I.va, P + ".a") << "};\n"
+                     << "                if (!pred_in2<1>(x, " << P << ".lit, " << P << ".lit2)) continue;\n";
+                break;
+            case FK_SEL_COL:
+                loop << "                const int32_t x[1] = {" << tail_operand(I.va, P + ".a") << "}, y[1] = {"
+                     << tail_operand(I.vb, P + ".b") << "};\n"
+                     << "                if (!pred_col<1, " << C << ">(x, y)) continue;\n";
+                break;
+            case FK_BINOP_CC:
+            case FK_BINOP_LC:
+            case FK_BINOP_CL: {
+                const std::string a = I.kind == FK_BINOP_LC ? P + ".lit" : tail_operand(I.va, P + ".a");
+                const std::string b = I.kind == FK_BINOP_CL ? P + ".lit" : tail_operand(I.vb, P + ".b");
+                loop << "                const int32_t x[1] = {" << a << "}, y[1] = {" << b << "};\n"
+                     << "                int32_t r[1];\n                binop<1, " << (int)I.op << ">(x, y, r);\n"
+                     << "                s" << vreg(I.vout) << " = r[0];\n";
+                break;
+            }
+            case FK_PROBE_SEMI:
+            case FK_PROBE_FULL:
+            case FK_PROBE_BITS: {
+                const bool vout = I.kind == FK_PROBE_FULL || (I.kind == FK_PROBE_BITS && I.vkind != FV_NONE);
+                loop << "                const int32_t x[1] = {" << tail_operand(I.va, P + ".a") << "};\n"
+                     << "                ProbeTmp<1> t;\n                int32_t v[1] = {"
+                     << (vout ? "s" + vreg(I.vout) : std::string("0")) << "};\n";
+                if (I.kind == FK_PROBE_SEMI)
+                    loop << "                probe_semi_issue<1>(x, 1u, (const uint8_t *)" << P << ".ht, " << P << ".ht_min, "
+                         << P << ".ht_max, t);\n                if (!probe_semi_finish<1>(1u, t)) continue;\n";
+                else if (I.kind == FK_PROBE_FULL)
+                    loop << "                probe_full_issue<1>(x, 1u, (const int2 *)" << P << ".ht, " << P << ".ht_min, "
+                         << P << ".ht_max, t);\n                if (!probe_full_finish<1>(1u, t, v)) continue;\n";
+                else {
+                    const std::string bits =
+                        I.in_lds ? "(const uint32_t *)(lds + " + P + ".lds_off)" : "(const uint32_t *)" + P + ".ht";
+                    const int vk = late_value(i) ? 0 : (int)I.vkind;
+                    const std::string inl = I.in_lds ? "true" : "false";
+                    loop << "                probe_bits_issue<1, " << vk << ", " << inl << ">(x, 1u, " << bits << ", "
+                         << P << ".ht_min, " << P << ".ht_max, " << P << ".vals, " << P << ".vmin, " << P
+                         << ".ht_len, t);\n"
+                         << "                if (!probe_bits_finish<1, " << vk << ", " << inl << ">(x, 1u, " << P
+                         << ".ht_min, " << P << ".ht_max, t, " << bits << ", " << P << ".vmin, " << P
+                         << ".ht_len, v)) continue;\n";
+                    if (late_value(i))
+                        loop << "                probe_value_after<1, " << (int)I.vkind << ">(x, 1u, " << P << ".vals, "
+                             << P << ".ht_min, " << P << ".vmin, v);\n";
+                }
+                if (vout) loop << "                s" << vreg(I.vout) << " = v[0];\n";
+                break;
+            }
+            case FK_SUM:
+                loop << "                acc += " << tail_operand(I.va, P + ".a") << ";\n";
+                break;
+            case FK_GROUPBY: {
+                const FGroup &g = p.gb;
+                loop << "                const int32_t kv[F_MAX_KEYS] = {";
+                for (int j = 0; j < F_MAX_KEYS; j++)
+                    loop << (j ? ", " : "")
+                         << (j < g.k ? tail_operand(g.key_reg[j], "P.gb.key_mem[" + std::to_string(j) + "]") : "0");
+                loop << "};\n                fdev::gb_row_mode<" << g.lds << ">(P.gb, lds, kv, "
+                     << tail_operand(g.agg_reg, "P.gb.agg_mem") << ");\n";
+                break;
+            }
+            default: fail(SDB_ERR_RUNTIME, "jit: instruction kind outside the row tail");
+            }
+            loop << "            }\n";
+        }
+        // value registers: the row's value, from the chunk registers
+        std::vector<bool> reads(p.nvals, false);
+        const std::string body = loop.str();
+        for (int r = 0; r < p.nvals; r++) reads[r] = body.find("s" + vreg(r)) != std::string::npos;
+        o << "    { // instructions " << T << ".. row at a time over the rows that reach them\n";
+        o << "        typename Lanes<UU * R>::type";
+        bool first = true;
+        for (auto &f : tail_fill) {
+            o << (first ? " " : ", ") << f.first << " = {}";
+            first = false;
+        }
+        for (int r = 0; r < p.nvals; r++)
+            if (reads[r]) {
+                o << (first ? " " : ", ") << "tv_" << vreg(r) << " = {}";
+                first = false;
+            }
+        if (first) o << " unused_ = {}";
+        o << ";\n        uint32_t rows = 0;\n";
+        o << "#pragma unroll\n        for (int u = 0; u < UU; u++) {\n";
+        o << "            rows |= " << freg(F) << "[u] << (u * R);\n            if (!" << freg(F) << "[u]) continue;\n";
+        o << "            int32_t t[R];\n";
+        for (auto &f : tail_fill)
+            o << "            " << f.second << "\n#pragma unroll\n            for (int k = 0; k < R; k++) " << f.first
+              << "[u * R + k] = t[k];\n";
+        for (int r = 0; r < p.nvals; r++)
+            if (reads[r])
+                o << "#pragma unroll\n            for (int k = 0; k < R; k++) tv_" << vreg(r) << "[u * R + k] = "
+                  << vreg(r) << "[u][k];\n";
+        o << "        }\n#pragma unroll 1\n        while (rows) {\n";
+        o << "            const int b = __builtin_ctz(rows);\n            rows &= rows - 1;\n";
+        for (int r = 0; r < p.nvals; r++)
+            if (reads[r]) o << "            int32_t s" << vreg(r) << " = tv_" << vreg(r) << "[b];\n";
+        o << body << "        }\n    }\n";
+    }
+
     std::string body()
     {
         // registers the program touches
@@ -548,7 +764,9 @@ struct Gen {
         for (int r = 0; r < p.nvals; r++) o << "    int32_t " << vreg(r) << "[UU][R] = {};\n";
         plan_preloads();
         o << preload_decls;
-        for (int i = 0; i < p.ninstr; i++)
+        plan_tail();
+        const int dense_end = tail_start >= 0 ? tail_start : p.ninstr;
+        for (int i = 0; i < dense_end; i++)
             if (is_probe(p.ins[i].kind) && !dropped(i))
                 o << "    int32_t x" << i << "[UU][R];\n    ProbeTmp<R> t" << i << "[UU];\n";
         // the next tile's column loads go out after the loads of the first
@@ -557,7 +775,7 @@ struct Gen {
         const std::string prefetch =
             prefetch_in_body ? "        if (has_next) pf_load<R, UU>(P, nrow0, next);\n" : std::string();
         int first_wait = -1;
-        for (int i = 0; i < p.ninstr && first_wait < 0; i++)
+        for (int i = 0; i < dense_end && first_wait < 0; i++)
             if (waits(i)) first_wait = i;
         // early issue of chained probes trades extra table reads for one
         // fewer wait; measured slower on the star joins of a single
@@ -572,7 +790,8 @@ struct Gen {
         const std::string at_wait =
             prefetch_in_body ? prefetch : (hoist_code.empty() ? std::string() : "        // issued early above\n");
         if (at_wait.empty()) first_wait = -1;
-        for (int i = 0; i < p.ninstr; i++) instr(i, i == first_wait ? at_wait : std::string());
+        for (int i = 0; i < dense_end; i++) instr(i, i == first_wait ? at_wait : std::string());
+        if (tail_start >= 0) tail_code();
         for (int j = 0; j < F_MAX_FLAGS; j++)
             if (p.flag_store[j])
                 o << "#pragma unroll\n    for (int u = 0; u < UU; u++) fdev::flags_out<R>(P.flag_store[" << j
@@ -587,8 +806,7 @@ using namespace sdb::fops;
 namespace {
 )";
 
-const char *KERNEL_TEMPLATE_HEAD = R"(constexpr int BLOCK = fdev::BLOCK;
-template <int R, int UU> struct Pf {
+const char *KERNEL_TEMPLATE_HEAD = R"(template <int R, int UU> struct Pf {
     typename Vec<R>::type c[NPF > 0 ? NPF : 1][UU];
 };
 template <int R, int UU>
@@ -608,14 +826,14 @@ __device__ __forceinline__ void body(const FProgram &P, const int64_t (&row0)[UU
 const char *KERNEL_TEMPLATE_TAIL = R"(}
 } // namespace
 
-extern "C" __global__ void __launch_bounds__(256) sdb_fused_program(const FProgram P_arg, int64_t n)
+extern "C" __global__ void __launch_bounds__(BLOCK) sdb_fused_program(const FProgram P_arg, int64_t n)
 {
     // the program is read in place from the kernel-argument segment
     const FProgram &P = *(const FProgram __attribute__((address_space(4))) *)__builtin_amdgcn_kernarg_segment_ptr();
     if (P.magic != F_MAGIC) return;
     extern __shared__ uint4 dyn_lds[];
     uint8_t *lds = reinterpret_cast<uint8_t *>(dyn_lds);
-    fdev::block_prologue(P, lds);
+    fdev::block_prologue<BLOCK>(P, lds);
     int64_t acc = 0;
     if constexpr (VEC) {
     // tiles of U * BLOCK 4-row chunks; chunk u of a thread in tile t starts
@@ -671,7 +889,7 @@ extern "C" __global__ void __launch_bounds__(256) sdb_fused_program(const FProgr
         i = in;
     }
     }
-    fdev::block_epilogue(P, lds, acc);
+    fdev::block_epilogue<BLOCK>(P, lds, acc);
 }
 )";
 
@@ -948,22 +1166,24 @@ int jit_kernels_compiled()
     return g_compiled;
 }
 
-std::string jit_source(const FProgram &p, int unroll)
+std::string jit_source(const FProgram &p, int unroll, int block)
 {
     if (unroll != 1 && unroll != 2 && unroll != 4) fail(SDB_ERR_INVALID_ARG, "jit: unroll must be 1, 2 or 4");
+    if (block != 256 && block != 512 && block != 1024) fail(SDB_ERR_INVALID_ARG, "jit: block must be 256, 512 or 1024");
     Gen g(p);
     std::ostringstream s;
     s << KERNEL_TEMPLATE_INCLUDES << "constexpr int NPF = " << p.npf << ";\nconstexpr int U = " << unroll
-      << ";\nconstexpr bool VEC = " << (p.vec ? "true" : "false") << ";\n";
+      << ";\nconstexpr bool VEC = " << (p.vec ? "true" : "false") << ";\nconstexpr int BLOCK = " << block << ";\n";
     s << KERNEL_TEMPLATE_HEAD << g.body() << KERNEL_TEMPLATE_TAIL;
     return s.str();
 }
 
 namespace {
 
-// Launches the compiled kernel of `src` with `per_cu_mult` times the blocks
-// per CU the occupancy calculator allows.
-void launch_compiled(const FProgram &p, int64_t n, hipStream_t s, int unroll, int per_cu_mult, const std::string &src)
+// Launches the compiled kernel of `src` (blocks of `block` threads) with
+// `per_cu_mult` times the blocks per CU the occupancy calculator allows.
+void launch_compiled(const FProgram &p, int64_t n, hipStream_t s, int unroll, int per_cu_mult, const std::string &src,
+                     int block = JIT_BLOCK)
 {
     Compiled &k = get_kernel(src);
     // the value-register file (the last LDS area) belongs to the interpreter
@@ -974,36 +1194,39 @@ void launch_compiled(const FProgram &p, int64_t n, hipStream_t s, int unroll, in
         auto it = k.blocks_per_cu.find(lds);
         if (it == k.blocks_per_cu.end()) {
             int b = 0;
-            SDB_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&b, k.fn, JIT_BLOCK, lds));
+            SDB_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&b, k.fn, block, lds));
             it = k.blocks_per_cu.emplace(lds, std::max(1, b)).first;
         }
         per_cu = it->second * per_cu_mult;
     }
     const int64_t units = p.vec ? (n + 3) / 4 : n;
-    const unsigned grid = stream_grid(units, (int64_t)JIT_BLOCK * (p.vec ? unroll : 1), per_cu);
+    const unsigned grid = stream_grid(units, (int64_t)block * (p.vec ? unroll : 1), per_cu);
     struct Args {
         FProgram p;
         int64_t n;
     } args{p, n};
     size_t size = sizeof(args);
     void *config[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, &args, HIP_LAUNCH_PARAM_BUFFER_SIZE, &size, HIP_LAUNCH_PARAM_END};
-    SDB_HIP(hipModuleLaunchKernel(k.fn, grid, 1, 1, JIT_BLOCK, 1, 1, lds, s, nullptr, config));
+    SDB_HIP(hipModuleLaunchKernel(k.fn, grid, 1, 1, block, 1, 1, lds, s, nullptr, config));
 }
 
 // ------------------------------------------------------------- autotuner
 //
 // Per program shape (and device), the first launches over at least
 // TUNE_MIN_ROWS rows each try one launch configuration -- chunks per thread
-// per iteration x blocks per CU -- timed with HIP events; after TUNE_ROUNDS
-// timings of every candidate the fastest per row is used from then on.
-// Every configuration computes the same result; only the time differs.
+// per iteration x threads per block x blocks per CU -- timed with HIP
+// events; after TUNE_ROUNDS timings of every candidate the fastest per row is
+// used from then on.  Every configuration computes the same result; only the
+// time differs.  Larger blocks share one copy of the program's LDS among
+// more waves, so a program whose staged tables limit the blocks per CU runs
+// more waves per CU.
 constexpr int64_t TUNE_MIN_ROWS = 4 << 20;
 constexpr int TUNE_ROUNDS = 1;
 
 struct TuneCfg {
-    int unroll, per_cu_mult;
+    int unroll, per_cu_mult, block;
 };
-const TuneCfg TUNE_CANDIDATES[] = {{2, 1}, {1, 1}, {4, 1}, {2, 2}, {1, 2}};
+const TuneCfg TUNE_CANDIDATES[] = {{2, 1, 256}, {1, 1, 256}, {4, 1, 256}, {2, 1, 512}, {1, 1, 512}, {2, 1, 1024}};
 constexpr int N_CAND = (int)(sizeof(TUNE_CANDIDATES) / sizeof(TUNE_CANDIDATES[0]));
 
 struct Tune {
@@ -1044,11 +1267,11 @@ void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll)
     }
     if (cand < 0) {
         const TuneCfg &c = TUNE_CANDIDATES[-1 - cand];
-        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, jit_source(p, c.unroll));
+        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, jit_source(p, c.unroll, c.block), c.block);
         return;
     }
     const TuneCfg &c = TUNE_CANDIDATES[cand];
-    const std::string src = jit_source(p, c.unroll);
+    const std::string src = jit_source(p, c.unroll, c.block);
     get_kernel(src); // compile outside the timed region
     hipEvent_t a, b;
     SDB_HIP(hipEventCreate(&a));
@@ -1056,7 +1279,7 @@ void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll)
     float ms = 0;
     hipError_t e = hipEventRecord(a, s);
     if (e == hipSuccess) {
-        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, src);
+        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, src, c.block);
         e = hipEventRecord(b, s);
     }
     if (e == hipSuccess) e = hipEventSynchronize(b);
@@ -1699,13 +1922,14 @@ std::string jit_tuning_report()
         const Tune &t = kv.second;
         o << "shape " << shape++ << " device " << kv.first.first << ": ";
         if (t.chosen >= 0)
-            o << "unroll " << TUNE_CANDIDATES[t.chosen].unroll << ", blocks/CU x"
-              << TUNE_CANDIDATES[t.chosen].per_cu_mult;
+            o << "unroll " << TUNE_CANDIDATES[t.chosen].unroll << ", block " << TUNE_CANDIDATES[t.chosen].block
+              << ", blocks/CU x" << TUNE_CANDIDATES[t.chosen].per_cu_mult;
         else
             o << "tuning (" << t.trials << " trials)";
         o << "; ns/row";
         for (int i = 0; i < N_CAND; i++) {
-            o << " [u" << TUNE_CANDIDATES[i].unroll << " x" << TUNE_CANDIDATES[i].per_cu_mult << "] ";
+            o << " [u" << TUNE_CANDIDATES[i].unroll << " b" << TUNE_CANDIDATES[i].block << " x"
+              << TUNE_CANDIDATES[i].per_cu_mult << "] ";
             if (t.best_ns_per_row[i] < 1e299) o << t.best_ns_per_row[i];
             else o << "-";
         }
