@@ -560,19 +560,31 @@ struct Gen {
             const char *e = getenv("SDB_JIT_ROW_TAIL");
             return !(e && *e == '0');
         }();
-        if (!on || batch_act) return;
-        int last_global = -1;
-        for (int i = 0; i < p.ninstr; i++)
-            if (global_probe(i)) last_global = i;
-        const int T = last_global + 1;
-        // a terminal alone already runs row at a time (group-by) or is cheap
-        if (last_global < 0 || T >= p.ninstr - 1) return;
-        const int F = p.ins[T].fin;
-        if (p.flag_store[F]) return;
+        if (!on) return;
+        int T;
+        if (batch_act) {
+            // a batched body sees only its `act` rows, a few per cent of the
+            // tile: the whole body runs row at a time when it starts from the
+            // all-true flags (which become `act`)
+            T = 0;
+            if (p.ninstr < 1 || p.flag_init[p.ins[0].fin] != FF_TRUE) return;
+        } else {
+            int last_global = -1;
+            for (int i = 0; i < p.ninstr; i++)
+                if (global_probe(i)) last_global = i;
+            T = last_global + 1;
+            // a terminal alone already runs row at a time (group-by) or is cheap
+            if (last_global < 0 || T >= p.ninstr - 1) return;
+        }
+        for (int j = 0; j < F_MAX_FLAGS; j++)
+            if (p.flag_store[j]) return;
+        // one chain: every instruction reads the flags the previous one wrote
+        // (a conjunctive instruction may write them to another register)
+        int cur = p.ins[T].fin;
         for (int i = T; i < p.ninstr; i++) {
             const FInstr &I = p.ins[i];
-            if (!tail_kind(I) || I.fin != F || I.store) return;
-            if (writes_flags(I.kind) && I.fout != F) return;
+            if (!tail_kind(I) || I.fin != cur || I.store) return;
+            if (writes_flags(I.kind)) cur = I.fout;
             const std::string P = "P.ins[" + std::to_string(i) + "]";
             const bool has_b = I.kind == FK_SEL_COL || I.kind == FK_BINOP_CC || I.kind == FK_BINOP_LC;
             const bool has_a = I.kind != FK_BINOP_LC && I.kind != FK_GROUPBY;
@@ -659,6 +671,20 @@ struct Gen {
             case FK_PROBE_FULL:
             case FK_PROBE_BITS: {
                 const bool vout = I.kind == FK_PROBE_FULL || (I.kind == FK_PROBE_BITS && I.vkind != FV_NONE);
+                if (dropped(i)) {
+                    loop << "                // decided by the shared masks\n";
+                    break;
+                }
+                if (is_decided(i)) {
+                    // the masks established the match: the value alone
+                    const int vk = I.kind == FK_PROBE_FULL ? (int)FV_NONE : (int)I.vkind;
+                    const std::string vals = I.kind == FK_PROBE_FULL ? P + ".ht" : P + ".vals";
+                    loop << "                const int32_t x[1] = {" << tail_operand(I.va, P + ".a") << "};\n"
+                         << "                ProbeTmp<1> t;\n                probe_value_issue<1, " << vk << ">(x, 1u, "
+                         << vals << ", " << P << ".ht_min, " << P << ".ht_max, " << P << ".vmin, t);\n"
+                         << "                s" << vreg(I.vout) << " = t.val[0];\n";
+                    break;
+                }
                 loop << "                const int32_t x[1] = {" << tail_operand(I.va, P + ".a") << "};\n"
                      << "                ProbeTmp<1> t;\n                int32_t v[1] = {"
                      << (vout ? "s" + vreg(I.vout) : std::string("0")) << "};\n";
