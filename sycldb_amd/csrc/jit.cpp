@@ -1342,17 +1342,27 @@ void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll)
 namespace {
 
 constexpr int BATCH_MAX_PF = 16;
-// threads per block of the batched kernel: one block per CU holds 16 waves
-// and the whole LDS budget of the batch's programs
-constexpr int BATCH_BLOCK = 1024;
+// threads per block of the batched kernel (the batch's programs keep little
+// LDS since they stage no tables: one block's regions fit several times per
+// CU); measured on 32 pipelines at 600M rows, 512-thread blocks with one tile
+// of columns in registers ran 17.4 ms against 19.4 ms for 1024 threads with
+// the next tile's columns loading meanwhile (the registers that double
+// buffering takes cost more waves than the overlap gains)
+constexpr int BATCH_BLOCK = 512;
 constexpr int64_t BATCH_MIN_TILES = 8;
 constexpr int BATCH_MAX_SHAPES = 32;
 // a probe joins the shared masks only over a key range of at most this many
-// keys (4 bytes of table per key)
-constexpr int64_t BATCH_MASK_PROBE_LEN = int64_t(1) << 22;
+// keys (4 bytes of table per key; SDB_BATCH_MASK_MAX_KEYS overrides)
+int64_t batch_mask_probe_len()
+{
+    static const int64_t v = [] {
+        const char *e = getenv("SDB_BATCH_MASK_MAX_KEYS");
+        return e && *e ? (int64_t)atoll(e) : (int64_t(1) << 22);
+    }();
+    return v;
+}
 
-const char *BATCH_HEAD = R"(constexpr int BLOCK = 1024;
-struct BatchArgs {
+const char *BATCH_HEAD = R"(struct BatchArgs {
     const FProgram *progs;
     int64_t n;
     int32_t nprog;
@@ -1449,24 +1459,35 @@ extern "C" __global__ void __launch_bounds__(BLOCK) sdb_fused_batch(const BatchA
             for (int u = 0; u < U; u++) r[u] = (t * tile + u * BLOCK + threadIdx.x) << 2;
         };
         int64_t t = blockIdx.x;
-        Pf<4, U> next = {};
-        if (t < ntiles) {
-            int64_t r[U];
-            rows_of(t, r);
-            pf_load<4, U>(A, r, next);
-        }
-        while (t < ntiles) {
-            const Pf<4, U> cur = next;
-            int64_t r[U];
-            rows_of(t, r);
-            const int64_t tn = t + gridDim.x;
-            if (tn < ntiles) {
-                int64_t rn[U];
-                rows_of(tn, rn);
-                pf_load<4, U>(A, rn, next);
+        if constexpr (DOUBLE_BUFFER) {
+            Pf<4, U> next = {};
+            if (t < ntiles) {
+                int64_t r[U];
+                rows_of(t, r);
+                pf_load<4, U>(A, r, next);
             }
-            run_all<4, U>(A, r, cur, lds, sums);
-            t = tn;
+            while (t < ntiles) {
+                const Pf<4, U> cur = next;
+                int64_t r[U];
+                rows_of(t, r);
+                const int64_t tn = t + gridDim.x;
+                if (tn < ntiles) {
+                    int64_t rn[U];
+                    rows_of(tn, rn);
+                    pf_load<4, U>(A, rn, next);
+                }
+                run_all<4, U>(A, r, cur, lds, sums);
+                t = tn;
+            }
+        } else {
+            // one tile's columns in registers: fewer registers, more waves
+            for (; t < ntiles; t += gridDim.x) {
+                int64_t r[U];
+                rows_of(t, r);
+                Pf<4, U> cur = {};
+                pf_load<4, U>(A, r, cur);
+                run_all<4, U>(A, r, cur, lds, sums);
+            }
         }
         for (int64_t c = ntiles * tile + (int64_t)blockIdx.x * BLOCK + threadIdx.x; c < nch;
              c += (int64_t)gridDim.x * BLOCK) {
@@ -1636,7 +1657,7 @@ std::vector<int> shared_probes(const FProgram &q)
         // stay whole)
         if (((c >> i) & 1u) && is_probe_kind(I.kind) && I.va >= F_PF_BASE &&
             !(I.kind == FK_PROBE_BITS && I.vkind == FV_RANK16) &&
-            (int64_t)I.ht_max - I.ht_min + 1 <= BATCH_MASK_PROBE_LEN)
+            (int64_t)I.ht_max - I.ht_min + 1 <= batch_mask_probe_len())
             out.push_back(i);
     }
     return out;
@@ -1664,10 +1685,34 @@ struct BatchPlan {
     bool vec = true;
 };
 
+// Launch shape of the batched kernel: threads per block and whether the
+// next tile's columns load while the current one runs (SDB_BATCH_BLOCK,
+// SDB_BATCH_DOUBLE_BUFFER override).
+int batch_block()
+{
+    static const int v = [] {
+        const char *e = getenv("SDB_BATCH_BLOCK");
+        const int b = e && *e ? atoi(e) : BATCH_BLOCK;
+        return b == 256 || b == 512 || b == 1024 ? b : BATCH_BLOCK;
+    }();
+    return v;
+}
+
+bool batch_double_buffer()
+{
+    static const bool v = [] {
+        const char *e = getenv("SDB_BATCH_DOUBLE_BUFFER");
+        return e && *e == '1';
+    }();
+    return v;
+}
+
 std::string batch_source(const BatchLibrary &lib, int unroll, bool vec)
 {
     std::ostringstream s;
-    s << KERNEL_TEMPLATE_INCLUDES << "constexpr int NPF = " << std::max<size_t>(lib.slots.size(), 1)
+    s << KERNEL_TEMPLATE_INCLUDES << "constexpr int BLOCK = " << batch_block()
+      << ";\nconstexpr bool DOUBLE_BUFFER = " << (batch_double_buffer() ? "true" : "false")
+      << ";\nconstexpr int NPF = " << std::max<size_t>(lib.slots.size(), 1)
       << ";\nconstexpr int U = " << unroll << ";\nconstexpr bool VEC = " << (vec ? "true" : "false")
       << ";\nconstexpr int NSHAPE = " << lib.bodies.size() << ";\nconstexpr int NMASK = " << lib.mask_slots.size()
       << ";\nconstexpr int MSLOT[" << std::max<size_t>(lib.mask_slots.size(), 1) << "] = {";
@@ -1921,7 +1966,7 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
         auto it = k.blocks_per_cu.find(b.lds);
         if (it == k.blocks_per_cu.end()) {
             int bl = 0;
-            SDB_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&bl, k.fn, BATCH_BLOCK, b.lds));
+            SDB_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&bl, k.fn, batch_block(), b.lds));
             it = k.blocks_per_cu.emplace(b.lds, std::max(1, bl)).first;
         }
         per_cu = it->second;
@@ -1929,14 +1974,14 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     const int64_t units = b.vec ? (n + 3) / 4 : n;
     // at least BATCH_MIN_TILES tiles per block: each block also stages and
     // flushes every program's LDS state, which small tables must amortise
-    const int64_t tiles = (units + (int64_t)BATCH_BLOCK * (b.vec ? unroll : 1) - 1) / ((int64_t)BATCH_BLOCK * (b.vec ? unroll : 1));
+    const int64_t tiles = (units + (int64_t)batch_block() * (b.vec ? unroll : 1) - 1) / ((int64_t)batch_block() * (b.vec ? unroll : 1));
     const unsigned grid = (unsigned)std::max<int64_t>(
-        1, std::min<int64_t>(stream_grid(units, (int64_t)BATCH_BLOCK * (b.vec ? unroll : 1), per_cu),
+        1, std::min<int64_t>(stream_grid(units, (int64_t)batch_block() * (b.vec ? unroll : 1), per_cu),
                              (tiles + BATCH_MIN_TILES - 1) / BATCH_MIN_TILES));
     size_t size = args.size();
     void *config[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, args.data(), HIP_LAUNCH_PARAM_BUFFER_SIZE, &size,
                       HIP_LAUNCH_PARAM_END};
-    SDB_HIP(hipModuleLaunchKernel(k.fn, grid, 1, 1, BATCH_BLOCK, 1, 1, b.lds, s, nullptr, config));
+    SDB_HIP(hipModuleLaunchKernel(k.fn, grid, 1, 1, batch_block(), 1, 1, b.lds, s, nullptr, config));
 }
 
 std::string jit_tuning_report()
