@@ -557,9 +557,20 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
     // ---------------------------------------------------------- translation
     std::vector<Draft> d;
     bool terminal_seen = false;
-    for (const KernelData &k : kernels_) {
-        if (k.type == KernelType::EmptyKernel) continue;
+    for (const KernelData &k0 : kernels_) {
+        if (k0.type == KernelType::EmptyKernel) continue;
         if (terminal_seen) return false;
+        // a probe's compact tables, with their statistics read
+        KernelData k = k0;
+        if (k.compact) {
+            if (!k.compact->resolved) ctx.eng->resolve_compact();
+            if (!k.compact->resolved) fail(SDB_ERR_RUNTIME, "compact table statistics not read");
+            k.vals = k.compact->vals;
+            k.vkind = k.compact->vkind;
+            k.rank = k.compact->rank;
+            k.rank_bytes = k.compact->rank_bytes;
+            k.sel = k.compact->sel;
+        }
         Draft D;
         FInstr &I = D.I;
         I.va = I.vb = I.vout = -1;

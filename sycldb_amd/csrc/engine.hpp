@@ -54,7 +54,8 @@ uint64_t groupby_lds_max_slots();
 void launch_bits_from_keys(const uint8_t *ht, int32_t len, uint32_t *bits, uint32_t *count, hipStream_t s);
 // Rank dictionary (FV_RANK16 layout) of `words` presence words and their
 // per-slot 16-bit values.
-void launch_rank_dict(const uint32_t *bits, int32_t words, const uint16_t *vals, uint8_t *dict, hipStream_t s);
+void launch_rank_dict(const uint32_t *bits, int32_t words, const uint16_t *vals, uint8_t *dict, uint32_t dict_cap,
+                      hipStream_t s);
 void launch_bits_from_pairs(const int32_t *ht, int32_t len, uint32_t *bits, uint16_t *vals, int32_t vmin,
                             uint32_t *count, uint32_t *bad, hipStream_t s);
 
@@ -162,6 +163,30 @@ struct GroupBySpec {
     uint64_t prod = 0;
 };
 
+// Compact probe tables of one join, built on the device when the join is
+// applied.  Their statistics (set slots, whether every value fits 16 bits)
+// decide the value layout, the LDS dictionary and the predicate order; the
+// host reads them when a fused program first needs them, one stream sync
+// for every join of the plan so far (Engine::resolve_compact).
+struct CompactState {
+    // built
+    const uint32_t *bits = nullptr;
+    const uint32_t *stats = nullptr; // device: set slots, value overflow
+    bool pairs = false;
+    const void *ht = nullptr;       // reference-layout table
+    const uint16_t *narrow = nullptr; // 16-bit values, when the value range fits
+    uint8_t *dict = nullptr;        // rank dictionary buffer (capacity dict_cap bytes), or null
+    uint32_t dict_cap = 0;
+    uint64_t words = 0, build_rows = 0;
+    // resolved
+    bool resolved = false;
+    float sel = 1.0f;
+    const void *vals = nullptr;
+    int vkind = 0;
+    const void *rank = nullptr;
+    uint32_t rank_bytes = 0;
+};
+
 // One functor of the reference (the constructor arguments of the classes in
 // kernels/*.hpp), with device pointers already offset to its segment.
 struct KernelData {
@@ -188,6 +213,9 @@ struct KernelData {
     uint32_t rank_bytes = 0;
     int32_t vmin = 0;
     float sel = 1.0f;    // estimated fraction of rows a predicate keeps
+    // probes: the compact tables' statistics, read on first use (they then
+    // supply vals, vkind, rank and sel)
+    std::shared_ptr<CompactState> compact;
 };
 
 struct Engine;
@@ -276,10 +304,19 @@ struct Engine {
     int unroll = 0;                 // chunks per thread per iteration (0: default)
     AllocRegistry registry;
     std::unique_ptr<MemoryManager> mm;
+    // compact tables whose statistics the host has not read yet
+    std::vector<std::weak_ptr<CompactState>> pending_compact;
+    // Reads the statistics of every pending compact table (one stream sync).
+    void resolve_compact();
     std::map<std::string, std::unique_ptr<Table>> tables;
     int64_t launches = 0;
     double last_device_ms = 0, last_wall_ms = 0;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    // pinned host staging of result downloads (grown on demand, freed with
+    // the engine): one asynchronous copy per array and one sync
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
+    void *pinned_buffer(size_t bytes);
 
     // Optional per-launch timing with HIP events on the engine stream.
     struct KernelStat {

@@ -1,5 +1,6 @@
 // Plan analysis and the plan walk: operations/preprocessing.cpp:14-275 and
 // executor/ddor_executor.cpp:22-245.
+#include <cstring>
 #include <deque>
 #include <iostream>
 #include <set>
@@ -221,31 +222,68 @@ FinalTable run_ddor_plan(Engine &eng, const PlanResult &plan)
     return t;
 }
 
+void *Engine::pinned_buffer(size_t bytes)
+{
+    if (bytes > pinned_bytes) {
+        if (pinned) SDB_HIP(hipHostFree(pinned));
+        pinned = nullptr;
+        pinned_bytes = 0;
+        const size_t want = std::max<size_t>(bytes, 1 << 20);
+        SDB_HIP(hipHostMalloc(&pinned, want, hipHostMallocDefault));
+        pinned_bytes = want;
+    }
+    return pinned;
+}
+
 ResultTable copy_result(Engine &eng, const FinalTable &t)
 {
     // host copy of the result (the materialize_host_view + save_result path,
-    // executor/result_writer.cpp:28-67)
+    // executor/result_writer.cpp:28-67): every array into one pinned staging
+    // buffer, one sync, then into the result
     ResultTable res;
     res.nrows = t.nrows;
     res.flags.resize(res.nrows);
+    struct Part {
+        void *dst;
+        size_t off, bytes;
+    };
+    std::vector<Part> parts;
+    size_t total = 0;
+    auto stage = [&](const void *src, void *dst, size_t bytes) {
+        total = (total + 15) / 16 * 16;
+        parts.push_back({dst, total, bytes});
+        total += bytes;
+        (void)src;
+    };
     if (res.nrows) {
         eng.registry.check(t.flags, res.nrows, "result flags");
-        SDB_HIP(hipMemcpyAsync(res.flags.data(), t.flags, res.nrows, hipMemcpyDeviceToHost, eng.stream));
+        stage(t.flags, res.flags.data(), res.nrows);
     }
+    std::vector<const void *> srcs;
+    if (res.nrows) srcs.push_back(t.flags);
     for (const FinalTable::Col &c : t.cols) {
         ResultTable::Col rc;
         rc.is_u64 = c.is_u64;
         const uint64_t elem = rc.is_u64 ? 8 : 4;
         if (rc.is_u64) rc.u64.assign(res.nrows, 0);
         else rc.i32.assign(res.nrows, 0);
+        res.cols.push_back(std::move(rc));
         if (c.rows) {
             eng.registry.check(c.data, c.rows * elem, "result column");
-            void *dst = rc.is_u64 ? (void *)rc.u64.data() : (void *)rc.i32.data();
-            SDB_HIP(hipMemcpyAsync(dst, c.data, c.rows * elem, hipMemcpyDeviceToHost, eng.stream));
+            ResultTable::Col &r = res.cols.back();
+            stage(c.data, r.is_u64 ? (void *)r.u64.data() : (void *)r.i32.data(), c.rows * elem);
+            srcs.push_back(c.data);
         }
-        res.cols.push_back(std::move(rc));
     }
-    SDB_HIP(hipStreamSynchronize(eng.stream));
+    if (!parts.empty()) {
+        uint8_t *host = static_cast<uint8_t *>(eng.pinned_buffer(total));
+        for (size_t i = 0; i < parts.size(); i++)
+            SDB_HIP(hipMemcpyAsync(host + parts[i].off, srcs[i], parts[i].bytes, hipMemcpyDeviceToHost, eng.stream));
+        SDB_HIP(hipStreamSynchronize(eng.stream));
+        for (const Part &pt : parts) std::memcpy(pt.dst, host + pt.off, pt.bytes);
+    } else {
+        SDB_HIP(hipStreamSynchronize(eng.stream));
+    }
     return res;
 }
 

@@ -25,6 +25,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <unordered_map>
 #include <mutex>
 #include <sstream>
 #include <vector>
@@ -1192,26 +1193,106 @@ int jit_kernels_compiled()
     return g_compiled;
 }
 
+namespace {
+
+// The fields of a program that decide its generated code: instruction kinds
+// and code-selecting parameters, register numbers, which pointers are set,
+// the group-by layout.  Everything else (pointers, literals, key ranges, LDS
+// offsets) the kernel reads from the program at run time, so programs with
+// equal keys share one kernel.  Code generation and kernel lookup by source
+// text cost tens of microseconds per launch; the key is a few hundred bytes.
+std::string shape_key(const FProgram &p)
+{
+    std::string k;
+    k.reserve(32 + 14 * p.ninstr);
+    auto put = [&](int v) { k.push_back((char)v); };
+    put(p.ninstr);
+    put(p.vec);
+    put(p.nvals);
+    put(p.npf);
+    for (int j = 0; j < F_MAX_FLAGS; j++) {
+        put(p.flag_init[j]);
+        put(p.flag_load[j] != nullptr);
+        put(p.flag_store[j] != nullptr);
+    }
+    for (int i = 0; i < p.ninstr; i++) {
+        const FInstr &I = p.ins[i];
+        put(I.kind);
+        put(I.cmp);
+        put(I.logic);
+        put(I.op);
+        put(I.fin);
+        put(I.fout);
+        put(I.floc);
+        put(I.in_lds);
+        put(I.va);
+        put(I.vb);
+        put(I.vout);
+        put(I.vkind);
+        put(I.store != nullptr);
+    }
+    put(p.gb.k);
+    put(p.gb.lds);
+    for (int j = 0; j < F_MAX_KEYS; j++) put(p.gb.key_reg[j]);
+    put(p.gb.agg_reg);
+    return k;
+}
+
+std::mutex g_src_mu;
+std::unordered_map<std::string, std::string> g_src;           // shape key + launch shape -> source
+std::unordered_map<std::string, Compiled *> g_kernel_by_shape; // device + shape key + launch shape
+
+std::string launch_shape(int unroll, int block) { return std::string(1, (char)unroll) + (char)(block >> 8); }
+
+} // namespace
+
 std::string jit_source(const FProgram &p, int unroll, int block)
 {
     if (unroll != 1 && unroll != 2 && unroll != 4) fail(SDB_ERR_INVALID_ARG, "jit: unroll must be 1, 2 or 4");
     if (block != 256 && block != 512 && block != 1024) fail(SDB_ERR_INVALID_ARG, "jit: block must be 256, 512 or 1024");
+    const std::string key = shape_key(p) + launch_shape(unroll, block);
+    {
+        std::lock_guard<std::mutex> lk(g_src_mu);
+        auto it = g_src.find(key);
+        if (it != g_src.end()) return it->second;
+    }
     Gen g(p);
     std::ostringstream s;
     s << KERNEL_TEMPLATE_INCLUDES << "constexpr int NPF = " << p.npf << ";\nconstexpr int U = " << unroll
       << ";\nconstexpr bool VEC = " << (p.vec ? "true" : "false") << ";\nconstexpr int BLOCK = " << block << ";\n";
     s << KERNEL_TEMPLATE_HEAD << g.body() << KERNEL_TEMPLATE_TAIL;
-    return s.str();
+    std::lock_guard<std::mutex> lk(g_src_mu);
+    return g_src.emplace(key, s.str()).first->second;
 }
+
+namespace {
+
+// The compiled kernel of a program's shape for a launch shape.
+Compiled &kernel_for(const FProgram &p, int unroll, int block)
+{
+    int dev = 0;
+    SDB_HIP(hipGetDevice(&dev));
+    const std::string key = std::to_string(dev) + ":" + shape_key(p) + launch_shape(unroll, block);
+    {
+        std::lock_guard<std::mutex> lk(g_src_mu);
+        auto it = g_kernel_by_shape.find(key);
+        if (it != g_kernel_by_shape.end()) return *it->second;
+    }
+    Compiled &k = get_kernel(jit_source(p, unroll, block));
+    std::lock_guard<std::mutex> lk(g_src_mu);
+    g_kernel_by_shape[key] = &k;
+    return k;
+}
+
+} // namespace
 
 namespace {
 
 // Launches the compiled kernel of `src` (blocks of `block` threads) with
 // `per_cu_mult` times the blocks per CU the occupancy calculator allows.
-void launch_compiled(const FProgram &p, int64_t n, hipStream_t s, int unroll, int per_cu_mult, const std::string &src,
-                     int block = JIT_BLOCK)
+void launch_compiled(const FProgram &p, int64_t n, hipStream_t s, int unroll, int per_cu_mult, int block = JIT_BLOCK)
 {
-    Compiled &k = get_kernel(src);
+    Compiled &k = kernel_for(p, unroll, block);
     // the value-register file (the last LDS area) belongs to the interpreter
     const uint32_t lds = p.vreg_off;
     int per_cu;
@@ -1275,16 +1356,16 @@ void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll)
     if (n <= 0) return;
     if (p.magic != F_MAGIC) fail(SDB_ERR_RUNTIME, "fused program without magic");
     if (unroll != 0) {
-        launch_compiled(p, n, s, unroll, 1, jit_source(p, unroll));
+        launch_compiled(p, n, s, unroll, 1);
         return;
     }
     if (n < TUNE_MIN_ROWS) {
-        launch_compiled(p, n, s, F_DEFAULT_UNROLL, 1, jit_source(p, F_DEFAULT_UNROLL));
+        launch_compiled(p, n, s, F_DEFAULT_UNROLL, 1);
         return;
     }
     int dev = 0;
     SDB_HIP(hipGetDevice(&dev));
-    const std::pair<int, std::string> key{dev, jit_source(p, 1)};
+    const std::pair<int, std::string> key{dev, shape_key(p)};
     int cand;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -1293,19 +1374,18 @@ void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll)
     }
     if (cand < 0) {
         const TuneCfg &c = TUNE_CANDIDATES[-1 - cand];
-        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, jit_source(p, c.unroll, c.block), c.block);
+        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, c.block);
         return;
     }
     const TuneCfg &c = TUNE_CANDIDATES[cand];
-    const std::string src = jit_source(p, c.unroll, c.block);
-    get_kernel(src); // compile outside the timed region
+    kernel_for(p, c.unroll, c.block); // compile outside the timed region
     hipEvent_t a, b;
     SDB_HIP(hipEventCreate(&a));
     SDB_HIP(hipEventCreate(&b));
     float ms = 0;
     hipError_t e = hipEventRecord(a, s);
     if (e == hipSuccess) {
-        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, src, c.block);
+        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, c.block);
         e = hipEventRecord(b, s);
     }
     if (e == hipSuccess) e = hipEventSynchronize(b);
@@ -1531,6 +1611,12 @@ struct BatchLibrary {
     std::vector<std::string> bodies;
     std::vector<const int32_t *> slots;
     std::vector<int> mask_slots;
+    std::unordered_map<std::string, std::string> body_of; // shape key + decided probes -> body
+    // the kernel of the library's current state (rebuilt when it grows)
+    size_t src_state[3] = {~size_t(0), 0, 0}; // bodies, slots, masked slots the source covers
+    std::string src;
+    uint64_t version = 0;             // bumped whenever src changes
+    std::map<int, Compiled *> kernel; // per device, for src
 };
 
 std::mutex g_lib_mu;
@@ -1681,6 +1767,8 @@ struct BatchPlan {
     std::vector<int> mask_slots;      // the library's masked slots (kernel order)
     std::vector<MaskPlan> masks;      // this batch's tables, one per masked slot it uses
     std::string src;
+    BatchLibrary *lib = nullptr; // the library the source is of (libraries are never erased)
+    uint64_t version = 0;        // its version
     uint32_t lds = 0, sums_off = 0, full = 0;
     bool vec = true;
 };
@@ -1775,7 +1863,13 @@ BatchPlan plan_batch(const std::vector<FProgram> &in, int unroll, const std::str
             }
             if (!used) q[i].blob_bytes[bl] = 0;
         }
-        const std::string body = g.body();
+        // a body depends on the program's shape and on which probes the
+        // masks decided
+        std::string body_key = shape_key(q[i]);
+        for (int j = 0; j < q[i].ninstr; j++) body_key.push_back(g.decided[j] ? 1 : 0);
+        auto cached = lib.body_of.find(body_key);
+        if (cached == lib.body_of.end()) cached = lib.body_of.emplace(body_key, g.body()).first;
+        const std::string &body = cached->second;
         auto it = std::find(lib.bodies.begin(), lib.bodies.end(), body);
         if (it == lib.bodies.end()) {
             if ((int)lib.bodies.size() >= BATCH_MAX_SHAPES) fail(SDB_ERR_UNSUPPORTED, "batched launch: too many program shapes");
@@ -1843,7 +1937,18 @@ BatchPlan plan_batch(const std::vector<FProgram> &in, int unroll, const std::str
     }
     b.sums_off = off;
     b.lds = off + 8u * (uint32_t)b.progs.size();
-    b.src = batch_source(lib, unroll, b.vec);
+    if (lib.src_state[0] != lib.bodies.size() || lib.src_state[1] != lib.slots.size() ||
+        lib.src_state[2] != lib.mask_slots.size()) {
+        lib.src = batch_source(lib, unroll, b.vec);
+        lib.src_state[0] = lib.bodies.size();
+        lib.src_state[1] = lib.slots.size();
+        lib.src_state[2] = lib.mask_slots.size();
+        lib.version++;
+        lib.kernel.clear();
+    }
+    b.lib = &lib;
+    b.version = lib.version;
+    b.src = lib.src;
     return b;
 }
 
@@ -1874,7 +1979,18 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     const BatchPlan b =
         plan_batch(in, unroll, device_arch() + "/" + std::to_string(dev) + "/" + scope_key + (masks ? "/m" : ""), masks);
     if (b.lds > F_BATCH_LDS + 8u * F_BATCH_MAX) fail(SDB_ERR_UNSUPPORTED, "batched launch: LDS regions exceed the budget");
-    Compiled &k = get_kernel(b.src, "sdb_fused_batch");
+    Compiled *kp = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_lib_mu);
+        auto it = b.lib->kernel.find(dev);
+        if (it != b.lib->kernel.end() && b.lib->version == b.version) kp = it->second;
+    }
+    if (!kp) {
+        kp = &get_kernel(b.src, "sdb_fused_batch");
+        std::lock_guard<std::mutex> lk(g_lib_mu);
+        if (b.lib->version == b.version) b.lib->kernel[dev] = kp;
+    }
+    Compiled &k = *kp;
     SDB_HIP(hipMemcpyAsync(dev_progs, b.progs.data(), b.progs.size() * sizeof(FProgram), hipMemcpyHostToDevice, s));
 
     // mask tables: descriptors, entries and tables in one allocation, plus

@@ -613,7 +613,7 @@ __global__ void __launch_bounds__(BLOCK) k_bits_from_pairs(const int2 *ht, int32
 // thread writes its words, their prefix counts and the values of their set
 // slots in slot order.
 __global__ void __launch_bounds__(BLOCK) k_rank_dict(const uint32_t *bits, int32_t words, const uint16_t *vals,
-                                                     uint8_t *dict)
+                                                     uint8_t *dict, uint32_t cap_vals)
 {
     __shared__ uint32_t run_total[BLOCK];
     const int32_t per = (words + BLOCK - 1) / BLOCK;
@@ -632,7 +632,10 @@ __global__ void __launch_bounds__(BLOCK) k_rank_dict(const uint32_t *bits, int32
         const uint32_t m = bits[w];
         out_bits[w] = m;
         out_prefix[w] = (uint16_t)base;
-        for (uint32_t r = m; r; r &= r - 1) out_vals[base++] = vals[(int64_t)w * 32 + __builtin_ctz(r)];
+        // values past the buffer's capacity are dropped (the dictionary is
+        // then not used)
+        for (uint32_t r = m; r; r &= r - 1, base++)
+            if (base < cap_vals) out_vals[base] = vals[(int64_t)w * 32 + __builtin_ctz(r)];
     }
 }
 
@@ -1036,10 +1039,13 @@ void launch_bits_from_pairs(const int32_t *ht, int32_t len, uint32_t *bits, uint
     check_launch("bits_from_pairs");
 }
 
-void launch_rank_dict(const uint32_t *bits, int32_t words, const uint16_t *vals, uint8_t *dict, hipStream_t s)
+void launch_rank_dict(const uint32_t *bits, int32_t words, const uint16_t *vals, uint8_t *dict, uint32_t dict_cap,
+                      hipStream_t s)
 {
     if (words <= 0) return;
-    hipLaunchKernelGGL(k_rank_dict, dim3(1), dim3(BLOCK), 0, s, bits, words, vals, dict);
+    if (dict_cap < rank_values_off((uint32_t)words)) fail(SDB_ERR_RUNTIME, "rank dictionary buffer too small");
+    const uint32_t cap_vals = (dict_cap - rank_values_off((uint32_t)words)) / 2u;
+    hipLaunchKernelGGL(k_rank_dict, dim3(1), dim3(BLOCK), 0, s, bits, words, vals, dict, cap_vals);
     check_launch("rank_dict");
 }
 

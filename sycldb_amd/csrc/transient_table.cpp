@@ -531,22 +531,15 @@ std::tuple<int32_t *, int32_t, int32_t> TransientTable::build_key_vals_hash_tabl
 
 // Compact probe tables for fused bundles, derived from the reference-layout
 // table once the build side ran: a presence bitmap (and 16-bit value offsets
-// for a full join when the values fit), plus the fraction of set slots, read
-// back to order predicates.  Results never depend on them: a bit is set
-// exactly when the reference table reports a match.
-struct CompactTable {
-    const uint32_t *bits = nullptr;
-    const void *vals = nullptr;
-    int vkind = FV_NONE;
-    const void *rank = nullptr; // FV_RANK16 dictionary, when small enough for LDS
-    uint32_t rank_bytes = 0;
-    float sel = 1.0f;
-};
-
-static CompactTable build_compact(Engine &eng, const void *ht, bool pairs, int32_t ht_min, int32_t ht_max,
-                                  int32_t vmin, int32_t vmax, uint64_t build_rows)
+// for a full join when the values fit, with their rank dictionary when small
+// enough for LDS), plus the fraction of set slots, which orders predicates.
+// Results never depend on them: a bit is set exactly when the reference
+// table reports a match.  Only the launches happen here; the statistics are
+// read on first use (Engine::resolve_compact).
+static std::shared_ptr<CompactState> build_compact(Engine &eng, const void *ht, bool pairs, int32_t ht_min,
+                                                   int32_t ht_max, int32_t vmin, int32_t vmax, uint64_t build_rows)
 {
-    CompactTable c;
+    auto c = std::make_shared<CompactState>();
     const int64_t len = (int64_t)ht_max - ht_min + 1;
     const uint64_t words = (uint64_t)(len + 31) / 32;
     uint32_t *bits = eng.mm->alloc<uint32_t>((words + 3) / 4 * 4);
@@ -559,32 +552,59 @@ static CompactTable build_compact(Engine &eng, const void *ht, bool pairs, int32
     else
         launch_bits_from_keys(static_cast<const uint8_t *>(ht), (int32_t)len, bits, stats, eng.stream);
     eng.launches++;
-    uint32_t host[2] = {0, 0};
-    SDB_HIP(hipMemcpyAsync(host, stats, sizeof(host), hipMemcpyDeviceToHost, eng.stream));
-    SDB_HIP(hipStreamSynchronize(eng.stream));
-    c.bits = bits;
-    // fraction of the build rows selected: the share of uniformly drawn
-    // foreign keys that find a match
-    c.sel = std::min(1.0f, (float)host[0] / (float)std::max<uint64_t>(build_rows, 1));
-    if (pairs) {
-        if (narrow && host[1] == 0) {
-            c.vals = vals;
-            c.vkind = FV_U16;
-            // a rank dictionary small enough to stage in LDS
-            const uint32_t dict_bytes = (rank_values_off((uint32_t)words) + 2u * host[0] + 15u) / 16u * 16u;
-            if (host[0] <= 0xFFFFu && words <= 0xFFFFu && dict_bytes <= F_RANK_DICT_MAX) {
-                uint8_t *dict = eng.mm->alloc<uint8_t>(dict_bytes);
-                launch_rank_dict(bits, (int32_t)words, vals, dict, eng.stream);
-                eng.launches++;
-                c.rank = dict;
-                c.rank_bytes = dict_bytes;
-            }
-        } else {
-            c.vals = ht;
-            c.vkind = FV_PAIR;
-        }
+    c->bits = bits;
+    c->stats = stats;
+    c->pairs = pairs;
+    c->ht = ht;
+    c->narrow = vals;
+    c->words = words;
+    c->build_rows = build_rows;
+    // the dictionary goes into a buffer of the largest size staged in LDS;
+    // whether it is used depends on the number of set slots
+    if (narrow && words <= 0xFFFFu && rank_values_off((uint32_t)words) < F_RANK_DICT_MAX) {
+        c->dict = eng.mm->alloc<uint8_t>(F_RANK_DICT_MAX);
+        c->dict_cap = F_RANK_DICT_MAX;
+        launch_rank_dict(bits, (int32_t)words, vals, c->dict, c->dict_cap, eng.stream);
+        eng.launches++;
     }
+    eng.pending_compact.push_back(c);
     return c;
+}
+
+void Engine::resolve_compact()
+{
+    std::vector<std::shared_ptr<CompactState>> todo;
+    for (auto &w : pending_compact)
+        if (auto c = w.lock())
+            if (!c->resolved) todo.push_back(c);
+    pending_compact.clear();
+    if (todo.empty()) return;
+    std::vector<uint32_t> host(2 * todo.size(), 0);
+    for (size_t i = 0; i < todo.size(); i++)
+        SDB_HIP(hipMemcpyAsync(&host[2 * i], todo[i]->stats, 8, hipMemcpyDeviceToHost, stream));
+    SDB_HIP(hipStreamSynchronize(stream));
+    for (size_t i = 0; i < todo.size(); i++) {
+        CompactState &c = *todo[i];
+        const uint32_t set = host[2 * i], overflow = host[2 * i + 1];
+        // fraction of the build rows selected: the share of uniformly drawn
+        // foreign keys that find a match
+        c.sel = std::min(1.0f, (float)set / (float)std::max<uint64_t>(c.build_rows, 1));
+        if (c.pairs) {
+            if (c.narrow && overflow == 0) {
+                c.vals = c.narrow;
+                c.vkind = FV_U16;
+                const uint32_t dict_bytes = (rank_values_off((uint32_t)c.words) + 2u * set + 15u) / 16u * 16u;
+                if (c.dict && set <= 0xFFFFu && dict_bytes <= c.dict_cap) {
+                    c.rank = c.dict;
+                    c.rank_bytes = dict_bytes;
+                }
+            } else {
+                c.vals = c.ht;
+                c.vkind = FV_PAIR;
+            }
+        }
+        c.resolved = true;
+    }
 }
 
 void TransientTable::apply_join(TransientTable &right, const RelNode &rel)
@@ -601,7 +621,7 @@ void TransientTable::apply_join(TransientTable &right, const RelNode &rel)
         auto ht = right.build_keys_hash_table((int)right_column);
         ensure_secondary_flags();
         right.execute_pending_kernels(false);
-        CompactTable ct;
+        std::shared_ptr<CompactState> ct;
         if (eng_.fusion)
             ct = build_compact(eng_, std::get<0>(ht), false, std::get<1>(ht), std::get<2>(ht), 0, 0, right.nrows_);
         std::vector<KernelBundle> ops(segment_num());
@@ -616,8 +636,8 @@ void TransientTable::apply_join(TransientTable &right, const RelNode &rel)
             k.ht = std::get<0>(ht);
             k.ht_min = std::get<1>(ht);
             k.ht_max = std::get<2>(ht);
-            k.bits = ct.bits;
-            k.sel = ct.sel;
+            if (ct) k.bits = ct->bits;
+            k.compact = ct;
             ops[s].add_kernel(k);
         }
         add_pending_kernels(std::move(ops));
@@ -637,7 +657,7 @@ void TransientTable::apply_join(TransientTable &right, const RelNode &rel)
     auto ht = right.build_key_vals_hash_table((int)right_column);
     ensure_secondary_flags();
     right.execute_pending_kernels(false);
-    CompactTable ct;
+    std::shared_ptr<CompactState> ct;
     if (eng_.fusion)
         ct = build_compact(eng_, std::get<0>(ht), true, std::get<1>(ht), std::get<2>(ht), gb_mm.first, gb_mm.second,
                            right.nrows_);
@@ -654,13 +674,9 @@ void TransientTable::apply_join(TransientTable &right, const RelNode &rel)
         k.ht = std::get<0>(ht);
         k.ht_min = std::get<1>(ht);
         k.ht_max = std::get<2>(ht);
-        k.bits = ct.bits;
-        k.vals = ct.vals;
-        k.vkind = ct.vkind;
-        k.rank = ct.rank;
-        k.rank_bytes = ct.rank_bytes;
+        if (ct) k.bits = ct->bits;
+        k.compact = ct;
         k.vmin = gb_mm.first;
-        k.sel = ct.sel;
         nc.seg_min[s] = gb_mm.first; // models/models.hpp:1609-1610
         nc.seg_max[s] = gb_mm.second;
         ops[s].add_kernel(k);
