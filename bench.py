@@ -56,7 +56,7 @@ def batched_measure(eng, n_pipes, seed, reps):
     import torch
     from sycldb_amd import plans as PL
     pipes = PL.random_pipelines(n_pipes, seed)
-    plans = [PL.build(t, **p) for t, p in pipes]
+    plans = [PL.to_json(PL.build(t, **p)) for t, p in pipes]  # JSON text, as the C ABI takes plans
     alone = [eng.execute(p).rows() for p in plans]  # the reference results
     # warm: the autotuner settles every plan shape (and compiles its
     # candidates) before anything is timed
@@ -233,6 +233,9 @@ def main():
     rows, builder, fact_cols = WORKLOADS[args.workload]
     rows = args.fact_rows or rows
     plan = getattr(PL, builder)()
+    # the plan as the C ABI takes it (JSON text), serialised once, as a
+    # client that runs a query repeatedly holds it
+    plan_text = PL.to_json(plan)
     eng = Engine(codegen=args.codegen, unroll=args.unroll)
     t0 = time.time()
     # all fact columns: the batched and served pipelines read up to nine
@@ -241,7 +244,7 @@ def main():
     log("generated %d fact rows on the device in %.1fs" % (rows, time.time() - t0))
 
     for _ in range(args.warmup):
-        res = eng.execute(plan)
+        res = eng.execute(plan_text)
     groups = len(res.rows())
     tuning = codegen_tuning_report()
     if tuning:
@@ -252,7 +255,7 @@ def main():
     t0 = time.perf_counter()
     dev_ms = []
     for _ in range(args.steps):
-        eng.execute(plan)
+        eng.execute(plan_text)
         dev_ms.append(eng.last_timing()[0])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -277,11 +280,11 @@ def main():
     unfused = None
     if not args.no_unfused:
         eng.set_fusion(False)
-        eng.execute(plan)
+        eng.execute(plan_text)
         t0 = time.perf_counter()
         n_unf = max(2, args.steps // 3)
         for _ in range(n_unf):
-            r2 = eng.execute(plan)
+            r2 = eng.execute(plan_text)
         torch.cuda.synchronize()
         unf_ms = (time.perf_counter() - t0) * 1000.0 / n_unf
         assert r2.rows() == res.rows(), "unfused and fused results differ"

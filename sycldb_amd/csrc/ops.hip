@@ -566,19 +566,26 @@ __global__ void __launch_bounds__(BLOCK) k_seen_to_flags(const uint32_t *seen, u
 // One thread per 32-slot bitmap word; deterministic (no atomics on the
 // tables), one atomic per word for the population count.
 
+// Presence bitmaps of the reference tables: one thread per slot, each wave
+// covering 64 consecutive slots (coalesced reads), its ballot giving two
+// bitmap words; the set slots are counted per wave.
 __global__ void __launch_bounds__(BLOCK) k_bits_from_keys(const uint8_t *ht, int32_t len, uint32_t *bits,
                                                           uint32_t *count)
 {
     const int64_t words = ((int64_t)len + 31) / 32;
-    for (int64_t w = (int64_t)blockIdx.x * BLOCK + threadIdx.x; w < words; w += (int64_t)gridDim.x * BLOCK) {
-        uint32_t m = 0;
-        for (int b = 0; b < 32; b++) {
-            const int64_t s = w * 32 + b;
-            if (s < len && ht[s] != 0) m |= 1u << b;
+    const int lane = threadIdx.x & 63;
+    const int64_t nwaves = ((int64_t)gridDim.x * BLOCK) >> 6;
+    uint32_t local = 0;
+    for (int64_t g = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) >> 6; g * 64 < (int64_t)len; g += nwaves) {
+        const int64_t s = g * 64 + lane;
+        const uint64_t m = __ballot(s < len && ht[s] != 0);
+        if (lane == 0) {
+            bits[2 * g] = (uint32_t)m;
+            if (2 * g + 1 < words) bits[2 * g + 1] = (uint32_t)(m >> 32);
+            local += (uint32_t)__popcll(m);
         }
-        bits[w] = m;
-        if (m) atomicAdd(count, (uint32_t)__popc(m));
     }
+    if (lane == 0 && local) atomicAdd(count, local);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_bits_from_pairs(const int2 *ht, int32_t len, uint32_t *bits,
@@ -586,45 +593,55 @@ __global__ void __launch_bounds__(BLOCK) k_bits_from_pairs(const int2 *ht, int32
                                                            uint32_t *bad)
 {
     const int64_t words = ((int64_t)len + 31) / 32;
+    const int lane = threadIdx.x & 63;
+    const int64_t nwaves = ((int64_t)gridDim.x * BLOCK) >> 6;
+    uint32_t local = 0;
     bool ok = true;
-    for (int64_t w = (int64_t)blockIdx.x * BLOCK + threadIdx.x; w < words; w += (int64_t)gridDim.x * BLOCK) {
-        uint32_t m = 0;
-        for (int b = 0; b < 32; b++) {
-            const int64_t s = w * 32 + b;
-            if (s >= len) break;
-            const int2 e = ht[s];
-            uint32_t d = 0;
-            if (e.x == 1) {
-                m |= 1u << b;
-                d = (uint32_t)e.y - (uint32_t)vmin;
-                if (d > 0xFFFFu) ok = false;
-            }
-            if (vals) vals[s] = (uint16_t)d;
+    for (int64_t g = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) >> 6; g * 64 < (int64_t)len; g += nwaves) {
+        const int64_t s = g * 64 + lane;
+        const int2 e = s < len ? ht[s] : make_int2(0, 0);
+        const bool set = e.x == 1;
+        const uint32_t d = set ? (uint32_t)e.y - (uint32_t)vmin : 0u;
+        ok &= d <= 0xFFFFu;
+        if (vals && s < len) vals[s] = (uint16_t)d;
+        const uint64_t m = __ballot(set);
+        if (lane == 0) {
+            bits[2 * g] = (uint32_t)m;
+            if (2 * g + 1 < words) bits[2 * g + 1] = (uint32_t)(m >> 32);
+            local += (uint32_t)__popcll(m);
         }
-        bits[w] = m;
-        if (m) atomicAdd(count, (uint32_t)__popc(m));
     }
+    if (lane == 0 && local) atomicAdd(count, local);
     if (!ok) atomicOr(bad, 1u);
 }
 
 // Rank dictionary of a presence bitmap with per-slot 16-bit values (the
-// layout FV_RANK16 describes in fused_types.hpp): one block; each thread owns
-// a run of consecutive words, the block scans the run totals, then every
-// thread writes its words, their prefix counts and the values of their set
-// slots in slot order.
-__global__ void __launch_bounds__(BLOCK) k_rank_dict(const uint32_t *bits, int32_t words, const uint16_t *vals,
-                                                     uint8_t *dict, uint32_t cap_vals)
+// layout FV_RANK16 describes in fused_types.hpp): one block of RANK_BLOCK
+// threads; each thread owns a run of consecutive words, the block scans the
+// run totals (within each wave by shuffles, then across the waves), then
+// every thread writes its words, their prefix counts and the values of their
+// set slots in slot order.
+constexpr int RANK_BLOCK = 1024;
+
+__global__ void __launch_bounds__(RANK_BLOCK) k_rank_dict(const uint32_t *bits, int32_t words, const uint16_t *vals,
+                                                          uint8_t *dict, uint32_t cap_vals)
 {
-    __shared__ uint32_t run_total[BLOCK];
-    const int32_t per = (words + BLOCK - 1) / BLOCK;
+    __shared__ uint32_t wave_total[RANK_BLOCK / 64];
+    const int32_t per = (words + RANK_BLOCK - 1) / RANK_BLOCK;
     const int32_t w0 = min(words, (int32_t)threadIdx.x * per), w1 = min(words, w0 + per);
     uint32_t local = 0;
     for (int32_t w = w0; w < w1; w++) local += __popc(bits[w]);
-    run_total[threadIdx.x] = local;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = local;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) wave_total[wave] = incl;
     __syncthreads();
-    // exclusive scan of BLOCK run totals (serial per thread over a small array)
-    uint32_t base = 0;
-    for (int t = 0; t < (int)threadIdx.x; t++) base += run_total[t];
+    uint32_t base = incl - local;
+    for (int v = 0; v < wave; v++) base += wave_total[v];
     uint32_t *out_bits = reinterpret_cast<uint32_t *>(dict);
     uint16_t *out_prefix = reinterpret_cast<uint16_t *>(dict + rank_prefix_off((uint32_t)words));
     uint16_t *out_vals = reinterpret_cast<uint16_t *>(dict + rank_values_off((uint32_t)words));
@@ -1024,7 +1041,8 @@ void launch_bits_from_keys(const uint8_t *ht, int32_t len, uint32_t *bits, uint3
 {
     if (len <= 0) return;
     const int64_t words = ((int64_t)len + 31) / 32;
-    hipLaunchKernelGGL(k_bits_from_keys, dim3(grid_one(words)), dim3(BLOCK), 0, s, ht, len, bits, count);
+    (void)words;
+    hipLaunchKernelGGL(k_bits_from_keys, dim3(grid_one(len)), dim3(BLOCK), 0, s, ht, len, bits, count);
     check_launch("bits_from_keys");
 }
 
@@ -1034,7 +1052,8 @@ void launch_bits_from_pairs(const int32_t *ht, int32_t len, uint32_t *bits, uint
     if (len <= 0) return;
     if (!aligned(ht, 8)) fail(SDB_ERR_INVALID_ARG, "key-value hash table must be 8-byte aligned");
     const int64_t words = ((int64_t)len + 31) / 32;
-    hipLaunchKernelGGL(k_bits_from_pairs, dim3(grid_one(words)), dim3(BLOCK), 0, s,
+    (void)words;
+    hipLaunchKernelGGL(k_bits_from_pairs, dim3(grid_one(len)), dim3(BLOCK), 0, s,
                        reinterpret_cast<const int2 *>(ht), len, bits, vals, vmin, count, bad);
     check_launch("bits_from_pairs");
 }
@@ -1045,7 +1064,7 @@ void launch_rank_dict(const uint32_t *bits, int32_t words, const uint16_t *vals,
     if (words <= 0) return;
     if (dict_cap < rank_values_off((uint32_t)words)) fail(SDB_ERR_RUNTIME, "rank dictionary buffer too small");
     const uint32_t cap_vals = (dict_cap - rank_values_off((uint32_t)words)) / 2u;
-    hipLaunchKernelGGL(k_rank_dict, dim3(1), dim3(BLOCK), 0, s, bits, words, vals, dict, cap_vals);
+    hipLaunchKernelGGL(k_rank_dict, dim3(1), dim3(RANK_BLOCK), 0, s, bits, words, vals, dict, cap_vals);
     check_launch("rank_dict");
 }
 

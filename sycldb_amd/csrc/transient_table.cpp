@@ -579,10 +579,12 @@ void Engine::resolve_compact()
             if (!c->resolved) todo.push_back(c);
     pending_compact.clear();
     if (todo.empty()) return;
-    std::vector<uint32_t> host(2 * todo.size(), 0);
+    // through the pinned staging buffer: asynchronous copies, one sync
+    uint32_t *staged = static_cast<uint32_t *>(pinned_buffer(8 * todo.size()));
     for (size_t i = 0; i < todo.size(); i++)
-        SDB_HIP(hipMemcpyAsync(&host[2 * i], todo[i]->stats, 8, hipMemcpyDeviceToHost, stream));
+        SDB_HIP(hipMemcpyAsync(&staged[2 * i], todo[i]->stats, 8, hipMemcpyDeviceToHost, stream));
     SDB_HIP(hipStreamSynchronize(stream));
+    const std::vector<uint32_t> host(staged, staged + 2 * todo.size());
     for (size_t i = 0; i < todo.size(); i++) {
         CompactState &c = *todo[i];
         const uint32_t set = host[2 * i], overflow = host[2 * i + 1];
