@@ -1333,7 +1333,8 @@ constexpr int TUNE_ROUNDS = 1;
 struct TuneCfg {
     int unroll, per_cu_mult, block;
 };
-const TuneCfg TUNE_CANDIDATES[] = {{2, 1, 256}, {1, 1, 256}, {4, 1, 256}, {2, 1, 512}, {1, 1, 512}, {2, 1, 1024}};
+const TuneCfg TUNE_CANDIDATES[] = {{2, 1, 256}, {1, 1, 256}, {4, 1, 256}, {2, 1, 512}, {1, 1, 512}, {2, 1, 1024},
+                                   {4, 1, 512}, {4, 1, 1024}, {1, 1, 1024}};
 constexpr int N_CAND = (int)(sizeof(TUNE_CANDIDATES) / sizeof(TUNE_CANDIDATES[0]));
 
 struct Tune {
