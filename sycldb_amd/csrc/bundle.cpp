@@ -341,6 +341,28 @@ void fold_search(std::vector<Draft> &d)
     }
 }
 
+// LDS bytes one fused program may give to staged probe tables
+// (SDB_STAGE_MAX_BYTES; default 48 KiB): staging trades occupancy (LDS per
+// block) for probes that never leave the CU.
+uint32_t stage_max_bytes()
+{
+    static const uint32_t v = [] {
+        const char *e = getenv("SDB_STAGE_MAX_BYTES");
+        return e && *e ? (uint32_t)atol(e) : 48u * 1024u;
+    }();
+    return v;
+}
+
+// Largest single table staged (SDB_STAGE_TABLE_MAX_BYTES; default 32 KiB).
+uint32_t stage_table_max_bytes()
+{
+    static const uint32_t v = [] {
+        const char *e = getenv("SDB_STAGE_TABLE_MAX_BYTES");
+        return e && *e ? (uint32_t)atol(e) : 32u * 1024u;
+    }();
+    return v;
+}
+
 // Tables of FK_PROBE_BITS staged into LDS at kernel start, smallest first,
 // each at most 32 KiB, together at most 48 KiB: a probe's rank dictionary when
 // it has one (its values then come from LDS too), else its presence bitmap;
@@ -365,14 +387,14 @@ std::vector<Staging> choose_staged(const std::vector<Draft> &d, uint32_t total_c
         if (I.kind != FK_PROBE_BITS) continue;
         if (D.rank && I.vkind == FV_U16) add(D.rank, D.rank_bytes, true);
         const uint64_t len = (uint64_t)((int64_t)I.ht_max - I.ht_min + 1);
-        if (len <= 32u * 1024u * 8u) add(I.ht, bitmap_stage_bytes(I), false);
+        if (len <= (uint64_t)stage_table_max_bytes() * 8u) add(I.ht, bitmap_stage_bytes(I), false);
     }
     std::stable_sort(cand.begin(), cand.end(), [](const Cand &a, const Cand &b) { return a.bytes < b.bytes; });
     std::vector<const void *> chosen;
     uint32_t staged = 0;
     int nblobs = 0;
     for (const Cand &c : cand) {
-        if (c.bytes > 32u * 1024u || nblobs >= F_MAX_BLOBS || staged + c.bytes > total_cap) continue;
+        if (c.bytes > stage_table_max_bytes() || nblobs >= F_MAX_BLOBS || staged + c.bytes > total_cap) continue;
         chosen.push_back(c.src);
         staged += c.bytes;
         nblobs++;
@@ -386,18 +408,6 @@ std::vector<Staging> choose_staged(const std::vector<Draft> &d, uint32_t total_c
         else if (has(D.I.ht)) out[i] = {D.I.ht, bitmap_stage_bytes(D.I), false};
     }
     return out;
-}
-
-// LDS bytes one fused program may give to staged probe tables
-// (SDB_STAGE_MAX_BYTES; default 48 KiB): staging trades occupancy (LDS per
-// block) for probes that never leave the CU.
-uint32_t stage_max_bytes()
-{
-    static const uint32_t v = [] {
-        const char *e = getenv("SDB_STAGE_MAX_BYTES");
-        return e && *e ? (uint32_t)atol(e) : 48u * 1024u;
-    }();
-    return v;
 }
 
 // SDB_GROUPBY_HASH=0 sends group-by rows whose dense slot range is too large
@@ -736,8 +746,11 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
     // LDS left for staged probe tables.  A program captured for a batched
     // launch stages none: the batch decides its probes on fact key columns
     // through shared masks, and its LDS share goes to the group-by partials.
+    // (the interpreter's value-register file, 4 KiB per register, shares
+    // the budget)
+    const uint32_t reserved = group_area + (uint32_t)vptr.size() * 256u * 16u;
     const uint32_t stage_cap =
-        captured ? 0u : std::min<uint32_t>(stage_max_bytes(), lds_budget - std::min(lds_budget, group_area));
+        captured ? 0u : std::min<uint32_t>(stage_max_bytes(), lds_budget - std::min(lds_budget, reserved));
     // probe costs follow where the tables will live
     {
         const std::vector<Staging> st = choose_staged(d, stage_cap);
