@@ -14,6 +14,7 @@ every launch of that kernel), the unfused per-operator path for comparison,
 and the CPU oracle timed on a bounded sample of the same table on this host.
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -45,6 +46,19 @@ DESCRIPTIONS = {
 }
 
 
+class no_gc:
+    """Timed regions run with Python's cyclic garbage collector paused (after
+    a collection): a collection pass over the interpreter's objects takes
+    milliseconds and would land at random in a timed step."""
+
+    def __enter__(self):
+        gc.collect()
+        gc.disable()
+
+    def __exit__(self, *exc):
+        gc.enable()
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -58,26 +72,33 @@ def batched_measure(eng, n_pipes, seed, reps):
     pipes = PL.random_pipelines(n_pipes, seed)
     plans = [PL.to_json(PL.build(t, **p)) for t, p in pipes]  # JSON text, as the C ABI takes plans
     alone = [eng.execute(p).rows() for p in plans]  # the reference results
-    # warm: the autotuner settles every plan shape (and compiles its
+    # warm: the autotuner settles every plan shape (and compiles its nine
     # candidates) before anything is timed
+    for _ in range(11):
+        for p in plans:
+            eng.execute(p)
+    torch.cuda.synchronize()
+    with no_gc():
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for p in plans:
+                eng.execute(p)
+        torch.cuda.synchronize()
+        seq_s = (time.perf_counter() - t0) / reps
+    # warm: the batched kernel's autotuner tries its launch shapes (and
+    # compiles them) on the first batches
     for _ in range(6):
-        for p in plans:
-            eng.execute(p)
+        results, nb = eng.execute_batch(plans)
+        assert [r.rows() for r in results] == alone, "batched results differ from running each plan alone"
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        for p in plans:
-            eng.execute(p)
-    torch.cuda.synchronize()
-    seq_s = (time.perf_counter() - t0) / reps
-    results, nb = eng.execute_batch(plans)  # warm
-    assert [r.rows() for r in results] == alone, "batched results differ from running each plan alone"
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        eng.execute_batch(plans)
-    torch.cuda.synchronize()
-    bat_s = (time.perf_counter() - t0) / reps
+    with no_gc():
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            eng.execute_batch(plans)
+        torch.cuda.synchronize()
+        bat_s = (time.perf_counter() - t0) / reps
+    from sycldb_amd.engine import codegen_tuning_report
+    log("batched kernel tuning:\n" + "\n".join(l for l in codegen_tuning_report().splitlines() if "batched" in l))
     return {"pipelines": n_pipes, "pipeline_seed": seed, "sharing_the_scan": nb,
             "sequential_ms": seq_s * 1e3, "batched_ms": bat_s * 1e3,
             "sequential_pipelines_per_s": n_pipes / seq_s, "batched_pipelines_per_s": n_pipes / bat_s,
@@ -118,7 +139,7 @@ def served_measure(eng_big, seed, n_requests, rates, max_batch):
                     engines[e].execute_batch(sub[k:k + max_batch])
 
         def serve(serial, arrivals):
-            with Server(engines, max_batch=max_batch, serial=serial) as srv:
+            with Server(engines, max_batch=max_batch, serial=serial) as srv, no_gc():
                 t0 = time.perf_counter()
                 out = replay(srv, reqs, arrivals)
                 wall = time.perf_counter() - t0
@@ -252,13 +273,14 @@ def main():
     eng.set_profiling(True)
     eng.profile_reset()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
     dev_ms = []
-    for _ in range(args.steps):
-        eng.execute(plan_text)
-        dev_ms.append(eng.last_timing()[0])
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    with no_gc():
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            eng.execute(plan_text)
+            dev_ms.append(eng.last_timing()[0])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
     prof = eng.profile()
     eng.set_profiling(False)
     ms_per_step = wall * 1000.0 / args.steps

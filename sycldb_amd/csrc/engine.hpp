@@ -317,6 +317,12 @@ struct Engine {
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
     void *pinned_buffer(size_t bytes);
+    // Waits until the stream's work so far is done, polling an event: the
+    // engine waits only between device phases that follow each other
+    // closely, where a sleeping wait adds the thread's wake-up latency
+    // (measured up to ~0.5 ms per query step).
+    hipEvent_t sync_ev = nullptr;
+    void wait_stream();
 
     // Optional per-launch timing with HIP events on the engine stream.
     struct KernelStat {

@@ -136,6 +136,7 @@ void sdb_engine_destroy(sdb_engine *e)
     }
     for (hipEvent_t ev : eng.event_pool) (void)hipEventDestroy(ev);
     if (eng.pinned) (void)hipHostFree(eng.pinned);
+    if (eng.sync_ev) (void)hipEventDestroy(eng.sync_ev);
     if (eng.ev_start) (void)hipEventDestroy(eng.ev_start);
     if (eng.ev_stop) (void)hipEventDestroy(eng.ev_stop);
     if (eng.stream) (void)hipStreamDestroy(eng.stream);

@@ -222,6 +222,16 @@ FinalTable run_ddor_plan(Engine &eng, const PlanResult &plan)
     return t;
 }
 
+void Engine::wait_stream()
+{
+    if (!sync_ev) SDB_HIP(hipEventCreateWithFlags(&sync_ev, hipEventDisableTiming));
+    SDB_HIP(hipEventRecord(sync_ev, stream));
+    hipError_t e;
+    while ((e = hipEventQuery(sync_ev)) == hipErrorNotReady) {
+    }
+    SDB_HIP(e);
+}
+
 void *Engine::pinned_buffer(size_t bytes)
 {
     if (bytes > pinned_bytes) {
@@ -279,10 +289,10 @@ ResultTable copy_result(Engine &eng, const FinalTable &t)
         uint8_t *host = static_cast<uint8_t *>(eng.pinned_buffer(total));
         for (size_t i = 0; i < parts.size(); i++)
             SDB_HIP(hipMemcpyAsync(host + parts[i].off, srcs[i], parts[i].bytes, hipMemcpyDeviceToHost, eng.stream));
-        SDB_HIP(hipStreamSynchronize(eng.stream));
+        eng.wait_stream();
         for (const Part &pt : parts) std::memcpy(pt.dst, host + pt.off, pt.bytes);
     } else {
-        SDB_HIP(hipStreamSynchronize(eng.stream));
+        eng.wait_stream();
     }
     return res;
 }

@@ -1613,12 +1613,39 @@ struct BatchLibrary {
     std::vector<const int32_t *> slots;
     std::vector<int> mask_slots;
     std::unordered_map<std::string, std::string> body_of; // shape key + decided probes -> body
-    // the kernel of the library's current state (rebuilt when it grows)
-    size_t src_state[3] = {~size_t(0), 0, 0}; // bodies, slots, masked slots the source covers
-    std::string src;
-    uint64_t version = 0;             // bumped whenever src changes
-    std::map<int, Compiled *> kernel; // per device, for src
+    // the kernel of the library's current state per launch shape (rebuilt
+    // when the library grows)
+    struct Src {
+        size_t state[3] = {~size_t(0), 0, 0}; // bodies, slots, masked slots the source covers
+        std::string text;
+        uint64_t version = 0;             // bumped whenever text changes
+        std::map<int, Compiled *> kernel; // per device, for text
+    };
+    std::map<int, Src> srcs; // by launch shape (BatchShape::id)
 };
+
+// Batched-kernel autotuner state per library: the best time per row and
+// program measured for each candidate launch shape.
+struct BatchTune {
+    double best[8];
+    int trials = 0, chosen = -1;
+    BatchTune()
+    {
+        for (double &b : best) b = 1e300;
+    }
+};
+std::map<std::string, BatchTune> g_batch_tune; // guarded by g_tune_mu
+
+// Launch shape of the batched kernel: threads per block, and whether the
+// next tile's columns load while the current one runs.
+struct BatchShape {
+    int block;
+    bool double_buffer;
+    int id() const { return block * 2 + (double_buffer ? 1 : 0); }
+};
+// autotuner candidates; the first is the default below the tuning size
+const BatchShape BATCH_CANDIDATES[] = {{512, false}, {256, false}, {1024, false}, {1024, true}};
+constexpr int N_BATCH_CAND = (int)(sizeof(BATCH_CANDIDATES) / sizeof(BATCH_CANDIDATES[0]));
 
 std::mutex g_lib_mu;
 std::map<std::string, BatchLibrary> g_libs;
@@ -1768,39 +1795,30 @@ struct BatchPlan {
     std::vector<int> mask_slots;      // the library's masked slots (kernel order)
     std::vector<MaskPlan> masks;      // this batch's tables, one per masked slot it uses
     std::string src;
-    BatchLibrary *lib = nullptr; // the library the source is of (libraries are never erased)
-    uint64_t version = 0;        // its version
+    BatchLibrary::Src *src_entry = nullptr; // the library source it is (never erased)
+    uint64_t version = 0;                   // its version
     uint32_t lds = 0, sums_off = 0, full = 0;
     bool vec = true;
 };
 
-// Launch shape of the batched kernel: threads per block and whether the
-// next tile's columns load while the current one runs (SDB_BATCH_BLOCK,
-// SDB_BATCH_DOUBLE_BUFFER override).
-int batch_block()
+// A launch shape fixed by SDB_BATCH_BLOCK / SDB_BATCH_DOUBLE_BUFFER (no
+// tuning), if either is set.
+bool batch_shape_fixed(BatchShape &out)
 {
-    static const int v = [] {
-        const char *e = getenv("SDB_BATCH_BLOCK");
-        const int b = e && *e ? atoi(e) : BATCH_BLOCK;
-        return b == 256 || b == 512 || b == 1024 ? b : BATCH_BLOCK;
-    }();
-    return v;
+    const char *b = getenv("SDB_BATCH_BLOCK");
+    const char *d = getenv("SDB_BATCH_DOUBLE_BUFFER");
+    if (!(b && *b) && !(d && *d)) return false;
+    const int blk = b && *b ? atoi(b) : BATCH_BLOCK;
+    out.block = blk == 256 || blk == 512 || blk == 1024 ? blk : BATCH_BLOCK;
+    out.double_buffer = d && *d == '1';
+    return true;
 }
 
-bool batch_double_buffer()
-{
-    static const bool v = [] {
-        const char *e = getenv("SDB_BATCH_DOUBLE_BUFFER");
-        return e && *e == '1';
-    }();
-    return v;
-}
-
-std::string batch_source(const BatchLibrary &lib, int unroll, bool vec)
+std::string batch_source(const BatchLibrary &lib, int unroll, bool vec, const BatchShape &shape)
 {
     std::ostringstream s;
-    s << KERNEL_TEMPLATE_INCLUDES << "constexpr int BLOCK = " << batch_block()
-      << ";\nconstexpr bool DOUBLE_BUFFER = " << (batch_double_buffer() ? "true" : "false")
+    s << KERNEL_TEMPLATE_INCLUDES << "constexpr int BLOCK = " << shape.block
+      << ";\nconstexpr bool DOUBLE_BUFFER = " << (shape.double_buffer ? "true" : "false")
       << ";\nconstexpr int NPF = " << std::max<size_t>(lib.slots.size(), 1)
       << ";\nconstexpr int U = " << unroll << ";\nconstexpr bool VEC = " << (vec ? "true" : "false")
       << ";\nconstexpr int NSHAPE = " << lib.bodies.size() << ";\nconstexpr int NMASK = " << lib.mask_slots.size()
@@ -1832,7 +1850,8 @@ std::string batch_source(const BatchLibrary &lib, int unroll, bool vec)
     return s.str();
 }
 
-BatchPlan plan_batch(const std::vector<FProgram> &in, int unroll, const std::string &lib_key, bool masks)
+BatchPlan plan_batch(const std::vector<FProgram> &in, int unroll, const std::string &lib_key, bool masks,
+                     const BatchShape &shape)
 {
     BatchPlan b;
     if (in.size() > 32) fail(SDB_ERR_UNSUPPORTED, "batched launch: more than 32 programs");
@@ -1938,18 +1957,19 @@ BatchPlan plan_batch(const std::vector<FProgram> &in, int unroll, const std::str
     }
     b.sums_off = off;
     b.lds = off + 8u * (uint32_t)b.progs.size();
-    if (lib.src_state[0] != lib.bodies.size() || lib.src_state[1] != lib.slots.size() ||
-        lib.src_state[2] != lib.mask_slots.size()) {
-        lib.src = batch_source(lib, unroll, b.vec);
-        lib.src_state[0] = lib.bodies.size();
-        lib.src_state[1] = lib.slots.size();
-        lib.src_state[2] = lib.mask_slots.size();
-        lib.version++;
-        lib.kernel.clear();
+    BatchLibrary::Src &src = lib.srcs[shape.id()];
+    if (src.state[0] != lib.bodies.size() || src.state[1] != lib.slots.size() ||
+        src.state[2] != lib.mask_slots.size()) {
+        src.text = batch_source(lib, unroll, b.vec, shape);
+        src.state[0] = lib.bodies.size();
+        src.state[1] = lib.slots.size();
+        src.state[2] = lib.mask_slots.size();
+        src.version++;
+        src.kernel.clear();
     }
-    b.lib = &lib;
-    b.version = lib.version;
-    b.src = lib.src;
+    b.src_entry = &src;
+    b.version = src.version;
+    b.src = src.text;
     return b;
 }
 
@@ -1962,7 +1982,7 @@ int jit_batch_check(const std::string &arch)
     // two copies of each shape, as pipelines with different constants
     std::vector<FProgram> two = progs;
     two.insert(two.end(), progs.begin(), progs.end());
-    compile_source(plan_batch(two, 2, "check:" + arch, true).src, arch);
+    compile_source(plan_batch(two, 2, "check:" + arch, true, BATCH_CANDIDATES[0]).src, arch);
     return 1;
 }
 
@@ -1977,19 +1997,32 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     SDB_HIP(hipGetDevice(&dev));
     char scope_key[32];
     snprintf(scope_key, sizeof(scope_key), "%p", scope);
-    const BatchPlan b =
-        plan_batch(in, unroll, device_arch() + "/" + std::to_string(dev) + "/" + scope_key + (masks ? "/m" : ""), masks);
+    const std::string lib_key = device_arch() + "/" + std::to_string(dev) + "/" + scope_key + (masks ? "/m" : "");
+    // launch shape: fixed by the environment, else the autotuner's choice
+    // for this library (trials on launches of at least TUNE_MIN_ROWS rows)
+    BatchShape shape = BATCH_CANDIDATES[0];
+    int trial = -1;
+    if (!batch_shape_fixed(shape) && n >= TUNE_MIN_ROWS) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        BatchTune &t = g_batch_tune[lib_key];
+        if (t.chosen >= 0) shape = BATCH_CANDIDATES[t.chosen];
+        else {
+            trial = t.trials % N_BATCH_CAND;
+            shape = BATCH_CANDIDATES[trial];
+        }
+    }
+    const BatchPlan b = plan_batch(in, unroll, lib_key, masks, shape);
     if (b.lds > F_BATCH_LDS + 8u * F_BATCH_MAX) fail(SDB_ERR_UNSUPPORTED, "batched launch: LDS regions exceed the budget");
     Compiled *kp = nullptr;
     {
         std::lock_guard<std::mutex> lk(g_lib_mu);
-        auto it = b.lib->kernel.find(dev);
-        if (it != b.lib->kernel.end() && b.lib->version == b.version) kp = it->second;
+        auto it = b.src_entry->kernel.find(dev);
+        if (it != b.src_entry->kernel.end() && b.src_entry->version == b.version) kp = it->second;
     }
     if (!kp) {
         kp = &get_kernel(b.src, "sdb_fused_batch");
         std::lock_guard<std::mutex> lk(g_lib_mu);
-        if (b.lib->version == b.version) b.lib->kernel[dev] = kp;
+        if (b.src_entry->version == b.version) b.src_entry->kernel[dev] = kp;
     }
     Compiled &k = *kp;
     SDB_HIP(hipMemcpyAsync(dev_progs, b.progs.data(), b.progs.size() * sizeof(FProgram), hipMemcpyHostToDevice, s));
@@ -2083,7 +2116,7 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
         auto it = k.blocks_per_cu.find(b.lds);
         if (it == k.blocks_per_cu.end()) {
             int bl = 0;
-            SDB_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&bl, k.fn, batch_block(), b.lds));
+            SDB_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&bl, k.fn, shape.block, b.lds));
             it = k.blocks_per_cu.emplace(b.lds, std::max(1, bl)).first;
         }
         per_cu = it->second;
@@ -2091,20 +2124,64 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     const int64_t units = b.vec ? (n + 3) / 4 : n;
     // at least BATCH_MIN_TILES tiles per block: each block also stages and
     // flushes every program's LDS state, which small tables must amortise
-    const int64_t tiles = (units + (int64_t)batch_block() * (b.vec ? unroll : 1) - 1) / ((int64_t)batch_block() * (b.vec ? unroll : 1));
+    const int64_t tiles = (units + (int64_t)shape.block * (b.vec ? unroll : 1) - 1) / ((int64_t)shape.block * (b.vec ? unroll : 1));
     const unsigned grid = (unsigned)std::max<int64_t>(
-        1, std::min<int64_t>(stream_grid(units, (int64_t)batch_block() * (b.vec ? unroll : 1), per_cu),
+        1, std::min<int64_t>(stream_grid(units, (int64_t)shape.block * (b.vec ? unroll : 1), per_cu),
                              (tiles + BATCH_MIN_TILES - 1) / BATCH_MIN_TILES));
     size_t size = args.size();
     void *config[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, args.data(), HIP_LAUNCH_PARAM_BUFFER_SIZE, &size,
                       HIP_LAUNCH_PARAM_END};
-    SDB_HIP(hipModuleLaunchKernel(k.fn, grid, 1, 1, batch_block(), 1, 1, b.lds, s, nullptr, config));
+    if (trial < 0) {
+        SDB_HIP(hipModuleLaunchKernel(k.fn, grid, 1, 1, shape.block, 1, 1, b.lds, s, nullptr, config));
+        return;
+    }
+    // a tuning trial: the kernel alone, timed with HIP events; compared per
+    // row and program, since batches of one library differ in size
+    hipEvent_t ea, eb;
+    SDB_HIP(hipEventCreate(&ea));
+    SDB_HIP(hipEventCreate(&eb));
+    float ms = 0;
+    hipError_t e = hipEventRecord(ea, s);
+    if (e == hipSuccess) e = hipModuleLaunchKernel(k.fn, grid, 1, 1, shape.block, 1, 1, b.lds, s, nullptr, config);
+    if (e == hipSuccess) e = hipEventRecord(eb, s);
+    if (e == hipSuccess) e = hipEventSynchronize(eb);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ea, eb);
+    (void)hipEventDestroy(ea);
+    (void)hipEventDestroy(eb);
+    hip_check(e, "batched kernel tuning");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    BatchTune &t = g_batch_tune[lib_key];
+    if (t.chosen >= 0) return;
+    const double cost = ms * 1e6 / ((double)n * (double)b.progs.size());
+    t.best[trial] = std::min(t.best[trial], cost);
+    if (++t.trials >= N_BATCH_CAND) {
+        int best = 0;
+        for (int c = 1; c < N_BATCH_CAND; c++)
+            if (t.best[c] < t.best[best]) best = c;
+        t.chosen = best;
+    }
 }
 
 std::string jit_tuning_report()
 {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     std::ostringstream o;
+    for (auto &kv : g_batch_tune) {
+        const BatchTune &t = kv.second;
+        o << "batched kernel " << kv.first << ": ";
+        if (t.chosen >= 0)
+            o << "block " << BATCH_CANDIDATES[t.chosen].block
+              << (BATCH_CANDIDATES[t.chosen].double_buffer ? ", double-buffered" : ", one tile in registers");
+        else
+            o << "tuning (" << t.trials << " trials)";
+        o << "; ns per row and program";
+        for (int c = 0; c < N_BATCH_CAND; c++) {
+            o << " [b" << BATCH_CANDIDATES[c].block << (BATCH_CANDIDATES[c].double_buffer ? "d" : "") << "] ";
+            if (t.best[c] < 1e299) o << t.best[c];
+            else o << "-";
+        }
+        o << "\n";
+    }
     int shape = 0;
     for (auto &kv : g_tune) {
         const Tune &t = kv.second;

@@ -583,7 +583,7 @@ void Engine::resolve_compact()
     uint32_t *staged = static_cast<uint32_t *>(pinned_buffer(8 * todo.size()));
     for (size_t i = 0; i < todo.size(); i++)
         SDB_HIP(hipMemcpyAsync(&staged[2 * i], todo[i]->stats, 8, hipMemcpyDeviceToHost, stream));
-    SDB_HIP(hipStreamSynchronize(stream));
+    wait_stream();
     const std::vector<uint32_t> host(staged, staged + 2 * todo.size());
     for (size_t i = 0; i < todo.size(); i++) {
         CompactState &c = *todo[i];
