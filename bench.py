@@ -237,7 +237,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the fact table would be "
-              "sharded by row ranges with replicated dimension tables; see DESIGN.md)", file=sys.stderr)
+              "sharded by row ranges with replicated dimension tables, and the int64 partial results "
+              "reduced across GPUs)", file=sys.stderr)
         sys.exit(2)
 
     import torch
