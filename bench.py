@@ -236,7 +236,7 @@ def main():
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the fact table would be "
+        print("bench.py measures one GPU: the operator path has no multi-GPU form (the fact table would be "
               "sharded by row ranges with replicated dimension tables, and the int64 partial results "
               "reduced across GPUs)", file=sys.stderr)
         sys.exit(2)
