@@ -352,7 +352,12 @@ int sdb_engine_profile_count(sdb_engine *eng);
 int sdb_engine_profile_get(sdb_engine *eng, int idx, char *name, int name_cap, int64_t *launches,
                            int64_t *rows, double *ms);
 
-/* Result table: nrows rows, ncols non-null columns, row selection flags. */
+/* Result table: nrows rows, ncols non-null columns, row selection flags
+ * (the final TransientTable's host view, TransientTable::materialize_host_view
+ * as executor/result_writer.cpp:28-40 calls it).  Only selected rows carry
+ * meaning: the key columns of a group-by table are unspecified at
+ * unselected slots (the reference never writes them); large group-by tables
+ * are compacted on the device and read 0 there. */
 uint64_t sdb_result_nrows(const sdb_result *r);
 int sdb_result_ncols(const sdb_result *r);
 /* 1 when column c holds uint64 sums, 0 when it holds int32 values. */
@@ -363,6 +368,11 @@ int sdb_result_col_copy(const sdb_result *r, int c, void *host);
 int sdb_result_flags_copy(const sdb_result *r, uint8_t *host);
 /* Number of selected rows. */
 uint64_t sdb_result_selected(const sdb_result *r);
+/* The selected rows only, as result_writer.cpp:41-67 visits them: their
+ * row numbers in ascending order (sdb_result_selected values) and column c's
+ * values at those rows (sdb_result_selected values, int32 or uint64). */
+int sdb_result_selected_rows(const sdb_result *r, uint64_t *host);
+int sdb_result_col_copy_selected(const sdb_result *r, int c, void *host);
 /* Writes the .res text of executor/result_writer.cpp:41-67 (selected rows,
  * space-separated, int32 keys and uint64 sums) to `path`. */
 int sdb_result_write(const sdb_result *r, const char *path);

@@ -150,6 +150,8 @@ SIGNATURES = {
     "sdb_result_col_copy": (INT, [P, INT, P]),
     "sdb_result_flags_copy": (INT, [P, P]),
     "sdb_result_selected": (U64, [P]),
+    "sdb_result_selected_rows": (INT, [P, P]),
+    "sdb_result_col_copy_selected": (INT, [P, INT, P]),
     "sdb_result_write": (INT, [P, CSTR]),
     "sdb_result_destroy": (None, [P]),
     "sdb_plan_dag_order": (INT, [CSTR, P, INT, ctypes.POINTER(INT)]),

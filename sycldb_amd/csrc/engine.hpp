@@ -418,15 +418,28 @@ std::vector<int> dag_topological_sort(const PlanResult &plan);   // preprocessin
 ExecutionInfo parse_execution_info(const PlanResult &plan);     // preprocessing.cpp:104-275
 
 // Host copy of a final table.
+// Host copy of a final table.  Dense form: flags and every column hold nrows
+// entries.  Sparse form (large group-by tables, compacted on the device):
+// `sel` holds the selected rows in ascending order and every column only its
+// values at those rows; flags are empty and unselected rows read as 0 (the
+// sums of unselected group-by slots are 0, their keys are never written).
 struct ResultTable {
     uint64_t nrows = 0;
+    bool sparse = false;
     std::vector<uint8_t> flags;
+    std::vector<uint32_t> sel;
     struct Col {
         bool is_u64;
         std::vector<int32_t> i32;
         std::vector<uint64_t> u64;
     };
     std::vector<Col> cols;
+
+    uint64_t selected() const;
+    void flags_to(uint8_t *host) const;                 // nrows flags
+    void col_to(int c, void *host) const;               // nrows values
+    void selected_rows_to(uint64_t *host) const;        // selected() row ids
+    void col_selected_to(int c, void *host) const;      // selected() values
 };
 
 // The final table of a plan, on the device (valid until the allocator resets).
@@ -446,6 +459,7 @@ struct FinalTable {
 // the final table (result_writer.cpp's materialize_host_view).
 FinalTable run_ddor_plan(Engine &eng, const PlanResult &plan);
 ResultTable copy_result(Engine &eng, const FinalTable &t);
+std::vector<ResultTable> copy_results(Engine &eng, const std::vector<FinalTable> &ts);
 ResultTable execute_ddor_plan(Engine &eng, const PlanResult &plan);
 // Several plans with a shared fact scan (sdb_engine_execute_batch); the
 // results are in plan order.  *batched = plans that shared the scan.

@@ -430,8 +430,9 @@ std::vector<ResultTable> execute_batch(Engine &eng, const std::vector<PlanResult
     for (int i = 0; i < n; i++)
         if (!pipes[i].batched) pipes[i].fin = run_ddor_plan(eng, parsed[i]);
     const double t_plain = ms_since(t0);
-    std::vector<ResultTable> out;
-    for (int i = 0; i < n; i++) out.push_back(copy_result(eng, pipes[i].fin));
+    std::vector<FinalTable> fins;
+    for (int i = 0; i < n; i++) fins.push_back(pipes[i].fin);
+    std::vector<ResultTable> out = copy_results(eng, fins);
     if (batched) *batched = (int)members.size();
     if (debug_batch())
         fprintf(stderr, "[batch] %d plans, %zu shared: host ms capture %.2f, shared pass %.2f, plain %.2f, results %.2f\n",
@@ -600,9 +601,7 @@ int sdb_result_col_copy(const sdb_result *r, int c, void *host)
     return guarded([&] {
         require(r && host, "null argument");
         require(c >= 0 && c < (int)r->t.cols.size(), "column out of range");
-        const auto &col = r->t.cols[c];
-        if (col.is_u64) std::memcpy(host, col.u64.data(), col.u64.size() * 8);
-        else std::memcpy(host, col.i32.data(), col.i32.size() * 4);
+        r->t.col_to(c, host);
     });
 }
 
@@ -610,16 +609,27 @@ int sdb_result_flags_copy(const sdb_result *r, uint8_t *host)
 {
     return guarded([&] {
         require(r && host, "null argument");
-        std::memcpy(host, r->t.flags.data(), r->t.flags.size());
+        r->t.flags_to(host);
     });
 }
 
-uint64_t sdb_result_selected(const sdb_result *r)
+uint64_t sdb_result_selected(const sdb_result *r) { return r ? r->t.selected() : 0; }
+
+int sdb_result_selected_rows(const sdb_result *r, uint64_t *host)
 {
-    if (!r) return 0;
-    uint64_t n = 0;
-    for (uint8_t f : r->t.flags) n += f != 0;
-    return n;
+    return guarded([&] {
+        require(r && host, "null argument");
+        r->t.selected_rows_to(host);
+    });
+}
+
+int sdb_result_col_copy_selected(const sdb_result *r, int c, void *host)
+{
+    return guarded([&] {
+        require(r && host, "null argument");
+        require(c >= 0 && c < (int)r->t.cols.size(), "column out of range");
+        r->t.col_selected_to(c, host);
+    });
 }
 
 int sdb_result_write(const sdb_result *r, const char *path)
@@ -629,13 +639,17 @@ int sdb_result_write(const sdb_result *r, const char *path)
         std::ofstream out(path);
         if (!out) fail(SDB_ERR_INVALID_ARG, std::string("cannot open ") + path);
         const ResultTable &t = r->t;
-        for (uint64_t row = 0; row < t.nrows; row++) {
-            if (!t.flags[row]) continue;
+        const uint64_t k = t.selected();
+        std::vector<uint64_t> rows(k);
+        t.selected_rows_to(rows.data());
+        for (uint64_t j = 0; j < k; j++) {
+            // dense: the value at the row; sparse: the j-th stored value
+            const uint64_t at = t.sparse ? j : rows[j];
             bool first = true;
             for (const auto &c : t.cols) {
                 if (!first) out << ' ';
-                if (c.is_u64) out << c.u64[row];
-                else out << c.i32[row];
+                if (c.is_u64) out << c.u64[at];
+                else out << c.i32[at];
                 first = false;
             }
             out << '\n';

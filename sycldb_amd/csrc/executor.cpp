@@ -245,56 +245,212 @@ void *Engine::pinned_buffer(size_t bytes)
     return pinned;
 }
 
-ResultTable copy_result(Engine &eng, const FinalTable &t)
+namespace {
+
+// Group-by results of at least this many slots come back compacted on the
+// device: only their selected rows cross PCIe and reach host memory.
+constexpr uint64_t RESULT_SPARSE_MIN_ROWS = 1u << 15;
+
+bool sparse_result(const FinalTable &t)
 {
-    // host copy of the result (the materialize_host_view + save_result path,
-    // executor/result_writer.cpp:28-67): every array into one pinned staging
-    // buffer, one sync, then into the result
-    ResultTable res;
-    res.nrows = t.nrows;
-    res.flags.resize(res.nrows);
+    if (t.nrows < RESULT_SPARSE_MIN_ROWS || t.nrows >= (1ull << 31) || !t.flags ||
+        t.cols.size() > (size_t)GATHER_MAX_COLS)
+        return false;
+    for (const FinalTable::Col &c : t.cols)
+        if (c.is_u64) return true;
+    return false;
+}
+
+// Device-to-host copies through the engine's pinned staging buffer: every
+// copy queued, one wait for the stream, then into host memory.
+struct HostCopies {
     struct Part {
+        const void *src;
         void *dst;
         size_t off, bytes;
     };
     std::vector<Part> parts;
     size_t total = 0;
-    auto stage = [&](const void *src, void *dst, size_t bytes) {
+    void add(const void *src, void *dst, size_t bytes)
+    {
+        if (!bytes) return;
         total = (total + 15) / 16 * 16;
-        parts.push_back({dst, total, bytes});
+        parts.push_back({src, dst, total, bytes});
         total += bytes;
-        (void)src;
-    };
-    if (res.nrows) {
-        eng.registry.check(t.flags, res.nrows, "result flags");
-        stage(t.flags, res.flags.data(), res.nrows);
     }
-    std::vector<const void *> srcs;
-    if (res.nrows) srcs.push_back(t.flags);
-    for (const FinalTable::Col &c : t.cols) {
-        ResultTable::Col rc;
-        rc.is_u64 = c.is_u64;
-        const uint64_t elem = rc.is_u64 ? 8 : 4;
-        if (rc.is_u64) rc.u64.assign(res.nrows, 0);
-        else rc.i32.assign(res.nrows, 0);
-        res.cols.push_back(std::move(rc));
-        if (c.rows) {
-            eng.registry.check(c.data, c.rows * elem, "result column");
-            ResultTable::Col &r = res.cols.back();
-            stage(c.data, r.is_u64 ? (void *)r.u64.data() : (void *)r.i32.data(), c.rows * elem);
-            srcs.push_back(c.data);
+    void run(Engine &eng)
+    {
+        if (parts.empty()) {
+            eng.wait_stream();
+            return;
+        }
+        uint8_t *host = static_cast<uint8_t *>(eng.pinned_buffer(total));
+        for (const Part &p : parts)
+            SDB_HIP(hipMemcpyAsync(host + p.off, p.src, p.bytes, hipMemcpyDeviceToHost, eng.stream));
+        eng.wait_stream();
+        for (const Part &p : parts) std::memcpy(p.dst, host + p.off, p.bytes);
+        parts.clear();
+        total = 0;
+    }
+};
+
+} // namespace
+
+std::vector<ResultTable> copy_results(Engine &eng, const std::vector<FinalTable> &ts)
+{
+    // host copies of the final tables (the materialize_host_view +
+    // save_result path, executor/result_writer.cpp:28-67), all tables in
+    // at most two waits for the stream
+    std::vector<ResultTable> out(ts.size());
+    HostCopies copies;
+    std::vector<size_t> sparse;
+    for (size_t i = 0; i < ts.size(); i++)
+        if (sparse_result(ts[i])) sparse.push_back(i);
+    // large group-by tables: the selected row ids, compacted in row order on
+    // the device; their counts come back with the dense tables
+    std::vector<int32_t *> ids(ts.size(), nullptr);
+    std::vector<uint32_t> counts(sparse.size(), 0);
+    if (!sparse.empty()) {
+        uint32_t *d_counts = eng.mm->alloc<uint32_t>(sparse.size());
+        for (size_t j = 0; j < sparse.size(); j++) {
+            const FinalTable &t = ts[sparse[j]];
+            eng.registry.check(t.flags, t.nrows, "result flags");
+            int32_t *row_ids = eng.mm->alloc<int32_t>(t.nrows);
+            uint32_t *tiles = eng.mm->alloc<uint32_t>((uint64_t)std::max<int64_t>(compact_tiles((int64_t)t.nrows), 1));
+            launch_compact_into(t.flags, (int64_t)t.nrows, row_ids, d_counts + j, tiles, eng.stream);
+            eng.launches += 3;
+            ids[sparse[j]] = row_ids;
+        }
+        copies.add(d_counts, counts.data(), counts.size() * 4);
+    }
+    for (size_t i = 0; i < ts.size(); i++) {
+        const FinalTable &t = ts[i];
+        ResultTable &res = out[i];
+        res.nrows = t.nrows;
+        res.sparse = ids[i] != nullptr;
+        for (const FinalTable::Col &c : t.cols) {
+            ResultTable::Col rc;
+            rc.is_u64 = c.is_u64;
+            res.cols.push_back(std::move(rc));
+        }
+        if (res.sparse) continue;
+        res.flags.resize(res.nrows);
+        if (res.nrows) {
+            eng.registry.check(t.flags, res.nrows, "result flags");
+            copies.add(t.flags, res.flags.data(), res.nrows);
+        }
+        for (size_t c = 0; c < t.cols.size(); c++) {
+            const FinalTable::Col &fc = t.cols[c];
+            ResultTable::Col &rc = res.cols[c];
+            const uint64_t elem = rc.is_u64 ? 8 : 4;
+            if (rc.is_u64) rc.u64.assign(res.nrows, 0);
+            else rc.i32.assign(res.nrows, 0);
+            if (fc.rows) {
+                eng.registry.check(fc.data, fc.rows * elem, "result column");
+                copies.add(fc.data, rc.is_u64 ? (void *)rc.u64.data() : (void *)rc.i32.data(), fc.rows * elem);
+            }
         }
     }
-    if (!parts.empty()) {
-        uint8_t *host = static_cast<uint8_t *>(eng.pinned_buffer(total));
-        for (size_t i = 0; i < parts.size(); i++)
-            SDB_HIP(hipMemcpyAsync(host + parts[i].off, srcs[i], parts[i].bytes, hipMemcpyDeviceToHost, eng.stream));
-        eng.wait_stream();
-        for (const Part &pt : parts) std::memcpy(pt.dst, host + pt.off, pt.bytes);
-    } else {
-        eng.wait_stream();
+    copies.run(eng);
+    // large group-by tables: every column gathered at the selected rows on
+    // the device, then copied
+    bool any = false;
+    for (size_t j = 0; j < sparse.size(); j++) {
+        const FinalTable &t = ts[sparse[j]];
+        ResultTable &res = out[sparse[j]];
+        const uint32_t k = counts[j];
+        if (k > t.nrows) fail(SDB_ERR_RUNTIME, "result compaction: selected count out of range");
+        res.sel.resize(k);
+        if (!k) continue;
+        any = true;
+        GatherCols g{};
+        g.ncols = (int32_t)t.cols.size();
+        for (size_t c = 0; c < t.cols.size(); c++) {
+            const FinalTable::Col &fc = t.cols[c];
+            ResultTable::Col &rc = res.cols[c];
+            const uint64_t elem = fc.is_u64 ? 8 : 4;
+            if (fc.rows) eng.registry.check(fc.data, fc.rows * elem, "result column");
+            g.src[c] = fc.data;
+            g.src_rows[c] = fc.rows;
+            if (fc.is_u64) g.is_u64 |= 1u << c;
+            void *d = eng.mm->alloc<uint8_t>((uint64_t)k * elem);
+            g.dst[c] = d;
+            if (fc.is_u64) rc.u64.resize(k);
+            else rc.i32.resize(k);
+            copies.add(d, fc.is_u64 ? (void *)rc.u64.data() : (void *)rc.i32.data(), (size_t)k * elem);
+        }
+        launch_gather_cols(g, ids[sparse[j]], (int64_t)k, eng.stream);
+        eng.launches++;
+        copies.add(ids[sparse[j]], res.sel.data(), (size_t)k * 4);
     }
-    return res;
+    if (any) copies.run(eng);
+    return out;
+}
+
+ResultTable copy_result(Engine &eng, const FinalTable &t) { return std::move(copy_results(eng, {t})[0]); }
+
+uint64_t ResultTable::selected() const
+{
+    if (sparse) return sel.size();
+    uint64_t n = 0;
+    for (uint8_t f : flags) n += f != 0;
+    return n;
+}
+
+void ResultTable::flags_to(uint8_t *host) const
+{
+    if (!sparse) {
+        std::memcpy(host, flags.data(), flags.size());
+        return;
+    }
+    std::memset(host, 0, nrows);
+    for (uint32_t r : sel) host[r] = 1;
+}
+
+void ResultTable::col_to(int c, void *host) const
+{
+    const Col &col = cols[c];
+    if (!sparse) {
+        if (col.is_u64) std::memcpy(host, col.u64.data(), col.u64.size() * 8);
+        else std::memcpy(host, col.i32.data(), col.i32.size() * 4);
+        return;
+    }
+    if (col.is_u64) {
+        uint64_t *h = static_cast<uint64_t *>(host);
+        std::memset(h, 0, nrows * 8);
+        for (size_t j = 0; j < sel.size(); j++) h[sel[j]] = col.u64[j];
+    } else {
+        int32_t *h = static_cast<int32_t *>(host);
+        std::memset(h, 0, nrows * 4);
+        for (size_t j = 0; j < sel.size(); j++) h[sel[j]] = col.i32[j];
+    }
+}
+
+void ResultTable::selected_rows_to(uint64_t *host) const
+{
+    if (sparse) {
+        for (size_t j = 0; j < sel.size(); j++) host[j] = sel[j];
+        return;
+    }
+    uint64_t j = 0;
+    for (uint64_t r = 0; r < flags.size(); r++)
+        if (flags[r]) host[j++] = r;
+}
+
+void ResultTable::col_selected_to(int c, void *host) const
+{
+    const Col &col = cols[c];
+    if (sparse) {
+        if (col.is_u64) std::memcpy(host, col.u64.data(), col.u64.size() * 8);
+        else std::memcpy(host, col.i32.data(), col.i32.size() * 4);
+        return;
+    }
+    uint64_t j = 0;
+    for (uint64_t r = 0; r < flags.size(); r++) {
+        if (!flags[r]) continue;
+        if (col.is_u64) static_cast<uint64_t *>(host)[j++] = col.u64[r];
+        else static_cast<int32_t *>(host)[j++] = col.i32[r];
+    }
 }
 
 ResultTable execute_ddor_plan(Engine &eng, const PlanResult &plan) { return copy_result(eng, run_ddor_plan(eng, plan)); }

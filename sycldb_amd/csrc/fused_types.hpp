@@ -158,12 +158,17 @@ struct FMaskEntry {
     int32_t ht_min, ht_max;
     uint32_t bit, kind;
 };
+// A table stores, per key, only the window of `wbytes` bytes of the word at
+// bit `shift` that covers the programs probing the column (programs take bit
+// positions that keep those windows narrow); the bits outside the window are
+// all ones (programs without a probe on the column).
 struct FMaskCol {
-    uint32_t *tab;
+    uint8_t *tab;          // len entries of wbytes bytes (padded to 4 bytes)
     int32_t lo;
     uint32_t len;
     uint32_t full;         // bits of every program of the launch
     uint32_t first, count; // its entries
+    uint32_t shift, wbytes;
 };
 constexpr int F_MASK_MAX_COLS = 8;
 constexpr int64_t F_MASK_MAX_LEN = int64_t(1) << 24;

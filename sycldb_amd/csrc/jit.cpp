@@ -1451,10 +1451,13 @@ const char *BATCH_HEAD = R"(struct BatchArgs {
     uint32_t full;
     int32_t first[NSHAPE + 1];
     const int32_t *pf[NPF > 0 ? NPF : 1];
-    const uint32_t *mtab[NMASK > 0 ? NMASK : 1];
+    const uint8_t *mtab[NMASK > 0 ? NMASK : 1];
     int32_t mlo[NMASK > 0 ? NMASK : 1];
     uint32_t mlen[NMASK > 0 ? NMASK : 1];
     uint32_t mdflt[NMASK > 0 ? NMASK : 1];
+    uint32_t mshift[NMASK > 0 ? NMASK : 1];
+    uint32_t mwbytes[NMASK > 0 ? NMASK : 1];
+    int32_t bitpos[32]; // program i's bit in the mask words
 };
 template <int R, int UU> struct Pf {
     typename Vec<R>::type c[NPF > 0 ? NPF : 1][UU];
@@ -1471,7 +1474,9 @@ __device__ __forceinline__ void pf_load(const BatchArgs &A, const int64_t (&row0
 // Program bits of every row: the AND of the row's words in the mask tables
 // of the masked columns (a column this batch does not mask has a one-word
 // table of all ones).  Indexes are clamped into the table so every load is
-// unconditional; out-of-table keys then take the column's default word.
+// unconditional (an aligned dword holding the entry); out-of-table keys then
+// take the column's default word.  An entry is the window of the word that
+// covers the column's probing programs, widened back with ones outside it.
 template <int R, int UU>
 __device__ __forceinline__ void mask_rows(const BatchArgs &A, const Pf<R, UU> &pf, uint32_t (&mw)[UU][R])
 {
@@ -1485,7 +1490,8 @@ __device__ __forceinline__ void mask_rows(const BatchArgs &A, const Pf<R, UU> &p
 #pragma unroll
             for (int k = 0; k < R; k++) {
                 const uint32_t idx = (uint32_t)x[k] - (uint32_t)A.mlo[m];
-                w[m][u][k] = A.mtab[m][idx < A.mlen[m] ? idx : A.mlen[m] - 1u];
+                const uint32_t byte = (idx < A.mlen[m] ? idx : A.mlen[m] - 1u) * A.mwbytes[m];
+                w[m][u][k] = *reinterpret_cast<const uint32_t *>(A.mtab[m] + (byte & ~3u)) >> ((byte & 3u) * 8u);
             }
         }
 #pragma unroll
@@ -1498,7 +1504,9 @@ __device__ __forceinline__ void mask_rows(const BatchArgs &A, const Pf<R, UU> &p
                 int32_t x[R];
                 vec_rows<R>(pf.c[MSLOT[m]][u], x);
                 const uint32_t idx = (uint32_t)x[k] - (uint32_t)A.mlo[m];
-                v &= idx < A.mlen[m] ? w[m][u][k] : A.mdflt[m];
+                const uint32_t wm = A.mwbytes[m] >= 4u ? ~0u : (1u << (8u * A.mwbytes[m])) - 1u;
+                const uint32_t word = ((w[m][u][k] & wm) << A.mshift[m]) | ~(wm << A.mshift[m]);
+                v &= idx < A.mlen[m] ? word : A.mdflt[m];
             }
             mw[u][k] = v;
         }
@@ -1786,6 +1794,7 @@ struct MaskPlan {
     int slot;
     int64_t lo = 0, hi = -1;
     std::vector<FMaskEntry> ents;
+    uint32_t shift = 0, wbytes = 4; // the window of the word the table stores
 };
 
 struct BatchPlan {
@@ -1794,6 +1803,7 @@ struct BatchPlan {
     std::vector<const int32_t *> pf;  // slot -> column this batch reads (null: unused)
     std::vector<int> mask_slots;      // the library's masked slots (kernel order)
     std::vector<MaskPlan> masks;      // this batch's tables, one per masked slot it uses
+    std::vector<int> bitpos;          // program (batch order) -> its bit in the mask words
     std::string src;
     BatchLibrary::Src *src_entry = nullptr; // the library source it is (never erased)
     uint64_t version = 0;                   // its version
@@ -1841,8 +1851,9 @@ std::string batch_source(const BatchLibrary &lib, int unroll, bool vec, const Ba
          "    const uint32_t wave = __builtin_amdgcn_readfirstlane(__ockl_wfred_or_u32(any));\n";
     for (size_t k = 0; k < lib.bodies.size(); k++)
         s << "#pragma unroll 1\n    for (int i = A.first[" << k << "]; i < A.first[" << k + 1 << "]; i++) {\n"
-          << "        if (!((wave >> i) & 1u)) continue;\n"
-          << "        uint32_t act[UU];\n        rows_of_program<R, UU>(mw, i, act);\n"
+          << "        const int bit = A.bitpos[i];\n"
+          << "        if (!((wave >> bit) & 1u)) continue;\n"
+          << "        uint32_t act[UU];\n        rows_of_program<R, UU>(mw, bit, act);\n"
           << "        const FProgram &P = ((ProgPtr)A.progs)[i];\n        int64_t acc = 0;\n"
           << "        body" << k << "<R, UU>(P, row0, pf, acc, lds + P.lds_base, act);\n"
           << "        if (acc) atomicAdd(&sums[i], (unsigned long long)acc);\n    }\n";
@@ -1911,6 +1922,41 @@ BatchPlan plan_batch(const std::vector<FProgram> &in, int unroll, const std::str
     b.first.push_back((int)b.progs.size());
     b.full = b.progs.size() >= 32 ? ~0u : (1u << b.progs.size()) - 1u;
     b.mask_slots = lib.mask_slots;
+    // Bit positions: programs ordered by which masked columns they probe
+    // (the set as a Gray code, columns probed by more programs first), so
+    // that each column's probing programs take nearby bits and its table
+    // stores a narrow window of the word.
+    {
+        std::vector<int> mslot_count(lib.mask_slots.size(), 0);
+        std::vector<uint32_t> sig(order.size(), 0);
+        for (size_t pos = 0; pos < order.size(); pos++)
+            for (int j : shared[order[pos]]) {
+                const int slot = q[order[pos]].ins[j].va - F_PF_BASE;
+                const int m = (int)(std::find(lib.mask_slots.begin(), lib.mask_slots.end(), slot) - lib.mask_slots.begin());
+                if (!((sig[pos] >> m) & 1u)) mslot_count[m]++;
+                sig[pos] |= 1u << m;
+            }
+        std::vector<int> rank_of(lib.mask_slots.size());
+        {
+            std::vector<int> by(lib.mask_slots.size());
+            for (size_t m = 0; m < by.size(); m++) by[m] = (int)m;
+            std::stable_sort(by.begin(), by.end(), [&](int a, int c) { return mslot_count[a] > mslot_count[c]; });
+            for (size_t r = 0; r < by.size(); r++) rank_of[by[r]] = (int)r; // most probed column: top bit
+        }
+        const int nm = (int)lib.mask_slots.size();
+        std::vector<std::pair<uint32_t, int>> key(order.size());
+        for (size_t pos = 0; pos < order.size(); pos++) {
+            uint32_t g = 0; // signature with the most probed column as the top bit
+            for (int m = 0; m < nm; m++)
+                if ((sig[pos] >> m) & 1u) g |= 1u << (nm - 1 - rank_of[m]);
+            uint32_t bin = g; // Gray decode
+            for (uint32_t sh = 1; sh < 32; sh <<= 1) bin ^= bin >> sh;
+            key[pos] = {bin, (int)pos};
+        }
+        std::stable_sort(key.begin(), key.end());
+        b.bitpos.assign(order.size(), 0);
+        for (size_t r = 0; r < key.size(); r++) b.bitpos[key[r].second] = (int)r;
+    }
     for (size_t pos = 0; pos < order.size(); pos++)
         for (int j : shared[order[pos]]) {
             const FInstr &I = q[order[pos]].ins[j];
@@ -1929,11 +1975,21 @@ BatchPlan plan_batch(const std::vector<FProgram> &in, int unroll, const std::str
                     it->hi = std::max<int64_t>(it->hi, I.ht_max);
                 }
             }
-            it->ents.push_back(FMaskEntry{I.ht, I.ht_min, I.ht_max, (uint32_t)pos, mask_kind(I)});
+            it->ents.push_back(FMaskEntry{I.ht, I.ht_min, I.ht_max, (uint32_t)b.bitpos[pos], mask_kind(I)});
         }
-    for (const MaskPlan &m : b.masks)
+    for (MaskPlan &m : b.masks) {
         if (m.hi - m.lo + 1 > F_MASK_MAX_LEN)
             fail(SDB_ERR_UNSUPPORTED, "batched launch: the probed key ranges of one column span more than 2^24 keys");
+        // the narrowest window (8, 16 or 32 bits) holding every probing bit
+        uint32_t lo_bit = 31, hi_bit = 0;
+        for (const FMaskEntry &e : m.ents) {
+            lo_bit = std::min(lo_bit, e.bit);
+            hi_bit = std::max(hi_bit, e.bit);
+        }
+        const uint32_t span = hi_bit - lo_bit + 1;
+        m.wbytes = span <= 8 ? 1 : span <= 16 ? 2 : 4;
+        m.shift = m.wbytes == 4 ? 0 : std::min(lo_bit, 32 - 8 * m.wbytes);
+    }
     if ((int)b.masks.size() > F_MASK_MAX_COLS) fail(SDB_ERR_UNSUPPORTED, "batched launch: too many masked columns");
     // columns this batch reads (other slots stay unloaded)
     b.pf.assign(std::max<size_t>(lib.slots.size(), 1), nullptr);
@@ -2030,26 +2086,26 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     // mask tables: descriptors, entries and tables in one allocation, plus
     // one all-ones word for the library's masked slots this batch leaves out
     const int nmask = (int)b.mask_slots.size();
-    std::vector<const uint32_t *> mtab(std::max(nmask, 1), nullptr);
+    std::vector<const uint8_t *> mtab(std::max(nmask, 1), nullptr);
     std::vector<int32_t> mlo(std::max(nmask, 1), 0);
     std::vector<uint32_t> mlen(std::max(nmask, 1), 1), mdflt(std::max(nmask, 1), ~0u);
+    std::vector<uint32_t> mshift(std::max(nmask, 1), 0), mwbytes(std::max(nmask, 1), 4);
     if (nmask > 0) {
-        size_t nents = 0, words = 1;
+        size_t nents = 0, tab_bytes = 4; // the first 4 bytes: the all-ones word
         for (const MaskPlan &m : b.masks) {
             nents += m.ents.size();
-            words += (size_t)std::max<int64_t>(m.hi - m.lo + 1, 1);
+            tab_bytes += ((size_t)std::max<int64_t>(m.hi - m.lo + 1, 1) * m.wbytes + 3) / 4 * 4;
         }
         const size_t desc_bytes = (b.masks.size() * sizeof(FMaskCol) + 255) / 256 * 256;
         const size_t ent_bytes = (nents * sizeof(FMaskEntry) + 255) / 256 * 256;
-        uint8_t *mem = static_cast<uint8_t *>(alloc(alloc_ctx, desc_bytes + ent_bytes + words * 4));
+        uint8_t *mem = static_cast<uint8_t *>(alloc(alloc_ctx, desc_bytes + ent_bytes + tab_bytes));
         FMaskCol *d_cols = reinterpret_cast<FMaskCol *>(mem);
         FMaskEntry *d_ents = reinterpret_cast<FMaskEntry *>(mem + desc_bytes);
-        uint32_t *tabs = reinterpret_cast<uint32_t *>(mem + desc_bytes + ent_bytes);
-        // tabs[0]: the all-ones word
+        uint8_t *tabs = mem + desc_bytes + ent_bytes;
         std::vector<uint8_t> host(desc_bytes + ent_bytes + 4, 0);
         std::vector<FMaskCol> cols;
         std::vector<FMaskEntry> ents;
-        size_t at = 1;
+        size_t at = 4;
         uint32_t max_len = 0;
         for (const MaskPlan &m : b.masks) {
             FMaskCol c{};
@@ -2060,6 +2116,8 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
             c.full = b.full;
             c.first = (uint32_t)ents.size();
             c.count = (uint32_t)m.ents.size();
+            c.shift = m.shift;
+            c.wbytes = m.wbytes;
             uint32_t probing = 0;
             for (const FMaskEntry &e : m.ents) {
                 ents.push_back(e);
@@ -2072,7 +2130,9 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
             mlo[mi] = c.lo;
             mlen[mi] = c.len;
             mdflt[mi] = b.full & ~probing;
-            at += (size_t)len;
+            mshift[mi] = c.shift;
+            mwbytes[mi] = c.wbytes;
+            at += ((size_t)len * c.wbytes + 3) / 4 * 4;
         }
         std::memcpy(host.data(), cols.data(), cols.size() * sizeof(FMaskCol));
         std::memcpy(host.data() + desc_bytes, ents.data(), ents.size() * sizeof(FMaskEntry));
@@ -2080,12 +2140,13 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
         std::memcpy(host.data() + desc_bytes + ent_bytes, &ones, 4);
         SDB_HIP(hipMemcpyAsync(mem, host.data(), host.size(), hipMemcpyHostToDevice, s));
         for (int mi = 0; mi < nmask; mi++)
-            if (!mtab[mi]) mtab[mi] = tabs; // mlen 1, dflt all ones
+            if (!mtab[mi]) mtab[mi] = tabs; // mlen 1, 4-byte entries, dflt all ones
         launch_batch_masks(d_cols, (int)cols.size(), max_len, d_ents, s);
     }
 
     // kernel argument block: progs, n, nprog, sums_off, full, first[NSHAPE+1],
-    // pf[NPF], mtab[NMASK], mlo[NMASK], mlen[NMASK], mdflt[NMASK]
+    // pf[NPF], mtab[NMASK], mlo[NMASK], mlen[NMASK], mdflt[NMASK],
+    // mshift[NMASK], mwbytes[NMASK], bitpos[32]
     const int nshape = (int)b.first.size() - 1;
     std::vector<uint8_t> args;
     auto put = [&](const void *v, size_t bytes, size_t align) {
@@ -2105,10 +2166,16 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
         put(&f, 4, 4);
     }
     for (const int32_t *c : b.pf) put(&c, 8, 8);
-    for (const uint32_t *t : mtab) put(&t, 8, 8);
+    for (const uint8_t *t : mtab) put(&t, 8, 8);
     for (int32_t v : mlo) put(&v, 4, 4);
     for (uint32_t v : mlen) put(&v, 4, 4);
     for (uint32_t v : mdflt) put(&v, 4, 4);
+    for (uint32_t v : mshift) put(&v, 4, 4);
+    for (uint32_t v : mwbytes) put(&v, 4, 4);
+    for (int i = 0; i < 32; i++) {
+        const int32_t bp = i < (int)b.bitpos.size() ? b.bitpos[i] : i;
+        put(&bp, 4, 4);
+    }
     while (args.size() % 8) args.push_back(0);
     int per_cu;
     {
@@ -2132,6 +2199,18 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     void *config[] = {HIP_LAUNCH_PARAM_BUFFER_POINTER, args.data(), HIP_LAUNCH_PARAM_BUFFER_SIZE, &size,
                       HIP_LAUNCH_PARAM_END};
     if (trial < 0) {
+        static const bool debug = [] {
+            const char *e = getenv("SDB_DEBUG_BATCH");
+            return e && *e && *e != '0';
+        }();
+        if (debug) {
+            fprintf(stderr, "[batch] launch: %zu programs, block %d, grid %u (%d blocks/CU), LDS %u bytes; mask windows",
+                    b.progs.size(), shape.block, grid, per_cu, b.lds);
+            for (const MaskPlan &m : b.masks)
+                fprintf(stderr, " [slot %d: %lld keys x %u B at bit %u]", m.slot, (long long)(m.hi - m.lo + 1), m.wbytes,
+                        m.shift);
+            fprintf(stderr, "\n");
+        }
         SDB_HIP(hipModuleLaunchKernel(k.fn, grid, 1, 1, shape.block, 1, 1, b.lds, s, nullptr, config));
         return;
     }

@@ -679,7 +679,10 @@ __global__ void __launch_bounds__(BLOCK) k_batch_masks(const FMaskCol *cols, con
             }
             if (!pass) w &= ~(1u << E.bit);
         }
-        c.tab[i] = w;
+        const uint32_t v = w >> c.shift;
+        if (c.wbytes == 1) c.tab[i] = (uint8_t)v;
+        else if (c.wbytes == 2) reinterpret_cast<uint16_t *>(c.tab)[i] = (uint16_t)v;
+        else reinterpret_cast<uint32_t *>(c.tab)[i] = v;
     }
 }
 
@@ -752,6 +755,20 @@ __global__ void __launch_bounds__(BLOCK) k_compact_write(const uint8_t *flags, i
     uint32_t pos = offsets[blockIdx.x] + block_exclusive_scan(c, &total);
     for (int k = 0; k < COMPACT_PER_THREAD; k++)
         if (t0 + k < n && flags[t0 + k]) row_ids[pos++] = (int32_t)(t0 + k);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_gather_cols(GatherCols g, const int32_t *ids, int64_t count)
+{
+    for (int64_t j = (int64_t)blockIdx.x * BLOCK + threadIdx.x; j < count; j += (int64_t)gridDim.x * BLOCK) {
+        const uint64_t r = (uint64_t)(uint32_t)ids[j];
+        for (int c = 0; c < g.ncols; c++) {
+            if ((g.is_u64 >> c) & 1u)
+                static_cast<uint64_t *>(g.dst[c])[j] =
+                    r < g.src_rows[c] ? static_cast<const uint64_t *>(g.src[c])[r] : 0ull;
+            else
+                static_cast<int32_t *>(g.dst[c])[j] = r < g.src_rows[c] ? static_cast<const int32_t *>(g.src[c])[r] : 0;
+        }
+    }
 }
 
 __global__ void __launch_bounds__(BLOCK) k_gather_i32(const int32_t *src, const int32_t *ids, int64_t count,
@@ -1076,16 +1093,33 @@ void launch_batch_masks(const FMaskCol *cols, int ncols, uint32_t max_len, const
     check_launch("batch_masks");
 }
 
+int64_t compact_tiles(int64_t n) { return n > 0 ? (n + COMPACT_TILE - 1) / COMPACT_TILE : 0; }
+
 void launch_compact(const uint8_t *flags, int64_t n, int32_t *row_ids, uint32_t *count, hipStream_t s)
 {
-    const int64_t ntiles = n > 0 ? (n + COMPACT_TILE - 1) / COMPACT_TILE : 0;
+    const int64_t ntiles = compact_tiles(n);
     uint32_t *counts = reinterpret_cast<uint32_t *>(scratch_buffer((size_t)std::max<int64_t>(ntiles, 1) * 4, s));
+    launch_compact_into(flags, n, row_ids, count, counts, s);
+}
+
+void launch_compact_into(const uint8_t *flags, int64_t n, int32_t *row_ids, uint32_t *count, uint32_t *counts,
+                         hipStream_t s)
+{
+    const int64_t ntiles = compact_tiles(n);
     if (ntiles > 0)
         hipLaunchKernelGGL(k_compact_count, dim3((unsigned)ntiles), dim3(BLOCK), 0, s, flags, n, counts);
     hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(BLOCK), 0, s, counts, ntiles, count);
     if (ntiles > 0)
         hipLaunchKernelGGL(k_compact_write, dim3((unsigned)ntiles), dim3(BLOCK), 0, s, flags, n, counts, row_ids);
     check_launch("compact");
+}
+
+void launch_gather_cols(const GatherCols &g, const int32_t *ids, int64_t count, hipStream_t s)
+{
+    if (count <= 0 || g.ncols <= 0) return;
+    if (g.ncols > GATHER_MAX_COLS) fail(SDB_ERR_INVALID_ARG, "gather: too many columns");
+    hipLaunchKernelGGL(k_gather_cols, dim3(grid_one(count)), dim3(BLOCK), 0, s, g, ids, count);
+    check_launch("gather_cols");
 }
 
 void launch_gather_i32(const int32_t *src, const int32_t *ids, int64_t count, int32_t *dst, hipStream_t s)

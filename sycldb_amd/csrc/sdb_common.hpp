@@ -41,4 +41,24 @@ unsigned stream_grid(int64_t units, int64_t per_block, int blocks_per_cu = 8);
 // calls on the same device and stream-ordered by the caller's stream.
 void *scratch_buffer(size_t bytes, hipStream_t stream);
 
+// ------------------------------------------------------- result transfer
+
+// Ids of the rows whose flag is set, in row order, into row_ids (capacity n);
+// their number into *count.  tile_counts: compact_tiles(n) words of device
+// memory for the per-tile counts.
+int64_t compact_tiles(int64_t n);
+void launch_compact_into(const uint8_t *flags, int64_t n, int32_t *row_ids, uint32_t *count, uint32_t *tile_counts,
+                         hipStream_t s);
+// dst[c][j] = src[c][ids[j]] for j < count (0 where ids[j] >= src_rows[c]);
+// a column holds int32 values, or uint64 where bit c of is_u64 is set.
+constexpr int GATHER_MAX_COLS = 8;
+struct GatherCols {
+    const void *src[GATHER_MAX_COLS];
+    void *dst[GATHER_MAX_COLS];
+    uint64_t src_rows[GATHER_MAX_COLS];
+    uint32_t is_u64;
+    int32_t ncols;
+};
+void launch_gather_cols(const GatherCols &g, const int32_t *ids, int64_t count, hipStream_t s);
+
 } // namespace sdb

@@ -8,6 +8,7 @@ calciteserver.thrift -- run through the TransientTable / KernelBundle model
 """
 import ctypes
 import json
+import weakref
 
 import numpy as np
 
@@ -17,30 +18,56 @@ from . import schema as S
 
 class Result:
     """Host copy of a plan's final table (TransientTable after the last
-    flush): ``nrows`` rows, selection ``flags``, non-null ``columns``."""
+    flush): ``nrows`` rows, selection ``flags``, non-null ``columns``.
+
+    The selected rows (what result_writer.cpp:41-67 writes) are copied when
+    the result is made; the full ``flags`` and ``columns`` arrays are copied
+    on first access."""
 
     def __init__(self, handle):
         lib = L.lib
-        try:
-            self.nrows = int(lib.sdb_result_nrows(handle))
-            self.flags = np.zeros(self.nrows, dtype=np.uint8)
+        self._h = ctypes.c_void_p(handle.value if isinstance(handle, ctypes.c_void_p) else handle)
+        self._fin = weakref.finalize(self, lib.sdb_result_destroy, self._h)
+        self.nrows = int(lib.sdb_result_nrows(self._h))
+        self._is_u64 = [lib.sdb_result_col_is_u64(self._h, c) == 1 for c in range(lib.sdb_result_ncols(self._h))]
+        k = int(lib.sdb_result_selected(self._h))
+        self.selected_rows = np.zeros(k, dtype=np.uint64)
+        self.selected_columns = []
+        if k:
+            L.check(lib.sdb_result_selected_rows(self._h, self.selected_rows.ctypes.data_as(ctypes.c_void_p)))
+        for c, u64 in enumerate(self._is_u64):
+            arr = np.zeros(k, dtype=np.uint64 if u64 else np.int32)
+            if k:
+                L.check(lib.sdb_result_col_copy_selected(self._h, c, arr.ctypes.data_as(ctypes.c_void_p)))
+            self.selected_columns.append(arr)
+        self._flags = None
+        self._columns = None
+
+    @property
+    def flags(self):
+        if self._flags is None:
+            f = np.zeros(self.nrows, dtype=np.uint8)
             if self.nrows:
-                L.check(lib.sdb_result_flags_copy(handle, self.flags.ctypes.data_as(ctypes.c_void_p)))
-            self.columns = []
-            for c in range(lib.sdb_result_ncols(handle)):
-                is_u64 = lib.sdb_result_col_is_u64(handle, c) == 1
-                arr = np.zeros(self.nrows, dtype=np.uint64 if is_u64 else np.int32)
+                L.check(L.lib.sdb_result_flags_copy(self._h, f.ctypes.data_as(ctypes.c_void_p)))
+            self._flags = f
+        return self._flags
+
+    @property
+    def columns(self):
+        if self._columns is None:
+            cols = []
+            for c, u64 in enumerate(self._is_u64):
+                arr = np.zeros(self.nrows, dtype=np.uint64 if u64 else np.int32)
                 if self.nrows:
-                    L.check(lib.sdb_result_col_copy(handle, c, arr.ctypes.data_as(ctypes.c_void_p)))
-                self.columns.append((is_u64, arr))
-        finally:
-            lib.sdb_result_destroy(handle)
+                    L.check(L.lib.sdb_result_col_copy(self._h, c, arr.ctypes.data_as(ctypes.c_void_p)))
+                cols.append((u64, arr))
+            self._columns = cols
+        return self._columns
 
     def rows(self):
         """Selected rows as tuples of Python ints (result_writer.cpp:41-67)."""
-        idx = np.nonzero(self.flags)[0]
-        cols = [arr[idx] for _, arr in self.columns]
-        return [tuple(int(c[r]) for c in cols) for r in range(len(idx))]
+        cols = [c.tolist() for c in self.selected_columns]
+        return list(zip(*cols)) if cols else [() for _ in range(len(self.selected_rows))]
 
     def text(self):
         return "".join(" ".join(str(v) for v in row) + "\n" for row in self.rows())

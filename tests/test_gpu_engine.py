@@ -122,6 +122,32 @@ def test_result_text_matches_oracle(fused, cpu_tables, tmp_path):
     assert res.text() == want
 
 
+@pytest.mark.parametrize("idx", [3, 7])
+def test_result_arrays_match_oracle(fused, cpu_tables, tmp_path, idx):
+    """The full host view (flags, sums, keys at selected rows) and the .res
+    text, for a small group-by table (copied whole) and a 437,500-slot one
+    (compacted on the device before the copy)."""
+    from sycldb_amd import _lib as L
+    tmpl, params = CASES[idx]
+    plan = PL.build(tmpl, **params)
+    want = PO.execute_plan(plan, cpu_tables)
+    res = fused.execute(plan)
+    assert res.nrows == want.nrows
+    assert (res.flags == want.flags).all()
+    sel = np.nonzero(want.flags)[0]
+    assert (res.selected_rows == sel).all()
+    assert len(res.columns) == len(want.columns)
+    for (u64, got), (agg, ref) in zip(res.columns, want.columns):
+        assert u64 == agg
+        if agg:
+            assert (got == ref.astype(np.uint64)).all()      # unselected slots sum to 0
+        else:
+            assert (got[sel] == ref[sel]).all()
+    path = tmp_path / "out.res"
+    L.check(L.lib.sdb_result_write(res._h, str(path).encode()))
+    assert path.read_text() == want.text()
+
+
 def test_errors_follow_reference(fused):
     bad_op = PL.build("t1")
     bad_op["rels"][5]["exprs"][0]["op"] = "%"     # unknown arithmetic op -> invalid_argument
