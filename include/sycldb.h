@@ -346,6 +346,9 @@ int sdb_codegen_check(const char *arch, int *count);
  * operator launched alone) -- the counterpart of the reference's
  * add_profiling_info (models/execution.hpp:18-28). */
 int sdb_engine_set_profiling(sdb_engine *eng, int on);
+/* Restricts the timing to the bundles of table `table` (NULL or "": every
+ * bundle); the others then run without events around their launches. */
+int sdb_engine_set_profile_table(sdb_engine *eng, const char *table);
 int sdb_engine_profile_reset(sdb_engine *eng);
 int sdb_engine_profile_count(sdb_engine *eng);
 /* Entry idx (0..count-1): name, launches, rows processed, summed ms. */

@@ -140,6 +140,7 @@ SIGNATURES = {
     "sdb_codegen_tuning_report": (INT, [ctypes.c_char_p, INT]),
     "sdb_codegen_check": (INT, [CSTR, ctypes.POINTER(INT)]),
     "sdb_engine_set_profiling": (INT, [P, INT]),
+    "sdb_engine_set_profile_table": (INT, [P, CSTR]),
     "sdb_engine_profile_reset": (INT, [P]),
     "sdb_engine_profile_count": (INT, [P]),
     "sdb_engine_profile_get": (INT, [P, INT, CSTR, INT, ctypes.POINTER(I64), ctypes.POINTER(I64),

@@ -95,11 +95,12 @@ void KernelBundle::execute(const ExecContext &ctx) const
             validate_program(p, kernels_.front().n, eng.registry);
             dump_program(p, kernels_.front().n, ctx.table ? *ctx.table : std::string("?"));
             Engine::ProfRec rec;
-            if (eng.profile) eng.prof_begin("fused:" + (ctx.table ? *ctx.table : std::string("?")), kernels_.front().n, rec);
+            const bool timed = eng.profiles(ctx.table);
+            if (timed) eng.prof_begin("fused:" + (ctx.table ? *ctx.table : std::string("?")), kernels_.front().n, rec);
             const bool jit = eng.codegen == SDB_CODEGEN_COMPILE || (eng.codegen == SDB_CODEGEN_AUTO && jit_available());
             if (jit) launch_fused_jit(p, kernels_.front().n, ctx.stream, eng.unroll);
             else launch_fused(p, kernels_.front().n, ctx.stream);
-            if (eng.profile) eng.prof_end(rec);
+            if (timed) eng.prof_end(rec);
             eng.launches++;
             return;
         }
@@ -125,7 +126,8 @@ void KernelBundle::execute_unfused(const ExecContext &ctx) const
         if (k.type == KernelType::EmptyKernel) continue;
         validate_kernel(k, eng.registry);
         Engine::ProfRec rec;
-        if (eng.profile)
+        const bool timed = eng.profiles(ctx.table);
+        if (timed)
             eng.prof_begin(std::string("op:") + kernel_type_name(k.type) + ":" + (ctx.table ? *ctx.table : std::string("?")),
                            k.n, rec);
         switch (k.type) {
@@ -169,7 +171,7 @@ void KernelBundle::execute_unfused(const ExecContext &ctx) const
             break;
         }
         }
-        if (eng.profile) eng.prof_end(rec);
+        if (timed) eng.prof_end(rec);
         eng.launches++;
     }
 }

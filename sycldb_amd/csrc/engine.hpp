@@ -330,6 +330,11 @@ struct Engine {
         double ms = 0;
     };
     bool profile = false;
+    std::string prof_table; // when set, only the bundles of this table are timed
+    bool profiles(const std::string *table) const
+    {
+        return profile && (prof_table.empty() || (table && *table == prof_table));
+    }
     std::map<std::string, KernelStat> prof_stats;
     struct ProfRec {
         std::string name;

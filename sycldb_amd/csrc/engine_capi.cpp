@@ -407,12 +407,13 @@ std::vector<ResultTable> execute_batch(Engine &eng, const std::vector<PlanResult
             }
             eng.registry.check(dev_progs, members.size() * sizeof(FProgram), "batched programs");
             Engine::ProfRec rec;
-            if (eng.profile) eng.prof_begin("batch:" + fact, rows, rec);
+            const bool timed = eng.profiles(&fact);
+            if (timed) eng.prof_begin("batch:" + fact, rows, rec);
             // one 4-row chunk per thread whatever the plain path's unroll: the
             // shared scan's double-buffered columns fill the registers
             launch_fused_batch(progs, rows, eng.stream, dev_progs, 1, eng.find_table(fact), batch_alloc, &eng,
                                batch_masks());
-            if (eng.profile) eng.prof_end(rec);
+            if (timed) eng.prof_end(rec);
             eng.launches++;
         }
         } catch (const Error &e) {
@@ -556,6 +557,14 @@ int sdb_engine_set_profiling(sdb_engine *e, int on)
     return guarded([&] {
         require(e, "null engine");
         e->eng.profile = on != 0;
+    });
+}
+
+int sdb_engine_set_profile_table(sdb_engine *e, const char *table)
+{
+    return guarded([&] {
+        require(e, "null engine");
+        e->eng.prof_table = table ? table : "";
     });
 }
 

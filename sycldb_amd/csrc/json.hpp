@@ -4,10 +4,10 @@
 #pragma once
 
 #include <cstdint>
-#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace sdb {
@@ -19,15 +19,26 @@ struct Value {
     int64_t i = 0;
     std::string s;
     std::vector<Value> a;
-    std::map<std::string, Value> o;
+    // object members in document order (plan objects have a handful of
+    // keys: a linear search beats a tree); a repeated key keeps its last value
+    std::vector<std::pair<std::string, Value>> o;
 
-    bool has(const std::string &k) const { return kind == Object && o.count(k) > 0; }
-    const Value &at(const std::string &k) const
+    const Value *find(const char *k) const
     {
-        auto it = o.find(k);
-        if (kind != Object || it == o.end()) throw std::invalid_argument("plan: missing field '" + k + "'");
-        return it->second;
+        if (kind != Object) return nullptr;
+        for (auto it = o.rbegin(); it != o.rend(); ++it)
+            if (it->first == k) return &it->second;
+        return nullptr;
     }
+    bool has(const char *k) const { return find(k) != nullptr; }
+    bool has(const std::string &k) const { return find(k.c_str()) != nullptr; }
+    const Value &at(const char *k) const
+    {
+        const Value *v = find(k);
+        if (!v) throw std::invalid_argument(std::string("plan: missing field '") + k + "'");
+        return *v;
+    }
+    const Value &at(const std::string &k) const { return at(k.c_str()); }
     int64_t as_int() const
     {
         if (kind != Int) throw std::invalid_argument("plan: expected an integer");
@@ -98,7 +109,7 @@ class Parser {
                 ws();
                 if (p_ >= t_.size() || t_[p_] != ':') err("expected ':'");
                 p_++;
-                v.o[k] = value();
+                v.o.emplace_back(std::move(k), value());
                 ws();
                 if (p_ < t_.size() && t_[p_] == ',') {
                     p_++;
@@ -154,19 +165,37 @@ class Parser {
             while (p_ < t_.size() && t_[p_] >= '0' && t_[p_] <= '9') p_++;
             if (p_ < t_.size() && (t_[p_] == '.' || t_[p_] == 'e' || t_[p_] == 'E')) err("non-integer number");
             v.kind = Value::Int;
-            try {
-                v.i = std::stoll(t_.substr(s, p_ - s));
-            } catch (const std::exception &) {
-                err("integer out of range");
+            // digits accumulated in uint64 with an overflow check (the
+            // range of int64, as std::stoll accepts)
+            const bool neg = t_[s] == '-';
+            const size_t d0 = s + (neg ? 1 : 0);
+            if (d0 == p_) err("expected digits");
+            uint64_t mag = 0;
+            const uint64_t limit = neg ? uint64_t(1) << 63 : (uint64_t(1) << 63) - 1;
+            for (size_t q = d0; q < p_; q++) {
+                const uint64_t d = (uint64_t)(t_[q] - '0');
+                if (mag > (limit - d) / 10) err("integer out of range");
+                mag = mag * 10 + d;
             }
+            v.i = neg ? (int64_t)(uint64_t(0) - mag) : (int64_t)mag;
             return v;
         }
         err("unexpected character");
     }
     std::string str()
     {
-        std::string out;
         p_++; // opening quote
+        // fast path: no escape before the closing quote
+        {
+            size_t q = p_;
+            while (q < t_.size() && t_[q] != '"' && t_[q] != '\\') q++;
+            if (q < t_.size() && t_[q] == '"') {
+                std::string out(t_, p_, q - p_);
+                p_ = q + 1;
+                return out;
+            }
+        }
+        std::string out;
         while (p_ < t_.size() && t_[p_] != '"') {
             char c = t_[p_++];
             if (c == '\\') {

@@ -566,9 +566,31 @@ __global__ void __launch_bounds__(BLOCK) k_seen_to_flags(const uint32_t *seen, u
 // One thread per 32-slot bitmap word; deterministic (no atomics on the
 // tables), one atomic per word for the population count.
 
+// Adds every wave's count (held by its lane 0) into *count with one atomic
+// per block: same-address atomics from every wave serialise in L2 (22K waves
+// over the 1.4M-slot part table took 95 us).
+__device__ __forceinline__ void block_count_add(uint32_t local, uint32_t *count)
+{
+    __shared__ uint32_t wave_sum[BLOCK / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) wave_sum[wave] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < BLOCK / 64; w++) t += wave_sum[w];
+        if (t) atomicAdd(count, t);
+    }
+}
+
 // Presence bitmaps of the reference tables: one thread per slot, each wave
 // covering 64 consecutive slots (coalesced reads), its ballot giving two
-// bitmap words; the set slots are counted per wave.
+// bitmap words; the set slots are counted per wave, then per block.
+// Each wave takes BITS_GROUPS consecutive 64-slot groups per step, their
+// loads issued together, on a grid of at most BITS_MAX_BLOCKS blocks.
+constexpr int BITS_GROUPS = 4;
+constexpr unsigned BITS_MAX_BLOCKS = 512;
+
 __global__ void __launch_bounds__(BLOCK) k_bits_from_keys(const uint8_t *ht, int32_t len, uint32_t *bits,
                                                           uint32_t *count)
 {
@@ -576,16 +598,26 @@ __global__ void __launch_bounds__(BLOCK) k_bits_from_keys(const uint8_t *ht, int
     const int lane = threadIdx.x & 63;
     const int64_t nwaves = ((int64_t)gridDim.x * BLOCK) >> 6;
     uint32_t local = 0;
-    for (int64_t g = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) >> 6; g * 64 < (int64_t)len; g += nwaves) {
-        const int64_t s = g * 64 + lane;
-        const uint64_t m = __ballot(s < len && ht[s] != 0);
-        if (lane == 0) {
-            bits[2 * g] = (uint32_t)m;
-            if (2 * g + 1 < words) bits[2 * g + 1] = (uint32_t)(m >> 32);
-            local += (uint32_t)__popcll(m);
+    for (int64_t g0 = (((int64_t)blockIdx.x * BLOCK + threadIdx.x) >> 6) * BITS_GROUPS; g0 * 64 < (int64_t)len;
+         g0 += nwaves * BITS_GROUPS) {
+        bool set[BITS_GROUPS];
+#pragma unroll
+        for (int k = 0; k < BITS_GROUPS; k++) {
+            const int64_t s = (g0 + k) * 64 + lane;
+            set[k] = s < len && ht[s] != 0;
+        }
+#pragma unroll
+        for (int k = 0; k < BITS_GROUPS; k++) {
+            const int64_t g = g0 + k;
+            const uint64_t m = __ballot(set[k]);
+            if (lane == 0 && 2 * g < words) {
+                bits[2 * g] = (uint32_t)m;
+                if (2 * g + 1 < words) bits[2 * g + 1] = (uint32_t)(m >> 32);
+                local += (uint32_t)__popcll(m);
+            }
         }
     }
-    if (lane == 0 && local) atomicAdd(count, local);
+    block_count_add(local, count);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_bits_from_pairs(const int2 *ht, int32_t len, uint32_t *bits,
@@ -597,21 +629,30 @@ __global__ void __launch_bounds__(BLOCK) k_bits_from_pairs(const int2 *ht, int32
     const int64_t nwaves = ((int64_t)gridDim.x * BLOCK) >> 6;
     uint32_t local = 0;
     bool ok = true;
-    for (int64_t g = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) >> 6; g * 64 < (int64_t)len; g += nwaves) {
-        const int64_t s = g * 64 + lane;
-        const int2 e = s < len ? ht[s] : make_int2(0, 0);
-        const bool set = e.x == 1;
-        const uint32_t d = set ? (uint32_t)e.y - (uint32_t)vmin : 0u;
-        ok &= d <= 0xFFFFu;
-        if (vals && s < len) vals[s] = (uint16_t)d;
-        const uint64_t m = __ballot(set);
-        if (lane == 0) {
-            bits[2 * g] = (uint32_t)m;
-            if (2 * g + 1 < words) bits[2 * g + 1] = (uint32_t)(m >> 32);
-            local += (uint32_t)__popcll(m);
+    for (int64_t g0 = (((int64_t)blockIdx.x * BLOCK + threadIdx.x) >> 6) * BITS_GROUPS; g0 * 64 < (int64_t)len;
+         g0 += nwaves * BITS_GROUPS) {
+        int2 e[BITS_GROUPS];
+#pragma unroll
+        for (int k = 0; k < BITS_GROUPS; k++) {
+            const int64_t s = (g0 + k) * 64 + lane;
+            e[k] = s < len ? ht[s] : make_int2(0, 0);
+        }
+#pragma unroll
+        for (int k = 0; k < BITS_GROUPS; k++) {
+            const int64_t g = g0 + k, s = g * 64 + lane;
+            const bool set = e[k].x == 1;
+            const uint32_t d = set ? (uint32_t)e[k].y - (uint32_t)vmin : 0u;
+            ok &= d <= 0xFFFFu;
+            if (vals && s < len) vals[s] = (uint16_t)d;
+            const uint64_t m = __ballot(set);
+            if (lane == 0 && 2 * g < words) {
+                bits[2 * g] = (uint32_t)m;
+                if (2 * g + 1 < words) bits[2 * g + 1] = (uint32_t)(m >> 32);
+                local += (uint32_t)__popcll(m);
+            }
         }
     }
-    if (lane == 0 && local) atomicAdd(count, local);
+    block_count_add(local, count);
     if (!ok) atomicOr(bad, 1u);
 }
 
@@ -769,6 +810,24 @@ __global__ void __launch_bounds__(BLOCK) k_gather_cols(GatherCols g, const int32
                 static_cast<int32_t *>(g.dst[c])[j] = r < g.src_rows[c] ? static_cast<const int32_t *>(g.src[c])[r] : 0;
         }
     }
+}
+
+// blockIdx.y = part; the blocks of a part stride over its bytes (16 at a
+// time where source and destination allow, the rest one by one)
+__global__ void __launch_bounds__(BLOCK) k_pack_parts(PackParts p, uint8_t *dst)
+{
+    const int i = blockIdx.y;
+    const uint8_t *src = p.src[i];
+    uint8_t *out = dst + p.off[i];
+    const uint64_t bytes = p.bytes[i];
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    uint64_t head = 0;
+    if ((((uintptr_t)src | (uintptr_t)out) & 15) == 0) {
+        head = bytes & ~uint64_t(15);
+        for (uint64_t j = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; j < head / 16; j += stride)
+            reinterpret_cast<uint4 *>(out)[j] = reinterpret_cast<const uint4 *>(src)[j];
+    }
+    for (uint64_t j = head + (uint64_t)blockIdx.x * BLOCK + threadIdx.x; j < bytes; j += stride) out[j] = src[j];
 }
 
 __global__ void __launch_bounds__(BLOCK) k_gather_i32(const int32_t *src, const int32_t *ids, int64_t count,
@@ -1059,7 +1118,9 @@ void launch_bits_from_keys(const uint8_t *ht, int32_t len, uint32_t *bits, uint3
     if (len <= 0) return;
     const int64_t words = ((int64_t)len + 31) / 32;
     (void)words;
-    hipLaunchKernelGGL(k_bits_from_keys, dim3(grid_one(len)), dim3(BLOCK), 0, s, ht, len, bits, count);
+    const unsigned grid = std::min<unsigned>(
+        BITS_MAX_BLOCKS, (unsigned)(((int64_t)len + 64 * BITS_GROUPS * (BLOCK / 64) - 1) / (64 * BITS_GROUPS * (BLOCK / 64))));
+    hipLaunchKernelGGL(k_bits_from_keys, dim3(grid), dim3(BLOCK), 0, s, ht, len, bits, count);
     check_launch("bits_from_keys");
 }
 
@@ -1070,8 +1131,10 @@ void launch_bits_from_pairs(const int32_t *ht, int32_t len, uint32_t *bits, uint
     if (!aligned(ht, 8)) fail(SDB_ERR_INVALID_ARG, "key-value hash table must be 8-byte aligned");
     const int64_t words = ((int64_t)len + 31) / 32;
     (void)words;
-    hipLaunchKernelGGL(k_bits_from_pairs, dim3(grid_one(len)), dim3(BLOCK), 0, s,
-                       reinterpret_cast<const int2 *>(ht), len, bits, vals, vmin, count, bad);
+    const unsigned grid = std::min<unsigned>(
+        BITS_MAX_BLOCKS, (unsigned)(((int64_t)len + 64 * BITS_GROUPS * (BLOCK / 64) - 1) / (64 * BITS_GROUPS * (BLOCK / 64))));
+    hipLaunchKernelGGL(k_bits_from_pairs, dim3(grid), dim3(BLOCK), 0, s, reinterpret_cast<const int2 *>(ht), len, bits,
+                       vals, vmin, count, bad);
     check_launch("bits_from_pairs");
 }
 
@@ -1120,6 +1183,17 @@ void launch_gather_cols(const GatherCols &g, const int32_t *ids, int64_t count, 
     if (g.ncols > GATHER_MAX_COLS) fail(SDB_ERR_INVALID_ARG, "gather: too many columns");
     hipLaunchKernelGGL(k_gather_cols, dim3(grid_one(count)), dim3(BLOCK), 0, s, g, ids, count);
     check_launch("gather_cols");
+}
+
+void launch_pack_parts(const PackParts &p, uint8_t *dst, hipStream_t s)
+{
+    if (p.n <= 0) return;
+    if (p.n > PACK_MAX_PARTS) fail(SDB_ERR_INVALID_ARG, "pack: too many parts");
+    uint64_t most = 0;
+    for (int i = 0; i < p.n; i++) most = std::max(most, p.bytes[i]);
+    const unsigned gx = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((most / 16 + BLOCK - 1) / BLOCK, 256));
+    hipLaunchKernelGGL(k_pack_parts, dim3(gx, (unsigned)p.n), dim3(BLOCK), 0, s, p, dst);
+    check_launch("pack_parts");
 }
 
 void launch_gather_i32(const int32_t *src, const int32_t *ids, int64_t count, int32_t *dst, hipStream_t s)

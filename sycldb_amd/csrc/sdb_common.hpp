@@ -60,5 +60,15 @@ struct GatherCols {
     int32_t ncols;
 };
 void launch_gather_cols(const GatherCols &g, const int32_t *ids, int64_t count, hipStream_t s);
+// Copies device buffers src[i] (bytes[i] bytes) to dst + off[i]: several
+// small arrays packed into one buffer for a single transfer.
+constexpr int PACK_MAX_PARTS = 64;
+struct PackParts {
+    const uint8_t *src[PACK_MAX_PARTS];
+    uint64_t off[PACK_MAX_PARTS];
+    uint64_t bytes[PACK_MAX_PARTS];
+    int32_t n;
+};
+void launch_pack_parts(const PackParts &p, uint8_t *dst, hipStream_t s);
 
 } // namespace sdb

@@ -253,7 +253,9 @@ class Engine:
         L.check(L.lib.sdb_engine_set_fusion(self._h, 1 if on else 0))
         self.fusion = bool(on)
 
-    def set_profiling(self, on=True):
+    def set_profiling(self, on=True, table=None):
+        """Per-launch timing of the bundles (all of them, or those of `table`)."""
+        L.check(L.lib.sdb_engine_set_profile_table(self._h, table.encode() if table else None))
         L.check(L.lib.sdb_engine_set_profiling(self._h, 1 if on else 0))
 
     def profile_reset(self):
