@@ -134,3 +134,34 @@ def test_batch_mask_edge_cases(cpu_tables):
             assert r.rows() == expected[i], (i, pipes[i])
     finally:
         eng.close()
+
+
+def test_batch_irregular_date_keys(cpu_tables, pipelines):
+    """Fact date keys that are no dates (day 32, month 13, zero, negative,
+    far years), inside and outside the probed key range, and a date table
+    holding such a key: the shared masks' range tests and default words must
+    still match the oracle."""
+    import numpy as np
+    from sycldb_amd.engine import Engine
+    tables = {t: {c: np.array(a, dtype=np.int32, copy=True) for c, a in cols.items()} for t, cols in cpu_tables.items()}
+    od = tables["lineorder"][S.LO_ORDERDATE]
+    odd = [19920132, 19921301, 19920230, 19930000, 19991301, 0, -7, 2147483647, 19920101, 19981231]
+    rng = np.random.default_rng(7)
+    for k, v in enumerate(odd):
+        od[rng.choice(len(od), 500, replace=False)] = v
+    dk = tables["ddate"][S.D_DATEKEY]
+    dk[-1] = 19921301            # a date-table key that no calendar slot names
+    dk[-2] = 19920230            # one that a slot names but no generated row holds
+    pipes = pipelines[:16] + [("t1", {"date": ("year", 1992), "disc": (0, 10), "qty": ("lt", 51)})]
+    expected = [PO.execute_plan(PL.build(t, **p), tables).rows() for t, p in pipes]
+    eng = Engine()
+    try:
+        eng.load_tables(tables)
+        plans = [PL.build(t, **p) for t, p in pipes]
+        print("calendar batch", flush=True)
+        results, batched = eng.execute_batch(plans)
+        assert batched == len(plans)
+        for i, r in enumerate(results):
+            assert r.rows() == expected[i], (i, pipes[i])
+    finally:
+        eng.close()
