@@ -204,10 +204,21 @@ def cpu_baseline(plan, fact_cols, sample_rows, seed):
         PO.execute_plan(plan, tables)
         elapsed += time.perf_counter() - t0
         reps += 1
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": sample_rows * reps / elapsed, "unit": "fact rows/s", "cores": O.num_threads(),
             "kind": "port",
             "sample": "first %d rows of the same seeded lineorder (full dimensions), %d timed repetitions, "
-                      "%.1f s of CPU work" % (sample_rows, reps, elapsed)}
+                      "%.1f s of CPU work" % (sample_rows, reps, elapsed),
+            "nproc": os.cpu_count(), "cpu_model": cpu_model,
+            "host_copy_GBps": round(O.host_copy_gbps(1 << 30), 1)}
 
 
 def main():

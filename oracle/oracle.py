@@ -65,6 +65,7 @@ _sigs = {
     "orc_gen_lookup": [_P, _I64, _U64, _U64, _I64, _P, _I32],
     "orc_gen_dates": [_P, _P, _P, _I32],
     "orc_num_threads": [],
+    "orc_host_copy_gbps": [_I64],
 }
 for _name, _args in _sigs.items():
     _fn = getattr(_lib, _name)
@@ -73,6 +74,7 @@ for _name, _args in _sigs.items():
 _lib.orc_count_flags.restype = _U64
 _lib.orc_compact.restype = _I64
 _lib.orc_num_threads.restype = _INT
+_lib.orc_host_copy_gbps.restype = ctypes.c_double
 
 
 def _ptr(a):
@@ -93,6 +95,11 @@ def _u8(a):
 
 def num_threads():
     return _lib.orc_num_threads()
+
+
+def host_copy_gbps(nbytes=1 << 30):
+    """Host memory bandwidth (parallel copy, read + write), GB/s."""
+    return _lib.orc_host_copy_gbps(int(nbytes))
 
 
 def select_lit(col, in_flags, out_flags, cmp, logic, value):

@@ -449,6 +449,39 @@ void orc_gen_dates(int32_t *datekey, int32_t *year, int32_t *yearmonthnum,
     }
 }
 
+/* Host memory bandwidth next to the CPU baseline (not part of the
+ * restatement): the best of five parallel copies of `bytes` bytes, counted
+ * as read + write, in GB/s (a STREAM "copy"). */
+double orc_host_copy_gbps(int64_t bytes)
+{
+    const int64_t n = bytes / 8;
+    if (n <= 0) return 0.0;
+    uint64_t *a = (uint64_t *)malloc((size_t)n * 8), *b = (uint64_t *)malloc((size_t)n * 8);
+    if (!a || !b) {
+        free(a);
+        free(b);
+        return 0.0;
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        a[i] = (uint64_t)i;
+        b[i] = 0;
+    }
+    double best = 0.0;
+    for (int r = 0; r < 5; r++) {
+        const double t0 = omp_get_wtime();
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; i++) b[i] = a[i];
+        const double t = omp_get_wtime() - t0;
+        if (t > 0 && 2.0 * (double)n * 8 / t / 1e9 > best) best = 2.0 * (double)n * 8 / t / 1e9;
+    }
+    volatile uint64_t sink = b[n / 2];
+    (void)sink;
+    free(a);
+    free(b);
+    return best;
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP
