@@ -77,16 +77,18 @@ void KernelBundle::execute(const ExecContext &ctx) const
     if (kernels_.empty()) return;
     Engine &eng = *ctx.eng;
     if (eng.capture.active && ctx.table && *ctx.table == eng.capture.table) {
-        // batched execution: record the fused program of the fact-table bundle
-        FProgram p;
-        if (!build_program(ctx, p)) throw NotBatchable("a fact-table bundle does not fuse");
-        try {
-            validate_program(p, kernels_.front().n, eng.registry);
-        } catch (const Error &e) {
-            throw NotBatchable(std::string("a fact-table bundle touches a row-sized intermediate: ") + e.what());
-        }
-        dump_program(p, kernels_.front().n, *ctx.table + " (captured)");
-        eng.capture.bundles.push_back({p, kernels_.front().n, ctx.segment});
+        // batched execution: record the fact-table bundle; its program is
+        // built once the whole batch is captured (build_captured)
+        CapturedBundle cb;
+        cb.n = kernels_.front().n;
+        cb.segment = ctx.segment;
+        cb.kernels = kernels_;
+        if (ctx.live) cb.live = *ctx.live;
+        cb.table = *ctx.table;
+        cb.virgin_begin = ctx.virgin_begin;
+        cb.virgin_end = ctx.virgin_end;
+        cb.virgin_filled = ctx.virgin_filled;
+        eng.capture.bundles.push_back(std::move(cb));
         return;
     }
     if (eng.fusion) {
@@ -106,6 +108,24 @@ void KernelBundle::execute(const ExecContext &ctx) const
         }
     }
     execute_unfused(ctx);
+}
+
+void KernelBundle::build_captured(Engine &eng, CapturedBundle &cb)
+{
+    KernelBundle b;
+    b.kernels_ = cb.kernels;
+    ExecContext ctx{&eng, eng.stream, &cb.live, &cb.table};
+    ctx.segment = cb.segment;
+    ctx.virgin_begin = cb.virgin_begin;
+    ctx.virgin_end = cb.virgin_end;
+    ctx.virgin_filled = cb.virgin_filled;
+    if (!b.build_program(ctx, cb.p)) throw NotBatchable("a fact-table bundle does not fuse");
+    try {
+        validate_program(cb.p, cb.n, eng.registry);
+    } catch (const Error &e) {
+        throw NotBatchable(std::string("a fact-table bundle touches a row-sized intermediate: ") + e.what());
+    }
+    dump_program(cb.p, cb.n, cb.table + " (captured)");
 }
 
 void KernelBundle::execute_unfused(const ExecContext &ctx) const

@@ -251,11 +251,17 @@ struct ExecContext {
 // KernelBundle, models/execution.hpp:500-651: the operators queued for one
 // segment, run in order on one stream.  With fusion on, the bundle becomes a
 // single fused kernel when its operators allow it.
+struct CapturedBundle;
+
 class KernelBundle {
   public:
     void add_kernel(const KernelData &k) { kernels_.push_back(k); }
     const std::vector<KernelData> &get_kernels() const { return kernels_; }
     void execute(const ExecContext &ctx) const;
+    // Builds and validates the fused program of a captured fact-table bundle
+    // (throws NotBatchable when it does not fuse or touches a row-sized
+    // intermediate).  The engine's capture must describe the batch.
+    static void build_captured(Engine &eng, CapturedBundle &cb);
 
   private:
     void execute_unfused(const ExecContext &ctx) const;
@@ -281,6 +287,15 @@ struct CapturedBundle {
     FProgram p;
     int64_t n;
     uint64_t segment;
+    // The program is built once every pipeline of the batch has been
+    // captured (KernelBundle::build_captured), so that the join statistics
+    // of all of them are read in one stream sync; until then, the bundle and
+    // what its build reads from the execution context.
+    std::vector<KernelData> kernels;
+    LiveSet live;
+    std::string table;
+    const uint8_t *virgin_begin = nullptr, *virgin_end = nullptr;
+    bool virgin_filled = false;
 };
 
 struct Capture {

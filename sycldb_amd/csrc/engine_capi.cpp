@@ -387,6 +387,29 @@ std::vector<ResultTable> execute_batch(Engine &eng, const std::vector<PlanResult
         }
         eng.capture = Capture{};
     }
+    // the captured programs, built now: every plan's join statistics are
+    // read in one sync (resolve_compact) rather than one per plan
+    if (can) {
+        eng.capture = Capture{};
+        eng.capture.active = true;
+        eng.capture.table = fact;
+        eng.capture.lds_budget = lds_each;
+        try {
+            for (int i = 0; i < n; i++) {
+                if (!pipes[i].batched) continue;
+                try {
+                    for (CapturedBundle &cb : pipes[i].bundles) KernelBundle::build_captured(eng, cb);
+                } catch (const NotBatchable &ex) {
+                    if (debug_batch()) fprintf(stderr, "[batch] plan %d runs alone: %s\n", i, ex.what());
+                    pipes[i].batched = false;
+                }
+            }
+        } catch (...) {
+            eng.capture = Capture{};
+            throw;
+        }
+        eng.capture = Capture{};
+    }
     const double t_capture = ms_since(t0);
     // the shared fact passes
     std::vector<int> members;
