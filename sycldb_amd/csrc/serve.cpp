@@ -109,9 +109,10 @@ struct Server {
     bool stopping = false;
     int64_t batches = 0, batched_requests = 0;
     // per engine (worker thread only): running mean of one request's plain
-    // run time (ms; 0 = none seen yet) and the smallest batch worth sharing
+    // run time and the learned fixed part of a batch's time (ms; 0 = none
+    // seen yet)
     std::vector<double> plain_ms;
-    std::vector<double> batch_fixed_ms; // learned fixed part of a batch's time (0 = none yet)
+    std::vector<double> batch_fixed_ms;
     std::vector<std::thread> workers;
 
     // the queue whose first request arrived first (-1: all empty)
@@ -220,21 +221,19 @@ struct Server {
         // request per call)
         const int e = rs.front()->engine;
         Engine &eng = *engines[e];
-        const int n = (int)rs.size();
         bool ok = false;
-        // until a plain run time is known, the first request runs alone
-        size_t first_shared = plain_ms[e] > 0 ? 0 : 1;
-        if (n - (int)first_shared > 1) {
+        // (a group of several is admitted only for an engine whose plain
+        // run time is known: batch_pays)
+        if (rs.size() > 1) {
             try {
-                for (size_t i = 0; i < first_shared; i++) run_alone(eng, *rs[i]);
                 const auto b0 = Clock::now();
                 SDB_HIP(hipSetDevice(eng.device));
                 std::vector<PlanResult> plans;
-                for (size_t i = first_shared; i < rs.size(); i++) plans.push_back(rs[i]->plan);
+                for (auto &r : rs) plans.push_back(r->plan);
                 std::vector<ResultTable> out = execute_batch(eng, plans, nullptr);
-                for (size_t i = first_shared; i < rs.size(); i++) rs[i]->result = std::move(out[i - first_shared]);
+                for (size_t i = 0; i < rs.size(); i++) rs[i]->result = std::move(out[i]);
                 ok = true;
-                for (size_t i = first_shared; i < rs.size(); i++) finish(*rs[i]);
+                for (auto &r : rs) finish(*r);
                 // the batch time's fixed part A, learned from every batch
                 const double batch_ms = std::chrono::duration<double, std::milli>(Clock::now() - b0).count();
                 const double A = std::max(0.0, batch_ms - BATCH_MARGINAL * plain_ms[e] * (double)plans.size());
