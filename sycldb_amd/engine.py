@@ -20,28 +20,51 @@ class Result:
     """Host copy of a plan's final table (TransientTable after the last
     flush): ``nrows`` rows, selection ``flags``, non-null ``columns``.
 
-    The selected rows (what result_writer.cpp:41-67 writes) are copied when
-    the result is made; the full ``flags`` and ``columns`` arrays are copied
-    on first access."""
+    The table stays in the library's host memory; the selected rows (what
+    result_writer.cpp:41-67 writes) and the full ``flags`` and ``columns``
+    arrays are copied into numpy on first access."""
 
     def __init__(self, handle):
         lib = L.lib
         self._h = ctypes.c_void_p(handle.value if isinstance(handle, ctypes.c_void_p) else handle)
         self._fin = weakref.finalize(self, lib.sdb_result_destroy, self._h)
         self.nrows = int(lib.sdb_result_nrows(self._h))
-        self._is_u64 = [lib.sdb_result_col_is_u64(self._h, c) == 1 for c in range(lib.sdb_result_ncols(self._h))]
-        k = int(lib.sdb_result_selected(self._h))
-        self.selected_rows = np.zeros(k, dtype=np.uint64)
-        self.selected_columns = []
-        if k:
-            L.check(lib.sdb_result_selected_rows(self._h, self.selected_rows.ctypes.data_as(ctypes.c_void_p)))
-        for c, u64 in enumerate(self._is_u64):
-            arr = np.zeros(k, dtype=np.uint64 if u64 else np.int32)
-            if k:
-                L.check(lib.sdb_result_col_copy_selected(self._h, c, arr.ctypes.data_as(ctypes.c_void_p)))
-            self.selected_columns.append(arr)
+        self._ncols = int(lib.sdb_result_ncols(self._h))
+        self._is_u64 = None
+        self._selected = None
         self._flags = None
         self._columns = None
+
+    def _u64(self):
+        if self._is_u64 is None:
+            self._is_u64 = [L.lib.sdb_result_col_is_u64(self._h, c) == 1 for c in range(self._ncols)]
+        return self._is_u64
+
+    def _fetch_selected(self):
+        if self._selected is None:
+            lib = L.lib
+            k = int(lib.sdb_result_selected(self._h))
+            rows = np.zeros(k, dtype=np.uint64)
+            if k:
+                L.check(lib.sdb_result_selected_rows(self._h, rows.ctypes.data_as(ctypes.c_void_p)))
+            cols = []
+            for c, u64 in enumerate(self._u64()):
+                arr = np.zeros(k, dtype=np.uint64 if u64 else np.int32)
+                if k:
+                    L.check(lib.sdb_result_col_copy_selected(self._h, c, arr.ctypes.data_as(ctypes.c_void_p)))
+                cols.append(arr)
+            self._selected = (rows, cols)
+        return self._selected
+
+    @property
+    def selected_rows(self):
+        """Row numbers of the selected rows, ascending."""
+        return self._fetch_selected()[0]
+
+    @property
+    def selected_columns(self):
+        """Each column's values at the selected rows."""
+        return self._fetch_selected()[1]
 
     @property
     def flags(self):
@@ -56,7 +79,7 @@ class Result:
     def columns(self):
         if self._columns is None:
             cols = []
-            for c, u64 in enumerate(self._is_u64):
+            for c, u64 in enumerate(self._u64()):
                 arr = np.zeros(self.nrows, dtype=np.uint64 if u64 else np.int32)
                 if self.nrows:
                     L.check(L.lib.sdb_result_col_copy(self._h, c, arr.ctypes.data_as(ctypes.c_void_p)))
