@@ -152,10 +152,19 @@ __global__ void __launch_bounds__(BLOCK) k_fill_i32(int32_t *dst, int32_t v, int
     if (blockIdx.x == 0 && threadIdx.x < (m & 3)) body[(nchunks << 2) + threadIdx.x] = v;
 }
 
+// 16-byte stores between the first and the last 16-byte boundary of the
+// buffer; the head and tail bytes (fewer than 16 each) one per thread
 __global__ void __launch_bounds__(BLOCK) k_fill_u8(uint8_t *dst, uint8_t v, int64_t n)
 {
-    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK)
-        dst[i] = v;
+    const int64_t head = min(n, (int64_t)((16u - ((uintptr_t)dst & 15u)) & 15u));
+    const int64_t nq = (n - head) >> 4, tail0 = head + (nq << 4);
+    const uint32_t w = 0x01010101u * v;
+    const uint4 q = make_uint4(w, w, w, w);
+    uint4 *dq = reinterpret_cast<uint4 *>(dst + head);
+    const int64_t tid = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    for (int64_t i = tid; i < nq; i += (int64_t)gridDim.x * BLOCK) dq[i] = q;
+    if (tid < head) dst[tid] = v;
+    if (tid < n - tail0) dst[tail0 + tid] = v;
 }
 
 __global__ void k_zero_u64(uint64_t *p) { *p = 0; }
@@ -988,7 +997,7 @@ void launch_fill_i32(int32_t *dst, int32_t v, int64_t n, hipStream_t s)
 void launch_fill_u8(uint8_t *dst, uint8_t v, int64_t n, hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_fill_u8, dim3(grid_one(n)), dim3(BLOCK), 0, s, dst, v, n);
+    hipLaunchKernelGGL(k_fill_u8, dim3(grid_one(std::max<int64_t>(n / 16, 1))), dim3(BLOCK), 0, s, dst, v, n);
     check_launch("fill_u8");
 }
 
