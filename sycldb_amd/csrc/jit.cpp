@@ -1521,7 +1521,7 @@ __device__ __forceinline__ void rows_of_program(const uint32_t (&mw)[UU][R], int
     for (int u = 0; u < UU; u++) {
         act[u] = 0;
 #pragma unroll
-        for (int k = 0; k < R; k++) act[u] |= ((mw[u][k] >> i) & 1u) << k;
+        for (int k = 0; k < R; k++) act[u] |= __builtin_amdgcn_ubfe(mw[u][k], (uint32_t)i, 1u) << k;
     }
 }
 )";
