@@ -107,6 +107,27 @@ def test_fusion_launches_fewer_kernels(fused):
     assert n_fused < n_unfused
 
 
+def test_every_tuner_candidate_matches_oracle():
+    """Above 4M rows the autotuner times one launch configuration per
+    execution (nine of them): every one of those executions returns the
+    oracle's result."""
+    from oracle import datagen
+    rows = 4_500_011
+    tables = datagen.generate_tables(S, rows, seed=SEED + 1)
+    plans = [PL.build("t2"), PL.build("t3")]
+    want = [PO.execute_plan(p, tables).rows() for p in plans]
+    from sycldb_amd.engine import Engine
+    eng = Engine()
+    try:
+        eng.generate(rows, SEED + 1)
+        for rep in range(11):
+            for i, p in enumerate(plans):
+                print("tuner execution", rep, i, flush=True)
+                assert eng.execute(p).rows() == want[i], (rep, i)
+    finally:
+        eng.close()
+
+
 def test_repeated_execution_is_stable(fused, expected):
     plan = PL.build("t3")
     for _ in range(3):
