@@ -224,6 +224,38 @@ __device__ __forceinline__ uint32_t probe_bits_finish(const int32_t (&x)[R], uin
     return out;
 }
 
+// A bitmap probe answered by its FV_U16 value table alone: a slot reads
+// VALS_UNSET exactly when its presence bit is clear, so one read gives both
+// the match and the value (no bitmap read).  Rows outside the key range or
+// the input flags read nothing and keep VALS_UNSET.
+template <int R>
+__device__ __forceinline__ void probe_unset_issue(const int32_t (&x)[R], uint32_t in, const uint16_t *vals,
+                                                  int32_t mn, int32_t mx, ProbeTmp<R> &t)
+{
+    const uint32_t span = (uint32_t)mx - (uint32_t)mn;
+    const bool nonempty = mx >= mn;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        t.word[k] = VALS_UNSET;
+        const uint32_t s = (uint32_t)x[k] - (uint32_t)mn;
+        if (nonempty & (((in >> k) & 1u) != 0) & (s <= span)) t.word[k] = vals[s];
+    }
+}
+
+template <int R>
+__device__ __forceinline__ uint32_t probe_unset_finish(uint32_t in, const ProbeTmp<R> &t, int32_t vmin,
+                                                       int32_t (&v)[R])
+{
+    uint32_t out = 0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const bool hit = (((in >> k) & 1u) != 0) & (t.word[k] != VALS_UNSET);
+        out |= (uint32_t)hit << k;
+        v[k] = hit ? (int32_t)((uint32_t)vmin + t.word[k]) : v[k];
+    }
+    return out;
+}
+
 template <int R>
 __device__ __forceinline__ void probe_semi_issue(const int32_t (&x)[R], uint32_t in, const uint8_t *ht, int32_t mn,
                                                  int32_t mx, ProbeTmp<R> &t)

@@ -54,7 +54,9 @@ enum FKind : uint8_t {
 };
 
 // value table of FK_PROBE_BITS
-//   FV_U16     16-bit offsets from vmin, one per slot (global memory)
+//   FV_U16     16-bit offsets from vmin, one per slot (global memory); a slot
+//              whose presence bit is clear holds VALS_UNSET, so the table
+//              alone also answers the presence test
 //   FV_PAIR    the reference (flag, value) pairs (global memory)
 //   FV_RANK16  a rank dictionary staged in LDS at lds_off: the presence words
 //              (u32 x ht_len words), the number of set bits before each word
@@ -62,6 +64,7 @@ enum FKind : uint8_t {
 //              slots in slot order (u16 offsets from vmin); the value of a set
 //              slot is values[prefix[w] + popcount(word & lower bits)]
 enum FValKind : uint8_t { FV_NONE = 0, FV_U16 = 1, FV_PAIR = 2, FV_RANK16 = 3 };
+constexpr uint32_t VALS_UNSET = 0xFFFFu;
 
 // byte offsets of the parts of a rank dictionary of `words` presence words
 __host__ __device__ constexpr uint32_t rank_prefix_off(uint32_t words) { return 4u * words; }

@@ -277,6 +277,9 @@ struct Gen {
         else if (I.kind == FK_PROBE_FULL)
             a << "probe_full_issue<R>(" << x << ", " << gate << ", (const int2 *)" << P << ".ht, " << P << ".ht_min, "
               << P << ".ht_max, " << t << ");\n";
+        else if (unset_probe(i))
+            a << "probe_unset_issue<R>(" << x << ", " << gate << ", (const uint16_t *)" << P << ".vals, " << P
+              << ".ht_min, " << P << ".ht_max, " << t << ");\n";
         else {
             const std::string bits =
                 I.in_lds ? "(const uint32_t *)(lds + " + P + ".lds_off)" : "(const uint32_t *)" + P + ".ht";
@@ -302,7 +305,23 @@ struct Gen {
             return !(e && *e == '0');
         }();
         const FInstr &I = p.ins[i];
-        return on && !is_decided(i) && I.kind == FK_PROBE_BITS && (I.vkind == FV_U16 || I.vkind == FV_PAIR);
+        return on && !is_decided(i) && !unset_probe(i) && I.kind == FK_PROBE_BITS &&
+               (I.vkind == FV_U16 || I.vkind == FV_PAIR);
+    }
+
+    // A plain program's bitmap probe whose bitmap is in global memory and
+    // whose values are FV_U16 reads the value table alone (VALS_UNSET marks
+    // the slots without a match): one round trip through the caches instead
+    // of the presence word's and then the value's.  SDB_JIT_UNSET_PROBE=0
+    // reads the bitmap first.
+    bool unset_probe(int i) const
+    {
+        static const bool on = [] {
+            const char *e = getenv("SDB_JIT_UNSET_PROBE");
+            return !(e && *e == '0');
+        }();
+        const FInstr &I = p.ins[i];
+        return on && !batch_act && !is_decided(i) && I.kind == FK_PROBE_BITS && I.vkind == FV_U16 && !I.in_lds;
     }
 
     static bool writes_flags(uint8_t k)
@@ -459,6 +478,9 @@ struct Gen {
             if (I.kind == FK_PROBE_SEMI) c << "                o = probe_semi_finish<R>(" << fin << ", " << t << ");\n";
             else if (I.kind == FK_PROBE_FULL)
                 c << "                o = probe_full_finish<R>(" << fin << ", " << t << ", " << vdst << ");\n";
+            else if (unset_probe(i))
+                c << "                o = probe_unset_finish<R>(" << fin << ", " << t << ", " << P << ".vmin, " << vdst
+                  << ");\n";
             else {
                 const std::string bits =
                     I.in_lds ? "(const uint32_t *)(lds + " + P + ".lds_off)" : "(const uint32_t *)" + P + ".ht";
@@ -695,6 +717,10 @@ struct Gen {
                 else if (I.kind == FK_PROBE_FULL)
                     loop << "                probe_full_issue<1>(x, 1u, (const int2 *)" << P << ".ht, " << P << ".ht_min, "
                          << P << ".ht_max, t);\n                if (!probe_full_finish<1>(1u, t, v)) continue;\n";
+                else if (unset_probe(i))
+                    loop << "                probe_unset_issue<1>(x, 1u, (const uint16_t *)" << P << ".vals, " << P
+                         << ".ht_min, " << P << ".ht_max, t);\n                if (!probe_unset_finish<1>(1u, t, " << P
+                         << ".vmin, v)) continue;\n";
                 else {
                     const std::string bits =
                         I.in_lds ? "(const uint32_t *)(lds + " + P + ".lds_off)" : "(const uint32_t *)" + P + ".ht";

@@ -596,7 +596,10 @@ __device__ __forceinline__ void block_count_add(uint32_t local, uint32_t *count)
 // covering 64 consecutive slots (coalesced reads), its ballot giving two
 // bitmap words; the set slots are counted per wave, then per block.
 // Each wave takes BITS_GROUPS consecutive 64-slot groups per step, their
-// loads issued together, on a grid of at most BITS_MAX_BLOCKS blocks.
+// loads issued together, on a grid of at most BITS_MAX_BLOCKS blocks.  From
+// the pairs, the 16-bit value table is written too: value - vmin in the set
+// slots, VALS_UNSET in the others (`bad` is raised when a set slot's offset
+// does not fit below VALS_UNSET, and the table is then not used).
 constexpr int BITS_GROUPS = 4;
 constexpr unsigned BITS_MAX_BLOCKS = 512;
 
@@ -650,8 +653,9 @@ __global__ void __launch_bounds__(BLOCK) k_bits_from_pairs(const int2 *ht, int32
         for (int k = 0; k < BITS_GROUPS; k++) {
             const int64_t g = g0 + k, s = g * 64 + lane;
             const bool set = e[k].x == 1;
-            const uint32_t d = set ? (uint32_t)e[k].y - (uint32_t)vmin : 0u;
-            ok &= d <= 0xFFFFu;
+            // unset slots hold VALS_UNSET; a set slot's offset must stay below it
+            const uint32_t d = set ? (uint32_t)e[k].y - (uint32_t)vmin : VALS_UNSET;
+            ok &= !set || d < VALS_UNSET;
             if (vals && s < len) vals[s] = (uint16_t)d;
             const uint64_t m = __ballot(set);
             if (lane == 0 && 2 * g < words) {
