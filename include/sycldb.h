@@ -328,8 +328,14 @@ int sdb_engine_set_fusion(sdb_engine *eng, int on);
  * `unroll` = 4-row chunks per thread per loop iteration of a compiled kernel
  * (1, 2 or 4), or 0: the autotuner times the candidate launch configurations
  * (chunks per thread x blocks per CU) on the first large launches of each
- * program shape and keeps the fastest. */
+ * program shape and keeps the fastest, then times that one once more with
+ * its row tail queued (below) and keeps the faster form.  1, 2 or 4 may be
+ * combined with SDB_UNROLL_QUEUED: the instructions after a program's last
+ * probe through global memory then run from a per-wave queue in LDS that is
+ * drained 64 rows at a time, where the program has such a tail (same
+ * results). */
 enum sdb_codegen { SDB_CODEGEN_AUTO = 0, SDB_CODEGEN_INTERPRET = 1, SDB_CODEGEN_COMPILE = 2 };
+#define SDB_UNROLL_QUEUED 16
 int sdb_engine_set_codegen(sdb_engine *eng, int mode, int unroll);
 /* 1 when runtime compilation is available, else 0 with the reason in `why`
  * (may be NULL). */

@@ -537,10 +537,12 @@ int sdb_engine_set_fusion(sdb_engine *e, int on)
 int sdb_engine_set_codegen(sdb_engine *e, int mode, int unroll)
 {
     return guarded([&] {
-        require(e, "null engine");
         require(mode == SDB_CODEGEN_AUTO || mode == SDB_CODEGEN_INTERPRET || mode == SDB_CODEGEN_COMPILE,
                 "unknown codegen mode");
-        require(unroll == 0 || unroll == 1 || unroll == 2 || unroll == 4, "unroll must be 0, 1, 2 or 4");
+        const int u = unroll & ~SDB_UNROLL_QUEUED;
+        require(unroll == 0 || u == 1 || u == 2 || u == 4,
+                "unroll must be 0, or 1, 2 or 4 (optionally | SDB_UNROLL_QUEUED)");
+        require(e, "null engine");
         if (mode == SDB_CODEGEN_COMPILE && !jit_available())
             fail(SDB_ERR_UNSUPPORTED, "runtime compilation unavailable: " + jit_unavailable_reason());
         e->eng.codegen = mode;

@@ -31,7 +31,7 @@ void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll);
 // measured ns/row of every candidate.
 std::string jit_tuning_report();
 // Generated source for the program shape (exposed for tests and debugging).
-std::string jit_source(const FProgram &p, int unroll, int block = 256);
+std::string jit_source(const FProgram &p, int unroll, int block = 256, bool queue = true);
 // Number of distinct kernels compiled so far in this process.
 int jit_kernels_compiled();
 // Device memory for one batched launch (mask tables and their descriptors),

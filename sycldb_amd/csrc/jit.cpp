@@ -643,6 +643,17 @@ struct Gen {
             key = arr;
             fill = "copy_rows<R>(" + arr + "[u], t);";
         }
+        if (tail_q) {
+            // queued rows: a prefetched column's value travels with the row,
+            // a lazily read column is read by row index when the queue drains
+            const std::string name = "tq_" + key;
+            auto &list = reg >= F_PF_BASE ? q_carry : q_gather;
+            const std::string src = reg >= F_PF_BASE ? "pf.c[" + std::to_string(reg - F_PF_BASE) + "][u]" : mem;
+            bool seen = false;
+            for (auto &e : list) seen |= e.first == name;
+            if (!seen) list.push_back({name, src});
+            return name;
+        }
         auto it = tail_vecs.find(key);
         if (it != tail_vecs.end()) return it->second + "[b]";
         const std::string name = "tv_" + key;
@@ -651,11 +662,10 @@ struct Gen {
         return name + "[b]";
     }
 
-    void tail_code()
+    // The per-row code of instructions T.. (one row in scope; `continue`
+    // drops it).
+    std::string tail_loop(int T)
     {
-        const int T = tail_start, F = p.ins[T].fin;
-        tail_vecs.clear();
-        tail_fill.clear();
         std::ostringstream loop;
         for (int i = T; i < p.ninstr; i++) {
             const FInstr &I = p.ins[i];
@@ -756,10 +766,24 @@ struct Gen {
             }
             loop << "            }\n";
         }
+        return loop.str();
+    }
+
+    // queued form: values carried with the row (name, chunk-register source)
+    // and lazily read columns read at drain time (name, column pointer)
+    bool tail_q = false;
+    std::vector<std::pair<std::string, std::string>> q_carry, q_gather;
+
+    void tail_code()
+    {
+        const int T = tail_start, F = p.ins[T].fin;
+        tail_vecs.clear();
+        tail_fill.clear();
+        const std::string body = tail_loop(T);
         // value registers: the row's value, from the chunk registers
         std::vector<bool> reads(p.nvals, false);
-        const std::string body = loop.str();
         for (int r = 0; r < p.nvals; r++) reads[r] = body.find("s" + vreg(r)) != std::string::npos;
+        if (queue) o << "    if constexpr (Q) {\n" << tail_queued(T, reads) << "    } else\n";
         o << "    { // instructions " << T << ".. row at a time over the rows that reach them\n";
         o << "        typename Lanes<UU * R>::type";
         bool first = true;
@@ -791,6 +815,96 @@ struct Gen {
         o << body << "        }\n    }\n";
     }
 
+    // Queued tail.  The rows that reach instruction T are appended to a
+    // queue of 64 entries per wave in LDS (row index, then the values the
+    // tail reads from registers); when the next rows would overflow it, and
+    // once after the last tile, the wave drains it: lane l takes entry l,
+    // reads the lazily read columns at its row and runs the tail for it.
+    // The tile then never waits for those reads; a drain waits once for up
+    // to 64 rows.  Field f of entry e sits at int32 index 64 * f + e.
+    bool queue_ok = false; // the caller allows a queue
+    bool queue = false;    // this kernel has one
+    int queue_fields = 0;
+    std::string queue_fn;  // queue_base / queue_flush, emitted before the body
+
+    std::string tail_queued(int T, const std::vector<bool> &reads)
+    {
+        q_carry.clear();
+        q_gather.clear();
+        tail_q = true;
+        const std::string body = tail_loop(T);
+        tail_q = false;
+        const int F = p.ins[T].fin;
+        // fields: 1 row, then the value registers read, then carried columns
+        std::vector<std::pair<std::string, std::string>> fields; // vector name, chunk source
+        std::ostringstream load;                                  // drain-time declarations
+        int f = 1;
+        for (int r = 0; r < p.nvals; r++)
+            if (reads[r]) {
+                fields.push_back({"tq_vec_" + vreg(r), vreg(r) + "[u][k]"});
+                load << "        int32_t s" << vreg(r) << " = qb[" << 64 * f++ << " + lane];\n";
+            }
+        for (auto &c : q_carry) {
+            fields.push_back({"tq_vec_" + c.first.substr(3), "t[k]"});
+            load << "        const int32_t " << c.first << " = qb[" << 64 * f++ << " + lane];\n";
+        }
+        for (auto &g : q_gather) load << "        const int32_t " << g.first << " = " << g.second << "[row];\n";
+        queue_fields = f;
+
+        std::ostringstream fn;
+        fn << "__device__ __forceinline__ void wave_sync()\n{\n"
+           << "    __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"wavefront\");\n"
+           << "    __builtin_amdgcn_wave_barrier();\n"
+           << "    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, \"wavefront\");\n}\n";
+        fn << "__device__ __forceinline__ int32_t *queue_base(const FProgram &P, uint8_t *lds)\n{\n"
+           << "    return reinterpret_cast<int32_t *>(lds + ((P.vreg_off + 15u) & ~15u)) + (threadIdx.x >> 6) * (64 * QF);\n}\n"
+           << "__device__ __forceinline__ void queue_flush(const FProgram &P, uint8_t *lds, int64_t &acc, uint32_t &qn)\n{\n"
+           << "    int32_t *const qb = queue_base(P, lds);\n    wave_sync();\n"
+           << "    const uint32_t lane = threadIdx.x & 63u;\n    if (lane < qn) {\n"
+           << "        const int64_t row = (int64_t)(uint32_t)qb[lane];\n"
+           << load.str() << "        do {\n" << body << "        } while (0);\n    }\n"
+           << "    wave_sync();\n    qn = 0;\n}\n";
+        queue_fn = fn.str();
+
+        std::ostringstream a;
+        a << "    { // instructions " << T << ".. queued for the rows that reach them\n";
+        a << "        int32_t *const qb = queue_base(P, lds);\n";
+        if (!fields.empty()) {
+            a << "        typename Lanes<UU * R>::type";
+            for (size_t i = 0; i < fields.size(); i++) a << (i ? ", " : " ") << fields[i].first << " = {}";
+            a << ";\n";
+        }
+        a << "        uint32_t rows = 0;\n#pragma unroll\n        for (int u = 0; u < UU; u++) {\n";
+        a << "            rows |= " << freg(F) << "[u] << (u * R);\n";
+        if (!fields.empty()) {
+            a << "            if (!" << freg(F) << "[u]) continue;\n            int32_t t[R];\n";
+            for (size_t i = 0; i < fields.size(); i++) {
+                const bool carried = i >= fields.size() - q_carry.size();
+                if (carried)
+                    a << "            vec_rows<R>(" << q_carry[i - (fields.size() - q_carry.size())].second << ", t);\n";
+                a << "#pragma unroll\n            for (int k = 0; k < R; k++) " << fields[i].first << "[u * R + k] = "
+                  << (carried ? std::string("t[k]") : fields[i].second) << ";\n";
+            }
+        }
+        a << "        }\n#pragma unroll 1\n        while (true) {\n"
+          << "            const bool has = rows != 0;\n"
+          << "            const uint64_t m = __ballot(has);\n"
+          << "            if (m == 0) break;\n"
+          << "            const uint32_t cnt = (uint32_t)__popcll(m);\n"
+          << "            if (qn + cnt > 64u) queue_flush(P, lds, acc, qn);\n"
+          << "            if (has) {\n"
+          << "                const int b = __builtin_ctz(rows);\n                rows &= rows - 1;\n"
+          << "                const uint32_t pos = qn + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), "
+             "__builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));\n"
+          << "                int64_t rb = row0[0];\n#pragma unroll\n"
+          << "                for (int u = 1; u < UU; u++) rb = b / R == u ? row0[u] : rb;\n"
+          << "                qb[pos] = (int32_t)(rb + b % R);\n";
+        for (size_t i = 0; i < fields.size(); i++)
+            a << "                qb[" << 64 * (i + 1) << " + pos] = " << fields[i].first << "[b];\n";
+        a << "            }\n            qn += cnt;\n        }\n    }\n";
+        return a.str();
+    }
+
     std::string body()
     {
         // registers the program touches
@@ -818,6 +932,7 @@ struct Gen {
         plan_preloads();
         o << preload_decls;
         plan_tail();
+        queue = queue_ok && !batch_act && prefetch_in_body && p.vec && tail_start >= 0;
         const int dense_end = tail_start >= 0 ? tail_start : p.ninstr;
         for (int i = 0; i < dense_end; i++)
             if (is_probe(p.ins[i].kind) && !dropped(i))
@@ -870,10 +985,14 @@ __device__ __forceinline__ void pf_load(const FProgram &P, const int64_t (&row0)
 #pragma unroll
         for (int u = 0; u < UU; u++) pf.c[j][u] = load_vec<R>(P.pf_mem[j], row0[u]);
 }
-template <int R, int UU>
+template <int R, int UU, bool Q>
 __device__ __forceinline__ void body(const FProgram &P, const int64_t (&row0)[UU], const Pf<R, UU> &pf, int64_t &acc,
-                                     uint8_t *lds, bool has_next, const int64_t (&nrow0)[UU], Pf<R, UU> &next)
+                                     uint8_t *lds, bool has_next, const int64_t (&nrow0)[UU], Pf<R, UU> &next,
+                                     uint32_t &qn)
 {
+)";
+
+const char *KERNEL_QUEUE_STUB = R"(__device__ __forceinline__ void queue_flush(const FProgram &, uint8_t *, int64_t &, uint32_t &) {}
 )";
 
 const char *KERNEL_TEMPLATE_TAIL = R"(}
@@ -888,6 +1007,7 @@ extern "C" __global__ void __launch_bounds__(BLOCK) sdb_fused_program(const FPro
     uint8_t *lds = reinterpret_cast<uint8_t *>(dyn_lds);
     fdev::block_prologue<BLOCK>(P, lds);
     int64_t acc = 0;
+    uint32_t qn = 0; // rows in this wave's queue (queued tail)
     if constexpr (VEC) {
     // tiles of U * BLOCK 4-row chunks; chunk u of a thread in tile t starts
     // at row 4 * (t * U * BLOCK + u * BLOCK + thread); the prefetched columns
@@ -910,20 +1030,24 @@ extern "C" __global__ void __launch_bounds__(BLOCK) sdb_fused_program(const FPro
         rows_of(t, r);
         const int64_t tn = t + gridDim.x;
         rows_of(tn, rn);
-        body<4, U>(P, r, cur, acc, lds, tn < ntiles, rn, next);
+        body<4, U, QUEUE>(P, r, cur, acc, lds, tn < ntiles, rn, next, qn);
         t = tn;
+    }
+    // every lane of the wave is here: drain its queue
+    if constexpr (QUEUE) {
+        if (qn) queue_flush(P, lds, acc, qn);
     }
     for (int64_t c = ntiles * tile + (int64_t)blockIdx.x * BLOCK + threadIdx.x; c < nch; c += (int64_t)gridDim.x * BLOCK) {
         const int64_t r[1] = {c << 2};
         Pf<4, 1> pf, unused;
         pf_load<4, 1>(P, r, pf);
-        body<4, 1>(P, r, pf, acc, lds, false, r, unused);
+        body<4, 1, false>(P, r, pf, acc, lds, false, r, unused, qn);
     }
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x < (n & 3)) {
         const int64_t r[1] = {(nch << 2) + threadIdx.x};
         Pf<1, 1> pf, unused;
         pf_load<1, 1>(P, r, pf);
-        body<1, 1>(P, r, pf, acc, lds, false, r, unused);
+        body<1, 1, false>(P, r, pf, acc, lds, false, r, unused, qn);
     }
     } else {
     const int64_t stride = (int64_t)gridDim.x * BLOCK;
@@ -938,7 +1062,7 @@ extern "C" __global__ void __launch_bounds__(BLOCK) sdb_fused_program(const FPro
         const int64_t r[1] = {i};
         const int64_t in = i + stride;
         const int64_t rn[1] = {in};
-        body<1, 1>(P, r, cur, acc, lds, in < n, rn, next);
+        body<1, 1, false>(P, r, cur, acc, lds, in < n, rn, next, qn);
         i = in;
     }
     }
@@ -1268,46 +1392,101 @@ std::mutex g_src_mu;
 std::unordered_map<std::string, std::string> g_src;           // shape key + launch shape -> source
 std::unordered_map<std::string, Compiled *> g_kernel_by_shape; // device + shape key + launch shape
 
-std::string launch_shape(int unroll, int block) { return std::string(1, (char)unroll) + (char)(block >> 8); }
+std::unordered_map<std::string, uint32_t> g_queue_bytes; // shape key + launch shape -> queue bytes per wave
+
+std::string launch_shape(int unroll, int block, bool queue)
+{
+    return std::string(1, (char)unroll) + (char)(block >> 8) + (queue ? "q" : "");
+}
+
+// Queued tails (SDB_JIT_QUEUE=0 runs every tail in the tile).
+bool queue_enabled()
+{
+    static const bool on = [] {
+        const char *e = getenv("SDB_JIT_QUEUE");
+        return !(e && *e == '0');
+    }();
+    return on;
+}
 
 } // namespace
 
-std::string jit_source(const FProgram &p, int unroll, int block)
+std::string jit_source(const FProgram &p, int unroll, int block, bool queue)
 {
     if (unroll != 1 && unroll != 2 && unroll != 4) fail(SDB_ERR_INVALID_ARG, "jit: unroll must be 1, 2 or 4");
     if (block != 256 && block != 512 && block != 1024) fail(SDB_ERR_INVALID_ARG, "jit: block must be 256, 512 or 1024");
-    const std::string key = shape_key(p) + launch_shape(unroll, block);
+    queue = queue && queue_enabled();
+    const std::string key = shape_key(p) + launch_shape(unroll, block, queue);
     {
         std::lock_guard<std::mutex> lk(g_src_mu);
         auto it = g_src.find(key);
         if (it != g_src.end()) return it->second;
     }
     Gen g(p);
+    g.queue_ok = queue;
+    const std::string body = g.body();
     std::ostringstream s;
     s << KERNEL_TEMPLATE_INCLUDES << "constexpr int NPF = " << p.npf << ";\nconstexpr int U = " << unroll
       << ";\nconstexpr bool VEC = " << (p.vec ? "true" : "false") << ";\nconstexpr int BLOCK = " << block << ";\n";
-    s << KERNEL_TEMPLATE_HEAD << g.body() << KERNEL_TEMPLATE_TAIL;
+    s << "constexpr bool QUEUE = " << (g.queue ? "true" : "false") << ";\nconstexpr int QF = " << g.queue_fields
+      << ";\n";
+    s << (g.queue ? g.queue_fn : std::string(KERNEL_QUEUE_STUB));
+    s << KERNEL_TEMPLATE_HEAD << body << KERNEL_TEMPLATE_TAIL;
     std::lock_guard<std::mutex> lk(g_src_mu);
+    g_queue_bytes[key] = g.queue ? 256u * (uint32_t)g.queue_fields : 0u;
     return g_src.emplace(key, s.str()).first->second;
 }
 
 namespace {
 
-// The compiled kernel of a program's shape for a launch shape.
-Compiled &kernel_for(const FProgram &p, int unroll, int block)
+// The compiled kernel of a program's shape for a launch shape, and the LDS
+// bytes per wave of its row queue (0: none).
+Compiled &kernel_for(const FProgram &p, int unroll, int block, bool queue, uint32_t *queue_bytes)
 {
     int dev = 0;
     SDB_HIP(hipGetDevice(&dev));
-    const std::string key = std::to_string(dev) + ":" + shape_key(p) + launch_shape(unroll, block);
+    queue = queue && queue_enabled();
+    const std::string skey = shape_key(p) + launch_shape(unroll, block, queue);
+    const std::string key = std::to_string(dev) + ":" + skey;
     {
         std::lock_guard<std::mutex> lk(g_src_mu);
         auto it = g_kernel_by_shape.find(key);
-        if (it != g_kernel_by_shape.end()) return *it->second;
+        if (it != g_kernel_by_shape.end()) {
+            *queue_bytes = g_queue_bytes.at(skey);
+            return *it->second;
+        }
     }
-    Compiled &k = get_kernel(jit_source(p, unroll, block));
+    Compiled &k = get_kernel(jit_source(p, unroll, block, queue));
     std::lock_guard<std::mutex> lk(g_src_mu);
     g_kernel_by_shape[key] = &k;
+    *queue_bytes = g_queue_bytes.at(skey);
     return k;
+}
+
+// Largest dynamic LDS of a launch with a row queue; past it (or past 2^31
+// rows, whose indices the queue does not hold) the kernel without one runs.
+constexpr uint64_t QUEUE_LDS_MAX = 96 * 1024;
+
+// The kernel a launch of n rows uses (with a row queue when `queue` allows
+// one and it fits), its dynamic LDS bytes, and whether it has the queue.
+Compiled &pick_kernel(const FProgram &p, int64_t n, int unroll, int block, bool queue, uint32_t &lds,
+                      bool *queued = nullptr)
+{
+    uint32_t qbytes = 0;
+    if (queued) *queued = false;
+    if (queue && n <= INT32_MAX) {
+        Compiled &k = kernel_for(p, unroll, block, true, &qbytes);
+        // the value-register file (the last LDS area) belongs to the
+        // interpreter; the queues of the block's waves go there instead
+        const uint64_t total = ((uint64_t)p.vreg_off + 15u) / 16u * 16u + (uint64_t)(block / 64) * qbytes;
+        if (!qbytes || total <= QUEUE_LDS_MAX) {
+            lds = qbytes ? (uint32_t)total : p.vreg_off;
+            if (queued) *queued = qbytes != 0;
+            return k;
+        }
+    }
+    lds = p.vreg_off;
+    return kernel_for(p, unroll, block, false, &qbytes);
 }
 
 } // namespace
@@ -1316,11 +1495,11 @@ namespace {
 
 // Launches the compiled kernel of `src` (blocks of `block` threads) with
 // `per_cu_mult` times the blocks per CU the occupancy calculator allows.
-void launch_compiled(const FProgram &p, int64_t n, hipStream_t s, int unroll, int per_cu_mult, int block = JIT_BLOCK)
+void launch_compiled(const FProgram &p, int64_t n, hipStream_t s, int unroll, int per_cu_mult, int block = JIT_BLOCK,
+                     bool queue = false)
 {
-    Compiled &k = kernel_for(p, unroll, block);
-    // the value-register file (the last LDS area) belongs to the interpreter
-    const uint32_t lds = p.vreg_off;
+    uint32_t lds = 0;
+    Compiled &k = pick_kernel(p, n, unroll, block, queue, lds);
     int per_cu;
     {
         std::lock_guard<std::mutex> lk(g_mu);
@@ -1363,10 +1542,19 @@ const TuneCfg TUNE_CANDIDATES[] = {{2, 1, 256}, {1, 1, 256}, {4, 1, 256}, {2, 1,
                                    {4, 1, 512}, {4, 1, 1024}, {1, 1, 1024}};
 constexpr int N_CAND = (int)(sizeof(TUNE_CANDIDATES) / sizeof(TUNE_CANDIDATES[0]));
 
+// After the candidates' trials, a program whose tail can run queued (the
+// rows that reach it drained 64 at a time per wave) times the fastest
+// candidate once more with the queue, and keeps the faster of the two: the
+// queue saves each tile a wait for the lazily read columns, but costs LDS and
+// a drain loop, and which wins depends on the program (measured: the q3.1
+// join at 600M rows 1.98 -> 1.89 ms, the q2.x join 1.39 -> 1.45 ms).
 struct Tune {
     double best_ns_per_row[N_CAND];
+    double queue_ns_per_row = 1e300;
     int trials = 0;
     int chosen = -1;
+    int base = -1;       // fastest candidate, while its queued trial is pending
+    bool chosen_queue = false;
     Tune()
     {
         for (double &b : best_ns_per_row) b = 1e300;
@@ -1383,7 +1571,7 @@ void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll)
     if (n <= 0) return;
     if (p.magic != F_MAGIC) fail(SDB_ERR_RUNTIME, "fused program without magic");
     if (unroll != 0) {
-        launch_compiled(p, n, s, unroll, 1);
+        launch_compiled(p, n, s, unroll & ~SDB_UNROLL_QUEUED, 1, JIT_BLOCK, (unroll & SDB_UNROLL_QUEUED) != 0);
         return;
     }
     if (n < TUNE_MIN_ROWS) {
@@ -1394,25 +1582,49 @@ void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll)
     SDB_HIP(hipGetDevice(&dev));
     const std::pair<int, std::string> key{dev, shape_key(p)};
     int cand;
+    bool queue = false;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         Tune &t = g_tune[key];
-        cand = t.chosen >= 0 ? -1 - t.chosen : t.trials % N_CAND;
+        if (t.chosen >= 0) {
+            cand = -1 - t.chosen;
+            queue = t.chosen_queue;
+        } else if (t.base >= 0) {
+            cand = t.base;
+            queue = true;
+        } else {
+            cand = t.trials % N_CAND;
+        }
     }
     if (cand < 0) {
         const TuneCfg &c = TUNE_CANDIDATES[-1 - cand];
-        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, c.block);
+        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, c.block, queue);
         return;
     }
     const TuneCfg &c = TUNE_CANDIDATES[cand];
-    kernel_for(p, c.unroll, c.block); // compile outside the timed region
+    uint32_t lds = 0;
+    bool queued = false;
+    pick_kernel(p, n, c.unroll, c.block, queue, lds, &queued); // compile outside the timed region
+    if (queue && !queued) {
+        // no queue for this program (no row tail, or its LDS does not fit)
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            Tune &t = g_tune[key];
+            if (t.chosen < 0 && t.base == cand) {
+                t.chosen = cand;
+                t.base = -1;
+            }
+        }
+        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, c.block);
+        return;
+    }
     hipEvent_t a, b;
     SDB_HIP(hipEventCreate(&a));
     SDB_HIP(hipEventCreate(&b));
     float ms = 0;
     hipError_t e = hipEventRecord(a, s);
     if (e == hipSuccess) {
-        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, c.block);
+        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, c.block, queue);
         e = hipEventRecord(b, s);
     }
     if (e == hipSuccess) e = hipEventSynchronize(b);
@@ -1423,12 +1635,22 @@ void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll)
     std::lock_guard<std::mutex> lk(g_tune_mu);
     Tune &t = g_tune[key];
     if (t.chosen >= 0) return;
-    t.best_ns_per_row[cand] = std::min(t.best_ns_per_row[cand], ms * 1e6 / (double)n);
+    const double ns = ms * 1e6 / (double)n;
+    if (queue) {
+        if (t.base != cand) return;
+        t.queue_ns_per_row = ns;
+        t.chosen_queue = ns < t.best_ns_per_row[cand];
+        t.chosen = cand;
+        t.base = -1;
+        return;
+    }
+    if (t.base >= 0) return;
+    t.best_ns_per_row[cand] = std::min(t.best_ns_per_row[cand], ns);
     if (++t.trials >= N_CAND * TUNE_ROUNDS) {
         int best = 0;
         for (int i = 1; i < N_CAND; i++)
             if (t.best_ns_per_row[i] < t.best_ns_per_row[best]) best = i;
-        t.chosen = best;
+        t.base = best; // next: its queued trial, when its kernel has a queue
     }
 }
 
@@ -2074,6 +2296,7 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     if (n <= 0 || in.empty()) return;
     // the shared scan holds up to BATCH_MAX_PF columns in registers, twice
     // (current and next tile): callers pass 1 (one 4-row chunk per thread)
+    unroll &= ~SDB_UNROLL_QUEUED; // batched bodies have no queued tails
     if (unroll != 1 && unroll != 2 && unroll != 4) unroll = 1;
     int dev = 0;
     SDB_HIP(hipGetDevice(&dev));
@@ -2293,7 +2516,8 @@ std::string jit_tuning_report()
         o << "shape " << shape++ << " device " << kv.first.first << ": ";
         if (t.chosen >= 0)
             o << "unroll " << TUNE_CANDIDATES[t.chosen].unroll << ", block " << TUNE_CANDIDATES[t.chosen].block
-              << ", blocks/CU x" << TUNE_CANDIDATES[t.chosen].per_cu_mult;
+              << ", blocks/CU x" << TUNE_CANDIDATES[t.chosen].per_cu_mult
+              << (t.chosen_queue ? ", queued tail" : "");
         else
             o << "tuning (" << t.trials << " trials)";
         o << "; ns/row";
@@ -2303,6 +2527,7 @@ std::string jit_tuning_report()
             if (t.best_ns_per_row[i] < 1e299) o << t.best_ns_per_row[i];
             else o << "-";
         }
+        if (t.queue_ns_per_row < 1e299) o << " [queued tail] " << t.queue_ns_per_row;
         o << "\n";
     }
     return o.str();

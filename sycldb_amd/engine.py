@@ -97,6 +97,7 @@ class Result:
 
 
 CODEGEN_MODES = {"auto": L.CODEGEN_AUTO, "interpret": L.CODEGEN_INTERPRET, "compile": L.CODEGEN_COMPILE}
+UNROLL_QUEUED = 16  # SDB_UNROLL_QUEUED
 
 
 def codegen_available():
@@ -128,6 +129,8 @@ class Engine:
              "interpret" or "compile".
     unroll:  4-row chunks per thread per iteration of a compiled kernel
              (1, 2, 4), or 0: chosen per program shape by the autotuner.
+             1, 2 or 4 | UNROLL_QUEUED also runs a program's row tail from
+             a per-wave queue drained 64 rows at a time.
     """
 
     def __init__(self, device_budget_bytes=0, segment_size=0, fusion=True, device=0, codegen="auto", unroll=0):

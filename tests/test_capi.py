@@ -100,6 +100,19 @@ def test_codegen_compiles_every_instruction_form():
     assert n.value == 8  # seven single-program kernels and one batched kernel
 
 
+def test_queued_unroll_flag_matches_header():
+    text = open(os.path.join(ROOT, "include", "sycldb.h")).read()
+    assert re.search(r"#define SDB_UNROLL_QUEUED (\d+)", text).group(1) == str(E.UNROLL_QUEUED) == "16"
+    # the flag goes with 1, 2 or 4 only (arguments are checked before the
+    # engine, which is null here)
+    for bad in (E.UNROLL_QUEUED, 3 | E.UNROLL_QUEUED, 8, 32 | 4):
+        with pytest.raises(ValueError, match="unroll"):
+            L.check(L.lib.sdb_engine_set_codegen(None, 0, bad))
+    for good in (0, 1, 4 | E.UNROLL_QUEUED, 2 | E.UNROLL_QUEUED):
+        with pytest.raises(ValueError, match="null engine"):
+            L.check(L.lib.sdb_engine_set_codegen(None, 0, good))
+
+
 def test_codegen_mode_validation():
     with pytest.raises(ValueError):
         E.Engine.set_codegen(E.Engine.__new__(E.Engine), "bogus")

@@ -40,6 +40,9 @@ def expected(cpu_tables):
     return {i: PO.execute_plan(PL.build(t, **p), cpu_tables).rows() for i, (t, p) in enumerate(CASES)}
 
 
+QUEUED = 16  # sycldb_amd.engine.UNROLL_QUEUED (SDB_UNROLL_QUEUED; test_capi checks the value)
+
+
 def make_engine(**kw):
     from sycldb_amd.engine import Engine
     eng = Engine(**kw)
@@ -72,7 +75,9 @@ def test_fused_matches_oracle(fused, expected, idx):
 @pytest.mark.parametrize("kw", [dict(fusion=False), dict(segment_size=50_000), dict(segment_size=77_777),
                                 dict(segment_size=77_777, fusion=False), dict(codegen="interpret"),
                                 dict(codegen="interpret", segment_size=77_777), dict(unroll=1), dict(unroll=4),
-                                dict(unroll=4, segment_size=77_777)])
+                                dict(unroll=4, segment_size=77_777), dict(unroll=1 | QUEUED),
+                                dict(unroll=2 | QUEUED), dict(unroll=4 | QUEUED),
+                                dict(unroll=4 | QUEUED, segment_size=77_777)])
 def test_unfused_and_segmented_match_oracle(expected, kw):
     print("engine", kw, flush=True)
     eng = make_engine(**kw)
