@@ -211,3 +211,39 @@ def test_autotuned_launches_match_oracle():
     report = E.codegen_tuning_report()
     print(report)
     assert "unroll" in report
+
+
+@pytest.mark.parametrize("span", [0xFFFE, 0xFFFF])
+def test_value_table_width_boundary(cpu_tables, span):
+    """Part brands spanning exactly 0xFFFE above their (negative) minimum:
+    the widest 16-bit value table, whose largest offset sits next to the
+    marker of unset slots; and 0xFFFF, one more, where the probe falls back
+    to the reference pairs.  The selected parts hold both ends of the range."""
+    from sycldb_amd.engine import Engine
+    tables = {t: {c: np.array(a, dtype=np.int32, copy=True) for c, a in cols.items()} for t, cols in cpu_tables.items()}
+    part = tables["part"]
+    n = len(part[S.P_BRAND1])
+    base = -5
+    brand = (base + (np.arange(n, dtype=np.int64) * 7919) % (span + 1)).astype(np.int32)
+    sel = np.flatnonzero(part[S.P_CATEGORY] == 1)
+    brand[sel[0]] = base + span
+    brand[sel[1]] = base
+    part[S.P_BRAND1] = brand
+    # fact rows that reach both ends: their part, and a supplier of region 1
+    lo = tables["lineorder"]
+    supp = tables["supplier"]
+    skey = supp[S.S_SUPPKEY][np.flatnonzero(supp[S.S_REGION] == 1)[0]]
+    rows = np.random.default_rng(11).choice(len(lo[S.LO_PARTKEY]), 100, replace=False)
+    lo[S.LO_PARTKEY][rows[:50]] = part[S.P_PARTKEY][sel[0]]
+    lo[S.LO_PARTKEY][rows[50:]] = part[S.P_PARTKEY][sel[1]]
+    lo[S.LO_SUPPKEY][rows] = skey
+    plan = PL.build("t2")
+    want = PO.execute_plan(plan, tables).rows()
+    assert any(r[1] == base + span for r in want) and any(r[1] == base for r in want)
+    eng = Engine()
+    try:
+        eng.load_tables(tables)
+        print("value table span", span, flush=True)
+        assert eng.execute(plan).rows() == want
+    finally:
+        eng.close()
