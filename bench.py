@@ -72,11 +72,17 @@ def batched_measure(eng, n_pipes, seed, reps):
     pipes = PL.random_pipelines(n_pipes, seed)
     plans = [PL.to_json(PL.build(t, **p)) for t, p in pipes]  # JSON text, as the C ABI takes plans
     alone = [eng.execute(p).rows() for p in plans]  # the reference results
-    # warm: the autotuner settles every plan shape (and compiles its nine
-    # candidates) before anything is timed
-    for _ in range(11):
+    # warm: every plan shape's kernels compile and the autotuner settles
+    # before anything is timed
+    from sycldb_amd.engine import codegen_pending, codegen_tuning_report, codegen_wait
+    for _ in range(64):
         for p in plans:
             eng.execute(p)
+        if codegen_pending() == 0:
+            break
+        codegen_wait(300)
+    for p in plans:
+        eng.execute(p)
     torch.cuda.synchronize()
     with no_gc():
         t0 = time.perf_counter()
@@ -87,9 +93,11 @@ def batched_measure(eng, n_pipes, seed, reps):
         seq_s = (time.perf_counter() - t0) / reps
     # warm: the batched kernel's autotuner tries its launch shapes (and
     # compiles them) on the first batches
-    for _ in range(6):
+    for k in range(16):
         results, nb = eng.execute_batch(plans)
         assert [r.rows() for r in results] == alone, "batched results differ from running each plan alone"
+        if k >= 1 and codegen_pending() == 0:
+            break
     torch.cuda.synchronize()
     with no_gc():
         t0 = time.perf_counter()
@@ -97,7 +105,6 @@ def batched_measure(eng, n_pipes, seed, reps):
             eng.execute_batch(plans)
         torch.cuda.synchronize()
         bat_s = (time.perf_counter() - t0) / reps
-    from sycldb_amd.engine import codegen_tuning_report
     log("batched kernel tuning:\n" + "\n".join(l for l in codegen_tuning_report().splitlines() if "batched" in l))
     return {"pipelines": n_pipes, "pipeline_seed": seed, "sharing_the_scan": nb,
             "sequential_ms": seq_s * 1e3, "batched_ms": bat_s * 1e3,
@@ -261,7 +268,8 @@ def main():
         log("copied %d bytes three times" % (y.numel() * 4))
         return
     from sycldb_amd import plans as PL
-    from sycldb_amd.engine import Engine, codegen_kernels_compiled, codegen_tuning_report
+    from sycldb_amd.engine import (Engine, codegen_kernels_compiled, codegen_pending, codegen_tuning_report,
+                                   codegen_wait)
 
     rows, builder, fact_cols = WORKLOADS[args.workload]
     rows = args.fact_rows or rows
@@ -276,12 +284,27 @@ def main():
     torch.cuda.synchronize()
     log("generated %d fact rows on the device in %.1fs" % (rows, time.time() - t0))
 
+    # The library compiles a new plan's kernels on a background thread (the
+    # plan runs on the interpreter kernel meanwhile) and then tunes the fact
+    # kernel's launch configuration: executions continue, untimed, until
+    # nothing is pending, whatever --warmup says.  Then the warmup steps.
+    t_tune = time.perf_counter()
+    res = eng.execute(plan_text)
+    tuning_runs = 1
+    while codegen_pending() > 0 and tuning_runs < 64:
+        codegen_wait(300)
+        res = eng.execute(plan_text)
+        tuning_runs += 1
+    torch.cuda.synchronize()
+    tuning_s = time.perf_counter() - t_tune
+    if codegen_pending() > 0:
+        log("warning: the autotuner has not settled after %d executions" % tuning_runs)
     for _ in range(args.warmup):
         res = eng.execute(plan_text)
     groups = len(res.rows())
     tuning = codegen_tuning_report()
     if tuning:
-        log("autotuner:\n" + tuning)
+        log("autotuner (%d executions, %.2f s):\n%s" % (tuning_runs, tuning_s, tuning))
     # HIP events around the fact-table launches only (the dimension
     # bundles run untimed, without the events' host cost)
     eng.set_profiling(True, table="lineorder")
@@ -362,6 +385,7 @@ def main():
                    "device_ms_per_step_median": sorted(dev_ms)[len(dev_ms) // 2], "groups": groups,
                    "fusion": True, "codegen": args.codegen, "unroll": args.unroll,
                    "kernels_compiled": codegen_kernels_compiled(), "tuning": tuning.strip().splitlines(),
+                   "tuning_runs": tuning_runs, "tuning_s": tuning_s,
                    "unfused": unfused},
         "roofline": roofline,
         "cpu_baseline": cpu,

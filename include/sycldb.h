@@ -176,6 +176,73 @@ int sdb_gather_i32(const int32_t *src, const int32_t *row_ids, int64_t count, in
 int sdb_minmax_i32(const int32_t *col, int64_t n, int32_t *min_out, int32_t *max_out,
                    sdb_stream stream);
 
+/* ------------------------------------------------------- fused bundle */
+
+/* Operator kinds of a bundle: KernelType, models/execution.hpp:30-45 (same
+ * values). */
+enum sdb_op_kind {
+    SDB_OP_EMPTY = 0,
+    SDB_OP_LOGICAL,       /* LogicalKernel: out = logical(logic, in, local) */
+    SDB_OP_SELECT_COL,    /* SelectionKernelColumns: out = logical(logic, in, cmp(col, col2)) */
+    SDB_OP_SELECT_LIT,    /* SelectionKernelLiteral: out = logical(logic, in, cmp(col, value)) */
+    SDB_OP_FILL,          /* FillKernel: res = value */
+    SDB_OP_BINOP_CC,      /* PerformOperationKernelColumns: where in: res = col binop col2 */
+    SDB_OP_BINOP_LC,      /* PerformOperationKernelLiteralFirst: where in: res = value binop col */
+    SDB_OP_BINOP_CL,      /* PerformOperationKernelLiteralSecond: where in: res = col binop value */
+    SDB_OP_BUILD_KEYS,    /* BuildKeysHTKernel: ht (uint8, ht_len slots from ht_min)[col] = in */
+    SDB_OP_PROBE_SEMI,    /* FilterJoinKernel: out = in && ht_min <= col <= ht_max && ht[col - ht_min] */
+    SDB_OP_BUILD_KEYVALS, /* BuildKeyValsHTKernel: ht (int32 pairs) from keys col, values col2, flags in */
+    SDB_OP_PROBE_FULL,    /* FullJoinKernel: match -> res = value, out = 1; else out = 0 */
+    SDB_OP_SUM,           /* AggregateOperationKernel: *acc += sum of (uint64)(int64)col where in */
+    SDB_OP_GROUPBY        /* GroupByAggregateKernel: as sdb_groupby_dense, aggregating col where in */
+};
+#define SDB_OP_MAX_KEYS 8
+#define SDB_BUNDLE_MAX_OPS 64
+
+/* One operator of a bundle: the arguments of one KernelData
+ * (models/execution.hpp:30-497).  The first fields are the reference's
+ * SYCLDBContext slot (kernels/jit_kernels.hpp:12-20): col_ptrs -> col,
+ * col_ptrs2 -> col2, res_ptrs -> res, values -> value, ht_ptrs and
+ * ht_int_ptrs -> ht, params[i*8 + 0..5] -> cmp, logic, binop, ht_min, ht_max,
+ * ht_len.  The rest is what the slot lacks: the operator's own flag buffers
+ * (the reference threads one pass flag through the fused prefix instead),
+ * the SUM accumulator and the group-by arguments.  Fields an operator kind
+ * does not use are ignored.  All pointers are device pointers, except that
+ * keys / out_keys hold device pointers in the struct itself. */
+typedef struct sdb_op {
+    int32_t kind;   /* sdb_op_kind */
+    int32_t cmp;    /* sdb_comp_op */
+    int32_t logic;  /* sdb_logical_op (the parent op of a selection) */
+    int32_t binop;  /* sdb_binop */
+    int32_t ht_min, ht_max, ht_len; /* probes: ht_len 0 means ht_max - ht_min + 1;
+                                       builds: ht_len slots from ht_min */
+    int32_t value;  /* literal */
+    const int32_t *col, *col2;
+    int32_t *res;
+    void *ht;
+    const uint8_t *in_flags;
+    uint8_t *out_flags;
+    const uint8_t *local_flags;
+    uint64_t *acc;
+    int32_t nkeys;
+    int32_t mins[SDB_OP_MAX_KEYS], maxs[SDB_OP_MAX_KEYS];
+    const int32_t *keys[SDB_OP_MAX_KEYS];
+    int32_t *out_keys[SDB_OP_MAX_KEYS];
+    uint64_t *sums;
+    uint32_t *seen;
+    uint64_t prod;
+} sdb_op;
+
+/* KernelBundle::execute (models/execution.hpp:528-650) with its fused path
+ * (546-615, switched off in the reference) switched on: the nops operators,
+ * all over rows [0, n) of one segment, run in order on `stream` with the
+ * results of running them one by one -- as one fused kernel when the
+ * operators allow it (every buffer an operator writes is written back),
+ * otherwise one kernel per operator.  The caller owns every buffer; each is
+ * checked against the size its operator implies (n rows, ht_len slots, prod
+ * slots).  n <= 2^31 - 1, nops <= SDB_BUNDLE_MAX_OPS. */
+int sdb_bundle_exec(const sdb_op *ops, int nops, int64_t n, sdb_stream stream);
+
 /* ------------------------------------------- device memory and streams */
 
 /* Conveniences for callers without the HIP headers (an FFI, the C++ layer
@@ -336,6 +403,10 @@ int sdb_engine_set_fusion(sdb_engine *eng, int on);
  * results). */
 enum sdb_codegen { SDB_CODEGEN_AUTO = 0, SDB_CODEGEN_INTERPRET = 1, SDB_CODEGEN_COMPILE = 2 };
 #define SDB_UNROLL_QUEUED 16
+/* with an explicit unroll: blocks of 512 or 1024 threads instead of 256 (the
+ * autotuner's other candidate launch shapes) */
+#define SDB_UNROLL_BLOCK512 256
+#define SDB_UNROLL_BLOCK1024 512
 int sdb_engine_set_codegen(sdb_engine *eng, int mode, int unroll);
 /* 1 when runtime compilation is available, else 0 with the reason in `why`
  * (may be NULL). */
@@ -345,6 +416,17 @@ int sdb_codegen_kernels_compiled(void);
 /* Autotuner state, one text line per program shape; returns the length
  * (the text is truncated to cap - 1 bytes). */
 int sdb_codegen_tuning_report(char *buf, int cap);
+/* Kernels are compiled on a background thread: in SDB_CODEGEN_AUTO mode a
+ * program shape runs on the interpreter kernel until its compiled kernel is
+ * ready, and the autotuner's candidates compile meanwhile (SDB_CODEGEN_COMPILE
+ * waits for them instead).  sdb_codegen_wait waits until every kernel queued
+ * so far is compiled (1) or timeout_s seconds have passed (0).
+ * sdb_codegen_pending counts the program shapes still compiling or, having
+ * run over at least 4M rows, not yet settled on a launch configuration, plus
+ * the batched-kernel libraries still tuning; 0 once every launch seen so far
+ * runs its final configuration. */
+int sdb_codegen_wait(double timeout_s);
+int sdb_codegen_pending(void);
 /* Compiles, without launching (no device needed), the runtime-generated
  * kernels of programs covering every instruction form for GPU architecture
  * `arch` (e.g. "gfx950"); *count = kernels compiled. */

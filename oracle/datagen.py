@@ -13,8 +13,17 @@ def generate_table(table, nrows, seed, spec, order, stream_id, dates, first_row=
                    columns=None):
     """spec(col) -> generator tuple; order: column generation order;
     dates: (datekey, year, yearmonthnum) arrays of the date dimension."""
+    need = None
+    if columns is not None:
+        # the requested columns and the columns they are derived from
+        need = set(columns)
+        for col in reversed(order):
+            if col in need and spec(col)[0] == "child":
+                need.add(spec(col)[1])
     out = {}
     for col in order:
+        if need is not None and col not in need:
+            continue
         kind = spec(col)
         s = stream_id(col)
         if kind[0] == "rowid":

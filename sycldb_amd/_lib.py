@@ -79,6 +79,25 @@ class RequestInfo(ctypes.Structure):
                 ("batch_size", ctypes.c_int)]
 
 
+# sdb_op_kind (KernelType order)
+(OP_EMPTY, OP_LOGICAL, OP_SELECT_COL, OP_SELECT_LIT, OP_FILL, OP_BINOP_CC, OP_BINOP_LC, OP_BINOP_CL,
+ OP_BUILD_KEYS, OP_PROBE_SEMI, OP_BUILD_KEYVALS, OP_PROBE_FULL, OP_SUM, OP_GROUPBY) = range(14)
+OP_MAX_KEYS = 8
+BUNDLE_MAX_OPS = 64
+
+
+class Op(ctypes.Structure):
+    """sdb_op: one operator of a bundle (sdb_bundle_exec)."""
+    _fields_ = [("kind", ctypes.c_int32), ("cmp", ctypes.c_int32), ("logic", ctypes.c_int32),
+                ("binop", ctypes.c_int32), ("ht_min", ctypes.c_int32), ("ht_max", ctypes.c_int32),
+                ("ht_len", ctypes.c_int32), ("value", ctypes.c_int32),
+                ("col", P), ("col2", P), ("res", P), ("ht", P),
+                ("in_flags", P), ("out_flags", P), ("local_flags", P), ("acc", P),
+                ("nkeys", ctypes.c_int32), ("mins", ctypes.c_int32 * OP_MAX_KEYS),
+                ("maxs", ctypes.c_int32 * OP_MAX_KEYS), ("keys", P * OP_MAX_KEYS), ("out_keys", P * OP_MAX_KEYS),
+                ("sums", P), ("seen", P), ("prod", ctypes.c_uint64)]
+
+
 # name: (restype, argtypes)
 SIGNATURES = {
     "sdb_last_error": (CSTR, []),
@@ -102,6 +121,7 @@ SIGNATURES = {
     "sdb_seen_to_flags": (INT, [P, P, U64, P]),
     "sdb_count_flags": (INT, [P, I64, P, INT, P]),
     "sdb_compact": (INT, [P, I64, P, P, P]),
+    "sdb_bundle_exec": (INT, [ctypes.POINTER(Op), INT, I64, P]),
     "sdb_gather_i32": (INT, [P, P, I64, P, P]),
     "sdb_minmax_i32": (INT, [P, I64, P, P, P]),
     "sdb_gen_uniform": (INT, [P, I64, U64, U64, I64, I32, I32, P]),
@@ -139,6 +159,8 @@ SIGNATURES = {
     "sdb_codegen_kernels_compiled": (INT, []),
     "sdb_codegen_tuning_report": (INT, [ctypes.c_char_p, INT]),
     "sdb_codegen_check": (INT, [CSTR, ctypes.POINTER(INT)]),
+    "sdb_codegen_wait": (INT, [ctypes.c_double]),
+    "sdb_codegen_pending": (INT, []),
     "sdb_engine_set_profiling": (INT, [P, INT]),
     "sdb_engine_set_profile_table": (INT, [P, CSTR]),
     "sdb_engine_profile_reset": (INT, [P]),

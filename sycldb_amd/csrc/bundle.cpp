@@ -96,12 +96,17 @@ void KernelBundle::execute(const ExecContext &ctx) const
         if (build_program(ctx, p)) {
             validate_program(p, kernels_.front().n, eng.registry);
             dump_program(p, kernels_.front().n, ctx.table ? *ctx.table : std::string("?"));
+            const int64_t n = kernels_.front().n;
+            const bool jit = eng.codegen == SDB_CODEGEN_COMPILE || (eng.codegen == SDB_CODEGEN_AUTO && jit_available());
+            // the kernel is chosen, compiled and tuned before the profiling
+            // events, which then bracket the program's own launch only
+            FusedLaunch L;
+            if (jit) L = prepare_fused_jit(p, n, ctx.stream, eng.unroll, eng.codegen == SDB_CODEGEN_COMPILE);
             Engine::ProfRec rec;
             const bool timed = eng.profiles(ctx.table);
-            if (timed) eng.prof_begin("fused:" + (ctx.table ? *ctx.table : std::string("?")), kernels_.front().n, rec);
-            const bool jit = eng.codegen == SDB_CODEGEN_COMPILE || (eng.codegen == SDB_CODEGEN_AUTO && jit_available());
-            if (jit) launch_fused_jit(p, kernels_.front().n, ctx.stream, eng.unroll);
-            else launch_fused(p, kernels_.front().n, ctx.stream);
+            if (timed) eng.prof_begin("fused:" + (ctx.table ? *ctx.table : std::string("?")), n, rec);
+            if (jit) run_fused_jit(L, p, n, ctx.stream);
+            else launch_fused(p, n, ctx.stream);
             if (timed) eng.prof_end(rec);
             eng.launches++;
             return;

@@ -539,9 +539,11 @@ int sdb_engine_set_codegen(sdb_engine *e, int mode, int unroll)
     return guarded([&] {
         require(mode == SDB_CODEGEN_AUTO || mode == SDB_CODEGEN_INTERPRET || mode == SDB_CODEGEN_COMPILE,
                 "unknown codegen mode");
-        const int u = unroll & ~SDB_UNROLL_QUEUED;
-        require(unroll == 0 || u == 1 || u == 2 || u == 4,
-                "unroll must be 0, or 1, 2 or 4 (optionally | SDB_UNROLL_QUEUED)");
+        const int u = unroll & ~(SDB_UNROLL_QUEUED | SDB_UNROLL_BLOCK512 | SDB_UNROLL_BLOCK1024);
+        require(unroll == 0 || ((u == 1 || u == 2 || u == 4) &&
+                                !((unroll & SDB_UNROLL_BLOCK512) && (unroll & SDB_UNROLL_BLOCK1024))),
+                "unroll must be 0, or 1, 2 or 4 (optionally | SDB_UNROLL_QUEUED and | one of "
+                "SDB_UNROLL_BLOCK512, SDB_UNROLL_BLOCK1024)");
         require(e, "null engine");
         if (mode == SDB_CODEGEN_COMPILE && !jit_available())
             fail(SDB_ERR_UNSUPPORTED, "runtime compilation unavailable: " + jit_unavailable_reason());
@@ -568,6 +570,10 @@ int sdb_codegen_tuning_report(char *buf, int cap)
     if (buf && cap > 0) snprintf(buf, (size_t)cap, "%s", r.c_str());
     return (int)r.size();
 }
+
+int sdb_codegen_wait(double timeout_s) { return jit_wait_idle(timeout_s) ? 1 : 0; }
+
+int sdb_codegen_pending(void) { return jit_pending(); }
 
 int sdb_codegen_check(const char *arch, int *count)
 {

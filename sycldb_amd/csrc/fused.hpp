@@ -4,6 +4,7 @@
 #pragma once
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "fused_types.hpp"
@@ -22,11 +23,37 @@ constexpr int F_DEFAULT_UNROLL = 2;
 // library cannot be loaded (the reason is in jit_unavailable_reason()).
 bool jit_available();
 std::string jit_unavailable_reason();
-// Launches the program through a kernel compiled for its shape (compiled on
-// first use and cached per device); `unroll` = 4-row chunks per thread per
-// iteration (1, 2 or 4), or 0 to let the autotuner choose the launch
-// configuration per program shape.
+// How one fused launch runs: on the interpreter kernel (while the shape's
+// compiled kernels are still being built), on an explicitly configured
+// compiled kernel, on the autotuner's current choice, or as one per-launch
+// tuning trial.
+struct FusedLaunch {
+    enum Mode { INTERPRET, EXPLICIT, TUNED, TRIAL } mode = INTERPRET;
+    int cand = 0;            // TUNED / TRIAL: autotuner candidate
+    int unroll = 0, block = 256; // EXPLICIT
+    bool queue = false;      // queued row tail
+    std::pair<int, std::string> key; // TRIAL: the tuning entry it reports to
+};
+// Decides how the program's next launch over n rows runs, doing first
+// whatever that needs: queuing the shape's kernels for the background
+// compiler, loading compiled kernels, and the autotuner's one-launch trials
+// (on a scratch copy of the program's accumulators, ordered on `s` after the
+// work that produces the program's inputs).  `unroll` = 4-row chunks per
+// thread per iteration (1, 2 or 4, optionally | SDB_UNROLL_QUEUED: compiled
+// now if needed), or 0 for the autotuner.  wait = true never returns
+// INTERPRET: it waits for the background compiler instead.
+FusedLaunch prepare_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll, bool wait);
+// Launches the program as prepared (a TRIAL also times itself and reports).
+void run_fused_jit(const FusedLaunch &L, const FProgram &p, int64_t n, hipStream_t s);
+// prepare_fused_jit (without waiting) followed by run_fused_jit.
 void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll);
+// Waits until the background compiler has built every kernel queued so far
+// (true), or until timeout_s seconds have passed (false).
+bool jit_wait_idle(double timeout_s);
+// Program shapes whose kernels are still compiling, or whose launches over
+// at least the tuning size have not yet settled on a configuration, plus
+// batched-kernel libraries still tuning.
+int jit_pending();
 // One line per tuned program shape: the chosen configuration and the best
 // measured ns/row of every candidate.
 std::string jit_tuning_report();

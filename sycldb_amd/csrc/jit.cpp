@@ -21,13 +21,18 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <unordered_map>
 #include <mutex>
 #include <sstream>
+#include <thread>
 #include <vector>
 
 #include "capi_util.hpp"
@@ -1093,7 +1098,7 @@ std::string device_arch()
 void dump_source(const std::string &src)
 {
     static const char *dir = getenv("SDB_JIT_DUMP_DIR");
-    static int seq = 0;
+    static std::atomic<int> seq{0};
     if (!dir || !*dir) return;
     const std::string path = std::string(dir) + "/sdb_fused_" + std::to_string(seq++) + ".hip";
     if (FILE *f = fopen(path.c_str(), "w")) {
@@ -1321,6 +1326,59 @@ Compiled &get_kernel(const std::string &src, const char *name = "sdb_fused_progr
     return g_loaded.emplace(std::make_pair(dev, key), c).first->second;
 }
 
+// Compiles every source of `srcs` for `arch`, several at a time on their own
+// threads (hipRTC programs are independent); the code objects in order.
+std::vector<std::vector<char>> compile_all(const std::vector<std::string> &srcs, const std::string &arch)
+{
+    std::vector<std::vector<char>> code(srcs.size());
+    std::vector<std::string> err(srcs.size());
+    std::atomic<size_t> next{0};
+    auto work = [&] {
+        for (size_t i; (i = next.fetch_add(1)) < srcs.size();) {
+            try {
+                code[i] = compile_source(srcs[i], arch);
+            } catch (const std::exception &e) {
+                err[i] = e.what();
+            }
+        }
+    };
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nthreads = std::min<size_t>(srcs.size(), std::min(hw, 16u));
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nthreads; t++) pool.emplace_back(work);
+    work();
+    for (std::thread &t : pool) t.join();
+    for (const std::string &e : err)
+        if (!e.empty()) fail(SDB_ERR_RUNTIME, e);
+    return code;
+}
+
+// Compiles for `arch` the sources of `srcs` that have no code object yet, in
+// parallel, and records their code objects, so that get_kernel then only
+// loads them.  A new program shape's autotuner candidates are compiled this
+// way at once, instead of one candidate per launch.  Returns how many were
+// compiled.
+int precompile(const std::vector<std::string> &srcs, const std::string &arch)
+{
+    std::vector<std::string> todo;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        for (const std::string &s : srcs)
+            if (!g_code.count(arch + "\n" + s) && std::find(todo.begin(), todo.end(), s) == todo.end())
+                todo.push_back(s);
+    }
+    if (todo.empty()) return 0;
+    std::vector<std::vector<char>> code = compile_all(todo, arch);
+    std::lock_guard<std::mutex> lk(g_mu);
+    int n = 0;
+    for (size_t i = 0; i < todo.size(); i++)
+        if (g_code.emplace(arch + "\n" + todo[i], std::move(code[i])).second) {
+            g_compiled++;
+            n++;
+        }
+    return n;
+}
+
 } // namespace
 
 bool jit_available() { return rtc().ok; }
@@ -1329,12 +1387,9 @@ std::string jit_unavailable_reason() { return rtc().ok ? std::string() : rtc().w
 
 int jit_check(const std::string &arch)
 {
-    int n = 0;
-    for (auto &pu : coverage_programs()) {
-        compile_source(jit_source(pu.first, pu.second), arch);
-        n++;
-    }
-    return n + jit_batch_check(arch);
+    std::vector<std::string> srcs;
+    for (auto &pu : coverage_programs()) srcs.push_back(jit_source(pu.first, pu.second));
+    return (int)compile_all(srcs, arch).size() + jit_batch_check(arch);
 }
 
 int jit_kernels_compiled()
@@ -1524,20 +1579,37 @@ void launch_compiled(const FProgram &p, int64_t n, hipStream_t s, int unroll, in
 
 // ------------------------------------------------------------- autotuner
 //
-// Per program shape (and device), the first launches over at least
-// TUNE_MIN_ROWS rows each try one launch configuration -- chunks per thread
-// per iteration x threads per block x blocks per CU -- timed with HIP
-// events; after TUNE_ROUNDS timings of every candidate the fastest per row is
-// used from then on.  Every configuration computes the same result; only the
-// time differs.  Larger blocks share one copy of the program's LDS among
-// more waves, so a program whose staged tables limit the blocks per CU runs
-// more waves per CU.
+// Per program shape (and device) the launch configuration -- 4-row chunks
+// per thread per iteration x threads per block, then with or without the
+// queued row tail -- is chosen by timing the candidates with HIP events over
+// launches of at least TUNE_MIN_ROWS rows.  Every configuration computes the
+// same result; only the time differs.  Larger blocks share one copy of the
+// program's LDS among more waves, so a program whose staged tables limit the
+// blocks per CU runs more waves per CU.
+//
+// No launch of the default mode waits for a compilation.  The first time a
+// shape is seen its kernels are handed to a background compiler thread
+// (hipRTC compiles one program at a time per process, about a quarter of a
+// second each), and until the default configuration's kernel is ready the
+// program runs on the compiled-in interpreter kernel (fused.hip), which
+// computes the same rows.  A launch under TUNE_MIN_ROWS needs only the
+// default configuration; a larger one queues every candidate, and after
+// their trials the fastest one's queued-tail variant.
+//
+// A program whose only outputs are its terminal accumulators (SUM, GROUP BY:
+// every fact-table bundle that ends in an aggregate) runs each phase's
+// trials all at once, in one launch, on a copy of the program whose
+// accumulators point into a scratch sink kept per device: its own result is
+// then computed once, by the kernel chosen so far.  Any other program tries
+// one candidate per launch on its real outputs.
 constexpr int64_t TUNE_MIN_ROWS = 4 << 20;
-constexpr int TUNE_ROUNDS = 1;
+// timings per candidate in a one-launch tuning (the fastest counts)
+constexpr int TUNE_ROUNDS = 2;
 
 struct TuneCfg {
     int unroll, per_cu_mult, block;
 };
+// the first is the default configuration (F_DEFAULT_UNROLL, JIT_BLOCK)
 const TuneCfg TUNE_CANDIDATES[] = {{2, 1, 256}, {1, 1, 256}, {4, 1, 256}, {2, 1, 512}, {1, 1, 512}, {2, 1, 1024},
                                    {4, 1, 512}, {4, 1, 1024}, {1, 1, 1024}};
 constexpr int N_CAND = (int)(sizeof(TUNE_CANDIDATES) / sizeof(TUNE_CANDIDATES[0]));
@@ -1555,6 +1627,14 @@ struct Tune {
     int chosen = -1;
     int base = -1;       // fastest candidate, while its queued trial is pending
     bool chosen_queue = false;
+    // background compilation of the shape's kernels
+    bool queued_default = false, queued_all = false, queued_tail = false;
+    bool default_ready = false, tail_ready = false;
+    int cands_ready = 0;  // of the N_CAND - 1 other candidates
+    std::string error;    // a failed compilation: the interpreter runs the shape
+    bool tuning = false;  // a thread is running one-launch trials
+    int tuned_launches = 0; // launches that ran trials
+    double tune_ms = 0;   // host time of the one-launch trials
     Tune()
     {
         for (double &b : best_ns_per_row) b = 1e300;
@@ -1562,69 +1642,400 @@ struct Tune {
 };
 
 std::mutex g_tune_mu;
+std::condition_variable g_tune_cv; // a background compilation finished
 std::map<std::pair<int, std::string>, Tune> g_tune;
+
+// One kernel for the background compiler: the program (for its source), the
+// launch shape, and the tuning entry it readies.
+enum JobKind { JOB_DEFAULT, JOB_CANDIDATE, JOB_TAIL };
+struct CompileJob {
+    FProgram p;
+    int unroll, block;
+    bool queue;
+    std::string arch;
+    std::pair<int, std::string> key;
+    JobKind kind;
+};
+
+// A thread that compiles queued kernels one at a time (the default
+// configurations first).  It only generates source and runs hipRTC, never
+// the HIP runtime; the launching thread loads the code objects.  Destroyed
+// (queue dropped, the compilation in progress finished) at process exit,
+// before the code cache and the hipRTC library it uses.
+class BackgroundCompiler {
+public:
+    static BackgroundCompiler &get()
+    {
+        static BackgroundCompiler c;
+        return c;
+    }
+    void submit(std::vector<CompileJob> jobs)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (CompileJob &j : jobs) {
+                if (j.kind == JOB_DEFAULT) q_.push_front(std::move(j));
+                else q_.push_back(std::move(j));
+            }
+        }
+        cv_.notify_all();
+    }
+    ~BackgroundCompiler()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+            q_.clear();
+        }
+        cv_.notify_all();
+        if (th_.joinable()) th_.join();
+    }
+
+private:
+    BackgroundCompiler()
+    {
+        rtc(); // the hipRTC library is loaded (and its exit handlers set) before this object's
+        th_ = std::thread([this] { run(); });
+    }
+    void run()
+    {
+        for (;;) {
+            CompileJob j;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || !q_.empty(); });
+                if (stop_) return;
+                j = std::move(q_.front());
+                q_.pop_front();
+            }
+            std::string err;
+            try {
+                const std::string src = jit_source(j.p, j.unroll, j.block, j.queue);
+                const std::string key = j.arch + "\n" + src;
+                bool have;
+                {
+                    std::lock_guard<std::mutex> lk(g_mu);
+                    have = g_code.count(key) != 0;
+                }
+                if (!have) {
+                    std::vector<char> code = compile_source(src, j.arch);
+                    std::lock_guard<std::mutex> lk(g_mu);
+                    if (g_code.emplace(key, std::move(code)).second) g_compiled++;
+                }
+            } catch (const std::exception &e) {
+                err = e.what();
+            }
+            {
+                std::lock_guard<std::mutex> lk(g_tune_mu);
+                Tune &t = g_tune[j.key];
+                if (!err.empty() && t.error.empty()) t.error = err;
+                if (j.kind == JOB_DEFAULT) t.default_ready = true;
+                else if (j.kind == JOB_CANDIDATE) t.cands_ready++;
+                else t.tail_ready = true;
+            }
+            g_tune_cv.notify_all();
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<CompileJob> q_;
+    bool stop_ = false;
+    std::thread th_;
+};
+
+// Scratch sink of the one-launch trials: per device, grown on demand and
+// kept; g_sink_mu also serialises the trials of all threads.
+std::mutex g_sink_mu;
+std::map<int, std::pair<void *, size_t>> g_sink;
+constexpr size_t SINK_MAX = size_t(256) << 20;
+
+// Bytes of sink a program's accumulators need, or 0 when the program has
+// another output (a stored flag or value column, a built table) or none.
+size_t sink_bytes(const FProgram &p)
+{
+    for (int j = 0; j < F_MAX_FLAGS; j++)
+        if (p.flag_store[j]) return 0;
+    bool terminal = false;
+    for (int i = 0; i < p.ninstr; i++) {
+        const FInstr &I = p.ins[i];
+        if (I.store || I.kind == FK_BUILD_KEYS || I.kind == FK_BUILD_KEYVALS || I.kind == FK_FILL) return 0;
+        terminal |= I.kind == FK_SUM || I.kind == FK_GROUPBY;
+    }
+    if (!terminal) return 0;
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    size_t bytes = 256; // the SUM accumulator
+    if (p.gb.k > 0) {
+        if (p.gb.prod == 0 || p.gb.prod > (uint64_t)SINK_MAX) return 0;
+        bytes += al(p.gb.prod * 8) + al(p.gb.prod * 4) + (size_t)p.gb.k * al(p.gb.prod * 4);
+    }
+    return bytes <= SINK_MAX ? bytes : 0;
+}
+
+// The program with its accumulators moved into `sink` (sink_bytes(p) bytes).
+FProgram sink_program(const FProgram &p, uint8_t *sink)
+{
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    FProgram q = p;
+    size_t off = 0;
+    if (q.sum_acc) q.sum_acc = reinterpret_cast<uint64_t *>(sink);
+    off += 256;
+    if (q.gb.k > 0) {
+        q.gb.sums = reinterpret_cast<uint64_t *>(sink + off);
+        off += al(q.gb.prod * 8);
+        q.gb.seen = reinterpret_cast<uint32_t *>(sink + off);
+        off += al(q.gb.prod * 4);
+        for (int j = 0; j < q.gb.k; j++) {
+            if (q.gb.out_keys[j]) q.gb.out_keys[j] = reinterpret_cast<int32_t *>(sink + off);
+            off += al(q.gb.prod * 4);
+        }
+    }
+    return q;
+}
+
+// Times the candidates `cands` (with the queued tail when `queue`) on the
+// sink copy of p, TUNE_ROUNDS launches each, queued back to back between
+// consecutive events and read after one synchronisation; the best ns/row of
+// each candidate.  Caller holds g_sink_mu.
+std::vector<double> time_in_sink(const FProgram &p, int64_t n, hipStream_t s, const std::vector<int> &cands, bool queue)
+{
+    int dev = 0;
+    SDB_HIP(hipGetDevice(&dev));
+    const size_t need = sink_bytes(p);
+    auto &sk = g_sink[dev];
+    if (sk.second < need) {
+        if (sk.first) SDB_HIP(hipFree(sk.first));
+        sk.first = nullptr;
+        sk.second = 0;
+        SDB_HIP(hipMalloc(&sk.first, need));
+        sk.second = need;
+    }
+    const FProgram q = sink_program(p, static_cast<uint8_t *>(sk.first));
+    for (int c : cands) { // load the modules before the first event
+        uint32_t lds = 0;
+        pick_kernel(q, n, TUNE_CANDIDATES[c].unroll, TUNE_CANDIDATES[c].block, queue, lds);
+    }
+    const int nl = (int)cands.size() * TUNE_ROUNDS;
+    std::vector<hipEvent_t> ev(nl + 1, nullptr);
+    hipError_t e = hipSuccess;
+    for (auto &x : ev)
+        if (e == hipSuccess) e = hipEventCreate(&x);
+    if (e == hipSuccess) e = hipEventRecord(ev[0], s);
+    for (int r = 0; r < TUNE_ROUNDS && e == hipSuccess; r++)
+        for (size_t i = 0; i < cands.size() && e == hipSuccess; i++) {
+            const TuneCfg &c = TUNE_CANDIDATES[cands[i]];
+            launch_compiled(q, n, s, c.unroll, c.per_cu_mult, c.block, queue);
+            e = hipEventRecord(ev[r * cands.size() + i + 1], s);
+        }
+    if (e == hipSuccess) e = hipEventSynchronize(ev[nl]);
+    std::vector<double> best(cands.size(), 1e300);
+    for (int l = 0; l < nl && e == hipSuccess; l++) {
+        float ms = 0;
+        e = hipEventElapsedTime(&ms, ev[l], ev[l + 1]);
+        best[l % cands.size()] = std::min(best[l % cands.size()], ms * 1e6 / (double)n);
+    }
+    for (auto &x : ev)
+        if (x) (void)hipEventDestroy(x);
+    hip_check(e, "fused kernel tuning");
+    return best;
+}
+
+std::vector<CompileJob> jobs_for(const FProgram &p, const std::pair<int, std::string> &key, const std::string &arch,
+                                 JobKind kind, int cand = 0)
+{
+    std::vector<CompileJob> out;
+    auto add = [&](int c, bool queue, JobKind k) {
+        out.push_back(CompileJob{p, TUNE_CANDIDATES[c].unroll, TUNE_CANDIDATES[c].block, queue, arch, key, k});
+    };
+    if (kind == JOB_DEFAULT) add(0, false, kind);
+    else if (kind == JOB_CANDIDATE)
+        for (int c = 1; c < N_CAND; c++) add(c, false, kind);
+    else add(cand, true, kind);
+    return out;
+}
 
 } // namespace
 
-void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll)
+FusedLaunch prepare_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll, bool wait)
 {
-    if (n <= 0) return;
+    FusedLaunch L;
+    if (n <= 0) return L;
     if (p.magic != F_MAGIC) fail(SDB_ERR_RUNTIME, "fused program without magic");
     if (unroll != 0) {
-        launch_compiled(p, n, s, unroll & ~SDB_UNROLL_QUEUED, 1, JIT_BLOCK, (unroll & SDB_UNROLL_QUEUED) != 0);
-        return;
-    }
-    if (n < TUNE_MIN_ROWS) {
-        launch_compiled(p, n, s, F_DEFAULT_UNROLL, 1);
-        return;
+        // an explicit configuration: compiled now if it is not yet
+        L.mode = FusedLaunch::EXPLICIT;
+        L.unroll = unroll & 7;
+        L.block = (unroll & SDB_UNROLL_BLOCK1024) ? 1024 : (unroll & SDB_UNROLL_BLOCK512) ? 512 : JIT_BLOCK;
+        L.queue = (unroll & SDB_UNROLL_QUEUED) != 0;
+        uint32_t lds = 0;
+        pick_kernel(p, n, L.unroll, L.block, L.queue, lds);
+        return L;
     }
     int dev = 0;
     SDB_HIP(hipGetDevice(&dev));
     const std::pair<int, std::string> key{dev, shape_key(p)};
-    int cand;
-    bool queue = false;
-    {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
+    const bool big = n >= TUNE_MIN_ROWS;
+    const bool sinkable = big && sink_bytes(p) > 0;
+    std::unique_lock<std::mutex> lk(g_tune_mu);
+    for (;;) {
         Tune &t = g_tune[key];
-        if (t.chosen >= 0) {
-            cand = -1 - t.chosen;
-            queue = t.chosen_queue;
-        } else if (t.base >= 0) {
-            cand = t.base;
-            queue = true;
-        } else {
-            cand = t.trials % N_CAND;
+        std::vector<CompileJob> jobs;
+        if (!t.queued_default) {
+            t.queued_default = true;
+            jobs = jobs_for(p, key, device_arch(), JOB_DEFAULT);
         }
-    }
-    if (cand < 0) {
-        const TuneCfg &c = TUNE_CANDIDATES[-1 - cand];
-        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, c.block, queue);
-        return;
-    }
-    const TuneCfg &c = TUNE_CANDIDATES[cand];
-    uint32_t lds = 0;
-    bool queued = false;
-    pick_kernel(p, n, c.unroll, c.block, queue, lds, &queued); // compile outside the timed region
-    if (queue && !queued) {
-        // no queue for this program (no row tail, or its LDS does not fit)
-        {
-            std::lock_guard<std::mutex> lk(g_tune_mu);
-            Tune &t = g_tune[key];
-            if (t.chosen < 0 && t.base == cand) {
-                t.chosen = cand;
-                t.base = -1;
+        if (big && !t.queued_all) {
+            t.queued_all = true;
+            for (CompileJob &j : jobs_for(p, key, device_arch(), JOB_CANDIDATE)) jobs.push_back(std::move(j));
+        }
+        if (big && t.base >= 0 && !t.queued_tail && t.chosen < 0) {
+            t.queued_tail = true;
+            if (queue_enabled())
+                for (CompileJob &j : jobs_for(p, key, device_arch(), JOB_TAIL, t.base)) jobs.push_back(std::move(j));
+            else
+                t.tail_ready = true;
+        }
+        if (!jobs.empty()) {
+            lk.unlock();
+            BackgroundCompiler::get().submit(std::move(jobs));
+            lk.lock();
+            continue; // the state may have moved meanwhile
+        }
+        if (!t.error.empty()) {
+            if (wait) fail(SDB_ERR_RUNTIME, t.error);
+            L.mode = FusedLaunch::INTERPRET;
+            return L;
+        }
+        if (big && t.chosen >= 0) {
+            L.mode = FusedLaunch::TUNED;
+            L.cand = t.chosen;
+            L.queue = t.chosen_queue;
+            return L;
+        }
+        const bool cands_ready = t.default_ready && t.cands_ready >= N_CAND - 1;
+        if (big && cands_ready && t.base < 0 && !t.tuning) {
+            if (sinkable) {
+                // every candidate, now, in the sink
+                t.tuning = true;
+                lk.unlock();
+                std::vector<int> all;
+                for (int c = 0; c < N_CAND; c++) all.push_back(c);
+                std::vector<double> ns;
+                const auto w0 = std::chrono::steady_clock::now();
+                try {
+                    std::lock_guard<std::mutex> sl(g_sink_mu);
+                    ns = time_in_sink(p, n, s, all, false);
+                } catch (...) {
+                    lk.lock();
+                    g_tune[key].tuning = false;
+                    throw;
+                }
+                const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+                lk.lock();
+                Tune &u = g_tune[key];
+                u.tuning = false;
+                u.tune_ms += ms;
+                u.tuned_launches++;
+                int best = 0;
+                for (int c = 0; c < N_CAND; c++) {
+                    u.best_ns_per_row[c] = std::min(u.best_ns_per_row[c], ns[c]);
+                    if (u.best_ns_per_row[c] < u.best_ns_per_row[best]) best = c;
+                }
+                u.trials = N_CAND;
+                u.base = best;
+                continue; // queue its tail variant
             }
+            // one candidate per launch, on the real outputs
+            L.mode = FusedLaunch::TRIAL;
+            L.cand = t.trials % N_CAND;
+            L.key = key;
+            return L;
         }
-        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, c.block);
+        if (big && t.base >= 0 && t.tail_ready && !t.tuning) {
+            uint32_t lds = 0;
+            bool queued = false;
+            {
+                const TuneCfg c = TUNE_CANDIDATES[t.base];
+                lk.unlock();
+                pick_kernel(p, n, c.unroll, c.block, true, lds, &queued); // compiled: loads it
+                lk.lock();
+            }
+            Tune &u = g_tune[key];
+            if (u.chosen >= 0 || u.tuning) continue;
+            if (!queued) {
+                // no queue for this program (no row tail, or its LDS does not fit)
+                u.chosen = u.base;
+                u.base = -1;
+                continue;
+            }
+            if (sinkable) {
+                u.tuning = true;
+                const int base = u.base;
+                lk.unlock();
+                std::vector<double> ns;
+                const auto w0 = std::chrono::steady_clock::now();
+                try {
+                    std::lock_guard<std::mutex> sl(g_sink_mu);
+                    ns = time_in_sink(p, n, s, {base}, true);
+                } catch (...) {
+                    lk.lock();
+                    g_tune[key].tuning = false;
+                    throw;
+                }
+                const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+                lk.lock();
+                Tune &v = g_tune[key];
+                v.tuning = false;
+                v.tune_ms += ms;
+                v.tuned_launches++;
+                v.queue_ns_per_row = ns[0];
+                v.chosen_queue = ns[0] < v.best_ns_per_row[base];
+                v.chosen = base;
+                v.base = -1;
+                continue;
+            }
+            L.mode = FusedLaunch::TRIAL;
+            L.cand = u.base;
+            L.queue = true;
+            L.key = key;
+            return L;
+        }
+        if (wait && !(t.default_ready && (!big || t.tuning))) {
+            // compiled kernels only: wait for the background compiler
+            g_tune_cv.wait(lk);
+            continue;
+        }
+        if (t.default_ready) {
+            L.mode = FusedLaunch::TUNED;
+            L.cand = 0;
+            return L;
+        }
+        L.mode = FusedLaunch::INTERPRET;
+        return L;
+    }
+}
+
+void run_fused_jit(const FusedLaunch &L, const FProgram &p, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return;
+    switch (L.mode) {
+    case FusedLaunch::INTERPRET: launch_fused(p, n, s); return;
+    case FusedLaunch::EXPLICIT: launch_compiled(p, n, s, L.unroll, 1, L.block, L.queue); return;
+    case FusedLaunch::TUNED: {
+        const TuneCfg &c = TUNE_CANDIDATES[L.cand];
+        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, c.block, L.queue);
         return;
     }
+    case FusedLaunch::TRIAL: break;
+    }
+    // a trial on the program's real outputs, timed with HIP events
+    const TuneCfg &c = TUNE_CANDIDATES[L.cand];
     hipEvent_t a, b;
     SDB_HIP(hipEventCreate(&a));
     SDB_HIP(hipEventCreate(&b));
     float ms = 0;
     hipError_t e = hipEventRecord(a, s);
     if (e == hipSuccess) {
-        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, c.block, queue);
+        launch_compiled(p, n, s, c.unroll, c.per_cu_mult, c.block, L.queue);
         e = hipEventRecord(b, s);
     }
     if (e == hipSuccess) e = hipEventSynchronize(b);
@@ -1633,24 +2044,48 @@ void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll)
     (void)hipEventDestroy(b);
     hip_check(e, "fused kernel tuning");
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    Tune &t = g_tune[key];
+    Tune &t = g_tune[L.key];
     if (t.chosen >= 0) return;
     const double ns = ms * 1e6 / (double)n;
-    if (queue) {
-        if (t.base != cand) return;
+    t.tuned_launches++;
+    if (L.queue) {
+        if (t.base != L.cand) return;
         t.queue_ns_per_row = ns;
-        t.chosen_queue = ns < t.best_ns_per_row[cand];
-        t.chosen = cand;
+        t.chosen_queue = ns < t.best_ns_per_row[L.cand];
+        t.chosen = L.cand;
         t.base = -1;
         return;
     }
     if (t.base >= 0) return;
-    t.best_ns_per_row[cand] = std::min(t.best_ns_per_row[cand], ns);
-    if (++t.trials >= N_CAND * TUNE_ROUNDS) {
+    t.best_ns_per_row[L.cand] = std::min(t.best_ns_per_row[L.cand], ns);
+    if (++t.trials >= N_CAND) {
         int best = 0;
         for (int i = 1; i < N_CAND; i++)
             if (t.best_ns_per_row[i] < t.best_ns_per_row[best]) best = i;
-        t.base = best; // next: its queued trial, when its kernel has a queue
+        t.base = best; // next: its queued trial, once compiled
+    }
+}
+
+void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll)
+{
+    run_fused_jit(prepare_fused_jit(p, n, s, unroll, false), p, n, s);
+}
+
+bool jit_wait_idle(double timeout_s)
+{
+    const auto until = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s);
+    std::unique_lock<std::mutex> lk(g_tune_mu);
+    for (;;) {
+        bool busy = false;
+        for (auto &kv : g_tune) {
+            const Tune &t = kv.second;
+            if (!t.error.empty()) continue;
+            busy |= t.queued_default && !t.default_ready;
+            busy |= t.queued_all && t.cands_ready < N_CAND - 1;
+            busy |= t.queued_tail && !t.tail_ready;
+        }
+        if (!busy) return true;
+        if (g_tune_cv.wait_until(lk, until) == std::cv_status::timeout) return false;
     }
 }
 
@@ -2490,6 +2925,21 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     }
 }
 
+int jit_pending()
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    int n = 0;
+    for (auto &kv : g_tune) {
+        const Tune &t = kv.second;
+        if (!t.error.empty()) continue;
+        const bool compiling = (t.queued_default && !t.default_ready) || (t.queued_all && t.cands_ready < N_CAND - 1) ||
+                               (t.queued_tail && !t.tail_ready);
+        n += compiling || (t.queued_all && t.chosen < 0);
+    }
+    for (auto &kv : g_batch_tune) n += kv.second.chosen < 0;
+    return n;
+}
+
 std::string jit_tuning_report()
 {
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -2513,13 +2963,19 @@ std::string jit_tuning_report()
     int shape = 0;
     for (auto &kv : g_tune) {
         const Tune &t = kv.second;
+        if (!t.queued_all) continue; // never launched over the tuning size
         o << "shape " << shape++ << " device " << kv.first.first << ": ";
-        if (t.chosen >= 0)
+        if (!t.error.empty())
+            o << "compilation failed, interpreted";
+        else if (t.chosen >= 0)
             o << "unroll " << TUNE_CANDIDATES[t.chosen].unroll << ", block " << TUNE_CANDIDATES[t.chosen].block
               << ", blocks/CU x" << TUNE_CANDIDATES[t.chosen].per_cu_mult
               << (t.chosen_queue ? ", queued tail" : "");
+        else if (!t.default_ready || t.cands_ready < N_CAND - 1 || (t.queued_tail && !t.tail_ready))
+            o << "compiling (" << (t.default_ready ? 1 : 0) + t.cands_ready + (t.tail_ready ? 1 : 0) << " kernels ready)";
         else
             o << "tuning (" << t.trials << " trials)";
+        o << "; " << t.tuned_launches << " tuning launches, " << t.tune_ms << " ms in one-launch trials";
         o << "; ns/row";
         for (int i = 0; i < N_CAND; i++) {
             o << " [u" << TUNE_CANDIDATES[i].unroll << " b" << TUNE_CANDIDATES[i].block << " x"

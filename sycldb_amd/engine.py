@@ -111,6 +111,19 @@ def codegen_kernels_compiled():
     return int(L.lib.sdb_codegen_kernels_compiled())
 
 
+def codegen_wait(timeout_s=600.0):
+    """Waits until every fused kernel queued for the background compiler so
+    far is compiled; False if timeout_s passed first."""
+    return L.lib.sdb_codegen_wait(float(timeout_s)) == 1
+
+
+def codegen_pending():
+    """Program shapes still compiling or still tuning (plus batched-kernel
+    libraries still tuning); 0 once every launch seen runs its final
+    configuration."""
+    return int(L.lib.sdb_codegen_pending())
+
+
 def codegen_tuning_report():
     """The autotuner's choice per program shape (text, one line each)."""
     n = L.lib.sdb_codegen_tuning_report(None, 0)

@@ -254,3 +254,11 @@ def count_true_flags(flags, length, stream=None):
     out = torch.zeros(1, dtype=torch.int64, device=flags.device)
     count_flags(flags, out, accumulate=False, n=length, stream=stream)
     return int(out.item())
+
+
+def bundle_exec(ops, n, stream=None):
+    """KernelBundle::execute with fusion (models/execution.hpp:528-650):
+    `ops` is a list of sycldb_amd._lib.Op (sdb_op) run in order over n rows,
+    as one fused kernel when they allow it."""
+    arr = (L.Op * max(len(ops), 1))(*ops)
+    L.check(L.lib.sdb_bundle_exec(arr, len(ops), int(n), _stream(stream)))

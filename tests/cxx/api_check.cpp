@@ -3,7 +3,10 @@
 //   api_check host    -- host-only checks (op parsing, exception mapping)
 //   api_check device  -- a small pipeline on the GPU against values computed
 //                        here: selection, semi join, full join, projection,
-//                        sum, group-by, count
+//                        sum, group-by, count; the allocating filter_join,
+//                        full_join(TableData) and group_by_aggregate
+//                        (ColumnData) overloads, count_true_flags_async and
+//                        the fused bundle entry
 // Prints "ok" and exits 0 on success.
 #include <cstdio>
 #include <cstring>
@@ -134,6 +137,76 @@ static int device_checks()
         threw = true;
     }
     REQUIRE(threw);
+    REQUIRE(count_true_flags_async(d_flags, n, q, {}).get() == want_cnt);
+
+    // the allocating overloads (kernels/join.hpp:210-254, 357-461,
+    // kernels/aggregation.hpp:411-539) with a device allocator
+    {
+        DeviceAllocator alloc;
+        bool *f2 = reinterpret_cast<bool *>(dev(one));
+        filter_join(d_dkey, d_dflags, dim, dim, 1, d_key, f2, n, nullptr, alloc, q, {}).wait();
+        const std::vector<unsigned char> hf2 = host(reinterpret_cast<unsigned char *>(f2), n);
+        for (int i = 0; i < n; i++) REQUIRE(hf2[i] == dflags[key[i] - 1]);
+
+        // probe table: one column (the foreign key), build table: key and value
+        int *pkey = dev(key);
+        ColumnData<int> pcols[1] = {{pkey, false, false, 1, dim}};
+        ColumnData<int> bcols[2] = {{d_dkey, false, false, 1, dim}, {d_dval, false, false, 0, 4}};
+        TableData<int> probe{pcols, 1, n, 1, f2, -1, nullptr, 0, 0, "lineorder", {{7, 0}}};
+        TableData<int> build{bcols, 2, dim, 2, d_dflags, 1, nullptr, 0, 0, "dim", {{0, 0}, {1, 1}}};
+        full_join(probe, build, 7, 0, alloc, q, {}).wait();
+        REQUIRE(probe.column_indices.count(7) == 0 && probe.column_indices.at(1 + 1) == 0);
+        REQUIRE(pcols[0].min_value == 0 && pcols[0].max_value == 4);
+        const std::vector<int> joined = host(pkey, n);
+        for (int i = 0; i < n; i++)
+            if (hf2[i]) REQUIRE(joined[i] == dval[key[i] - 1]);
+
+        // GROUP BY the joined value: SUM(price)
+        auto g = group_by_aggregate(pcols, d_price, f2, 1, n, "SUM", alloc, q, {});
+        std::get<4>(g).wait();
+        REQUIRE(std::get<1>(g) == 5);
+        std::vector<uint64_t> gs(5, 0);
+        std::vector<unsigned char> gf(5, 0);
+        for (int i = 0; i < n; i++)
+            if (hf2[i]) {
+                gs[dval[key[i] - 1]] += (uint64_t)(int64_t)price[i];
+                gf[dval[key[i] - 1]] = 1;
+            }
+        REQUIRE(host(std::get<3>(g), 5) == gs);
+        REQUIRE(host(reinterpret_cast<unsigned char *>(std::get<2>(g)), 5) == gf);
+        const std::vector<int> gk = host(std::get<0>(g)[0], 5);
+        for (int k = 0; k < 5; k++)
+            if (gf[k]) REQUIRE(gk[k] == k);
+        delete[] std::get<0>(g);
+        check(sdb_device_free(pkey));
+        check(sdb_device_free(f2));
+    }
+
+    // the fused bundle entry: discount in [1, 3], the semi join, price *
+    // discount and its sum in one call
+    {
+        bool *f3 = reinterpret_cast<bool *>(dev(one));
+        int *rev3 = dev(std::vector<int>(n, 0));
+        uint64_t *sum3 = dev(std::vector<uint64_t>(1, 0));
+        std::vector<sdb_op> ops(5);
+        for (sdb_op &o : ops) o = sdb_op{};
+        ops[0].kind = SDB_OP_SELECT_LIT, ops[0].cmp = SDB_GE, ops[0].logic = SDB_AND, ops[0].value = 1;
+        ops[1].kind = SDB_OP_SELECT_LIT, ops[1].cmp = SDB_LE, ops[1].logic = SDB_AND, ops[1].value = 3;
+        for (int k = 0; k < 2; k++) {
+            ops[k].col = d_disc;
+            ops[k].in_flags = detail::u8(f3);
+            ops[k].out_flags = detail::u8(f3);
+        }
+        ops[2].kind = SDB_OP_PROBE_SEMI, ops[2].col = d_key, ops[2].ht = d_ht, ops[2].ht_min = 1, ops[2].ht_max = dim;
+        ops[2].in_flags = detail::u8(f3), ops[2].out_flags = detail::u8(f3);
+        ops[3].kind = SDB_OP_BINOP_CC, ops[3].binop = SDB_MUL, ops[3].col = d_price, ops[3].col2 = d_disc;
+        ops[3].res = rev3, ops[3].in_flags = detail::u8(f3);
+        ops[4].kind = SDB_OP_SUM, ops[4].col = rev3, ops[4].in_flags = detail::u8(f3), ops[4].acc = sum3;
+        bundle_execute(ops, n, q).wait();
+        REQUIRE(host(sum3, 1)[0] == want_sum);
+        REQUIRE(count_true_flags(f3, n, q) == want_cnt);
+        for (void *p : {(void *)f3, (void *)rev3, (void *)sum3}) check(sdb_device_free(p));
+    }
     for (void *p : {(void *)d_key, (void *)d_price, (void *)d_disc, (void *)d_dkey, (void *)d_dval, (void *)d_flags,
                     (void *)d_dflags, (void *)d_ht, (void *)d_kv, (void *)d_val, (void *)d_rev, (void *)d_sum,
                     (void *)d_cnt, (void *)d_gkeys, (void *)d_gsum, (void *)d_seen})

@@ -143,3 +143,37 @@ def test_cxx_layer_device_pipeline():
     out = subprocess.run([API_CHECK, "device"], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     assert out.stdout.strip() == "ok"
+
+
+def test_sdb_op_layout_matches_ctypes(tmp_path):
+    """The ctypes mirror of sdb_op has the C struct's size and offsets."""
+    import subprocess
+    fields = [f for f, _ in L.Op._fields_]
+    src = tmp_path / "op_layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "sycldb.h"\nint main(void) {\n'
+                   '    printf("%zu\\n", sizeof(sdb_op));\n' +
+                   "".join('    printf("%%zu\\n", offsetof(sdb_op, %s));\n' % f for f in fields) + "    return 0;\n}\n")
+    exe = tmp_path / "op_layout"
+    subprocess.run(["gcc", "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert got[0] == ctypes.sizeof(L.Op)
+    assert got[1:] == [getattr(L.Op, f).offset for f in fields]
+
+
+def test_bundle_exec_rejects_bad_operators():
+    """Checked on the host before any device work."""
+    ops = (L.Op * 2)()
+    ops[0].kind = 99
+    with pytest.raises(ValueError, match="unknown operator kind"):
+        L.check(L.lib.sdb_bundle_exec(ops, 1, 10, None))
+    ops[0].kind = L.OP_SELECT_LIT  # null column
+    with pytest.raises(ValueError, match="null"):
+        L.check(L.lib.sdb_bundle_exec(ops, 1, 10, None))
+    with pytest.raises(ValueError, match="operator count"):
+        L.check(L.lib.sdb_bundle_exec(ops, L.BUNDLE_MAX_OPS + 1, 10, None))
+    ops[0].kind = L.OP_GROUPBY
+    ops[0].nkeys = 9
+    with pytest.raises(ValueError, match="key count"):
+        L.check(L.lib.sdb_bundle_exec(ops, 1, 10, None))
+    # nothing to do: no device needed
+    L.check(L.lib.sdb_bundle_exec(ops, 0, 10, None))

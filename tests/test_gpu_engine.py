@@ -52,7 +52,9 @@ def make_engine(**kw):
 
 @pytest.fixture(scope="module")
 def fused():
-    e = make_engine()
+    # compile mode: every execution runs the compiled kernels (the default
+    # mode runs a new shape on the interpreter until they are ready)
+    e = make_engine(codegen="compile")
     yield e
     e.close()
 
@@ -72,7 +74,7 @@ def test_fused_matches_oracle(fused, expected, idx):
     assert res.rows() == expected[idx]
 
 
-@pytest.mark.parametrize("kw", [dict(fusion=False), dict(segment_size=50_000), dict(segment_size=77_777),
+@pytest.mark.parametrize("kw", [dict(), dict(fusion=False), dict(segment_size=50_000), dict(segment_size=77_777),
                                 dict(segment_size=77_777, fusion=False), dict(codegen="interpret"),
                                 dict(codegen="interpret", segment_size=77_777), dict(unroll=1), dict(unroll=4),
                                 dict(unroll=4, segment_size=77_777), dict(unroll=1 | QUEUED),
@@ -90,14 +92,17 @@ def test_unfused_and_segmented_match_oracle(expected, kw):
 
 
 def test_default_engine_compiles_fused_kernels(fused):
-    """With hipRTC present the default engine runs runtime-compiled kernels."""
+    """With hipRTC present the default engine runs runtime-compiled kernels
+    (built on the background compiler thread; the interpreter runs a shape
+    meanwhile, with the same results)."""
     from sycldb_amd import engine as E
     ok, why = E.codegen_available()
     assert ok, why
-    before = E.codegen_kernels_compiled()
-    fused.execute(PL.build("t4", mfgr=(1, 5)))
-    assert E.codegen_kernels_compiled() >= before
+    plan = PL.build("t4", mfgr=(1, 5))
+    first = fused.execute(plan).rows()
+    assert E.codegen_wait(300), "background compilation did not finish"
     assert E.codegen_kernels_compiled() > 0
+    assert fused.execute(plan).rows() == first
 
 
 def test_fusion_launches_fewer_kernels(fused):
@@ -112,23 +117,86 @@ def test_fusion_launches_fewer_kernels(fused):
     assert n_fused < n_unfused
 
 
-def test_every_tuner_candidate_matches_oracle():
-    """Above 4M rows the autotuner times one launch configuration per
-    execution (nine of them): every one of those executions returns the
-    oracle's result."""
-    from oracle import datagen
-    rows = 4_500_011
-    tables = datagen.generate_tables(S, rows, seed=SEED + 1)
+BLOCK512, BLOCK1024 = 256, 512  # SDB_UNROLL_BLOCK512 / SDB_UNROLL_BLOCK1024
+TUNE_ROWS = 4_500_011  # over the autotuner's 4M-row threshold
+
+
+@pytest.fixture(scope="module")
+def tune_tables():
+    return datagen.generate_tables(S, TUNE_ROWS, seed=SEED + 1)
+
+
+@pytest.fixture(scope="module")
+def tune_case(tune_tables):
     plans = [PL.build("t2"), PL.build("t3")]
-    want = [PO.execute_plan(p, tables).rows() for p in plans]
+    return plans, [PO.execute_plan(p, tune_tables).rows() for p in plans]
+
+
+def test_every_tuner_candidate_matches_oracle(tune_case):
+    """Every launch configuration the autotuner chooses among (chunks per
+    thread x threads per block, with and without the queued row tail),
+    forced explicitly, returns the oracle's result."""
     from sycldb_amd.engine import Engine
+    plans, want = tune_case
     eng = Engine()
     try:
-        eng.generate(rows, SEED + 1)
-        for rep in range(11):
-            for i, p in enumerate(plans):
-                print("tuner execution", rep, i, flush=True)
-                assert eng.execute(p).rows() == want[i], (rep, i)
+        eng.generate(TUNE_ROWS, SEED + 1)
+        for unroll in (1, 2, 4):
+            for blk in (0, BLOCK512, BLOCK1024):
+                for q in (0, QUEUED):
+                    eng.set_codegen("auto", unroll | blk | q)
+                    for i, p in enumerate(plans):
+                        print("candidate", unroll, blk, q, i, flush=True)
+                        assert eng.execute(p).rows() == want[i], (unroll, blk, q, i)
+    finally:
+        eng.close()
+
+
+def _tuning_launches(report):
+    """Launches that ran autotuner trials, over every program shape."""
+    import re
+    return sum(int(m) for m in re.findall(r"(\d+) tuning launches", report))
+
+
+def test_autotuner_settles_in_one_execution(tune_case):
+    """In compile mode (waiting for the background compiler) the first
+    execution over 4M rows runs every trial on a scratch copy of the
+    accumulators, then the program itself once: its result is the oracle's,
+    and the shape is settled afterwards."""
+    from sycldb_amd.engine import Engine, codegen_tuning_report
+    plans, want = tune_case
+    eng = Engine(codegen="compile")
+    try:
+        eng.generate(TUNE_ROWS, SEED + 1)
+        for i, p in enumerate(plans):
+            print("tuning execution", i, flush=True)
+            assert eng.execute(p).rows() == want[i]
+        report = codegen_tuning_report()
+        print(report, flush=True)
+        # settled: executing the plans again runs no further trial
+        trials = _tuning_launches(report)
+        for i, p in enumerate(plans):
+            assert eng.execute(p).rows() == want[i]
+        assert _tuning_launches(codegen_tuning_report()) == trials
+    finally:
+        eng.close()
+
+
+def test_auto_mode_results_while_compiling(tune_tables):
+    """The default mode runs a new shape on the interpreter while its kernels
+    compile, then the tuned kernels: every execution returns the oracle's
+    result (the q4.x template has not run over 4M rows before here)."""
+    from sycldb_amd.engine import Engine, codegen_wait
+    eng = Engine()
+    try:
+        eng.generate(TUNE_ROWS, SEED + 1)
+        p = PL.build("t4", c_region=2, s_region=1, mfgr=(1, 3))
+        w = PO.execute_plan(p, tune_tables).rows()
+        for rep in range(3):
+            assert eng.execute(p).rows() == w, rep
+        assert codegen_wait(300)
+        for rep in range(3):
+            assert eng.execute(p).rows() == w, rep
     finally:
         eng.close()
 

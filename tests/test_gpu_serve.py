@@ -1,15 +1,17 @@
 """Serving loop (sdb_server_*): requests over three engines with resident
-tables of different sizes.  With continuous batching every request's result
-must equal the same plan run alone on its engine (Poisson arrivals), a burst
-of requests that queue up behind the first ones must have been batched
-together, and the serial baseline must give the same results one request at
-a time."""
+tables of different sizes.  Every served request's result must equal the
+plan oracle's result for that plan on the same seeded tables, and the same
+plan run alone on its engine: with continuous batching (Poisson arrivals), a
+burst of requests that queue up behind the first ones (which must have been
+batched together), and the serial baseline one request at a time."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
+from oracle import datagen, plan_oracle as PO  # noqa: E402
 from sycldb_amd import plans as PL  # noqa: E402
+from sycldb_amd import schema as S  # noqa: E402
 
 SIZES = (100_003, 200_003, 300_007)
 
@@ -37,12 +39,24 @@ def trace():
 
 
 @pytest.fixture(scope="module")
-def alone(engines, trace):
+def oracle_rows(trace):
+    """Each request's rows from the plan oracle on its engine's tables."""
     reqs, _ = trace
-    return [engines[e].execute(plan).rows() for e, plan in reqs]
+    tables = [datagen.generate_tables(S, n, seed=50 + i) for i, n in enumerate(SIZES)]
+    return [PO.execute_plan(plan, tables[e]).rows() for e, plan in reqs]
 
 
-def test_batched_serving_matches_alone(engines, trace, alone):
+@pytest.fixture(scope="module")
+def alone(engines, trace, oracle_rows):
+    """Each request run alone on its engine; equal to the oracle's rows."""
+    reqs, _ = trace
+    rows = [engines[e].execute(plan).rows() for e, plan in reqs]
+    for i, (r, w) in enumerate(zip(rows, oracle_rows)):
+        assert r == w, i
+    return rows
+
+
+def test_batched_serving_matches_oracle(engines, trace, alone, oracle_rows):
     from sycldb_amd.engine import Server, replay
     reqs, arrivals = trace
     print("serving", len(reqs), "requests", flush=True)
@@ -50,12 +64,12 @@ def test_batched_serving_matches_alone(engines, trace, alone):
         out = replay(srv, reqs, arrivals)
         stats = srv.stats()
     for i, (res, info) in enumerate(out):
-        assert res.rows() == alone[i], i
+        assert res.rows() == oracle_rows[i] == alone[i], i
         assert info["latency_ms"] >= info["run_ms"] >= 0
     assert stats["requests"] == len(reqs)
 
 
-def test_burst_is_batched(engines, trace, alone):
+def test_burst_is_batched(engines, trace, alone, oracle_rows):
     """48 requests at once: ~16 per engine queue behind the first ones, more
     than the initial batching threshold, so they share scans."""
     from sycldb_amd.engine import Server, replay
@@ -64,30 +78,30 @@ def test_burst_is_batched(engines, trace, alone):
         out = replay(srv, reqs, arrivals * 0)
         stats = srv.stats()
     for i, (res, info) in enumerate(out):
-        assert res.rows() == alone[i], i
+        assert res.rows() == oracle_rows[i], i
     assert stats["requests"] == len(reqs)
     assert max(info["batch_size"] for _, info in out) > 1
     assert stats["batches"] < len(reqs)
 
 
-def test_serial_serving_matches_alone(engines, trace, alone):
+def test_serial_serving_matches_oracle(engines, trace, alone, oracle_rows):
     from sycldb_amd.engine import Server, replay
     reqs, arrivals = trace
     with Server(engines, serial=True) as srv:
         out = replay(srv, reqs[:16], arrivals[:16])
     for i, (res, info) in enumerate(out):
-        assert res.rows() == alone[i]
+        assert res.rows() == oracle_rows[i]
         assert info["batch_size"] == 1
 
 
-def test_tight_memory_budget_still_serves(engines, trace, alone):
+def test_tight_memory_budget_still_serves(engines, trace, alone, oracle_rows):
     """A budget below one request's footprint admits one request per batch."""
     from sycldb_amd.engine import Server, replay
     reqs, arrivals = trace
     with Server(engines, max_batch=16, memory_budget_bytes=1) as srv:
         out = replay(srv, reqs[:12], arrivals[:12] * 0)
     for i, (res, info) in enumerate(out):
-        assert res.rows() == alone[i]
+        assert res.rows() == oracle_rows[i]
         assert info["batch_size"] == 1
 
 

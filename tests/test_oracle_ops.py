@@ -227,3 +227,66 @@ def test_minmax():
     rng = np.random.default_rng(7)
     c = rng.integers(-10**9, 10**9, 12345).astype(np.int32)
     assert O.minmax(c) == (int(c.min()), int(c.max()))
+
+
+def test_fill_i32_closed_form():
+    # FillKernel (kernels/projection.hpp:18-21) overwrites every row
+    for v in (0, -1, 7, 2**31 - 1, -2**31):
+        dst = np.arange(1000, dtype=np.int32)
+        O.fill_i32(dst, v)
+        assert (dst == v).all() and dst.dtype == np.int32
+    O.fill_i32(np.zeros(0, np.int32), 5)
+
+
+def _slot_bigint(row_keys, mins, maxs, prod):
+    """The group slot of kernels/aggregation.hpp:315-321 in Python integers:
+    `int hash`, `int mult` wrap at 32 bits, then `hash %= prod_ranges`
+    converts hash to uint64 (a negative hash becomes 2^64 + hash) and the
+    remainder goes back into the int."""
+    def i32(x):
+        x &= 0xFFFFFFFF
+        return x - (1 << 32) if x >= 1 << 31 else x
+    h, m = 0, 1
+    for c, lo, hi in zip(row_keys, mins, maxs):
+        h = i32(h + i32(i32(c - lo) * m))
+        m = i32(m * i32(hi - lo + 1))
+    return i32((h % (1 << 64)) % prod)
+
+
+def test_groupby_slot_outside_range_wraps_through_uint64():
+    # keys below the minimum give a negative int hash: (2^64 + hash) % prod,
+    # which differs from both C's and Python's signed remainder
+    assert _slot_bigint([3], [10], [16], 7) == 2          # 2^64 - 7 = 2 (mod 7)
+    assert (-7) % 7 == 0
+    rng = np.random.default_rng(9)
+    n = 50_000
+    # one narrow key column with keys far outside [10, 16], and a wide pair
+    # whose mixed-radix index overflows int32
+    cases = [
+        ([rng.integers(-2**31, 2**31, n).astype(np.int32)], [10], [16], 7),
+        ([rng.integers(-100, 100, n).astype(np.int32), rng.integers(0, 2**31, n).astype(np.int32)],
+         [-5, 3], [70_000, 70_003], 4375),
+        ([rng.integers(-2**31, 2**31, n).astype(np.int32) for _ in range(3)], [-7, 0, 123], [9, 999, 200_000],
+         97 * 89),
+    ]
+    for keys, mins, maxs, prod in cases:
+        fl = (rng.random(n) < 0.6).astype(np.uint8)
+        agg = rng.integers(-10**6, 10**6, n).astype(np.int32)
+        sums = np.zeros(prod, np.uint64)
+        seen = np.zeros(prod, np.uint32)
+        outk = [np.zeros(prod, np.int32) for _ in keys]
+        O.groupby_dense(keys, mins, maxs, agg, fl, outk, sums, seen, prod)
+        want_sum, want_first = {}, {}
+        cols = [k.tolist() for k in keys]
+        for i in range(n):
+            if not fl[i]:
+                continue
+            s = _slot_bigint([c[i] for c in cols], mins, maxs, prod)
+            assert 0 <= s < prod
+            want_sum[s] = want_sum.get(s, 0) + int(agg[i])
+            want_first.setdefault(s, i)
+        assert sorted(np.flatnonzero(seen).tolist()) == sorted(want_sum)
+        for s, v in want_sum.items():
+            assert int(sums[s]) == v % 2**64
+            # the oracle stores the keys of the first flagged row of the slot
+            assert [int(k[s]) for k in outk] == [c[want_first[s]] for c in cols]

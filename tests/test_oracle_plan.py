@@ -128,3 +128,66 @@ def test_full_join_without_group_column_fails():
               "supplier": {0: np.array([1, 2], np.int32), 5: np.array([1, 0], np.int32)}}
     with pytest.raises(PO.OracleError):
         PO.execute_plan(b.plan(), tables)
+
+
+# ---------------------------------------------------------------- quirk plans
+# The plan oracle on the quirk plans the GPU engines are also checked with
+# (tests/quirk_plans.py), against the rule each quirk fixes, written here as
+# plain numpy over the same seeded tables: the left fold, the NONE overwrite
+# after a join, literal fills and literal-operand arithmetic, and group slots
+# of keys outside their recorded range.
+
+def _lo(tables):
+    lo = tables["lineorder"]
+    return (lo[S.LO_QUANTITY].astype(np.int64), lo[S.LO_DISCOUNT].astype(np.int64),
+            lo[S.LO_EXTENDEDPRICE].astype(np.int64), lo[S.LO_REVENUE].astype(np.int64))
+
+
+def _scalar(v):
+    return [(int(v) % 2**64,)]
+
+
+def test_quirk_left_fold_or_and(tables):
+    import quirk_plans as Q
+    q, d, p, r = _lo(tables)
+    mask = ((d == 2) | (q < 10)) & (p > 5_000_000)          # not (d == 2) | ((q < 10) & (p > 5e6))
+    sql = (d == 2) | ((q < 10) & (p > 5_000_000))
+    assert mask.sum() != sql.sum()
+    assert PO.execute_plan(Q.PLANS["left_fold_or_and"], tables).rows() == _scalar(r[mask].sum())
+
+
+def test_quirk_left_fold_and_or(tables):
+    import quirk_plans as Q
+    q, d, p, r = _lo(tables)
+    # a gets NONE, the OR node gets AND: b gets AND, c and e get OR
+    mask = (((q < 30) & (d == 1)) | (d == 9)) | (p > 9_000_000)
+    want = [(k, int(q[mask & (d == k)].sum())) for k in range(11) if (mask & (d == k)).any()]
+    assert PO.execute_plan(Q.PLANS["left_fold_and_or"], tables).rows() == want
+
+
+def test_quirk_filter_after_join_overwrites(tables):
+    import quirk_plans as Q
+    q, d, p, r = _lo(tables)
+    assert PO.execute_plan(Q.PLANS["filter_after_join"], tables).rows() == _scalar(p[d == 2].sum())
+
+
+def test_quirk_literal_projections(tables):
+    import quirk_plans as Q
+    q, d, p, r = _lo(tables)
+    sel = q < 25
+    assert PO.execute_plan(Q.PLANS["literal_first"], tables).rows() == _scalar(7 * sel.sum())
+    assert PO.execute_plan(Q.PLANS["col_times_lit"], tables).rows() == _scalar((3 * q[sel]).sum())
+    assert PO.execute_plan(Q.PLANS["lit_minus_col"], tables).rows() == _scalar((100 - d[sel]).sum())
+    assert PO.execute_plan(Q.PLANS["col_div_lit"], tables).rows() == _scalar((r[sel] // 1000).sum())
+
+
+def test_quirk_group_slots_outside_recorded_range(tables):
+    # 100 - discount has the recorded range of discount (0..10): the slot of
+    # key k is k % 11, and groups come out in slot order
+    import quirk_plans as Q
+    q, d, p, r = _lo(tables)
+    sel = q < 25
+    keys = sorted({int(k) for k in (100 - d[sel])}, key=lambda k: k % 11)
+    want = [(k, int((3 * q[sel & (100 - d == k)]).sum())) for k in keys]
+    assert [k for k, _ in want][:3] == [99, 100, 90]
+    assert PO.execute_plan(Q.PLANS["group_by_lit_minus_col"], tables).rows() == want
