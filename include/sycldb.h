@@ -164,6 +164,19 @@ int sdb_count_flags(const uint8_t *flags, int64_t n, uint64_t *out, int accumula
 int sdb_compact(const uint8_t *flags, int64_t n, int32_t *row_ids, uint32_t *count,
                 sdb_stream stream);
 
+/* sdb_compact for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same results as sdb_compact; it
+ * allocates nothing: its per-tile counts live in a fixed 256 KiB device array
+ * of the library (one per GPU it runs on, held until the process ends), so n
+ * is at most SDB_COMPACT_NOALLOC_MAX_ROWS (2^28, the reference's
+ * SEGMENT_SIZE); a larger n returns SDB_ERR_INVALID_ARG.  Every call on one
+ * GPU shares that array: calls that overlap on one GPU (on different streams
+ * or threads) race. */
+#define SDB_COMPACT_NOALLOC_MAX_ROWS (1LL << 28)
+int sdb_compact_noalloc(const uint8_t *flags, int64_t n, int32_t *row_ids, uint32_t *count,
+                        sdb_stream stream);
+
 /* Gather of selected rows, the device half of Segment::compress_sync
  * (models/models.hpp:1019-1035): dst[j] = src[row_ids[j]] for j < count
  * (count read on the host by the caller). */

@@ -121,6 +121,7 @@ SIGNATURES = {
     "sdb_seen_to_flags": (INT, [P, P, U64, P]),
     "sdb_count_flags": (INT, [P, I64, P, INT, P]),
     "sdb_compact": (INT, [P, I64, P, P, P]),
+    "sdb_compact_noalloc": (INT, [P, I64, P, P, P]),
     "sdb_bundle_exec": (INT, [ctypes.POINTER(Op), INT, I64, P]),
     "sdb_gather_i32": (INT, [P, P, I64, P, P]),
     "sdb_minmax_i32": (INT, [P, I64, P, P, P]),

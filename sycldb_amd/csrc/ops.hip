@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 namespace sdb {
 namespace {
@@ -1190,6 +1191,45 @@ void launch_compact_into(const uint8_t *flags, int64_t n, int32_t *row_ids, uint
     check_launch("compact");
 }
 
+// ------------------------------------------- compaction without allocation
+//
+// The per-tile counts of sdb_compact_noalloc live in a fixed device array of
+// the library instead of library-allocated scratch: each GPU the library runs
+// on holds its own copy until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it only stands for the symbol); its
+// device address is looked up once per device, with that device current, by
+// hipGetSymbolAddress, and that address is what the kernels get.  Every call
+// on one GPU shares the array; the counting pass overwrites every count the
+// scan and write passes read, so nothing needs clearing between calls.
+constexpr int64_t NOALLOC_TILES = 65536; // 256 KiB of counts: 2^28 rows
+static_assert(NOALLOC_TILES * COMPACT_TILE == (int64_t(1) << 28), "noalloc compaction covers 2^28 rows");
+static __device__ __align__(256) uint32_t scratch[NOALLOC_TILES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+uint32_t *g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+uint32_t *noalloc_counts()
+{
+    int dev = 0;
+    SDB_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) fail(SDB_ERR_UNSUPPORTED, "device ordinal beyond the no-allocation table");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *ptr = nullptr;
+        hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!ptr) fail(SDB_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+        g_noalloc_ptr[dev] = static_cast<uint32_t *>(ptr);
+    }
+    return g_noalloc_ptr[dev];
+}
+
+void launch_compact_noalloc(const uint8_t *flags, int64_t n, int32_t *row_ids, uint32_t *count, hipStream_t s)
+{
+    if (n > NOALLOC_TILES * COMPACT_TILE)
+        fail(SDB_ERR_INVALID_ARG, "sdb_compact_noalloc: n exceeds 2^28 rows (the fixed tile-count array)");
+    launch_compact_into(flags, n, row_ids, count, noalloc_counts(), s);
+}
+
 void launch_gather_cols(const GatherCols &g, const int32_t *ids, int64_t count, hipStream_t s)
 {
     if (count <= 0 || g.ncols <= 0) return;
@@ -1439,6 +1479,17 @@ int sdb_compact(const uint8_t *flags, int64_t n, int32_t *row_ids, uint32_t *cou
         require(count, "null count");
         require(n == 0 || (flags && row_ids), "null pointer");
         launch_compact(flags, n, row_ids, count, as_stream(stream));
+    });
+}
+
+int sdb_compact_noalloc(const uint8_t *flags, int64_t n, int32_t *row_ids, uint32_t *count, sdb_stream stream)
+{
+    return guarded([&] {
+        check_n(n);
+        require(n <= SDB_COMPACT_NOALLOC_MAX_ROWS, "sdb_compact_noalloc: n exceeds SDB_COMPACT_NOALLOC_MAX_ROWS (2^28)");
+        require(count, "null count");
+        require(n == 0 || (flags && row_ids), "null pointer");
+        launch_compact_noalloc(flags, n, row_ids, count, as_stream(stream));
     });
 }
 
