@@ -175,6 +175,61 @@ def served_measure(eng_big, seed, n_requests, rates, max_batch):
             e.close()
 
 
+def config_measure(name, seed, steps, cpu_seconds):
+    """One of the smaller plain configurations at its own size: q1.1 on the
+    6M-row table (C1) or the q3.1 join on the 60M-row table (C2).  GPU: the
+    fused plan, settled first, then timed steps with HIP events around the
+    fact kernel.  CPU: the oracle (the reference's unfused operator
+    sequence) on the same seeded table at full size, on this host's cores.
+    The two results must agree."""
+    import torch
+    from oracle import datagen, oracle as O, plan_oracle as PO
+    from sycldb_amd import plans as PL, schema as S
+    from sycldb_amd.engine import Engine, codegen_pending, codegen_wait
+    rows, builder, cols = WORKLOADS[name]
+    plan = getattr(PL, builder)()
+    text = PL.to_json(plan)
+    out = {"workload": DESCRIPTIONS[name], "fact_rows": rows, "algorithmic_bytes_per_row": 4 * len(cols)}
+    with Engine() as eng:
+        eng.generate(rows, seed, fact_columns=cols)
+        res = eng.execute(text)
+        runs = 1
+        while codegen_pending() > 0 and runs < 64:
+            codegen_wait(300)
+            res = eng.execute(text)
+            runs += 1
+        for _ in range(3):
+            eng.execute(text)
+        eng.set_profiling(True, table="lineorder")
+        eng.profile_reset()
+        torch.cuda.synchronize()
+        with no_gc():
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                eng.execute(text)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+        launches, krows, kms = eng.profile().get("fused:lineorder", (0, 0, 0.0))
+        gpu_rows = res.rows()
+    out["gpu"] = {"ms_per_step": wall * 1e3 / steps, "fact_rows_per_s": rows * steps / wall,
+                  "kernel_ms": kms / max(launches, 1),
+                  "kernel_hbm_frac": (4 * len(cols) * krows / (kms / 1e3)) / (HBM_PEAK_GBS * 1e9) if kms else None}
+    tables = datagen.generate_tables(S, rows, seed, fact_columns=cols)
+    want = PO.execute_plan(plan, tables).rows()
+    assert want == gpu_rows, "%s: GPU and CPU oracle results differ" % name
+    reps, elapsed = 0, 0.0
+    while elapsed < cpu_seconds or reps < 2:
+        t0 = time.perf_counter()
+        PO.execute_plan(plan, tables)
+        elapsed += time.perf_counter() - t0
+        reps += 1
+    out["cpu_baseline"] = {"fact_rows_per_s": rows * reps / elapsed, "ms_per_step": elapsed * 1e3 / reps,
+                           "cores": O.num_threads(), "kind": "port", "reps": reps}
+    out["results_identical"] = True
+    out["gpu_over_cpu"] = out["gpu"]["fact_rows_per_s"] / out["cpu_baseline"]["fact_rows_per_s"]
+    return out
+
+
 def eng_big_rows(eng):
     return getattr(eng, "fact_rows", None)
 
@@ -243,6 +298,8 @@ def main():
     ap.add_argument("--batch-pipelines", type=int, default=32)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the q1.1 (6M rows) and q3.1 (60M rows) configurations at their own sizes")
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-qps", default="1000,3000",
                     help="arrival rates (requests/s), comma separated")
@@ -363,6 +420,13 @@ def main():
                                 [float(q) for q in str(args.serve_qps).split(",")], args.serve_max_batch)
         log("served: %s" % served)
 
+    configs = None
+    if not args.no_configs and args.workload == "c3":
+        configs = {}
+        for name in ("c1", "c2"):
+            configs[name] = config_measure(name, args.seed, 20, 3.0 if args.no_cpu else 6.0)
+            log("%s: %s" % (name, configs[name]))
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(plan, fact_cols, min(args.cpu_sample_rows, rows), args.seed)
@@ -391,6 +455,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
+        "configs": configs,
     }
     print(json.dumps(out), flush=True)
     eng.close()

@@ -183,6 +183,18 @@ int sdb_compact_noalloc(const uint8_t *flags, int64_t n, int32_t *row_ids, uint3
 int sdb_gather_i32(const int32_t *src, const int32_t *row_ids, int64_t count, int32_t *dst,
                    sdb_stream stream);
 
+/* Segment::compress_sync driven by TransientTable::build_row_ids
+ * (models/models.hpp:990-1068, models/transient_table.hpp:535-599), the
+ * sync of a device segment into its host copy: for every row whose device
+ * flag is set, host_cols[c][row] = cols[c][row], for the ncols (0..8)
+ * device columns of n rows; the host columns' other rows are left as they
+ * are.  The selected rows are compacted and gathered on the device, copied
+ * to the host, and scattered there.  *count (host) receives the number of
+ * rows synced.  `cols` and `host_cols` are HOST arrays of pointers; the
+ * host columns hold n int32 each.  Synchronises the stream. */
+int sdb_compress_sync(const int32_t *const *cols, int ncols, const uint8_t *flags, int64_t n,
+                      int32_t *const *host_cols, uint64_t *count, sdb_stream stream);
+
 /* Per-column min and max (the load-time statistics of
  * models/models.hpp:71-77), written to the host ints min_out and max_out.
  * Synchronises the stream.  n must be > 0. */

@@ -124,6 +124,7 @@ SIGNATURES = {
     "sdb_compact_noalloc": (INT, [P, I64, P, P, P]),
     "sdb_bundle_exec": (INT, [ctypes.POINTER(Op), INT, I64, P]),
     "sdb_gather_i32": (INT, [P, P, I64, P, P]),
+    "sdb_compress_sync": (INT, [ctypes.POINTER(P), INT, P, I64, ctypes.POINTER(P), ctypes.POINTER(U64), P]),
     "sdb_minmax_i32": (INT, [P, I64, P, P, P]),
     "sdb_gen_uniform": (INT, [P, I64, U64, U64, I64, I32, I32, P]),
     "sdb_gen_child": (INT, [P, P, I64, U64, U64, I64, I32, I32, I32, P]),

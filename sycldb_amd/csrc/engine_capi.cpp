@@ -220,9 +220,11 @@ int sdb_engine_table_load_dir(sdb_engine *e, const char *name, const char *dir)
         require(e && name && dir, "null argument");
         auto idx = TABLE_COLUMN_INDICES.find(name);
         if (idx == TABLE_COLUMN_INDICES.end()) fail(SDB_ERR_INVALID_ARG, std::string("unknown table ") + name);
-        // models/models.hpp:1659-1708: <DIR>/<TABLE><col>, raw int32; the
-        // first file fixes the row count; a file of another length leaves
-        // that column unloaded
+        // models/models.hpp:1659-1708: <DIR>/<TABLE><col>, raw int32, for
+        // the columns of table_column_indices in ascending order.  A file
+        // that cannot be read counts as 0 entries.  The first column's entry
+        // count fixes the row count; a column of another count stays
+        // unloaded, with a warning.
         std::string upper = name;
         for (char &ch : upper) ch = (char)toupper((unsigned char)ch);
         std::vector<std::pair<int, std::vector<int32_t>>> cols;
@@ -230,24 +232,28 @@ int sdb_engine_table_load_dir(sdb_engine *e, const char *name, const char *dir)
         for (int ci : idx->second) {
             const std::string path = std::string(dir) + "/" + upper + std::to_string(ci);
             std::ifstream f(path, std::ios::binary | std::ios::ate);
-            if (!f) fail(SDB_ERR_INVALID_ARG, "cannot open column file " + path);
-            const int64_t bytes = (int64_t)f.tellg();
-            const int64_t n = bytes / 4;
+            const std::streampos size = f ? f.tellg() : std::streampos(-1);
+            const int64_t n = size == std::streampos(-1) ? 0 : (int64_t)size / 4;
             if (nrows < 0) nrows = n;
             if (n != nrows) {
-                fprintf(stderr, "Warning: Column length mismatch in %s\n", path.c_str());
+                fprintf(stderr, "Warning: Column length mismatch in %s: expected %lld, got %lld\n", path.c_str(),
+                        (long long)nrows, (long long)n);
                 continue;
             }
             std::vector<int32_t> data((size_t)n);
-            f.seekg(0);
-            f.read(reinterpret_cast<char *>(data.data()), n * 4);
+            if (n > 0) {
+                f.seekg(0);
+                f.read(reinterpret_cast<char *>(data.data()), n * 4);
+                if (!f) fail(SDB_ERR_RUNTIME, "reading column file " + path + " failed");
+            }
             cols.emplace_back(ci, std::move(data));
         }
         if (nrows < 0) nrows = 0;
         int rc = sdb_engine_table_create(e, name, (uint64_t)nrows);
         if (rc) fail(rc, sdb_last_error());
+        static const int32_t none = 0; // the source of an empty column
         for (auto &c : cols) {
-            rc = sdb_engine_column_upload(e, name, c.first, c.second.data());
+            rc = sdb_engine_column_upload(e, name, c.first, c.second.empty() ? &none : c.second.data());
             if (rc) fail(rc, sdb_last_error());
         }
         rc = sdb_engine_table_finalize(e, name);

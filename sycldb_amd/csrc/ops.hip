@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 namespace sdb {
 namespace {
@@ -1499,6 +1500,67 @@ int sdb_gather_i32(const int32_t *src, const int32_t *row_ids, int64_t count, in
         check_n(count);
         require(count == 0 || (src && row_ids && dst), "null pointer");
         launch_gather_i32(src, row_ids, count, dst, as_stream(stream));
+    });
+}
+
+int sdb_compress_sync(const int32_t *const *cols, int ncols, const uint8_t *flags, int64_t n,
+                      int32_t *const *host_cols, uint64_t *count, sdb_stream stream)
+{
+    return guarded([&] {
+        check_n(n);
+        require(n <= INT32_MAX, "compress_sync is limited to 2^31-1 rows (int32 row ids)");
+        require(ncols >= 0 && ncols <= GATHER_MAX_COLS, "compress_sync: 0 to 8 columns");
+        require(count && (ncols == 0 || (cols && host_cols)), "null pointer");
+        *count = 0;
+        if (n == 0) return;
+        for (int c = 0; c < ncols; c++) require(cols[c] && host_cols[c], "null column");
+        require(flags, "null flags");
+        hipStream_t s = as_stream(stream);
+        // device half: the selected rows' ids (build_row_ids,
+        // models/transient_table.hpp:535-599, here in row order) ...
+        const int64_t ntiles = compact_tiles(n);
+        const size_t ids_bytes = ((size_t)n * 4 + 255) / 256 * 256, cnt_off = ids_bytes + (size_t)ntiles * 4;
+        uint8_t *scr = static_cast<uint8_t *>(scratch_buffer(cnt_off + 256, s));
+        int32_t *ids = reinterpret_cast<int32_t *>(scr);
+        uint32_t *dcount = reinterpret_cast<uint32_t *>(scr + cnt_off);
+        launch_compact_into(flags, n, ids, dcount, reinterpret_cast<uint32_t *>(scr + ids_bytes), s);
+        uint32_t selected = 0;
+        SDB_HIP(hipMemcpyAsync(&selected, dcount, 4, hipMemcpyDeviceToHost, s));
+        SDB_HIP(hipStreamSynchronize(s));
+        *count = selected;
+        if (selected == 0 || ncols == 0) return;
+        // ... and their values, gathered (Segment::compress_sync,
+        // models/models.hpp:1019-1035) and copied to the host with the ids
+        int32_t *vals = nullptr;
+        SDB_HIP(hipMalloc(&vals, (size_t)selected * 4 * (size_t)ncols));
+        std::vector<int32_t> hids(selected), hvals((size_t)selected * (size_t)ncols);
+        hipError_t e = hipSuccess;
+        try {
+            GatherCols g{};
+            for (int c = 0; c < ncols; c++) {
+                g.src[c] = cols[c];
+                g.dst[c] = vals + (size_t)c * selected;
+                g.src_rows[c] = (uint64_t)n;
+            }
+            g.ncols = ncols;
+            launch_gather_cols(g, ids, selected, s);
+            e = hipMemcpyAsync(hids.data(), ids, (size_t)selected * 4, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(hvals.data(), vals, hvals.size() * 4, hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+        } catch (...) {
+            (void)hipStreamSynchronize(s);
+            (void)hipFree(vals);
+            throw;
+        }
+        (void)hipFree(vals);
+        hip_check(e, "compress_sync copy");
+        // host half: the scatter into the host columns (models.hpp:1044-1063)
+        for (int c = 0; c < ncols; c++) {
+            const int32_t *v = hvals.data() + (size_t)c * selected;
+            int32_t *dst = host_cols[c];
+            for (uint32_t j = 0; j < selected; j++) dst[hids[j]] = v[j];
+        }
     });
 }
 

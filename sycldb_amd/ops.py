@@ -256,6 +256,27 @@ def count_true_flags(flags, length, stream=None):
     return int(out.item())
 
 
+def compress_sync(cols, flags, host_cols, n=None, stream=None):
+    """Segment::compress_sync (models/models.hpp:990-1068): the rows whose
+    device flag is set are copied from the device columns (torch int32) into
+    the same rows of the host columns (numpy int32, C-contiguous, written in
+    place); returns the number of rows synced."""
+    n = _n(n, flags)
+    k = len(cols)
+    if len(host_cols) != k:
+        raise ValueError("one host column per device column")
+    dp = (ctypes.c_void_p * max(k, 1))(*[_dev(c, torch.int32, "column", n).value for c in cols])
+    hp = (ctypes.c_void_p * max(k, 1))()
+    for i, h in enumerate(host_cols):
+        if h.dtype != "int32" or not h.flags["C_CONTIGUOUS"] or not h.flags["WRITEABLE"] or len(h) < n:
+            raise ValueError("host column %d must be a writable contiguous int32 array of n rows" % i)
+        hp[i] = h.ctypes.data
+    cnt = ctypes.c_uint64()
+    L.check(L.lib.sdb_compress_sync(dp, k, _dev(flags, torch.uint8, "flags", n), n, hp, ctypes.byref(cnt),
+                                    _stream(stream)))
+    return cnt.value
+
+
 def bundle_exec(ops, n, stream=None):
     """KernelBundle::execute with fusion (models/execution.hpp:528-650):
     `ops` is a list of sycldb_amd._lib.Op (sdb_op) run in order over n rows,

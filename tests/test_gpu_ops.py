@@ -292,3 +292,20 @@ def test_error_paths(ops):
         ops.binop(t, t, t, f, 7)                  # arithmetic op out of range
     with pytest.raises(ValueError):
         ops.probe_semi(t, f, f, f, 0, 100)        # table smaller than its range
+
+
+def test_compress_sync_scatters_into_host_columns(ops):
+    """Segment::compress_sync (models/models.hpp:990-1068): the flagged rows
+    of the device columns land in the same rows of the host columns, the
+    other host rows stay as they were."""
+    rng = np.random.default_rng(17)
+    for n, p, k in ((1_000_003, 0.3, 3), (4097, 0.0, 2), (65_536, 1.0, 1), (10_000, 0.5, 0), (0, 0.5, 2)):
+        print("compress_sync n=%d p=%.1f cols=%d" % (n, p, k), flush=True)
+        cols = [rng.integers(-2**31, 2**31, n).astype(np.int32) for _ in range(k)]
+        fl = (rng.random(n) < p).astype(np.uint8)
+        host = [np.full(n, -12345, np.int32) for _ in range(k)]
+        cnt = ops.compress_sync([dev(c) for c in cols], dev(fl), host, n=n)
+        assert cnt == int(fl.sum())
+        sel = fl.astype(bool)
+        for c, h in zip(cols, host):
+            assert (h[sel] == c[sel]).all() and (h[~sel] == -12345).all()
