@@ -99,17 +99,35 @@ def batched_measure(eng, n_pipes, seed, reps):
         if k >= 1 and codegen_pending() == 0:
             break
     torch.cuda.synchronize()
-    with no_gc():
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            eng.execute_batch(plans)
-        torch.cuda.synchronize()
-        bat_s = (time.perf_counter() - t0) / reps
+    from sycldb_amd.engine import batch_graph_stats, set_batch_graphs
+
+    def time_batches():
+        with no_gc():
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                eng.execute_batch(plans)
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) / reps
+    # the launches as plain stream operations (the default), then as
+    # replayed hipGraphs; each mode warmed first
+    set_batch_graphs(False)
+    eng.execute_batch(plans)
+    bat_s = time_batches()
+    set_batch_graphs(True)
+    eng.execute_batch(plans)
+    eng.execute_batch(plans)
+    c0, r0 = batch_graph_stats()
+    graph_s = time_batches()
+    c1, r1 = batch_graph_stats()
+    set_batch_graphs(None)
     log("batched kernel tuning:\n" + "\n".join(l for l in codegen_tuning_report().splitlines() if "batched" in l))
     return {"pipelines": n_pipes, "pipeline_seed": seed, "sharing_the_scan": nb,
             "sequential_ms": seq_s * 1e3, "batched_ms": bat_s * 1e3,
             "sequential_pipelines_per_s": n_pipes / seq_s, "batched_pipelines_per_s": n_pipes / bat_s,
-            "speedup": seq_s / bat_s, "results_identical": True}
+            "speedup": seq_s / bat_s, "results_identical": True,
+            "launch": "stream operations (copy, mask build, batched kernel)",
+            "batched_ms_graph_replay": graph_s * 1e3,
+            "graph": "%d captures, %d replays while timed" % (c1 - c0, r1 - r0)}
 
 
 def served_measure(eng_big, seed, n_requests, rates, max_batch):

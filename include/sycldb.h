@@ -399,6 +399,19 @@ int sdb_server_stats(sdb_server *srv, int64_t *batches, int64_t *requests);
 /* Finishes the queued requests, then stops the workers. */
 void sdb_server_destroy(sdb_server *srv);
 
+/* Batched launches (no counterpart in the reference): the programs and
+ * mask descriptors of a launch reach the device in one copy, then the mask
+ * build and the batched kernel run.  With graphs on (1), a launch met a
+ * second time (same geometry and kernel arguments) is captured into a
+ * hipGraph and replayed for later such launches, from device and pinned
+ * host buffers kept per shape library; 0 (the default unless
+ * SDB_BATCH_GRAPH=1) issues the three operations on the stream each time;
+ * -1 restores the default.  With graphs on, batches over one fact table must
+ * not run concurrently from several threads. */
+int sdb_set_batch_graphs(int on);
+/* Graph captures and replays of batched launches so far (either may be NULL). */
+int sdb_batch_graph_stats(int64_t *captures, int64_t *replays);
+
 /* Device time (ms, hipEvents) and host wall time (ms) of the last execute. */
 int sdb_engine_last_timing(sdb_engine *eng, double *device_ms, double *wall_ms);
 /* Resets the scratch allocator (memory_manager::reset, re-zeroing the

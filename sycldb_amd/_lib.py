@@ -163,6 +163,8 @@ SIGNATURES = {
     "sdb_codegen_check": (INT, [CSTR, ctypes.POINTER(INT)]),
     "sdb_codegen_wait": (INT, [ctypes.c_double]),
     "sdb_codegen_pending": (INT, []),
+    "sdb_set_batch_graphs": (INT, [INT]),
+    "sdb_batch_graph_stats": (INT, [ctypes.POINTER(I64), ctypes.POINTER(I64)]),
     "sdb_engine_set_profiling": (INT, [P, INT]),
     "sdb_engine_set_profile_table": (INT, [P, CSTR]),
     "sdb_engine_profile_reset": (INT, [P]),

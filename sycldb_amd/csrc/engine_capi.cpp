@@ -425,7 +425,6 @@ std::vector<ResultTable> execute_batch(Engine &eng, const std::vector<PlanResult
         const size_t nseg = pipes[members[0]].bundles.size();
         for (int i : members)
             if (pipes[i].bundles.size() != nseg) fail(SDB_ERR_RUNTIME, "batched pipelines disagree on segments");
-        FProgram *dev_progs = eng.mm->alloc<FProgram>(members.size());
         try {
         for (size_t sgi = 0; sgi < nseg; sgi++) {
             std::vector<FProgram> progs;
@@ -434,13 +433,12 @@ std::vector<ResultTable> execute_batch(Engine &eng, const std::vector<PlanResult
                 if (pipes[i].bundles[sgi].n != rows) fail(SDB_ERR_RUNTIME, "batched pipelines disagree on rows");
                 progs.push_back(pipes[i].bundles[sgi].p);
             }
-            eng.registry.check(dev_progs, members.size() * sizeof(FProgram), "batched programs");
             Engine::ProfRec rec;
             const bool timed = eng.profiles(&fact);
             if (timed) eng.prof_begin("batch:" + fact, rows, rec);
             // one 4-row chunk per thread whatever the plain path's unroll: the
             // shared scan's double-buffered columns fill the registers
-            launch_fused_batch(progs, rows, eng.stream, dev_progs, 1, eng.find_table(fact), batch_alloc, &eng,
+            launch_fused_batch(progs, rows, eng.stream, 1, eng.find_table(fact), batch_alloc, &eng,
                                batch_masks());
             if (timed) eng.prof_end(rec);
             eng.launches++;
@@ -578,6 +576,19 @@ int sdb_codegen_tuning_report(char *buf, int cap)
 }
 
 int sdb_codegen_wait(double timeout_s) { return jit_wait_idle(timeout_s) ? 1 : 0; }
+
+int sdb_set_batch_graphs(int on)
+{
+    return guarded([&] {
+        require(on >= -1 && on <= 1, "sdb_set_batch_graphs: -1, 0 or 1");
+        set_batch_graphs(on);
+    });
+}
+
+int sdb_batch_graph_stats(int64_t *captures, int64_t *replays)
+{
+    return guarded([&] { batch_graph_stats(captures, replays); });
+}
 
 int sdb_codegen_pending(void) { return jit_pending(); }
 

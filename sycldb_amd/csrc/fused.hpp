@@ -66,15 +66,21 @@ int jit_kernels_compiled();
 typedef void *(*BatchAlloc)(void *ctx, size_t bytes);
 
 // Runs the fact-table programs of several pipelines over rows [0, n) in one
-// launch (one scan of their columns).  dev_progs: device room for
-// in.size() programs.  unroll: 1, 2 or 4 (0: default).  scope: identity of
-// the fact table, whose batches share one library of compiled shapes.
-// Probes on fact key columns that every row of a program must pass are
-// evaluated once per row for all programs through shared mask tables
-// (FMaskCol) allocated with `alloc`; masks = false keeps every probe in its
-// program.
-void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t s, FProgram *dev_progs, int unroll,
+// launch (one scan of their columns).  unroll: 1, 2 or 4 (0: default).
+// scope: identity of the fact table, whose batches share one library of
+// compiled shapes.  Probes on fact key columns that every row of a program
+// must pass are evaluated once per row for all programs through shared mask
+// tables (FMaskCol); masks = false keeps every probe in its program.  The
+// launch's device block (arguments, programs, mask tables) comes from
+// `alloc`, or, with batched graphs on, from a block kept per library and
+// launch shape whose launch is a replayed hipGraph.
+void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t s, int unroll,
                         const void *scope, BatchAlloc alloc, void *alloc_ctx, bool masks = true);
+// Batched launches as replayed hipGraphs (1), as plain stream operations
+// (0), or as SDB_BATCH_GRAPH says (-1, the default: off).
+void set_batch_graphs(int on);
+// Graph captures and replays of batched launches so far.
+void batch_graph_stats(int64_t *captures, int64_t *replays);
 // Builds the mask tables of a batched launch (ops.hip).
 void launch_batch_masks(const FMaskCol *cols, int ncols, uint32_t max_len, const FMaskEntry *ents, hipStream_t s);
 // Compiles (no device needed) a batched kernel over the coverage programs.

@@ -312,13 +312,39 @@ __global__ void __launch_bounds__(BLOCK) k_probe_semi(const int32_t *__restrict_
     }
 }
 
+// The value table of a probe: the reference's (flag, value) pairs, or --
+// when the caller's table is much smaller than the probe column -- a 16-bit
+// copy built on the device just before (k_pairs_range, k_pairs_to_u16): one
+// uint16 per slot, value - vmin where the flag is 1, VALS_UNSET elsewhere.
+// A quarter of the bytes per gather keeps a table of millions of slots in
+// L2; the device decides (info[2]) whether every value fitted, so the host
+// never waits for it.
+struct PairsOrU16 {
+    const int2 *pairs;
+    const uint16_t *u16; // null: pairs only
+    const int32_t *info; // vmin, vmax, values do not fit in 16 bits
+};
+
+__device__ __forceinline__ bool probe_value(const PairsOrU16 &t, bool use16, int32_t vmin, uint32_t slot, int32_t &v)
+{
+    if (use16) {
+        const uint32_t x = t.u16[slot];
+        v = vmin + (int32_t)x;
+        return x != VALS_UNSET;
+    }
+    const int2 e = t.pairs[slot];
+    v = e.y;
+    return e.x == 1;
+}
+
 __global__ void __launch_bounds__(BLOCK) k_probe_full(const int32_t *__restrict__ probe,
                                                       int32_t *val_out, const uint8_t *in,
                                                       uint8_t *out, int64_t n,
-                                                      const int32_t *__restrict__ ht, int32_t min,
+                                                      PairsOrU16 t, int32_t min,
                                                       int32_t max, int vec)
 {
-    const int2 *ht2 = reinterpret_cast<const int2 *>(ht);
+    const bool use16 = t.u16 && t.info[2] == 0;
+    const int32_t vmin = t.u16 ? t.info[0] : 0;
     if (vec) {
         const int64_t nchunks = n >> 2;
         const int4 *p4 = reinterpret_cast<const int4 *>(probe);
@@ -343,12 +369,11 @@ __global__ void __launch_bounds__(BLOCK) k_probe_full(const int32_t *__restrict_
                     const int32_t xs[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
-                        if (byte_of(f[u], k) && xs[k] >= min && xs[k] <= max) {
-                            const int2 e = ht2[slot_of(xs[k], min)];
-                            if (e.x == 1) {
-                                val_out[4 * c + k] = e.y;
-                                r |= 1u << (8 * k);
-                            }
+                        int32_t v;
+                        if (byte_of(f[u], k) && xs[k] >= min && xs[k] <= max &&
+                            probe_value(t, use16, vmin, slot_of(xs[k], min), v)) {
+                            val_out[4 * c + k] = v;
+                            r |= 1u << (8 * k);
                         }
                     }
                 }
@@ -360,12 +385,10 @@ __global__ void __launch_bounds__(BLOCK) k_probe_full(const int32_t *__restrict_
             bool r = false;
             if (in[i]) {
                 const int32_t x = probe[i];
-                if (x >= min && x <= max) {
-                    const int2 e = ht2[slot_of(x, min)];
-                    if (e.x == 1) {
-                        val_out[i] = e.y;
-                        r = true;
-                    }
+                int32_t v;
+                if (x >= min && x <= max && probe_value(t, use16, vmin, slot_of(x, min), v)) {
+                    val_out[i] = v;
+                    r = true;
                 }
             }
             out[i] = r;
@@ -376,15 +399,70 @@ __global__ void __launch_bounds__(BLOCK) k_probe_full(const int32_t *__restrict_
         bool r = false;
         if (in[i]) {
             const int32_t x = probe[i];
-            if (x >= min && x <= max) {
-                const int2 e = ht2[slot_of(x, min)];
-                if (e.x == 1) {
-                    val_out[i] = e.y;
-                    r = true;
-                }
+            int32_t v;
+            if (x >= min && x <= max && probe_value(t, use16, vmin, slot_of(x, min), v)) {
+                val_out[i] = v;
+                r = true;
             }
         }
         out[i] = r;
+    }
+}
+
+__global__ void k_pairs_range_init(int32_t *info)
+{
+    info[0] = INT32_MAX;
+    info[1] = INT32_MIN;
+    info[2] = 0;
+}
+
+// min and max of the values of the slots whose flag is 1
+__global__ void __launch_bounds__(BLOCK) k_pairs_range(const int2 *pairs, int64_t len, int32_t *info)
+{
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    for (int64_t h = (int64_t)blockIdx.x * BLOCK + threadIdx.x; h < len; h += (int64_t)gridDim.x * BLOCK) {
+        const int2 e = pairs[h];
+        if (e.x == 1) {
+            lo = min(lo, e.y);
+            hi = max(hi, e.y);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = min(lo, __shfl_xor(lo, off, 64));
+        hi = max(hi, __shfl_xor(hi, off, 64));
+    }
+    // one atomic pair per block: same-address atomics from every wave
+    // serialise in L2
+    __shared__ int32_t wlo[BLOCK / 64], whi[BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) {
+        wlo[threadIdx.x >> 6] = lo;
+        whi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < BLOCK / 64; w++) {
+            lo = min(lo, wlo[w]);
+            hi = max(hi, whi[w]);
+        }
+        if (lo <= hi) {
+            atomicMin(&info[0], lo);
+            atomicMax(&info[1], hi);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_pairs_to_u16(const int2 *pairs, int64_t len, int32_t *info, uint16_t *u16)
+{
+    const int32_t vmin = info[0];
+    // every value offset must stay below VALS_UNSET
+    const bool fits = info[1] < info[0] || (int64_t)info[1] - (int64_t)vmin < (int64_t)VALS_UNSET;
+    if (!fits) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) info[2] = 1;
+        return;
+    }
+    for (int64_t h = (int64_t)blockIdx.x * BLOCK + threadIdx.x; h < len; h += (int64_t)gridDim.x * BLOCK) {
+        const int2 e = pairs[h];
+        u16[h] = e.x == 1 ? (uint16_t)(e.y - vmin) : (uint16_t)VALS_UNSET;
     }
 }
 
@@ -525,20 +603,21 @@ __global__ void __launch_bounds__(BLOCK) k_groupby_global(GroupArgs g, const int
 // all lie in [min, max] map to a unique slot whose key values are recovered
 // from the slot index at flush; any other row takes the global path so the
 // key it stores is its own.
-__global__ void __launch_bounds__(BLOCK) k_groupby_lds(GroupArgs g, const int32_t *agg,
-                                                       const uint8_t *flags, int64_t n,
-                                                       uint64_t *sums, uint32_t *seen,
-                                                       uint32_t prod)
+constexpr int GB_BLOCK = 1024; // threads per block of the LDS group-by
+
+__global__ void __launch_bounds__(GB_BLOCK) k_groupby_lds(GroupArgs g, const int32_t *agg,
+                                                          const uint8_t *flags, int64_t n,
+                                                          uint64_t *sums, uint32_t *seen,
+                                                          uint32_t prod, int vec)
 {
     extern __shared__ unsigned long long lds[];
     unsigned long long *lsum = lds;
     uint32_t *lbits = reinterpret_cast<uint32_t *>(lds + prod);
     const uint32_t nwords = (prod + 31) / 32;
-    for (uint32_t s = threadIdx.x; s < prod; s += BLOCK) lsum[s] = 0;
-    for (uint32_t w = threadIdx.x; w < nwords; w += BLOCK) lbits[w] = 0;
+    for (uint32_t s = threadIdx.x; s < prod; s += GB_BLOCK) lsum[s] = 0;
+    for (uint32_t w = threadIdx.x; w < nwords; w += GB_BLOCK) lbits[w] = 0;
     __syncthreads();
-    for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
-        if (!flags[i]) continue;
+    auto row = [&](int64_t i) {
         int32_t kv[GB_MAX_KEYS];
         bool in_range = true;
         for (int j = 0; j < g.k; j++) {
@@ -547,14 +626,44 @@ __global__ void __launch_bounds__(BLOCK) k_groupby_lds(GroupArgs g, const int32_
         }
         if (!in_range) {
             group_row_global(g, kv, agg[i], sums, seen, prod);
-            continue;
+            return;
         }
         const uint32_t s = group_slot(g, kv, prod);
         atomicAdd(&lsum[s], (unsigned long long)(int64_t)agg[i]);
         atomicOr(&lbits[s >> 5], 1u << (s & 31));
+    };
+    if (vec) {
+        // four flags per load (4-byte aligned), UNROLL loads in flight per
+        // thread; only flagged rows read their keys and value
+        const int64_t nchunks = n >> 2;
+        const uint32_t *f32 = reinterpret_cast<const uint32_t *>(flags);
+        for (int64_t base = (int64_t)blockIdx.x * GB_BLOCK * UNROLL; base < nchunks;
+             base += (int64_t)gridDim.x * GB_BLOCK * UNROLL) {
+            uint32_t f[UNROLL];
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) {
+                const int64_t c = base + u * GB_BLOCK + threadIdx.x;
+                f[u] = c < nchunks ? f32[c] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) {
+                if (!f[u]) continue;
+                const int64_t c = base + u * GB_BLOCK + threadIdx.x;
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (byte_of(f[u], k)) row(4 * c + k);
+            }
+        }
+        if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+            const int64_t i = (nchunks << 2) + threadIdx.x;
+            if (flags[i]) row(i);
+        }
+    } else {
+        for (int64_t i = (int64_t)blockIdx.x * GB_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * GB_BLOCK)
+            if (flags[i]) row(i);
     }
     __syncthreads();
-    for (uint32_t s = threadIdx.x; s < prod; s += BLOCK) {
+    for (uint32_t s = threadIdx.x; s < prod; s += GB_BLOCK) {
         if (!((lbits[s >> 5] >> (s & 31)) & 1u)) continue;
         atomicAdd(reinterpret_cast<unsigned long long *>(&sums[s]), lsum[s]);
         if (atomicExch(&seen[s], 1u) == 0u) {
@@ -1044,14 +1153,36 @@ void launch_probe_semi(const int32_t *probe, const uint8_t *in, uint8_t *out, in
     check_launch("probe_semi");
 }
 
+// a probe over at least this many rows, and at least this many rows per
+// table slot, gathers from a 16-bit copy of a table of up to 2^26 slots
+constexpr int64_t PROBE_U16_MIN_ROWS = 4 << 20;
+constexpr int64_t PROBE_U16_MIN_RATIO = 16;
+constexpr int64_t PROBE_U16_MAX_SLOTS = int64_t(1) << 26;
+
 void launch_probe_full(const int32_t *probe, int32_t *val_out, const uint8_t *in, uint8_t *out, int64_t n,
                        const int32_t *ht, int32_t min, int32_t max, hipStream_t s)
 {
     if (n <= 0) return;
     if (!aligned(ht, 8)) fail(SDB_ERR_INVALID_ARG, "key-value hash table must be 8-byte aligned");
     const int vec = aligned(probe, 16) && aligned(in, 4) && aligned(out, 4);
+    PairsOrU16 t{reinterpret_cast<const int2 *>(ht), nullptr, nullptr};
+    // a table much smaller than the probe column gets a 16-bit copy first
+    // (two passes over the table; the probe then gathers 2 bytes per row
+    // instead of 8)
+    const int64_t len = max >= min ? (int64_t)max - min + 1 : 0;
+    if (n >= PROBE_U16_MIN_ROWS && len > 0 && len * PROBE_U16_MIN_RATIO <= n && len <= PROBE_U16_MAX_SLOTS) {
+        uint8_t *scr = static_cast<uint8_t *>(scratch_buffer(256 + (size_t)len * 2, s));
+        int32_t *info = reinterpret_cast<int32_t *>(scr);
+        uint16_t *u16 = reinterpret_cast<uint16_t *>(scr + 256);
+        const unsigned g = (unsigned)std::min<int64_t>((len + BLOCK - 1) / BLOCK, 1024);
+        hipLaunchKernelGGL(k_pairs_range_init, dim3(1), dim3(1), 0, s, info);
+        hipLaunchKernelGGL(k_pairs_range, dim3(g), dim3(BLOCK), 0, s, t.pairs, len, info);
+        hipLaunchKernelGGL(k_pairs_to_u16, dim3(g), dim3(BLOCK), 0, s, t.pairs, len, info, u16);
+        t.u16 = u16;
+        t.info = info;
+    }
     hipLaunchKernelGGL(k_probe_full, dim3(vec ? grid_rows(n) : grid_one(n)), dim3(BLOCK), 0, s, probe, val_out, in,
-                       out, n, ht, min, max, vec);
+                       out, n, t, min, max, vec);
     check_launch("probe_full");
 }
 
@@ -1081,6 +1212,8 @@ void launch_count_flags(const uint8_t *flags, int64_t n, uint64_t *out, bool acc
 // LDS budget of the privatised group-by (bytes); the slot array is u64 plus a
 // one-bit occupancy map.
 constexpr uint64_t GROUPBY_LDS_BYTES = 64 * 1024;
+// LDS of one CU (gfx950)
+constexpr size_t GROUPBY_LDS_PER_CU = 160 * 1024;
 
 uint64_t groupby_lds_max_slots() { return (GROUPBY_LDS_BYTES * 8) / (64 + 1); }
 
@@ -1111,9 +1244,14 @@ void launch_groupby_dense(const int32_t *const *keys, int k, const int32_t *mins
     fits = fits && exact == prod;
     const bool lds = strategy != 1 && fits && prod <= groupby_lds_max_slots();
     if (lds) {
+        // 1024-thread blocks share one LDS copy of the sums among 16 waves;
+        // as many blocks per CU as that LDS allows (at most 2)
         const size_t bytes = prod * 8 + ((prod + 31) / 32) * 4;
-        hipLaunchKernelGGL(k_groupby_lds, dim3(stream_grid(n, BLOCK, 2)), dim3(BLOCK), bytes, s, g, agg, flags, n,
-                           sums, seen, (uint32_t)prod);
+        const int per_cu = std::max(1, std::min(2, (int)(GROUPBY_LDS_PER_CU / std::max<size_t>(bytes, 1))));
+        const int vec = ((uintptr_t)flags % 4) == 0;
+        const int64_t units = vec ? (n + 3) / 4 : n;
+        hipLaunchKernelGGL(k_groupby_lds, dim3(stream_grid(units, (int64_t)GB_BLOCK * (vec ? UNROLL : 1), per_cu)),
+                           dim3(GB_BLOCK), bytes, s, g, agg, flags, n, sums, seen, (uint32_t)prod, vec);
     } else {
         hipLaunchKernelGGL(k_groupby_global, dim3(grid_one(n)), dim3(BLOCK), 0, s, g, agg, flags, n, sums, seen,
                            prod);

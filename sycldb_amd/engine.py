@@ -124,6 +124,19 @@ def codegen_pending():
     return int(L.lib.sdb_codegen_pending())
 
 
+def set_batch_graphs(on):
+    """Batched launches as replayed hipGraphs (True), as plain stream
+    operations (False), or the default (None: off unless SDB_BATCH_GRAPH=1)."""
+    L.check(L.lib.sdb_set_batch_graphs(-1 if on is None else int(bool(on))))
+
+
+def batch_graph_stats():
+    """(captures, replays) of batched-launch graphs so far."""
+    c, r = ctypes.c_int64(), ctypes.c_int64()
+    L.check(L.lib.sdb_batch_graph_stats(ctypes.byref(c), ctypes.byref(r)))
+    return c.value, r.value
+
+
 def codegen_tuning_report():
     """The autotuner's choice per program shape (text, one line each)."""
     n = L.lib.sdb_codegen_tuning_report(None, 0)

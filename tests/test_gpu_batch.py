@@ -165,3 +165,59 @@ def test_batch_irregular_date_keys(cpu_tables, pipelines):
             assert r.rows() == expected[i], (i, pipes[i])
     finally:
         eng.close()
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(segment_size=77_777)])
+def test_batch_graph_replays_match_oracle(pipelines, expected, cpu_tables, kw):
+    """Batched launches captured once into a hipGraph and replayed: three
+    batches in a row, with plain executions and synchronisations between
+    them, each against the oracle; then the same batch as plain stream
+    operations.  Several segments replay one graph per segment."""
+    import torch
+    from sycldb_amd.engine import batch_graph_stats, set_batch_graphs
+    eng = _engine(**kw)
+    try:
+        plans = [PL.build(t, **p) for t, p in pipelines]
+        set_batch_graphs(True)
+        c0, r0 = batch_graph_stats()
+        for rep in range(3):
+            print("graph batch", kw, rep, flush=True)
+            results, batched = eng.execute_batch(plans)
+            assert batched >= 24
+            assert [r.rows() for r in results] == expected, rep
+            # eager work of the program between the replays
+            assert eng.execute(plans[rep]).rows() == expected[rep]
+            torch.cuda.synchronize()
+        c1, r1 = batch_graph_stats()
+        # the first batch of a geometry runs as stream operations, the later
+        # ones replay its graph
+        assert r1 - r0 >= 2
+        assert 1 <= c1 - c0 <= 4
+        # the same shapes with other constants: the replayed graph copies the
+        # new programs and probe descriptors
+        pipes2 = []
+        for t, p in pipelines:
+            p = dict(p)
+            if t == "t1" and "qty" in p:
+                q = p["qty"]
+                p["qty"] = ("lt", q[1] + 3) if q[0] == "lt" else ("range", q[1], q[2] + 1)
+            if t == "t2" and "s_region" in p:
+                p["s_region"] = (p["s_region"] + 1) % 5
+            pipes2.append((t, p))
+        want2 = [PO.execute_plan(PL.build(t, **p), cpu_tables).rows() for t, p in pipes2]
+        plans2 = [PL.build(t, **p) for t, p in pipes2]
+        for rep in range(2):
+            print("graph batch, other constants", kw, rep, flush=True)
+            results, _ = eng.execute_batch(plans2)
+            assert [r.rows() for r in results] == want2, rep
+            results, _ = eng.execute_batch(plans)
+            assert [r.rows() for r in results] == expected, rep
+        c1, r1 = batch_graph_stats()
+        set_batch_graphs(False)
+        results, _ = eng.execute_batch(plans)
+        assert [r.rows() for r in results] == expected
+        c2, r2 = batch_graph_stats()
+        assert r2 == r1
+    finally:
+        set_batch_graphs(None)
+        eng.close()

@@ -309,3 +309,36 @@ def test_compress_sync_scatters_into_host_columns(ops):
         sel = fl.astype(bool)
         for c, h in zip(cols, host):
             assert (h[sel] == c[sel]).all() and (h[~sel] == -12345).all()
+
+
+@pytest.mark.parametrize("case", ["u16", "wide_values", "odd_flags", "misaligned"])
+def test_probe_full_large_probe_small_table(ops, case):
+    """Probes over at least 4M rows and 16 rows per table slot gather from a
+    16-bit copy of the pairs built on the device; values spanning 2^16 or
+    more fall back to the pairs.  Flags other than 0/1 in the table never
+    match (the reference tests == 1).  All against the oracle."""
+    rng = np.random.default_rng(23)
+    n, mn, mx = 4_500_007, 1000, 200_999
+    length = mx - mn + 1
+    keys = rng.integers(mn - 500, mx + 500, n).astype(np.int32)       # some out of range
+    pfl = (rng.random(n) < 0.7).astype(np.uint8)
+    ht2 = np.zeros(2 * length, np.int32)
+    ht2[0::2] = (rng.random(length) < 0.6).astype(np.int32)
+    lo_v = -40_000
+    ht2[1::2] = rng.integers(lo_v, lo_v + 60_000, length).astype(np.int32)
+    if case == "wide_values":
+        ht2[1::2][::1000] = 10**9                                      # range >= 2^16: pairs path
+    if case == "odd_flags":
+        ht2[0::2][::7] = 2                                             # neither 0 nor 1: no match
+    off = 1 if case == "misaligned" else 0
+    print("probe_full large case", case, flush=True)
+    tp_buf, tp = padded(keys, off)
+    tf_buf, tf = padded(pfl, off)
+    out = torch.zeros(n + off, dtype=torch.uint8, device="cuda")[off:]
+    vo = torch.full((n,), -77, dtype=torch.int32, device="cuda")
+    ops.probe_full(tp, vo, tf, out, dev(ht2), mn, mx)
+    want_v = np.full(n, -77, np.int32)
+    want = np.zeros(n, np.uint8)
+    O.probe_full(keys, want_v, pfl, want, ht2, mn, mx)
+    assert (host(out) == want).all()
+    assert (host(vo) == want_v).all()
