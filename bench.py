@@ -135,8 +135,10 @@ def served_measure(eng_big, seed, n_requests, rates, max_batch):
     600M rows) with their dimensions; Poisson arrivals at a fixed rate of
     pipelines from the four templates with random constants and table sizes.
     At every rate the same trace is served one request at a time in arrival
-    order (the plain sequential baseline) and with continuous batching; every
-    result of the batched run must equal the sequential one."""
+    order (the plain sequential baseline), with continuous batching by one
+    worker per engine (the engines overlap on the device), and with
+    continuous batching by one worker for all engines; every result of the
+    batched runs must equal the sequential one."""
     import numpy as np
     from sycldb_amd import plans as PL
     from sycldb_amd.engine import Engine, Server, replay
@@ -163,8 +165,8 @@ def served_measure(eng_big, seed, n_requests, rates, max_batch):
                 if len(sub[k:k + max_batch]) > 1:
                     engines[e].execute_batch(sub[k:k + max_batch])
 
-        def serve(serial, arrivals):
-            with Server(engines, max_batch=max_batch, serial=serial) as srv, no_gc():
+        def serve(serial, arrivals, single_worker=False):
+            with Server(engines, max_batch=max_batch, serial=serial, single_worker=single_worker) as srv, no_gc():
                 t0 = time.perf_counter()
                 out = replay(srv, reqs, arrivals)
                 wall = time.perf_counter() - t0
@@ -179,10 +181,14 @@ def served_measure(eng_big, seed, n_requests, rates, max_batch):
         for qps in rates:
             arrivals = np.cumsum(gaps / qps)
             out_seq, seq = serve(True, arrivals)
+            out_one, one = serve(False, arrivals, single_worker=True)
             out_bat, bat = serve(False, arrivals)
-            identical = all(a.rows() == b.rows() for (a, _), (b, _) in zip(out_seq, out_bat))
+            identical = all(a.rows() == b.rows() == c.rows()
+                            for (a, _), (b, _), (c, _) in zip(out_seq, out_bat, out_one))
             assert identical, "served results differ from the sequential ones"
-            by_rate["%g" % qps] = {"sequential": seq, "batched": bat,
+            for m in (seq, one, bat):
+                m["of_offered"] = m["pipelines_per_s"] / qps
+            by_rate["%g" % qps] = {"sequential": seq, "batched": bat, "batched_one_worker": one,
                                    "throughput_ratio": bat["pipelines_per_s"] / seq["pipelines_per_s"],
                                    "p50_ratio": seq["p50_ms"] / bat["p50_ms"], "p99_ratio": seq["p99_ms"] / bat["p99_ms"],
                                    "results_identical": identical}
@@ -318,7 +324,7 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the q1.1 (6M rows) and q3.1 (60M rows) configurations at their own sizes")
-    ap.add_argument("--serve-requests", type=int, default=600)
+    ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-qps", default="1000,3000",
                     help="arrival rates (requests/s), comma separated")
     ap.add_argument("--serve-max-batch", type=int, default=32)

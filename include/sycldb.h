@@ -361,18 +361,18 @@ int sdb_engine_execute_batch(sdb_engine *eng, const char *const *plans, int n, s
 /* A serving loop over engines with resident tables (no counterpart in the
  * reference, which runs one query at a time).  Requests are plans for one
  * engine, queued per engine with their arrival time.  With batching
- * (serial == 0) one worker, whenever it is free, takes the engine whose
- * oldest waiting request arrived first.  When enough requests wait for that
- * engine for sharing its scan to pay (a threshold learned per engine from
- * the plain and batched run times it has seen), it admits them in arrival
- * order while a slot (max_batch) and the memory budget (the requests'
- * estimated build-table and group-by footprints) allow -- always at least
- * one -- and runs them as one batch sharing the fact scan
- * (sdb_engine_execute_batch); otherwise it runs the oldest request alone.
- * Requests arriving meanwhile join a later batch; each result is handed to
- * its waiter as soon as it is ready.  One batch runs at a time (a batched
- * launch fills the device).  With
- * serial != 0 the worker runs every request alone in arrival order (the
+ * (serial == 0) each engine has its own worker (single_worker: one worker
+ * for all, taking the engine whose oldest waiting request arrived first), so
+ * the engines' host work and kernels overlap on the device.  When enough
+ * requests wait for an engine for sharing its scan to pay (a threshold
+ * learned per engine from the plain and batched run times it has seen), its
+ * worker admits them in arrival order while a slot (max_batch) and the
+ * memory budget (the requests' estimated build-table and group-by
+ * footprints) allow -- always at least one -- and runs them as one batch
+ * sharing the fact scan (sdb_engine_execute_batch); otherwise it runs the
+ * oldest request alone.  Requests arriving meanwhile join a later batch;
+ * each result is handed to its waiter as soon as it is ready.  With
+ * serial != 0 one worker runs every request alone in arrival order (the
  * sequential baseline).  While a server runs, its engines must not be used
  * directly. */
 typedef struct sdb_server sdb_server;
@@ -380,6 +380,9 @@ typedef struct sdb_server_options {
     int max_batch;                /* slots per batch (default 32) */
     uint64_t memory_budget_bytes; /* admission budget per batch (default 2 GiB) */
     int serial;                   /* 1: one request at a time, arrival order */
+    int single_worker;            /* 1: one worker for every engine (default:
+                                     one per engine, so that the engines'
+                                     host work and kernels overlap) */
 } sdb_server_options;
 typedef struct sdb_request_info {
     double latency_ms; /* completion - arrival */

@@ -69,7 +69,8 @@ class EngineOptions(ctypes.Structure):
 class ServerOptions(ctypes.Structure):
     _fields_ = [("max_batch", ctypes.c_int),
                 ("memory_budget_bytes", ctypes.c_uint64),
-                ("serial", ctypes.c_int)]
+                ("serial", ctypes.c_int),
+                ("single_worker", ctypes.c_int)]
 
 
 class RequestInfo(ctypes.Structure):

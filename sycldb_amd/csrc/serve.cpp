@@ -124,20 +124,23 @@ struct Server {
         return q;
     }
 
-    void worker()
+    // The worker of engine `own`, or (own < 0) the one worker of every
+    // engine, which takes the engine whose first request is the oldest.
+    void worker(int own)
     {
         while (true) {
             std::vector<std::shared_ptr<Request>> admitted;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                work_cv.wait(lk, [&] { return stopping || oldest_queue() >= 0; });
-                const int q = oldest_queue();
+                auto pick = [&] { return own >= 0 ? (queues[own].empty() ? -1 : own) : oldest_queue(); };
+                work_cv.wait(lk, [&] { return stopping || pick() >= 0; });
+                const int q = pick();
                 if (q < 0) return; // stopping with nothing left
                 // a batch only when it lowers the summed latency of the
                 // waiting requests; otherwise the oldest request alone, the
                 // rest staying queued in arrival order with the other
                 // engines' requests
-                const int want = batch_pays(q) ? max_batch : 1;
+                const int want = batch_pays(q, own < 0) ? max_batch : 1;
                 uint64_t used = 0;
                 while (!queues[q].empty() && (int)admitted.size() < want) {
                     auto &r = queues[q].front();
@@ -161,12 +164,16 @@ struct Server {
     // for the other engines: the batch finishes all n (and lets the m start)
     // at A + B n; one by one the i-th finishes at i p and the m start after
     // n p.  Sharing pays when (A + B n)(n + m) < p n (n + 1) / 2 + n p m.
-    bool batch_pays(int q) const
+    bool batch_pays(int q, bool shared_worker) const
     {
         const double p = plain_ms[q];
         if (p <= 0) return false; // the engine's first request runs alone
         const double n = (double)std::min<size_t>(queues[q].size(), (size_t)max_batch);
         if (n < 2) return false;
+        // the other engines' waiting requests: with one worker for all they
+        // wait for this batch outright; with a worker per engine they still
+        // share the device with it, whose launches fill every CU
+        (void)shared_worker;
         double m = 0;
         for (int i = 0; i < (int)queues.size(); i++)
             if (i != q) m += (double)queues[i].size();
@@ -279,7 +286,13 @@ int sdb_server_create(sdb_engine *const *engines, int n_engines, const sdb_serve
         s.queues.resize(n_engines);
         s.plain_ms.assign(n_engines, 0.0);
         s.batch_fixed_ms.assign(n_engines, 0.0);
-        s.workers.emplace_back([&s] { s.worker(); });
+        // one worker per engine (each engine has its own stream, so one
+        // engine's host work and kernels overlap another's), except in the
+        // serial baseline and with single_worker
+        if (s.serial || (opts && opts->single_worker))
+            s.workers.emplace_back([&s] { s.worker(-1); });
+        else
+            for (int i = 0; i < n_engines; i++) s.workers.emplace_back([&s, i] { s.worker(i); });
         *out = srv.release();
     });
 }

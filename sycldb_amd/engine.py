@@ -352,15 +352,17 @@ def dag_order(plan):
 class Server:
     """Serving loop over engines with resident tables (sdb_server_*):
     requests are plans for one engine; with batching, each engine's worker
-    admits the waiting requests (slots and memory budget permitting) and runs
-    them as one batch sharing the fact scan, while later arrivals queue for
-    the next batch; serial=True runs every request alone in arrival order.
+    (one per engine, so engines overlap on the device; single_worker=True:
+    one for all) admits the waiting requests (slots and memory budget
+    permitting) and runs them as one batch sharing the fact scan, while later
+    arrivals queue for the next batch; serial=True runs every request alone
+    in arrival order.
     The engines must not be used directly while the server runs."""
 
-    def __init__(self, engines, max_batch=32, memory_budget_bytes=0, serial=False):
+    def __init__(self, engines, max_batch=32, memory_budget_bytes=0, serial=False, single_worker=False):
         self._engines = list(engines)
         arr = (ctypes.c_void_p * len(self._engines))(*[e._h for e in self._engines])
-        opts = L.ServerOptions(max_batch, memory_budget_bytes, 1 if serial else 0)
+        opts = L.ServerOptions(max_batch, memory_budget_bytes, 1 if serial else 0, 1 if single_worker else 0)
         h = ctypes.c_void_p()
         L.check(L.lib.sdb_server_create(arr, len(self._engines), ctypes.byref(opts), ctypes.byref(h)))
         self._h = h

@@ -70,11 +70,13 @@ def test_batched_serving_matches_oracle(engines, trace, alone, oracle_rows):
 
 
 def test_burst_is_batched(engines, trace, alone, oracle_rows):
-    """48 requests at once: ~16 per engine queue behind the first ones, more
-    than the initial batching threshold, so they share scans."""
+    """48 requests at once to one worker for all engines: ~16 per engine
+    queue behind the first ones, more than the initial batching threshold, so
+    they share scans.  (With a worker per engine these small tables drain
+    as fast as the requests are submitted, and nothing needs batching.)"""
     from sycldb_amd.engine import Server, replay
     reqs, arrivals = trace
-    with Server(engines, max_batch=16) as srv:
+    with Server(engines, max_batch=16, single_worker=True) as srv:
         out = replay(srv, reqs, arrivals * 0)
         stats = srv.stats()
     for i, (res, info) in enumerate(out):
@@ -117,3 +119,26 @@ def test_bad_request_fails_alone(engines):
             srv.wait(t_bad)
         res, _ = srv.wait(t_ok)
         assert res.rows() == engines[0].execute(PL.build("t1")).rows()
+
+
+def test_single_worker_serving_matches_oracle(engines, trace, alone, oracle_rows):
+    """The one-worker form (no overlap between engines) gives the same
+    results."""
+    from sycldb_amd.engine import Server, replay
+    reqs, arrivals = trace
+    with Server(engines, max_batch=16, single_worker=True) as srv:
+        out = replay(srv, reqs, arrivals)
+    for i, (res, info) in enumerate(out):
+        assert res.rows() == oracle_rows[i], i
+
+
+def test_burst_with_a_worker_per_engine(engines, trace, alone, oracle_rows):
+    """The same burst with a worker per engine: every result the oracle's."""
+    from sycldb_amd.engine import Server, replay
+    reqs, arrivals = trace
+    with Server(engines, max_batch=16) as srv:
+        out = replay(srv, reqs, arrivals * 0)
+        stats = srv.stats()
+    for i, (res, info) in enumerate(out):
+        assert res.rows() == oracle_rows[i], i
+    assert stats["requests"] == len(reqs)
