@@ -2332,13 +2332,22 @@ std::map<std::string, BatchTune> g_batch_tune; // guarded by g_tune_mu
 
 // Launch shape of the batched kernel: threads per block, and whether the
 // next tile's columns load while the current one runs.
+// Launch shape and mask-table layout of the batched kernel: threads per
+// block, whether the next tile's columns load while the current one runs,
+// and whether every mask table stores whole 32-bit words (wide) or only the
+// 8/16/32-bit window covering the programs that probe its column (narrow:
+// smaller tables, an extra shift and mask per lookup).  The layout is a
+// kernel argument, so both layouts run the same compiled kernel.
 struct BatchShape {
     int block;
     bool double_buffer;
-    int id() const { return block * 2 + (double_buffer ? 1 : 0); }
+    bool wide = false;
+    int id() const { return block * 2 + (double_buffer ? 1 : 0); } // the compiled source
+    int key() const { return id() * 2 + (wide ? 1 : 0); }
 };
 // autotuner candidates; the first is the default below the tuning size
-const BatchShape BATCH_CANDIDATES[] = {{512, false}, {256, false}, {1024, false}, {1024, true}};
+const BatchShape BATCH_CANDIDATES[] = {{512, false, false}, {256, false, false}, {1024, false, false},
+                                       {1024, true, false}, {256, false, true}, {512, false, true}};
 constexpr int N_BATCH_CAND = (int)(sizeof(BATCH_CANDIDATES) / sizeof(BATCH_CANDIDATES[0]));
 
 std::mutex g_lib_mu;
@@ -2497,16 +2506,18 @@ struct BatchPlan {
     bool vec = true;
 };
 
-// A launch shape fixed by SDB_BATCH_BLOCK / SDB_BATCH_DOUBLE_BUFFER (no
-// tuning), if either is set.
+// A launch shape fixed by SDB_BATCH_BLOCK / SDB_BATCH_DOUBLE_BUFFER /
+// SDB_BATCH_WIDE_MASKS (no tuning), if any is set.
 bool batch_shape_fixed(BatchShape &out)
 {
     const char *b = getenv("SDB_BATCH_BLOCK");
     const char *d = getenv("SDB_BATCH_DOUBLE_BUFFER");
-    if (!(b && *b) && !(d && *d)) return false;
+    const char *w = getenv("SDB_BATCH_WIDE_MASKS");
+    if (!(b && *b) && !(d && *d) && !(w && *w)) return false;
     const int blk = b && *b ? atoi(b) : BATCH_BLOCK;
     out.block = blk == 256 || blk == 512 || blk == 1024 ? blk : BATCH_BLOCK;
     out.double_buffer = d && *d == '1';
+    out.wide = w && *w == '1';
     return true;
 }
 
@@ -2673,7 +2684,7 @@ BatchPlan plan_batch(const std::vector<FProgram> &in, int unroll, const std::str
             hi_bit = std::max(hi_bit, e.bit);
         }
         const uint32_t span = hi_bit - lo_bit + 1;
-        m.wbytes = span <= 8 ? 1 : span <= 16 ? 2 : 4;
+        m.wbytes = shape.wide ? 4 : span <= 8 ? 1 : span <= 16 ? 2 : 4;
         m.shift = m.wbytes == 4 ? 0 : std::min(lo_bit, 32 - 8 * m.wbytes);
     }
     if ((int)b.masks.size() > F_MASK_MAX_COLS) fail(SDB_ERR_UNSUPPORTED, "batched launch: too many masked columns");
@@ -2878,7 +2889,7 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     BatchGraph *G = nullptr;
     if (trial < 0 && batch_graphs_on()) {
         std::lock_guard<std::mutex> lk(g_graph_mu);
-        auto &slot = g_graphs[lib_key + "#" + std::to_string(shape.id())];
+        auto &slot = g_graphs[lib_key + "#" + std::to_string(shape.key())];
         if (!slot) slot.reset(new BatchGraph);
         G = slot.get();
     }
@@ -3137,12 +3148,14 @@ std::string jit_tuning_report()
         o << "batched kernel " << kv.first << ": ";
         if (t.chosen >= 0)
             o << "block " << BATCH_CANDIDATES[t.chosen].block
-              << (BATCH_CANDIDATES[t.chosen].double_buffer ? ", double-buffered" : ", one tile in registers");
+              << (BATCH_CANDIDATES[t.chosen].double_buffer ? ", double-buffered" : ", one tile in registers")
+              << (BATCH_CANDIDATES[t.chosen].wide ? ", wide mask words" : ", narrow mask windows");
         else
             o << "tuning (" << t.trials << " trials)";
         o << "; ns per row and program";
         for (int c = 0; c < N_BATCH_CAND; c++) {
-            o << " [b" << BATCH_CANDIDATES[c].block << (BATCH_CANDIDATES[c].double_buffer ? "d" : "") << "] ";
+            o << " [b" << BATCH_CANDIDATES[c].block << (BATCH_CANDIDATES[c].double_buffer ? "d" : "")
+              << (BATCH_CANDIDATES[c].wide ? "w" : "") << "] ";
             if (t.best[c] < 1e299) o << t.best[c];
             else o << "-";
         }

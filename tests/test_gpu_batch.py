@@ -221,3 +221,22 @@ def test_batch_graph_replays_match_oracle(pipelines, expected, cpu_tables, kw):
     finally:
         set_batch_graphs(None)
         eng.close()
+
+
+@pytest.mark.parametrize("env", [{"SDB_BATCH_WIDE_MASKS": "1"}, {"SDB_BATCH_WIDE_MASKS": "1", "SDB_BATCH_BLOCK": "1024"},
+                                 {"SDB_BATCH_BLOCK": "1024", "SDB_BATCH_DOUBLE_BUFFER": "1"}])
+def test_batch_layouts_match_oracle(pipelines, expected, monkeypatch, env):
+    """The autotuner's other candidates, forced: whole 32-bit mask words
+    instead of narrow windows, and other launch shapes."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    eng = _engine()
+    try:
+        plans = [PL.build(t, **p) for t, p in pipelines]
+        print("batch layout", env, flush=True)
+        results, batched = eng.execute_batch(plans)
+        assert batched >= 24
+        for i, r in enumerate(results):
+            assert r.rows() == expected[i], (env, i, pipelines[i])
+    finally:
+        eng.close()
