@@ -120,13 +120,37 @@ def batched_measure(eng, n_pipes, seed, reps):
     graph_s = time_batches()
     c1, r1 = batch_graph_stats()
     set_batch_graphs(None)
+    # per template: its 8 pipelines one by one and as one batch
+    per_template = {}
+    for tmpl in ("t1", "t2", "t3", "t4"):
+        sub = [plans[i] for i, (t, _) in enumerate(pipes) if t == tmpl]
+        if len(sub) < 2:
+            continue
+        sub_alone = [alone[i] for i, (t, _) in enumerate(pipes) if t == tmpl]
+        results, _ = eng.execute_batch(sub)
+        assert [r.rows() for r in results] == sub_alone, "batched %s results differ" % tmpl
+        torch.cuda.synchronize()
+        with no_gc():
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                for p in sub:
+                    eng.execute(p)
+            torch.cuda.synchronize()
+            t_seq = (time.perf_counter() - t0) / reps
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                eng.execute_batch(sub)
+            torch.cuda.synchronize()
+            t_bat = (time.perf_counter() - t0) / reps
+        per_template[tmpl] = {"pipelines": len(sub), "sequential_ms": t_seq * 1e3, "batched_ms": t_bat * 1e3,
+                              "speedup": t_seq / t_bat}
     log("batched kernel tuning:\n" + "\n".join(l for l in codegen_tuning_report().splitlines() if "batched" in l))
     return {"pipelines": n_pipes, "pipeline_seed": seed, "sharing_the_scan": nb,
             "sequential_ms": seq_s * 1e3, "batched_ms": bat_s * 1e3,
             "sequential_pipelines_per_s": n_pipes / seq_s, "batched_pipelines_per_s": n_pipes / bat_s,
             "speedup": seq_s / bat_s, "results_identical": True,
             "launch": "stream operations (copy, mask build, batched kernel)",
-            "batched_ms_graph_replay": graph_s * 1e3,
+            "batched_ms_graph_replay": graph_s * 1e3, "per_template": per_template,
             "graph": "%d captures, %d replays while timed" % (c1 - c0, r1 - r0)}
 
 

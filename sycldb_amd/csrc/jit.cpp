@@ -2140,6 +2140,7 @@ const char *BATCH_HEAD = R"(struct BatchArgs {
     uint32_t mdflt[NMASK > 0 ? NMASK : 1];
     uint32_t mshift[NMASK > 0 ? NMASK : 1];
     uint32_t mwbytes[NMASK > 0 ? NMASK : 1];
+    uint32_t mlds[NMASK > 0 ? NMASK : 1]; // 1 + LDS byte offset of a staged table (0: read from memory)
     int32_t bitpos[32]; // program i's bit in the mask words
 };
 template <int R, int UU> struct Pf {
@@ -2161,11 +2162,14 @@ __device__ __forceinline__ void pf_load(const BatchArgs &A, const int64_t (&row0
 // take the column's default word.  An entry is the window of the word that
 // covers the column's probing programs, widened back with ones outside it.
 template <int R, int UU>
-__device__ __forceinline__ void mask_rows(const BatchArgs &A, const Pf<R, UU> &pf, uint32_t (&mw)[UU][R])
+__device__ __forceinline__ void mask_rows(const BatchArgs &A, const Pf<R, UU> &pf, const uint8_t *lds,
+                                          uint32_t (&mw)[UU][R])
 {
     uint32_t w[NMASK > 0 ? NMASK : 1][UU][R];
 #pragma unroll
-    for (int m = 0; m < NMASK; m++)
+    for (int m = 0; m < NMASK; m++) {
+        // a table staged in LDS is read there (the branch is uniform)
+        const bool staged = A.mlds[m] != 0;
 #pragma unroll
         for (int u = 0; u < UU; u++) {
             int32_t x[R];
@@ -2174,9 +2178,13 @@ __device__ __forceinline__ void mask_rows(const BatchArgs &A, const Pf<R, UU> &p
             for (int k = 0; k < R; k++) {
                 const uint32_t idx = (uint32_t)x[k] - (uint32_t)A.mlo[m];
                 const uint32_t byte = (idx < A.mlen[m] ? idx : A.mlen[m] - 1u) * A.mwbytes[m];
-                w[m][u][k] = *reinterpret_cast<const uint32_t *>(A.mtab[m] + (byte & ~3u)) >> ((byte & 3u) * 8u);
+                uint32_t word;
+                if (staged) word = *reinterpret_cast<const uint32_t *>(lds + (A.mlds[m] - 1u) + (byte & ~3u));
+                else word = *reinterpret_cast<const uint32_t *>(A.mtab[m] + (byte & ~3u));
+                w[m][u][k] = word >> ((byte & 3u) * 8u);
             }
         }
+    }
 #pragma unroll
     for (int u = 0; u < UU; u++)
 #pragma unroll
@@ -2225,6 +2233,14 @@ extern "C" __global__ void __launch_bounds__(BLOCK) sdb_fused_batch(const BatchA
     }
     unsigned long long *sums = reinterpret_cast<unsigned long long *>(lds + A.sums_off);
     for (int i = threadIdx.x; i < A.nprog; i += BLOCK) sums[i] = 0;
+    // mask tables staged in LDS (their sizes are padded to 16 bytes)
+    for (int m = 0; m < NMASK; m++)
+        if (A.mlds[m]) {
+            const uint4 *src = reinterpret_cast<const uint4 *>(A.mtab[m]);
+            uint4 *dst = reinterpret_cast<uint4 *>(lds + (A.mlds[m] - 1u));
+            const uint32_t n16 = (A.mlen[m] * A.mwbytes[m] + 15u) / 16u;
+            for (uint32_t v = threadIdx.x; v < n16; v += BLOCK) dst[v] = src[v];
+        }
     __syncthreads();
     const int64_t n = A.n;
     if constexpr (VEC) {
@@ -2321,7 +2337,7 @@ struct BatchLibrary {
 // Batched-kernel autotuner state per library: the best time per row and
 // program measured for each candidate launch shape.
 struct BatchTune {
-    double best[8];
+    double best[16];
     int trials = 0, chosen = -1;
     BatchTune()
     {
@@ -2342,12 +2358,17 @@ struct BatchShape {
     int block;
     bool double_buffer;
     bool wide = false;
+    bool stage = false; // the smallest mask tables copied into LDS per block
     int id() const { return block * 2 + (double_buffer ? 1 : 0); } // the compiled source
-    int key() const { return id() * 2 + (wide ? 1 : 0); }
+    int key() const { return (id() * 2 + (wide ? 1 : 0)) * 2 + (stage ? 1 : 0); }
 };
 // autotuner candidates; the first is the default below the tuning size
-const BatchShape BATCH_CANDIDATES[] = {{512, false, false}, {256, false, false}, {1024, false, false},
-                                       {1024, true, false}, {256, false, true}, {512, false, true}};
+const BatchShape BATCH_CANDIDATES[] = {{512, false, false, false}, {256, false, false, false},
+                                       {1024, false, false, false}, {1024, true, false, false},
+                                       {256, false, true, false},  {512, false, true, false},
+                                       {256, false, false, true},  {1024, false, false, true}};
+// LDS a batched block may take with staged mask tables (gfx950: 160 KB per CU)
+constexpr uint64_t BATCH_STAGE_LDS_MAX = 144 * 1024;
 constexpr int N_BATCH_CAND = (int)(sizeof(BATCH_CANDIDATES) / sizeof(BATCH_CANDIDATES[0]));
 
 std::mutex g_lib_mu;
@@ -2490,6 +2511,8 @@ struct MaskPlan {
     int64_t lo = 0, hi = -1;
     std::vector<FMaskEntry> ents;
     uint32_t shift = 0, wbytes = 4; // the window of the word the table stores
+    uint32_t lds = 0;               // 1 + byte offset of its LDS copy (0: read from memory)
+    uint64_t bytes() const { return ((uint64_t)std::max<int64_t>(hi - lo + 1, 1) * wbytes + 15) / 16 * 16; }
 };
 
 struct BatchPlan {
@@ -2507,17 +2530,19 @@ struct BatchPlan {
 };
 
 // A launch shape fixed by SDB_BATCH_BLOCK / SDB_BATCH_DOUBLE_BUFFER /
-// SDB_BATCH_WIDE_MASKS (no tuning), if any is set.
+// SDB_BATCH_WIDE_MASKS / SDB_BATCH_STAGE_MASKS (no tuning), if any is set.
 bool batch_shape_fixed(BatchShape &out)
 {
     const char *b = getenv("SDB_BATCH_BLOCK");
     const char *d = getenv("SDB_BATCH_DOUBLE_BUFFER");
     const char *w = getenv("SDB_BATCH_WIDE_MASKS");
-    if (!(b && *b) && !(d && *d) && !(w && *w)) return false;
+    const char *g = getenv("SDB_BATCH_STAGE_MASKS");
+    if (!(b && *b) && !(d && *d) && !(w && *w) && !(g && *g)) return false;
     const int blk = b && *b ? atoi(b) : BATCH_BLOCK;
     out.block = blk == 256 || blk == 512 || blk == 1024 ? blk : BATCH_BLOCK;
     out.double_buffer = d && *d == '1';
     out.wide = w && *w == '1';
+    out.stage = g && *g == '1';
     return true;
 }
 
@@ -2540,7 +2565,7 @@ std::string batch_source(const BatchLibrary &lib, int unroll, bool vec, const Ba
           << lib.bodies[k] << "}\n";
     s << "template <int R, int UU>\n__device__ __forceinline__ void run_all(const BatchArgs &A, const int64_t "
          "(&row0)[UU], const Pf<R, UU> &pf, uint8_t *lds, unsigned long long *sums)\n{\n"
-         "    uint32_t mw[UU][R];\n    mask_rows<R, UU>(A, pf, mw);\n    uint32_t any = 0;\n"
+         "    uint32_t mw[UU][R];\n    mask_rows<R, UU>(A, pf, lds, mw);\n    uint32_t any = 0;\n"
          "#pragma unroll\n    for (int u = 0; u < UU; u++)\n#pragma unroll\n"
          "        for (int k = 0; k < R; k++) any |= mw[u][k];\n"
          "    // programs some row of the wave keeps (read from the first lane so\n"
@@ -2710,6 +2735,20 @@ BatchPlan plan_batch(const std::vector<FProgram> &in, int unroll, const std::str
     }
     b.sums_off = off;
     b.lds = off + 8u * (uint32_t)b.progs.size();
+    if (shape.stage) {
+        // the smallest mask tables into LDS, while the block's LDS stays
+        // within BATCH_STAGE_LDS_MAX
+        std::vector<MaskPlan *> by;
+        for (MaskPlan &m : b.masks) by.push_back(&m);
+        std::stable_sort(by.begin(), by.end(), [](const MaskPlan *a, const MaskPlan *c) { return a->bytes() < c->bytes(); });
+        uint32_t at = (b.lds + 15u) & ~15u;
+        for (MaskPlan *m : by) {
+            if ((uint64_t)at + m->bytes() > BATCH_STAGE_LDS_MAX) break;
+            m->lds = at + 1u;
+            at += (uint32_t)m->bytes();
+        }
+        if (at > b.lds) b.lds = at;
+    }
     BatchLibrary::Src &src = lib.srcs[shape.id()];
     if (src.state[0] != lib.bodies.size() || src.state[1] != lib.slots.size() ||
         src.state[2] != lib.mask_slots.size()) {
@@ -2842,7 +2881,9 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
         }
     }
     const BatchPlan b = plan_batch(in, unroll, lib_key, masks, shape);
-    if (b.lds > F_BATCH_LDS + 8u * F_BATCH_MAX) fail(SDB_ERR_UNSUPPORTED, "batched launch: LDS regions exceed the budget");
+    static_assert(N_BATCH_CAND <= 16, "BatchTune::best holds 16 candidates");
+    if (b.lds > std::max<uint64_t>(F_BATCH_LDS + 8u * F_BATCH_MAX, BATCH_STAGE_LDS_MAX))
+        fail(SDB_ERR_UNSUPPORTED, "batched launch: LDS regions exceed the budget");
     Compiled *kp = nullptr;
     {
         std::lock_guard<std::mutex> lk(g_lib_mu);
@@ -2859,10 +2900,10 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     // ---- the block: [args | programs | mask descriptors | entries | ones word, tables]
     const int nmask = (int)b.mask_slots.size();
     const int nshape = (int)b.first.size() - 1;
-    size_t nents = 0, tab_bytes = 4; // the first 4 bytes: the all-ones word
+    size_t nents = 0, tab_bytes = 16; // the first 16 bytes: the all-ones word
     for (const MaskPlan &m : b.masks) {
         nents += m.ents.size();
-        tab_bytes += ((size_t)std::max<int64_t>(m.hi - m.lo + 1, 1) * m.wbytes + 3) / 4 * 4;
+        tab_bytes += m.bytes(); // padded to 16 bytes, so a staged table copies in uint4s
     }
     // argument block (the kernel's BatchArgs): progs, n, nprog, sums_off,
     // full, first[NSHAPE+1], pf[NPF], mtab[NMASK], mlo[NMASK], mlen[NMASK],
@@ -2874,7 +2915,7 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
         for (int i = 0; i <= nshape; i++) grow(4, 4);
         for (size_t i = 0; i < b.pf.size(); i++) grow(8, 8);
         for (int i = 0; i < std::max(nmask, 1); i++) grow(8, 8);
-        for (int f = 0; f < 5; f++)
+        for (int f = 0; f < 6; f++)
             for (int i = 0; i < std::max(nmask, 1); i++) grow(4, 4);
         for (int i = 0; i < 32; i++) grow(4, 4);
         args_bytes = (args_bytes + 7) / 8 * 8;
@@ -2883,7 +2924,7 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     const size_t off_cols = off_progs + al256(b.progs.size() * sizeof(FProgram));
     const size_t off_ents = off_cols + al256(std::max<size_t>(b.masks.size(), 1) * sizeof(FMaskCol));
     const size_t off_tabs = off_ents + al256(std::max<size_t>(nents, 1) * sizeof(FMaskEntry));
-    const size_t copy_bytes = off_tabs + 4, block_bytes = off_tabs + tab_bytes;
+    const size_t copy_bytes = off_tabs + 16, block_bytes = off_tabs + tab_bytes;
 
     // tuning trials and graphs off: the block from the caller's allocator
     BatchGraph *G = nullptr;
@@ -2934,9 +2975,10 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     std::vector<int32_t> mlo(std::max(nmask, 1), 0);
     std::vector<uint32_t> mlen(std::max(nmask, 1), 1), mdflt(std::max(nmask, 1), ~0u);
     std::vector<uint32_t> mshift(std::max(nmask, 1), 0), mwbytes(std::max(nmask, 1), 4);
+    std::vector<uint32_t> mlds(std::max(nmask, 1), 0);
     uint32_t max_len = 0;
     {
-        size_t at = 4, ei = 0;
+        size_t at = 16, ei = 0;
         for (size_t ci = 0; ci < b.masks.size(); ci++) {
             const MaskPlan &m = b.masks[ci];
             FMaskCol c{};
@@ -2964,10 +3006,11 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
             mdflt[mi] = b.full & ~probing;
             mshift[mi] = c.shift;
             mwbytes[mi] = c.wbytes;
-            at += ((size_t)len * c.wbytes + 3) / 4 * 4;
+            mlds[mi] = m.lds;
+            at += m.bytes();
         }
-        const uint32_t ones = ~0u;
-        std::memcpy(host + off_tabs, &ones, 4);
+        const uint32_t ones[4] = {~0u, ~0u, ~0u, ~0u};
+        std::memcpy(host + off_tabs, ones, 16);
     }
     std::vector<uint8_t> args(args_bytes, 0);
     {
@@ -2995,6 +3038,7 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
         for (uint32_t v : mdflt) put(&v, 4, 4);
         for (uint32_t v : mshift) put(&v, 4, 4);
         for (uint32_t v : mwbytes) put(&v, 4, 4);
+        for (uint32_t v : mlds) put(&v, 4, 4);
         for (int i = 0; i < 32; i++) {
             const int32_t bp = i < (int)b.bitpos.size() ? b.bitpos[i] : i;
             put(&bp, 4, 4);
@@ -3036,8 +3080,8 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
         fprintf(stderr, "[batch] launch: %zu programs, block %d, grid %u (%d blocks/CU), LDS %u bytes, graph %d; mask windows",
                 b.progs.size(), shape.block, grid, per_cu, b.lds, G != nullptr);
         for (const MaskPlan &m : b.masks)
-            fprintf(stderr, " [slot %d: %lld keys x %u B at bit %u]", m.slot, (long long)(m.hi - m.lo + 1), m.wbytes,
-                    m.shift);
+            fprintf(stderr, " [slot %d: %lld keys x %u B at bit %u%s]", m.slot, (long long)(m.hi - m.lo + 1), m.wbytes,
+                    m.shift, m.lds ? ", in LDS" : "");
         fprintf(stderr, "\n");
     }
     if (G) {
@@ -3149,13 +3193,14 @@ std::string jit_tuning_report()
         if (t.chosen >= 0)
             o << "block " << BATCH_CANDIDATES[t.chosen].block
               << (BATCH_CANDIDATES[t.chosen].double_buffer ? ", double-buffered" : ", one tile in registers")
-              << (BATCH_CANDIDATES[t.chosen].wide ? ", wide mask words" : ", narrow mask windows");
+              << (BATCH_CANDIDATES[t.chosen].wide ? ", wide mask words" : ", narrow mask windows")
+              << (BATCH_CANDIDATES[t.chosen].stage ? ", small mask tables in LDS" : "");
         else
             o << "tuning (" << t.trials << " trials)";
         o << "; ns per row and program";
         for (int c = 0; c < N_BATCH_CAND; c++) {
             o << " [b" << BATCH_CANDIDATES[c].block << (BATCH_CANDIDATES[c].double_buffer ? "d" : "")
-              << (BATCH_CANDIDATES[c].wide ? "w" : "") << "] ";
+              << (BATCH_CANDIDATES[c].wide ? "w" : "") << (BATCH_CANDIDATES[c].stage ? "s" : "") << "] ";
             if (t.best[c] < 1e299) o << t.best[c];
             else o << "-";
         }

@@ -224,10 +224,14 @@ def test_batch_graph_replays_match_oracle(pipelines, expected, cpu_tables, kw):
 
 
 @pytest.mark.parametrize("env", [{"SDB_BATCH_WIDE_MASKS": "1"}, {"SDB_BATCH_WIDE_MASKS": "1", "SDB_BATCH_BLOCK": "1024"},
-                                 {"SDB_BATCH_BLOCK": "1024", "SDB_BATCH_DOUBLE_BUFFER": "1"}])
+                                 {"SDB_BATCH_BLOCK": "1024", "SDB_BATCH_DOUBLE_BUFFER": "1"},
+                                 {"SDB_BATCH_STAGE_MASKS": "1", "SDB_BATCH_BLOCK": "256"},
+                                 {"SDB_BATCH_STAGE_MASKS": "1", "SDB_BATCH_BLOCK": "1024"},
+                                 {"SDB_BATCH_STAGE_MASKS": "1", "SDB_BATCH_WIDE_MASKS": "1"}])
 def test_batch_layouts_match_oracle(pipelines, expected, monkeypatch, env):
     """The autotuner's other candidates, forced: whole 32-bit mask words
-    instead of narrow windows, and other launch shapes."""
+    instead of narrow windows, mask tables staged in LDS, and other launch
+    shapes."""
     for k, v in env.items():
         monkeypatch.setenv(k, v)
     eng = _engine()
