@@ -895,6 +895,7 @@ bool KernelBundle::build_program(const ExecContext &ctx, FProgram &p) const
             if (mem) vec &= aligned(mem, 16);
         if (I.store) vec &= aligned(I.store, 16);
         p.ins[i] = I;
+        p.ins[i].sel = d[i].sel;
     }
     p.ninstr = (int)d.size();
     for (int j = 0; j < p.gb.k; j++)

@@ -76,6 +76,8 @@ struct FInstr {
     int8_t va, vb, vout, vkind;      // value registers (F_PF_BASE + j: prefetched
                                      // column j); -1 = memory operand a / b
     int32_t lit, lit2;
+    float sel;                 // host estimate of the share of rows a predicate keeps
+                               // (batched launches order their mask lookups by it)
     const int32_t *a;          // memory operand (row-indexed) when va < 0
     const int32_t *b;          // memory operand when vb < 0
     int32_t *store;            // write-back of vout (masked as the functor writes), or null
@@ -85,6 +87,7 @@ struct FInstr {
     uint32_t lds_off;          // byte offset of the staged bitmap / dictionary in dynamic LDS
     uint32_t lds_bytes;        // its size
 };
+static_assert(sizeof(FInstr) == 88, "FInstr: sel sits in the padding before the pointers");
 
 enum FGroupMode : int { FG_GLOBAL = 0, FG_DENSE_LDS = 1, FG_HASH_LDS = 2 };
 

@@ -1568,6 +1568,13 @@ void launch_compiled(const FProgram &p, int64_t n, hipStream_t s, int unroll, in
     }
     const int64_t units = p.vec ? (n + 3) / 4 : n;
     const unsigned grid = stream_grid(units, (int64_t)block * (p.vec ? unroll : 1), per_cu);
+    static const bool debug = [] {
+        const char *e = getenv("SDB_DEBUG_FUSED");
+        return e && *e && *e != '0';
+    }();
+    if (debug && n >= ((int64_t)1 << 22))
+        fprintf(stderr, "[fused launch] rows %lld, unroll %d, block %d, queue %d, LDS %u bytes, %d blocks/CU, grid %u\n",
+                (long long)n, unroll, block, (int)queue, lds, per_cu, grid);
     struct Args {
         FProgram p;
         int64_t n;
@@ -2132,6 +2139,7 @@ const char *BATCH_HEAD = R"(struct BatchArgs {
     int32_t nprog;
     uint32_t sums_off;
     uint32_t full;
+    uint32_t lazy; // 1: mask tables read in morder, each only for rows some program still keeps
     int32_t first[NSHAPE + 1];
     const int32_t *pf[NPF > 0 ? NPF : 1];
     const uint8_t *mtab[NMASK > 0 ? NMASK : 1];
@@ -2141,6 +2149,7 @@ const char *BATCH_HEAD = R"(struct BatchArgs {
     uint32_t mshift[NMASK > 0 ? NMASK : 1];
     uint32_t mwbytes[NMASK > 0 ? NMASK : 1];
     uint32_t mlds[NMASK > 0 ? NMASK : 1]; // 1 + LDS byte offset of a staged table (0: read from memory)
+    int32_t morder[NMASK > 0 ? NMASK : 1]; // lazy reads: masked columns, fewest rows kept first
     int32_t bitpos[32]; // program i's bit in the mask words
 };
 template <int R, int UU> struct Pf {
@@ -2156,51 +2165,88 @@ __device__ __forceinline__ void pf_load(const BatchArgs &A, const int64_t (&row0
             for (int u = 0; u < UU; u++) pf.c[j][u] = load_vec<R>(A.pf[j], row0[u]);
 }
 // Program bits of every row: the AND of the row's words in the mask tables
-// of the masked columns (a column this batch does not mask has a one-word
-// table of all ones).  Indexes are clamped into the table so every load is
-// unconditional (an aligned dword holding the entry); out-of-table keys then
-// take the column's default word.  An entry is the window of the word that
-// covers the column's probing programs, widened back with ones outside it.
+// of the masked columns (a column this batch does not mask has mlen 0 and
+// is skipped; the branch is uniform).  An entry is the window of the word
+// that covers the column's probing programs, widened back with ones outside
+// it; keys outside the table take the column's default word.
+//
+// Eager form: every table is read for every row, all loads issued before
+// any is combined, the indexes clamped into the table so each load is an
+// unconditional aligned dword holding the entry.  Lazy form (A.lazy): the
+// tables are read in A.morder, the column expected to keep the fewest rows
+// first, and a row reads the next table only while some program still keeps
+// it, so a selective first column saves most later reads; the result is the
+// same AND.
+template <int R>
+__device__ __forceinline__ uint32_t mask_word(const BatchArgs &A, const uint8_t *lds, int m, bool staged, int32_t x)
+{
+    const uint32_t idx = (uint32_t)x - (uint32_t)A.mlo[m];
+    const uint32_t byte = (idx < A.mlen[m] ? idx : A.mlen[m] - 1u) * A.mwbytes[m];
+    const uint32_t raw = staged ? *reinterpret_cast<const uint32_t *>(lds + (A.mlds[m] - 1u) + (byte & ~3u))
+                                : *reinterpret_cast<const uint32_t *>(A.mtab[m] + (byte & ~3u));
+    return raw >> ((byte & 3u) * 8u);
+}
+__device__ __forceinline__ uint32_t mask_widen(const BatchArgs &A, int m, int32_t x, uint32_t w)
+{
+    const uint32_t idx = (uint32_t)x - (uint32_t)A.mlo[m];
+    const uint32_t wm = A.mwbytes[m] >= 4u ? ~0u : (1u << (8u * A.mwbytes[m])) - 1u;
+    const uint32_t word = ((w & wm) << A.mshift[m]) | ~(wm << A.mshift[m]);
+    return idx < A.mlen[m] ? word : A.mdflt[m];
+}
 template <int R, int UU>
 __device__ __forceinline__ void mask_rows(const BatchArgs &A, const Pf<R, UU> &pf, const uint8_t *lds,
                                           uint32_t (&mw)[UU][R])
 {
-    uint32_t w[NMASK > 0 ? NMASK : 1][UU][R];
-#pragma unroll
-    for (int m = 0; m < NMASK; m++) {
-        // a table staged in LDS is read there (the branch is uniform)
-        const bool staged = A.mlds[m] != 0;
-#pragma unroll
-        for (int u = 0; u < UU; u++) {
-            int32_t x[R];
-            vec_rows<R>(pf.c[MSLOT[m]][u], x);
-#pragma unroll
-            for (int k = 0; k < R; k++) {
-                const uint32_t idx = (uint32_t)x[k] - (uint32_t)A.mlo[m];
-                const uint32_t byte = (idx < A.mlen[m] ? idx : A.mlen[m] - 1u) * A.mwbytes[m];
-                uint32_t word;
-                if (staged) word = *reinterpret_cast<const uint32_t *>(lds + (A.mlds[m] - 1u) + (byte & ~3u));
-                else word = *reinterpret_cast<const uint32_t *>(A.mtab[m] + (byte & ~3u));
-                w[m][u][k] = word >> ((byte & 3u) * 8u);
-            }
-        }
-    }
 #pragma unroll
     for (int u = 0; u < UU; u++)
 #pragma unroll
-        for (int k = 0; k < R; k++) {
-            uint32_t v = A.full;
+        for (int k = 0; k < R; k++) mw[u][k] = A.full;
+    if (!A.lazy) {
+        uint32_t w[NMASK > 0 ? NMASK : 1][UU][R];
 #pragma unroll
-            for (int m = 0; m < NMASK; m++) {
+        for (int m = 0; m < NMASK; m++) {
+            if (!A.mlen[m]) continue;
+            // a table staged in LDS is read there (the branch is uniform)
+            const bool staged = A.mlds[m] != 0;
+#pragma unroll
+            for (int u = 0; u < UU; u++) {
                 int32_t x[R];
                 vec_rows<R>(pf.c[MSLOT[m]][u], x);
-                const uint32_t idx = (uint32_t)x[k] - (uint32_t)A.mlo[m];
-                const uint32_t wm = A.mwbytes[m] >= 4u ? ~0u : (1u << (8u * A.mwbytes[m])) - 1u;
-                const uint32_t word = ((w[m][u][k] & wm) << A.mshift[m]) | ~(wm << A.mshift[m]);
-                v &= idx < A.mlen[m] ? word : A.mdflt[m];
+#pragma unroll
+                for (int k = 0; k < R; k++) w[m][u][k] = mask_word<R>(A, lds, m, staged, x[k]);
             }
-            mw[u][k] = v;
         }
+#pragma unroll
+        for (int m = 0; m < NMASK; m++) {
+            if (!A.mlen[m]) continue;
+#pragma unroll
+            for (int u = 0; u < UU; u++) {
+                int32_t x[R];
+                vec_rows<R>(pf.c[MSLOT[m]][u], x);
+#pragma unroll
+                for (int k = 0; k < R; k++) mw[u][k] &= mask_widen(A, m, x[k], w[m][u][k]);
+            }
+        }
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < NMASK; j++) {
+        const int m = A.morder[j];
+        if (m < 0 || !A.mlen[m]) continue;
+        const bool staged = A.mlds[m] != 0;
+#pragma unroll
+        for (int u = 0; u < UU; u++) {
+            // the key column of mask m (m is uniform: one register is selected)
+            int32_t x[R];
+            vec_rows<R>(pf.c[MSLOT[0]][u], x);
+#pragma unroll
+            for (int c = 1; c < NMASK; c++)
+                if (m == c) vec_rows<R>(pf.c[MSLOT[c]][u], x);
+#pragma unroll
+            for (int k = 0; k < R; k++)
+                if (mw[u][k]) mw[u][k] &= mask_widen(A, m, x[k], mask_word<R>(A, lds, m, staged, x[k]));
+        }
+    }
 }
 // The programs of the launch, read-only while it runs: reading them through
 // the constant address space lets their fields go through scalar loads.
@@ -2522,6 +2568,8 @@ struct BatchPlan {
     std::vector<int> mask_slots;      // the library's masked slots (kernel order)
     std::vector<MaskPlan> masks;      // this batch's tables, one per masked slot it uses
     std::vector<int> bitpos;          // program (batch order) -> its bit in the mask words
+    bool lazy = false;                // mask tables read lazily, in morder
+    std::vector<int> morder;          // kernel mask indexes, in reading order (-1: none)
     std::string src;
     BatchLibrary::Src *src_entry = nullptr; // the library source it is (never erased)
     uint64_t version = 0;                   // its version
@@ -2581,6 +2629,82 @@ std::string batch_source(const BatchLibrary &lib, int unroll, bool vec, const Ba
           << "        if (acc) atomicAdd(&sums[i], (unsigned long long)acc);\n    }\n";
     s << "}\n} // namespace\n" << BATCH_TAIL;
     return s.str();
+}
+
+// The batch's mask tables as the autotuner tells batches apart: per masked
+// column (in slot order) its slot, entry bytes and table size (a power of
+// two), and the lazy form.
+std::string mask_signature(const BatchPlan &b)
+{
+    std::vector<const MaskPlan *> ms;
+    for (const MaskPlan &m : b.masks) ms.push_back(&m);
+    std::sort(ms.begin(), ms.end(), [](const MaskPlan *x, const MaskPlan *y) { return x->slot < y->slot; });
+    std::ostringstream o;
+    for (const MaskPlan *m : ms) {
+        int lg = 0;
+        while ((1ull << lg) < m->bytes()) lg++;
+        o << m->slot << ":" << m->wbytes << "B:2^" << lg << " ";
+    }
+    o << (b.lazy ? "lazy" : "eager");
+    return o.str();
+}
+
+// SDB_BATCH_LAZY_MASKS: 0 reads every mask table for every row, 1 always
+// reads them lazily, unset decides by the estimate below.
+int batch_lazy_mode()
+{
+    const char *e = getenv("SDB_BATCH_LAZY_MASKS");
+    return e && *e ? (*e == '1' ? 1 : 0) : -1;
+}
+
+// Reading order of the mask tables and whether to read them lazily.  From
+// the host's estimate of the share of rows each probe keeps (the share of
+// its dimension's rows the build side selected), with programs and columns
+// taken as independent: a row still read for column j is one that some
+// program keeps after the columns before j.  Greedily, the next column is
+// the one after which the fewest rows stay kept; the reads per row that
+// order saves decide the lazy form (its reads wait on the previous ones, so
+// it has to save a quarter of them).
+void order_masks(BatchPlan &b, const std::vector<FProgram> &q, const std::vector<int> &order,
+                 const std::vector<std::vector<int>> &shared)
+{
+    const int nm = (int)b.masks.size(), np = (int)order.size();
+    b.morder.assign(std::max<size_t>(b.mask_slots.size(), 1), -1);
+    b.lazy = false;
+    if (nm == 0) return;
+    // keep[c][pos]: share of rows program pos keeps through column c's probes
+    std::vector<std::vector<double>> keep(nm, std::vector<double>(np, 1.0));
+    for (int pos = 0; pos < np; pos++)
+        for (int j : shared[order[pos]]) {
+            const FInstr &I = q[order[pos]].ins[j];
+            const int slot = I.va - F_PF_BASE;
+            for (int c = 0; c < nm; c++)
+                if (b.masks[c].slot == slot) keep[c][pos] *= std::min(1.0, std::max(0.0, (double)I.sel));
+        }
+    std::vector<double> alive(np, 1.0);
+    std::vector<bool> used(nm, false);
+    double reads = 0, still = 1.0;
+    for (int step = 0; step < nm; step++) {
+        int best = -1;
+        double best_after = 2.0;
+        for (int c = 0; c < nm; c++) {
+            if (used[c]) continue;
+            double none = 1.0;
+            for (int pos = 0; pos < np; pos++) none *= 1.0 - alive[pos] * keep[c][pos];
+            if (1.0 - none < best_after) {
+                best_after = 1.0 - none;
+                best = c;
+            }
+        }
+        used[best] = true;
+        reads += still;
+        still = best_after;
+        for (int pos = 0; pos < np; pos++) alive[pos] *= keep[best][pos];
+        b.morder[step] = (int)(std::find(b.mask_slots.begin(), b.mask_slots.end(), b.masks[best].slot) -
+                               b.mask_slots.begin());
+    }
+    const int mode = batch_lazy_mode();
+    b.lazy = mode == 1 || (mode < 0 && reads <= 0.75 * nm);
 }
 
 BatchPlan plan_batch(const std::vector<FProgram> &in, int unroll, const std::string &lib_key, bool masks,
@@ -2713,6 +2837,7 @@ BatchPlan plan_batch(const std::vector<FProgram> &in, int unroll, const std::str
         m.shift = m.wbytes == 4 ? 0 : std::min(lo_bit, 32 - 8 * m.wbytes);
     }
     if ((int)b.masks.size() > F_MASK_MAX_COLS) fail(SDB_ERR_UNSUPPORTED, "batched launch: too many masked columns");
+    order_masks(b, q, order, shared);
     // columns this batch reads (other slots stay unloaded)
     b.pf.assign(std::max<size_t>(lib.slots.size(), 1), nullptr);
     auto use = [&](int8_t reg) {
@@ -2726,8 +2851,8 @@ BatchPlan plan_batch(const std::vector<FProgram> &in, int unroll, const std::str
         for (int j = 0; j < p.gb.k; j++) use(p.gb.key_reg[j]);
         use(p.gb.agg_reg);
     }
-    // a masked slot of the library this batch does not read keeps its
-    // zero-initialised registers; its lookups read the all-ones word
+    // a masked slot of the library this batch does not mask is skipped by
+    // the kernel (its mlen is 0)
     uint32_t off = 0;
     for (FProgram &p : b.progs) {
         p.lds_base = off;
@@ -2869,18 +2994,28 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     const std::string lib_key = device_arch() + "/" + std::to_string(dev) + "/" + scope_key + (masks ? "/m" : "");
     // launch shape: fixed by the environment, else the autotuner's choice
     // for this library (trials on launches of at least TUNE_MIN_ROWS rows)
+    // for this library and the batch's mask tables (their columns, entry
+    // widths and sizes: what the layouts trade)
     BatchShape shape = BATCH_CANDIDATES[0];
+    const bool fixed = batch_shape_fixed(shape);
+    BatchPlan b = plan_batch(in, unroll, lib_key, masks, shape);
     int trial = -1;
-    if (!batch_shape_fixed(shape) && n >= TUNE_MIN_ROWS) {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        BatchTune &t = g_batch_tune[lib_key];
-        if (t.chosen >= 0) shape = BATCH_CANDIDATES[t.chosen];
-        else {
-            trial = t.trials % N_BATCH_CAND;
-            shape = BATCH_CANDIDATES[trial];
+    std::string tune_key;
+    if (!fixed && n >= TUNE_MIN_ROWS) {
+        tune_key = lib_key + " masks " + mask_signature(b);
+        BatchShape want = shape;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            BatchTune &t = g_batch_tune[tune_key];
+            if (t.chosen >= 0) want = BATCH_CANDIDATES[t.chosen];
+            else {
+                trial = t.trials % N_BATCH_CAND;
+                want = BATCH_CANDIDATES[trial];
+            }
         }
+        if (want.key() != shape.key()) b = plan_batch(in, unroll, lib_key, masks, want);
+        shape = want;
     }
-    const BatchPlan b = plan_batch(in, unroll, lib_key, masks, shape);
     static_assert(N_BATCH_CAND <= 16, "BatchTune::best holds 16 candidates");
     if (b.lds > std::max<uint64_t>(F_BATCH_LDS + 8u * F_BATCH_MAX, BATCH_STAGE_LDS_MAX))
         fail(SDB_ERR_UNSUPPORTED, "batched launch: LDS regions exceed the budget");
@@ -2906,16 +3041,17 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
         tab_bytes += m.bytes(); // padded to 16 bytes, so a staged table copies in uint4s
     }
     // argument block (the kernel's BatchArgs): progs, n, nprog, sums_off,
-    // full, first[NSHAPE+1], pf[NPF], mtab[NMASK], mlo[NMASK], mlen[NMASK],
-    // mdflt[NMASK], mshift[NMASK], mwbytes[NMASK], bitpos[32]
+    // full, lazy, first[NSHAPE+1], pf[NPF], mtab[NMASK], mlo[NMASK],
+    // mlen[NMASK], mdflt[NMASK], mshift[NMASK], mwbytes[NMASK], mlds[NMASK],
+    // morder[NMASK], bitpos[32]
     size_t args_bytes = 0;
     {
         auto grow = [&](size_t bytes, size_t align) { args_bytes = (args_bytes + align - 1) / align * align + bytes; };
-        grow(8, 8), grow(8, 8), grow(4, 4), grow(4, 4), grow(4, 4);
+        grow(8, 8), grow(8, 8), grow(4, 4), grow(4, 4), grow(4, 4), grow(4, 4);
         for (int i = 0; i <= nshape; i++) grow(4, 4);
         for (size_t i = 0; i < b.pf.size(); i++) grow(8, 8);
         for (int i = 0; i < std::max(nmask, 1); i++) grow(8, 8);
-        for (int f = 0; f < 6; f++)
+        for (int f = 0; f < 7; f++)
             for (int i = 0; i < std::max(nmask, 1); i++) grow(4, 4);
         for (int i = 0; i < 32; i++) grow(4, 4);
         args_bytes = (args_bytes + 7) / 8 * 8;
@@ -2971,9 +3107,9 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     FMaskCol *d_cols = reinterpret_cast<FMaskCol *>(dblock + off_cols);
     FMaskEntry *d_ents = reinterpret_cast<FMaskEntry *>(dblock + off_ents);
     uint8_t *tabs = dblock + off_tabs;
-    std::vector<const uint8_t *> mtab(std::max(nmask, 1), tabs); // unused slots: the all-ones word
+    std::vector<const uint8_t *> mtab(std::max(nmask, 1), tabs); // unused slots (mlen 0) are skipped
     std::vector<int32_t> mlo(std::max(nmask, 1), 0);
-    std::vector<uint32_t> mlen(std::max(nmask, 1), 1), mdflt(std::max(nmask, 1), ~0u);
+    std::vector<uint32_t> mlen(std::max(nmask, 1), 0), mdflt(std::max(nmask, 1), ~0u); // mlen 0: unused
     std::vector<uint32_t> mshift(std::max(nmask, 1), 0), mwbytes(std::max(nmask, 1), 4);
     std::vector<uint32_t> mlds(std::max(nmask, 1), 0);
     uint32_t max_len = 0;
@@ -3027,6 +3163,8 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
         put(&nprog, 4, 4);
         put(&b.sums_off, 4, 4);
         put(&b.full, 4, 4);
+        const uint32_t lazy = b.lazy ? 1u : 0u;
+        put(&lazy, 4, 4);
         for (int i = 0; i <= nshape; i++) {
             const int32_t f = b.first[i];
             put(&f, 4, 4);
@@ -3039,6 +3177,10 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
         for (uint32_t v : mshift) put(&v, 4, 4);
         for (uint32_t v : mwbytes) put(&v, 4, 4);
         for (uint32_t v : mlds) put(&v, 4, 4);
+        for (int i = 0; i < std::max(nmask, 1); i++) {
+            const int32_t m = i < (int)b.morder.size() ? b.morder[i] : -1;
+            put(&m, 4, 4);
+        }
         for (int i = 0; i < 32; i++) {
             const int32_t bp = i < (int)b.bitpos.size() ? b.bitpos[i] : i;
             put(&bp, 4, 4);
@@ -3077,8 +3219,11 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
         return e && *e && *e != '0';
     }();
     if (trial < 0 && debug) {
-        fprintf(stderr, "[batch] launch: %zu programs, block %d, grid %u (%d blocks/CU), LDS %u bytes, graph %d; mask windows",
-                b.progs.size(), shape.block, grid, per_cu, b.lds, G != nullptr);
+        fprintf(stderr, "[batch] launch: %zu programs, block %d, grid %u (%d blocks/CU), LDS %u bytes, graph %d, %s masks",
+                b.progs.size(), shape.block, grid, per_cu, b.lds, G != nullptr, b.lazy ? "lazy" : "eager");
+        for (int m : b.morder)
+            if (m >= 0) fprintf(stderr, " %d", b.mask_slots[m]);
+        fprintf(stderr, "; mask windows");
         for (const MaskPlan &m : b.masks)
             fprintf(stderr, " [slot %d: %lld keys x %u B at bit %u%s]", m.slot, (long long)(m.hi - m.lo + 1), m.wbytes,
                     m.shift, m.lds ? ", in LDS" : "");
@@ -3156,7 +3301,7 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     (void)hipEventDestroy(eb);
     hip_check(e, "batched kernel tuning");
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    BatchTune &t = g_batch_tune[lib_key];
+    BatchTune &t = g_batch_tune[tune_key];
     if (t.chosen >= 0) return;
     const double cost = ms * 1e6 / ((double)n * (double)b.progs.size());
     t.best[trial] = std::min(t.best[trial], cost);

@@ -107,10 +107,13 @@ def test_batch_without_shared_masks(pipelines, expected, monkeypatch):
         eng.close()
 
 
-def test_batch_mask_edge_cases(cpu_tables):
+@pytest.mark.parametrize("lazy", ["", "1", "0"])
+def test_batch_mask_edge_cases(cpu_tables, monkeypatch, lazy):
     """Shared probe masks at the extremes: a pipeline no row passes (a year
     outside the date table), near-empty ones, identical pipelines side by
-    side, and every template's widest predicates."""
+    side, and every template's widest predicates; with the mask tables read
+    as the estimate decides, always lazily, and always eagerly."""
+    monkeypatch.setenv("SDB_BATCH_LAZY_MASKS", lazy)
     pipes = [
         ("t1", {"date": ("year", 2050), "disc": (1, 3), "qty": ("lt", 25)}),
         ("t2", {"part": ("brand", 999), "s_region": 4}),
@@ -127,7 +130,7 @@ def test_batch_mask_edge_cases(cpu_tables):
     eng = _engine()
     try:
         plans = [PL.build(t, **p) for t, p in pipes]
-        print("mask edge batch", flush=True)
+        print("mask edge batch, lazy=%r" % lazy, flush=True)
         results, batched = eng.execute_batch(plans)
         assert batched == len(plans)
         for i, r in enumerate(results):
@@ -136,11 +139,13 @@ def test_batch_mask_edge_cases(cpu_tables):
         eng.close()
 
 
-def test_batch_irregular_date_keys(cpu_tables, pipelines):
+@pytest.mark.parametrize("lazy", ["", "1"])
+def test_batch_irregular_date_keys(cpu_tables, pipelines, monkeypatch, lazy):
     """Fact date keys that are no dates (day 32, month 13, zero, negative,
     far years), inside and outside the probed key range, and a date table
     holding such a key: the shared masks' range tests and default words must
-    still match the oracle."""
+    still match the oracle (mask tables also read lazily)."""
+    monkeypatch.setenv("SDB_BATCH_LAZY_MASKS", lazy)
     import numpy as np
     from sycldb_amd.engine import Engine
     tables = {t: {c: np.array(a, dtype=np.int32, copy=True) for c, a in cols.items()} for t, cols in cpu_tables.items()}
@@ -158,7 +163,7 @@ def test_batch_irregular_date_keys(cpu_tables, pipelines):
     try:
         eng.load_tables(tables)
         plans = [PL.build(t, **p) for t, p in pipes]
-        print("calendar batch", flush=True)
+        print("calendar batch, lazy=%r" % lazy, flush=True)
         results, batched = eng.execute_batch(plans)
         assert batched == len(plans)
         for i, r in enumerate(results):
@@ -227,11 +232,14 @@ def test_batch_graph_replays_match_oracle(pipelines, expected, cpu_tables, kw):
                                  {"SDB_BATCH_BLOCK": "1024", "SDB_BATCH_DOUBLE_BUFFER": "1"},
                                  {"SDB_BATCH_STAGE_MASKS": "1", "SDB_BATCH_BLOCK": "256"},
                                  {"SDB_BATCH_STAGE_MASKS": "1", "SDB_BATCH_BLOCK": "1024"},
-                                 {"SDB_BATCH_STAGE_MASKS": "1", "SDB_BATCH_WIDE_MASKS": "1"}])
+                                 {"SDB_BATCH_STAGE_MASKS": "1", "SDB_BATCH_WIDE_MASKS": "1"},
+                                 {"SDB_BATCH_LAZY_MASKS": "1"},
+                                 {"SDB_BATCH_LAZY_MASKS": "1", "SDB_BATCH_STAGE_MASKS": "1", "SDB_BATCH_BLOCK": "1024"},
+                                 {"SDB_BATCH_LAZY_MASKS": "1", "SDB_BATCH_WIDE_MASKS": "1"}])
 def test_batch_layouts_match_oracle(pipelines, expected, monkeypatch, env):
     """The autotuner's other candidates, forced: whole 32-bit mask words
-    instead of narrow windows, mask tables staged in LDS, and other launch
-    shapes."""
+    instead of narrow windows, mask tables staged in LDS, other launch
+    shapes, and mask tables read lazily in the estimated order."""
     for k, v in env.items():
         monkeypatch.setenv(k, v)
     eng = _engine()
@@ -242,5 +250,30 @@ def test_batch_layouts_match_oracle(pipelines, expected, monkeypatch, env):
         assert batched >= 24
         for i, r in enumerate(results):
             assert r.rows() == expected[i], (env, i, pipelines[i])
+    finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("lazy", ["", "1", "0"])
+def test_batch_per_template_matches_oracle(pipelines, expected, monkeypatch, lazy):
+    """Batches of one template each (the lazy form's case: every program
+    probes every masked column), in one engine whose shape library already
+    holds all four templates, so masked slots the batch does not use are
+    skipped."""
+    monkeypatch.setenv("SDB_BATCH_LAZY_MASKS", lazy)
+    eng = _engine()
+    try:
+        plans = [PL.build(t, **p) for t, p in pipelines]
+        results, _ = eng.execute_batch(plans)
+        assert [r.rows() for r in results] == expected
+        for tmpl in ("t1", "t2", "t3", "t4"):
+            idx = [i for i, (t, _) in enumerate(pipelines) if t == tmpl]
+            if len(idx) < 2:
+                continue
+            print("template batch", tmpl, "lazy=%r" % lazy, flush=True)
+            results, batched = eng.execute_batch([plans[i] for i in idx])
+            assert batched == len(idx)
+            for j, r in zip(idx, results):
+                assert r.rows() == expected[j], (tmpl, j, pipelines[j])
     finally:
         eng.close()
