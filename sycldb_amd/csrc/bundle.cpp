@@ -76,6 +76,7 @@ void KernelBundle::execute(const ExecContext &ctx) const
 {
     if (kernels_.empty()) return;
     Engine &eng = *ctx.eng;
+    if (eng.tracing) eng.mark("bundle " + (ctx.table ? *ctx.table : std::string("?")));
     if (eng.capture.active && ctx.table && *ctx.table == eng.capture.table) {
         // batched execution: record the fact-table bundle; its program is
         // built once the whole batch is captured (build_captured)
@@ -105,10 +106,12 @@ void KernelBundle::execute(const ExecContext &ctx) const
             Engine::ProfRec rec;
             const bool timed = eng.profiles(ctx.table);
             if (timed) eng.prof_begin("fused:" + (ctx.table ? *ctx.table : std::string("?")), n, rec);
+            eng.mark("program built");
             if (jit) run_fused_jit(L, p, n, ctx.stream);
             else launch_fused(p, n, ctx.stream);
             if (timed) eng.prof_end(rec);
             eng.launches++;
+            eng.mark("launched");
             return;
         }
     }

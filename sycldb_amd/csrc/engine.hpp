@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdint>
 #include <deque>
 #include <functional>
@@ -363,6 +364,16 @@ struct Engine {
     void prof_collect(); // after the stream is synchronised
 
     Table *find_table(const std::string &name);
+
+    // SDB_TRACE_STEPS=1: host timestamps of an execution's phases, printed
+    // to stderr when it ends (where host time goes between the kernels)
+    bool tracing = false;
+    std::vector<std::pair<std::chrono::steady_clock::time_point, std::string>> marks;
+    void mark(const std::string &what)
+    {
+        if (tracing) marks.emplace_back(std::chrono::steady_clock::now(), what);
+    }
+    void print_marks();
 
     Capture capture;
     // Buffer of `count` T for a row-sized intermediate of `table` (a

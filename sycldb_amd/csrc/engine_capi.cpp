@@ -284,8 +284,15 @@ int sdb_engine_execute(sdb_engine *e, const char *plan_json, sdb_result **out)
         require(e && plan_json && out, "null argument");
         *out = nullptr;
         Engine &eng = e->eng;
+        static const bool trace = [] {
+            const char *v = getenv("SDB_TRACE_STEPS");
+            return v && *v == '1';
+        }();
+        eng.tracing = trace;
+        eng.mark("start");
         SDB_HIP(hipSetDevice(eng.device));
         const PlanResult plan = parse_plan(plan_json);
+        eng.mark("parsed");
         // scratch of the previous query is released (runtime/runtime.cpp:331-345)
         eng.mm->reset(eng.stream);
         eng.launches = 0;
@@ -293,9 +300,12 @@ int sdb_engine_execute(sdb_engine *e, const char *plan_json, sdb_result **out)
         SDB_HIP(hipEventRecord(eng.ev_start, eng.stream));
         auto r = std::make_unique<sdb_result>();
         r->t = execute_ddor_plan(eng, plan);
+        eng.mark("plan done");
         SDB_HIP(hipEventRecord(eng.ev_stop, eng.stream));
         SDB_HIP(hipEventSynchronize(eng.ev_stop));
         if (eng.profile) eng.prof_collect();
+        eng.mark("end");
+        eng.print_marks();
         const auto w1 = std::chrono::steady_clock::now();
         float ms = 0;
         SDB_HIP(hipEventElapsedTime(&ms, eng.ev_start, eng.ev_stop));

@@ -222,6 +222,21 @@ FinalTable run_ddor_plan(Engine &eng, const PlanResult &plan)
     return t;
 }
 
+void Engine::print_marks()
+{
+    if (marks.empty()) return;
+    const auto t0 = marks.front().first;
+    std::string line = "[step]";
+    for (const auto &m : marks) {
+        char buf[64];
+        snprintf(buf, sizeof(buf), " %.1f", std::chrono::duration<double, std::micro>(m.first - t0).count());
+        line += buf;
+        line += " " + m.second + ";";
+    }
+    fprintf(stderr, "%s\n", line.c_str());
+    marks.clear();
+}
+
 void Engine::wait_stream()
 {
     if (!sync_ev) SDB_HIP(hipEventCreateWithFlags(&sync_ev, hipEventDisableTiming));
@@ -311,7 +326,9 @@ struct HostCopies {
             }
             SDB_HIP(hipMemcpyAsync(host, dev, total, hipMemcpyDeviceToHost, eng.stream));
         }
+        eng.mark("result wait");
         eng.wait_stream();
+        eng.mark("result ready");
         for (const Part &p : parts) std::memcpy(p.dst, host + p.off, p.bytes);
         parts.clear();
         total = 0;

@@ -598,7 +598,9 @@ void Engine::resolve_compact()
         launches++;
         SDB_HIP(hipMemcpyAsync(staged, dev, 8 * todo.size(), hipMemcpyDeviceToHost, stream));
     }
+    mark("stats wait");
     wait_stream();
+    mark("stats ready");
     const std::vector<uint32_t> host(staged, staged + 2 * todo.size());
     for (size_t i = 0; i < todo.size(); i++) {
         CompactState &c = *todo[i];
