@@ -302,7 +302,11 @@ int sdb_engine_execute(sdb_engine *e, const char *plan_json, sdb_result **out)
         r->t = execute_ddor_plan(eng, plan);
         eng.mark("plan done");
         SDB_HIP(hipEventRecord(eng.ev_stop, eng.stream));
-        SDB_HIP(hipEventSynchronize(eng.ev_stop));
+        // the result wait has drained the stream: polled, this returns at once
+        hipError_t q;
+        while ((q = hipEventQuery(eng.ev_stop)) == hipErrorNotReady) {
+        }
+        SDB_HIP(q);
         if (eng.profile) eng.prof_collect();
         eng.mark("end");
         eng.print_marks();

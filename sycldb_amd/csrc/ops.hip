@@ -712,7 +712,7 @@ __device__ __forceinline__ void block_count_add(uint32_t local, uint32_t *count)
 // slots, VALS_UNSET in the others (`bad` is raised when a set slot's offset
 // does not fit below VALS_UNSET, and the table is then not used).
 constexpr int BITS_GROUPS = 4;
-constexpr unsigned BITS_MAX_BLOCKS = 512;
+constexpr unsigned BITS_MAX_BLOCKS = 2048;
 
 __global__ void __launch_bounds__(BLOCK) k_bits_from_keys(const uint8_t *ht, int32_t len, uint32_t *bits,
                                                           uint32_t *count)
@@ -781,17 +781,25 @@ __global__ void __launch_bounds__(BLOCK) k_bits_from_pairs(const int2 *ht, int32
 }
 
 // Rank dictionary of a presence bitmap with per-slot 16-bit values (the
-// layout FV_RANK16 describes in fused_types.hpp): one block of RANK_BLOCK
-// threads; each thread owns a run of consecutive words, the block scans the
-// run totals (within each wave by shuffles, then across the waves), then
-// every thread writes its words, their prefix counts and the values of their
-// set slots in slot order.
+// layout FV_RANK16 describes in fused_types.hpp), in one block of
+// RANK_BLOCK threads.  Each thread owns a run of consecutive words; the block
+// scans the run totals (within each wave by shuffles, then across the
+// waves) and every thread writes its words and their prefix counts.  Then,
+// for a bitmap of at most RANK_LDS_WORDS words (kept in LDS with its
+// prefixes), one thread per slot writes the values of the set slots at their
+// ranks: coalesced reads of the value table, every load independent.  A
+// larger bitmap writes them per word, in its owner thread.
 constexpr int RANK_BLOCK = 1024;
+constexpr int RANK_LDS_WORDS = 4096;
 
-__global__ void __launch_bounds__(RANK_BLOCK) k_rank_dict(const uint32_t *bits, int32_t words, const uint16_t *vals,
-                                                          uint8_t *dict, uint32_t cap_vals)
+__global__ void __launch_bounds__(RANK_BLOCK) k_rank_dict(const uint32_t *__restrict__ bits, int32_t words,
+                                                          const uint16_t *__restrict__ vals, uint8_t *__restrict__ dict,
+                                                          uint32_t cap_vals)
 {
     __shared__ uint32_t wave_total[RANK_BLOCK / 64];
+    __shared__ uint32_t s_bits[RANK_LDS_WORDS];
+    __shared__ uint32_t s_prefix[RANK_LDS_WORDS];
+    const bool in_lds = words <= RANK_LDS_WORDS;
     const int32_t per = (words + RANK_BLOCK - 1) / RANK_BLOCK;
     const int32_t w0 = min(words, (int32_t)threadIdx.x * per), w1 = min(words, w0 + per);
     uint32_t local = 0;
@@ -814,10 +822,27 @@ __global__ void __launch_bounds__(RANK_BLOCK) k_rank_dict(const uint32_t *bits, 
         const uint32_t m = bits[w];
         out_bits[w] = m;
         out_prefix[w] = (uint16_t)base;
-        // values past the buffer's capacity are dropped (the dictionary is
-        // then not used)
-        for (uint32_t r = m; r; r &= r - 1, base++)
-            if (base < cap_vals) out_vals[base] = vals[(int64_t)w * 32 + __builtin_ctz(r)];
+        if (in_lds) {
+            s_bits[w] = m;
+            s_prefix[w] = base;
+            base += __popc(m);
+        } else {
+            // values past the buffer's capacity are dropped (the dictionary
+            // is then not used)
+            for (uint32_t r = m; r; r &= r - 1, base++)
+                if (base < cap_vals) out_vals[base] = vals[(int64_t)w * 32 + __builtin_ctz(r)];
+        }
+    }
+    if (!in_lds) return;
+    __syncthreads();
+    const int64_t slots = (int64_t)words * 32;
+    for (int64_t sl = threadIdx.x; sl < slots; sl += RANK_BLOCK) {
+        const uint32_t w = (uint32_t)(sl >> 5), b = (uint32_t)(sl & 31);
+        const uint32_t m = s_bits[w];
+        if ((m >> b) & 1u) {
+            const uint32_t r = s_prefix[w] + __popc(m & ((1u << b) - 1u));
+            if (r < cap_vals) out_vals[r] = vals[sl];
+        }
     }
 }
 
