@@ -137,6 +137,53 @@ __global__ void __launch_bounds__(BLOCK) k_logical(const uint8_t *in, const uint
         out[i] = logical(LOGIC, in[i] != 0, local[i] != 0);
 }
 
+// 0x01 in each byte of x that is nonzero, 0x00 elsewhere (no carry crosses a
+// byte: (b & 0x7f) + 0x7f <= 0xfe)
+__device__ __forceinline__ uint32_t nz_bytes(uint32_t x)
+{
+    return ((((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x) >> 7) & 0x01010101u;
+}
+
+template <int LOGIC> __device__ __forceinline__ uint32_t logical_bytes(uint32_t a, uint32_t b)
+{
+    return LOGIC == SDB_NONE ? nz_bytes(b) : LOGIC == SDB_AND ? (nz_bytes(a) & nz_bytes(b)) : (nz_bytes(a) | nz_bytes(b));
+}
+
+// The same, 16 flags per thread and step (the three buffers 16-byte
+// aligned); the last n % 16 flags one per thread.
+template <int LOGIC>
+__global__ void __launch_bounds__(BLOCK) k_logical_vec(const uint8_t *__restrict__ in,
+                                                       const uint8_t *__restrict__ local, uint8_t *__restrict__ out,
+                                                       int64_t n)
+{
+    const int64_t nq = n >> 4;
+    const uint4 *a4 = reinterpret_cast<const uint4 *>(in);
+    const uint4 *b4 = reinterpret_cast<const uint4 *>(local);
+    uint4 *o4 = reinterpret_cast<uint4 *>(out);
+    for (int64_t base = (int64_t)blockIdx.x * BLOCK * UNROLL; base < nq; base += (int64_t)gridDim.x * BLOCK * UNROLL) {
+        uint4 a[UNROLL], b[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            const int64_t c = base + u * BLOCK + threadIdx.x;
+            if (c < nq) {
+                a[u] = LOGIC == SDB_NONE ? make_uint4(0, 0, 0, 0) : a4[c];
+                b[u] = b4[c];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            const int64_t c = base + u * BLOCK + threadIdx.x;
+            if (c < nq)
+                o4[c] = make_uint4(logical_bytes<LOGIC>(a[u].x, b[u].x), logical_bytes<LOGIC>(a[u].y, b[u].y),
+                                   logical_bytes<LOGIC>(a[u].z, b[u].z), logical_bytes<LOGIC>(a[u].w, b[u].w));
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (n & 15)) {
+        const int64_t i = (nq << 4) + threadIdx.x;
+        out[i] = logical(LOGIC, in[i] != 0, local[i] != 0);
+    }
+}
+
 // ------------------------------------------------------------ projection
 
 __global__ void __launch_bounds__(BLOCK) k_fill_i32(int32_t *dst, int32_t v, int64_t n)
@@ -257,11 +304,15 @@ __global__ void __launch_bounds__(BLOCK) k_build_keyvals(const int32_t *keys, co
     }
 }
 
+// vec: 4 rows per thread and step (keys 16-byte, flags 4-byte aligned),
+// else one row.  The keys load beside the flags, not after them: one wait
+// fewer per step, for the keys of chunks without a flagged row.
 __global__ void __launch_bounds__(BLOCK) k_probe_semi(const int32_t *__restrict__ probe,
                                                       const uint8_t *in, uint8_t *out, int64_t n,
                                                       const uint8_t *__restrict__ ht, int32_t min,
                                                       int32_t max, int vec)
 {
+    auto present = [&](int32_t x) { return x >= min && x <= max && ht[slot_of(x, min)] != 0; };
     if (vec) {
         const int64_t nchunks = n >> 2;
         const int4 *p4 = reinterpret_cast<const int4 *>(probe);
@@ -275,7 +326,7 @@ __global__ void __launch_bounds__(BLOCK) k_probe_semi(const int32_t *__restrict_
             for (int u = 0; u < UNROLL; u++) {
                 const int64_t c = base + u * BLOCK + threadIdx.x;
                 f[u] = c < nchunks ? in32[c] : 0u;
-                if (f[u]) x[u] = p4[c];
+                x[u] = c < nchunks ? p4[c] : make_int4(0, 0, 0, 0);
             }
 #pragma unroll
             for (int u = 0; u < UNROLL; u++) {
@@ -286,8 +337,7 @@ __global__ void __launch_bounds__(BLOCK) k_probe_semi(const int32_t *__restrict_
                     const int32_t xs[4] = {x[u].x, x[u].y, x[u].z, x[u].w};
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
-                        const bool hit = byte_of(f[u], k) && xs[k] >= min && xs[k] <= max &&
-                                         ht[slot_of(xs[k], min)] != 0;
+                        const bool hit = byte_of(f[u], k) && present(xs[k]);
                         r |= (uint32_t)hit << (8 * k);
                     }
                 }
@@ -296,18 +346,14 @@ __global__ void __launch_bounds__(BLOCK) k_probe_semi(const int32_t *__restrict_
         }
         if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
             const int64_t i = (nchunks << 2) + threadIdx.x;
-            const int32_t x = probe[i];
-            out[i] = in[i] && x >= min && x <= max && ht[slot_of(x, min)] != 0;
+            out[i] = in[i] && present(probe[i]);
         }
         return;
     }
     for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
         const bool f = in[i] != 0;
         bool r = false;
-        if (f) {
-            const int32_t x = probe[i];
-            r = x >= min && x <= max && ht[slot_of(x, min)] != 0;
-        }
+        if (f) r = present(probe[i]);
         out[i] = r;
     }
 }
@@ -1119,6 +1165,16 @@ void launch_select_col(const int32_t *a, const int32_t *b, const uint8_t *in, ui
 void launch_logical(const uint8_t *in, const uint8_t *local, uint8_t *out, int64_t n, int logic, hipStream_t s)
 {
     if (n <= 0) return;
+    if (aligned(in, 16) && aligned(local, 16) && aligned(out, 16)) {
+        const unsigned grid = stream_grid((n >> 4) + 1, (int64_t)BLOCK * UNROLL);
+        switch (logic) {
+        case SDB_NONE: hipLaunchKernelGGL((k_logical_vec<SDB_NONE>), dim3(grid), dim3(BLOCK), 0, s, in, local, out, n); break;
+        case SDB_AND: hipLaunchKernelGGL((k_logical_vec<SDB_AND>), dim3(grid), dim3(BLOCK), 0, s, in, local, out, n); break;
+        default: hipLaunchKernelGGL((k_logical_vec<SDB_OR>), dim3(grid), dim3(BLOCK), 0, s, in, local, out, n); break;
+        }
+        check_launch("logical");
+        return;
+    }
     switch (logic) {
     case SDB_NONE: hipLaunchKernelGGL((k_logical<SDB_NONE>), dim3(grid_one(n)), dim3(BLOCK), 0, s, in, local, out, n); break;
     case SDB_AND: hipLaunchKernelGGL((k_logical<SDB_AND>), dim3(grid_one(n)), dim3(BLOCK), 0, s, in, local, out, n); break;

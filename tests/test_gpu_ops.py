@@ -12,7 +12,7 @@ torch = pytest.importorskip("torch")
 from oracle import oracle as O  # noqa: E402
 
 SIZES = [0, 1, 3, 4, 5, 1023, 4099, 100_003]
-OFFSETS = [0, 1]
+OFFSETS = [0, 1, 4]  # 16-byte aligned, misaligned, 4-byte aligned flags (16-byte aligned int32)
 
 
 @pytest.fixture(scope="module")
@@ -83,6 +83,27 @@ def test_select_inplace_and_columns(ops, n):
     assert (host(t) == want).all()
 
 
+@pytest.mark.parametrize("n", [0, 15, 16, 17, 100_003])
+@pytest.mark.parametrize("off", OFFSETS)
+def test_logical_any_nonzero_byte(ops, n, off):
+    """Flags are bool bytes: any nonzero byte is true (0x80, 0xff, 2, ...),
+    through the 16-flags-per-thread path (aligned) and the byte path."""
+    rng = np.random.default_rng(12)
+    a = np.where(rng.random(n) < 0.5, rng.integers(1, 256, n), 0).astype(np.uint8)
+    b = np.where(rng.random(n) < 0.5, rng.integers(1, 256, n), 0).astype(np.uint8)
+    a[: min(n, 4)] = [0x80, 0xFF, 0x01, 0x7F][: min(n, 4)]
+    for logic in range(3):
+        _, ta = padded(a, off)
+        _, tb = padded(b, off)
+        buf, out = padded(np.zeros(n, np.uint8), off)
+        ops.logical(ta, tb, out, logic)
+        want = np.zeros(n, np.uint8)
+        O.logical(a, b, want, logic)
+        got = host(buf)
+        assert (got[off:off + n] == want).all(), logic
+        assert (got[:off] == 0).all() and (got[off + n:] == 0).all()
+
+
 @pytest.mark.parametrize("n", [0, 7, 100_003])
 def test_logical_and_fill(ops, n):
     rng = np.random.default_rng(11)
@@ -129,6 +150,26 @@ def test_binops(ops, n, off):
             assert (host(r) == want).all(), (op, kind)
     with pytest.raises(ValueError):
         ops.perform_operation(r, ta, tb, tf, n, "%")
+
+
+@pytest.mark.parametrize("hoff", [0, 1])
+@pytest.mark.parametrize("length", [1, 31, 777, 2500])
+def test_probe_semi_tables(ops, hoff, length):
+    """Semi probes over key tables of odd lengths, a table pointer off
+    16-byte alignment, any nonzero byte present, keys on both sides of the
+    range, a row count off 4."""
+    rng = np.random.default_rng(length + hoff)
+    mn = -37
+    table = np.where(rng.random(length) < 0.4, rng.integers(1, 256, length), 0).astype(np.uint8)
+    _, ht = padded(table, hoff)
+    n = 300_001
+    probe = rng.integers(mn - 40, mn + length + 40, n).astype(np.int32)
+    pfl = rand_flags(rng, n, 0.7)
+    out = torch.full((n,), 3, dtype=torch.uint8, device="cuda")
+    ops.probe_semi(dev(probe), dev(pfl), out, ht, mn, mn + length - 1)
+    want = np.zeros(n, np.uint8)
+    O.probe_semi(probe, pfl, want, table, mn, mn + length - 1)
+    assert (host(out) == want).all()
 
 
 @pytest.mark.parametrize("nb,np_", [(1, 10), (1000, 100_003), (20_000, 4099)])
