@@ -1388,7 +1388,12 @@ std::string jit_unavailable_reason() { return rtc().ok ? std::string() : rtc().w
 int jit_check(const std::string &arch)
 {
     std::vector<std::string> srcs;
-    for (auto &pu : coverage_programs()) srcs.push_back(jit_source(pu.first, pu.second));
+    for (auto &pu : coverage_programs()) {
+        // with the queued row tail where the program has one, and without
+        const std::string q = jit_source(pu.first, pu.second, 256, true), plain = jit_source(pu.first, pu.second, 256, false);
+        srcs.push_back(q);
+        if (plain != q) srcs.push_back(plain);
+    }
     return (int)compile_all(srcs, arch).size() + jit_batch_check(arch);
 }
 
