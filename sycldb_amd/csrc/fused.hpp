@@ -81,6 +81,16 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
 void set_batch_graphs(int on);
 // Graph captures and replays of batched launches so far.
 void batch_graph_stats(int64_t *captures, int64_t *replays);
+// Whether batched launches on the calling thread may run autotuner trials
+// (default on).  Off, a batch takes the choice already tuned for its kind,
+// or the default shape: the server's workers run so, keeping trials off
+// their requests' latency.
+void set_batch_trials(bool on);
+// Waves per CU a batched launch on the calling thread may take (0, the
+// default: as many as fit).  The server's per-engine workers leave a share
+// of every CU free, so that one engine's kernels run beside another's
+// batched scan instead of behind it.
+void set_batch_waves_per_cu(int cap);
 // Builds the mask tables of a batched launch (ops.hip).
 void launch_batch_masks(const FMaskCol *cols, int ncols, uint32_t max_len, const FMaskEntry *ents, hipStream_t s);
 // Compiles (no device needed) a batched kernel over the coverage programs.

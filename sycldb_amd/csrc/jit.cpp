@@ -2387,6 +2387,12 @@ struct BatchLibrary {
 
 // Batched-kernel autotuner state per library: the best time per row and
 // program measured for each candidate launch shape.
+thread_local bool t_batch_trials = true;
+thread_local int t_batch_waves_cap = 0; // waves per CU a batched launch may take (0: what fits)
+// kinds of batch (mask-table signatures) a library tunes; batches of further
+// kinds take the default shape
+constexpr int BATCH_TUNE_MAX_KINDS = 16;
+
 struct BatchTune {
     double best[16];
     int trials = 0, chosen = -1;
@@ -3011,11 +3017,17 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
         BatchShape want = shape;
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
-            BatchTune &t = g_batch_tune[tune_key];
-            if (t.chosen >= 0) want = BATCH_CANDIDATES[t.chosen];
-            else {
-                trial = t.trials % N_BATCH_CAND;
-                want = BATCH_CANDIDATES[trial];
+            auto it = g_batch_tune.find(tune_key);
+            if (it != g_batch_tune.end() && it->second.chosen >= 0) want = BATCH_CANDIDATES[it->second.chosen];
+            else if (t_batch_trials) {
+                int kinds = 0;
+                const std::string prefix = lib_key + " masks ";
+                for (auto &kv : g_batch_tune) kinds += kv.first.compare(0, prefix.size(), prefix) == 0;
+                if (it != g_batch_tune.end() || kinds < BATCH_TUNE_MAX_KINDS) {
+                    BatchTune &t = g_batch_tune[tune_key];
+                    trial = t.trials % N_BATCH_CAND;
+                    want = BATCH_CANDIDATES[trial];
+                }
             }
         }
         if (want.key() != shape.key()) b = plan_batch(in, unroll, lib_key, masks, want);
@@ -3204,6 +3216,9 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
         }
         per_cu = it->second;
     }
+    // a share of each CU's wave slots, where the caller asks for one (other
+    // engines' kernels then run beside this launch)
+    if (t_batch_waves_cap > 0) per_cu = std::max(1, std::min(per_cu, t_batch_waves_cap / (shape.block / 64)));
     const int64_t units = b.vec ? (n + 3) / 4 : n;
     // at least BATCH_MIN_TILES tiles per block: each block also stages and
     // flushes every program's LDS state, which small tables must amortise
@@ -3317,6 +3332,9 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
         t.chosen = best;
     }
 }
+
+void set_batch_trials(bool on) { t_batch_trials = on; }
+void set_batch_waves_per_cu(int cap) { t_batch_waves_cap = cap; }
 
 int jit_pending()
 {

@@ -3,27 +3,28 @@
 //
 // A server holds several engines (each with its own resident tables, stream
 // and allocator).  Requests -- a plan for one engine -- are queued per engine
-// with their arrival time.  In batching mode one worker, as soon as it is
-// free, takes the engine whose first waiting request is the oldest and admits
-// that engine's waiting requests in arrival order while a slot (max_batch)
-// and the memory budget (the sum of the requests' estimated footprints) allow
-// -- always at least one -- and runs them as one batch (execute_batch: one
-// shared fact scan).  Requests that arrive while a batch runs wait for the
-// next one: continuous batching.  One batch runs at a time: a batched launch
-// fills every CU, so batches of different engines would only queue behind
-// each other on the device (and a small table's requests behind a large
-// table's scan).  A shared scan has a fixed cost that a few small requests do
-// not repay, so per engine the worker learns the batch size from which
-// batching pays: it keeps a running mean of a request's plain run time p and
-// models a batch of n as A + B n with B a fixed fraction of p, learning A
-// from every batch.  It batches an engine's n waiting requests only when
-// that lowers the summed latency of all waiting requests (the batch's n and
-// the m of the other engines, which wait for it); otherwise it takes only
-// the oldest and runs it alone, the others staying queued in arrival order.
+// with their arrival time.  In batching mode each engine has a worker (or,
+// with single_worker, one worker takes the engine whose first waiting request
+// is the oldest); when free, it admits its engine's waiting requests in
+// arrival order while a slot (max_batch) and the memory budget (the sum of
+// the requests' estimated footprints) allow -- always at least one -- and
+// runs them as one batch (execute_batch: one shared fact scan).  Requests
+// that arrive while a batch runs wait for the next one: continuous batching.
+// The engines' workers overlap on the device: one engine's host work and
+// kernels run beside another's batched scan, which therefore takes at most
+// SERVE_BATCH_WAVES_PER_CU of each CU's wave slots; and no worker runs
+// autotuner trials on requests.  A shared scan has a fixed cost that a few
+// small requests do not repay, so per engine the worker learns the batch size
+// from which batching pays: it keeps a running mean of a request's plain run
+// time p and models a batch of n as A + B n with B a fixed fraction of p,
+// learning A from every batch.  It batches an engine's n waiting requests
+// only when that lowers the summed latency of all waiting requests (the
+// batch's n and the m waiting for other engines); otherwise it takes only the
+// oldest and runs it alone, the others staying queued in arrival order.
 // Light load thus runs requests one by one, a growing queue shares scans.
-// Every request's result is handed to its waiter as soon as it is ready.  In serial mode the
-// same worker serves all requests one at a time in arrival order through the
-// plain path: the sequential baseline.
+// Every request's result is handed to its waiter as soon as it is ready.  In
+// serial mode one worker serves all requests one at a time in arrival order
+// through the plain path: the sequential baseline.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -95,6 +96,10 @@ uint64_t estimate_footprint(Engine &eng, const PlanResult &plan)
 // (measured at 600M rows: ~6 ms against 1.6 ms alone).
 constexpr double BATCH_MARGINAL = 0.3;
 constexpr double BATCH_FIXED_INITIAL = 4.0;
+// Wave slots per CU (of 32) a batched scan may take while other engines'
+// workers run: a batched kernel that fills every slot makes the others'
+// kernels wait for its whole scan.
+constexpr int SERVE_BATCH_WAVES_PER_CU = 24;
 
 struct Server {
     std::vector<Engine *> engines;
@@ -128,6 +133,12 @@ struct Server {
     // engine, which takes the engine whose first request is the oldest.
     void worker(int own)
     {
+        // no autotuner trials on requests: their batches take the choices
+        // already tuned for their kind, or the default shape
+        set_batch_trials(false);
+        // with a worker per engine, a batched scan leaves a quarter of each
+        // CU's wave slots to the other engines' kernels
+        if (own >= 0 && engines.size() > 1) set_batch_waves_per_cu(SERVE_BATCH_WAVES_PER_CU);
         while (true) {
             std::vector<std::shared_ptr<Request>> admitted;
             {
