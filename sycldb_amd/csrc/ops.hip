@@ -835,7 +835,9 @@ __global__ void __launch_bounds__(BLOCK) k_bits_from_pairs(const int2 *ht, int32
 // prefixes), one thread per slot writes the values of the set slots at their
 // ranks: coalesced reads of the value table, every load independent.  A
 // larger bitmap writes them per word, in its owner thread.
-constexpr int RANK_BLOCK = 1024;
+// (one 256-thread block: it fits beside a long batched scan of another
+// engine, where a 1024-thread block would wait for that scan to end)
+constexpr int RANK_BLOCK = 256;
 constexpr int RANK_LDS_WORDS = 4096;
 
 __global__ void __launch_bounds__(RANK_BLOCK) k_rank_dict(const uint32_t *__restrict__ bits, int32_t words,
