@@ -93,6 +93,8 @@ void set_batch_trials(bool on);
 void set_batch_waves_per_cu(int cap);
 // Builds the mask tables of a batched launch (ops.hip).
 void launch_batch_masks(const FMaskCol *cols, int ncols, uint32_t max_len, const FMaskEntry *ents, hipStream_t s);
+// Builds the rank dictionaries of some of them, after the tables (ops.hip).
+void launch_batch_dict(const FMaskDict *dicts, int ndicts, hipStream_t s);
 // Compiles (no device needed) a batched kernel over the coverage programs.
 int jit_batch_check(const std::string &arch);
 // Compiles (without launching; no device needed) the kernels of programs

@@ -179,6 +179,26 @@ struct FMaskCol {
 constexpr int F_MASK_MAX_COLS = 8;
 constexpr int64_t F_MASK_MAX_LEN = int64_t(1) << 24;
 
+// Rank dictionary of a batched mask table: the keys whose stored window is
+// not zero (some probing program passes), in key order.  In global memory a
+// 16-byte header (u32 ok: the entries fitted F_DICT_CAP; u32 count) precedes
+// the part a block copies into LDS: the presence bitmap over the table's len
+// keys (u32 words), the number of set bits before each word (u16, padded to
+// 16 bytes) and the windows of the set keys (u32 each, up to F_DICT_CAP).
+// A key whose presence bit is clear has window 0.
+struct FMaskDict {
+    const uint8_t *tab; // the mask table (len windows of wbytes bytes)
+    uint8_t *dict;      // header, then the LDS part
+    uint32_t len, wbytes;
+};
+constexpr uint32_t F_DICT_CAP = 4096;
+constexpr uint32_t F_DICT_MAX_LEN = 4096 * 32; // bitmap words the builder holds in LDS
+__host__ __device__ constexpr uint32_t dict_words(uint32_t len) { return (len + 31u) / 32u; }
+__host__ __device__ constexpr uint32_t dict_prefix_off(uint32_t len) { return 4u * dict_words(len); }
+__host__ __device__ constexpr uint32_t dict_vals_off(uint32_t len) { return (6u * dict_words(len) + 15u) / 16u * 16u; }
+// bytes of the LDS part with room for F_DICT_CAP windows
+__host__ __device__ constexpr uint32_t dict_lds_bytes(uint32_t len) { return dict_vals_off(len) + 4u * F_DICT_CAP; }
+
 // Largest rank dictionary built for LDS staging.
 constexpr uint32_t F_RANK_DICT_MAX = 32 * 1024;
 

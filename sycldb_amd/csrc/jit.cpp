@@ -2148,12 +2148,14 @@ const char *BATCH_HEAD = R"(struct BatchArgs {
     int32_t first[NSHAPE + 1];
     const int32_t *pf[NPF > 0 ? NPF : 1];
     const uint8_t *mtab[NMASK > 0 ? NMASK : 1];
+    const uint8_t *mdg[NMASK > 0 ? NMASK : 1]; // rank dictionary in global memory (header first), or null
     int32_t mlo[NMASK > 0 ? NMASK : 1];
     uint32_t mlen[NMASK > 0 ? NMASK : 1];
     uint32_t mdflt[NMASK > 0 ? NMASK : 1];
     uint32_t mshift[NMASK > 0 ? NMASK : 1];
     uint32_t mwbytes[NMASK > 0 ? NMASK : 1];
     uint32_t mlds[NMASK > 0 ? NMASK : 1]; // 1 + LDS byte offset of a staged table (0: read from memory)
+    uint32_t mdict[NMASK > 0 ? NMASK : 1]; // 1 + LDS byte offset of the rank dictionary's copy (0: none)
     int32_t morder[NMASK > 0 ? NMASK : 1]; // lazy reads: masked columns, fewest rows kept first
     int32_t bitpos[32]; // program i's bit in the mask words
 };
@@ -2191,6 +2193,19 @@ __device__ __forceinline__ uint32_t mask_word(const BatchArgs &A, const uint8_t 
                                 : *reinterpret_cast<const uint32_t *>(A.mtab[m] + (byte & ~3u));
     return raw >> ((byte & 3u) * 8u);
 }
+// The stored window of key x from column m's rank dictionary in LDS (0 for
+// a key without a set presence bit; out-of-table keys are left to
+// mask_widen, which gives them the default word).
+__device__ __forceinline__ uint32_t dict_window(const BatchArgs &A, const uint8_t *lds, int m, int32_t x)
+{
+    const uint32_t idx = (uint32_t)x - (uint32_t)A.mlo[m], len = A.mlen[m];
+    if (idx >= len) return 0u;
+    const uint8_t *base = lds + (A.mdict[m] - 1u);
+    const uint32_t bits = reinterpret_cast<const uint32_t *>(base)[idx >> 5], bit = idx & 31u;
+    const uint32_t r = reinterpret_cast<const uint16_t *>(base + dict_prefix_off(len))[idx >> 5] +
+                       (uint32_t)__popc(bits & ((1u << bit) - 1u));
+    return ((bits >> bit) & 1u) ? reinterpret_cast<const uint32_t *>(base + dict_vals_off(len))[r] : 0u;
+}
 __device__ __forceinline__ uint32_t mask_widen(const BatchArgs &A, int m, int32_t x, uint32_t w)
 {
     const uint32_t idx = (uint32_t)x - (uint32_t)A.mlo[m];
@@ -2199,7 +2214,7 @@ __device__ __forceinline__ uint32_t mask_widen(const BatchArgs &A, int m, int32_
     return idx < A.mlen[m] ? word : A.mdflt[m];
 }
 template <int R, int UU>
-__device__ __forceinline__ void mask_rows(const BatchArgs &A, const Pf<R, UU> &pf, const uint8_t *lds,
+__device__ __forceinline__ void mask_rows(const BatchArgs &A, const Pf<R, UU> &pf, const uint8_t *lds, uint32_t dmask,
                                           uint32_t (&mw)[UU][R])
 {
 #pragma unroll
@@ -2211,14 +2226,16 @@ __device__ __forceinline__ void mask_rows(const BatchArgs &A, const Pf<R, UU> &p
 #pragma unroll
         for (int m = 0; m < NMASK; m++) {
             if (!A.mlen[m]) continue;
-            // a table staged in LDS is read there (the branch is uniform)
-            const bool staged = A.mlds[m] != 0;
+            // a table staged in LDS, or its rank dictionary there, is read
+            // there (the branches are uniform)
+            const bool staged = A.mlds[m] != 0, dict = (dmask >> m) & 1u;
 #pragma unroll
             for (int u = 0; u < UU; u++) {
                 int32_t x[R];
                 vec_rows<R>(pf.c[MSLOT[m]][u], x);
 #pragma unroll
-                for (int k = 0; k < R; k++) w[m][u][k] = mask_word<R>(A, lds, m, staged, x[k]);
+                for (int k = 0; k < R; k++)
+                    w[m][u][k] = dict ? dict_window(A, lds, m, x[k]) : mask_word<R>(A, lds, m, staged, x[k]);
             }
         }
 #pragma unroll
@@ -2238,7 +2255,7 @@ __device__ __forceinline__ void mask_rows(const BatchArgs &A, const Pf<R, UU> &p
     for (int j = 0; j < NMASK; j++) {
         const int m = A.morder[j];
         if (m < 0 || !A.mlen[m]) continue;
-        const bool staged = A.mlds[m] != 0;
+        const bool staged = A.mlds[m] != 0, dict = (dmask >> m) & 1u;
 #pragma unroll
         for (int u = 0; u < UU; u++) {
             // the key column of mask m (m is uniform: one register is selected)
@@ -2249,7 +2266,8 @@ __device__ __forceinline__ void mask_rows(const BatchArgs &A, const Pf<R, UU> &p
                 if (m == c) vec_rows<R>(pf.c[MSLOT[c]][u], x);
 #pragma unroll
             for (int k = 0; k < R; k++)
-                if (mw[u][k]) mw[u][k] &= mask_widen(A, m, x[k], mask_word<R>(A, lds, m, staged, x[k]));
+                if (mw[u][k])
+                    mw[u][k] &= mask_widen(A, m, x[k], dict ? dict_window(A, lds, m, x[k]) : mask_word<R>(A, lds, m, staged, x[k]));
         }
     }
 }
@@ -2284,14 +2302,25 @@ extern "C" __global__ void __launch_bounds__(BLOCK) sdb_fused_batch(const BatchA
     }
     unsigned long long *sums = reinterpret_cast<unsigned long long *>(lds + A.sums_off);
     for (int i = threadIdx.x; i < A.nprog; i += BLOCK) sums[i] = 0;
-    // mask tables staged in LDS (their sizes are padded to 16 bytes)
+    // mask tables staged in LDS (their sizes are padded to 16 bytes), and
+    // the rank dictionaries that fitted (bitmap, counts and set windows)
+    uint32_t dmask = 0;
     for (int m = 0; m < NMASK; m++)
         if (A.mlds[m]) {
             const uint4 *src = reinterpret_cast<const uint4 *>(A.mtab[m]);
             uint4 *dst = reinterpret_cast<uint4 *>(lds + (A.mlds[m] - 1u));
             const uint32_t n16 = (A.mlen[m] * A.mwbytes[m] + 15u) / 16u;
             for (uint32_t v = threadIdx.x; v < n16; v += BLOCK) dst[v] = src[v];
+        } else if (A.mdict[m]) {
+            const uint32_t *hdr = reinterpret_cast<const uint32_t *>(A.mdg[m]);
+            if (!hdr[0]) continue; // more set keys than F_DICT_CAP: the table is read instead
+            dmask |= 1u << m;
+            const uint4 *src = reinterpret_cast<const uint4 *>(A.mdg[m] + 16);
+            uint4 *dst = reinterpret_cast<uint4 *>(lds + (A.mdict[m] - 1u));
+            const uint32_t n16 = (dict_vals_off(A.mlen[m]) + 4u * hdr[1] + 15u) / 16u;
+            for (uint32_t v = threadIdx.x; v < n16; v += BLOCK) dst[v] = src[v];
         }
+    dmask = __builtin_amdgcn_readfirstlane(dmask);
     __syncthreads();
     const int64_t n = A.n;
     if constexpr (VEC) {
@@ -2318,7 +2347,7 @@ extern "C" __global__ void __launch_bounds__(BLOCK) sdb_fused_batch(const BatchA
                     rows_of(tn, rn);
                     pf_load<4, U>(A, rn, next);
                 }
-                run_all<4, U>(A, r, cur, lds, sums);
+                run_all<4, U>(A, r, cur, lds, sums, dmask);
                 t = tn;
             }
         } else {
@@ -2328,7 +2357,7 @@ extern "C" __global__ void __launch_bounds__(BLOCK) sdb_fused_batch(const BatchA
                 rows_of(t, r);
                 Pf<4, U> cur = {};
                 pf_load<4, U>(A, r, cur);
-                run_all<4, U>(A, r, cur, lds, sums);
+                run_all<4, U>(A, r, cur, lds, sums, dmask);
             }
         }
         for (int64_t c = ntiles * tile + (int64_t)blockIdx.x * BLOCK + threadIdx.x; c < nch;
@@ -2336,20 +2365,20 @@ extern "C" __global__ void __launch_bounds__(BLOCK) sdb_fused_batch(const BatchA
             const int64_t r[1] = {c << 2};
             Pf<4, 1> pf = {};
             pf_load<4, 1>(A, r, pf);
-            run_all<4, 1>(A, r, pf, lds, sums);
+            run_all<4, 1>(A, r, pf, lds, sums, dmask);
         }
         if (blockIdx.x == gridDim.x - 1 && threadIdx.x < (n & 3)) {
             const int64_t r[1] = {(nch << 2) + threadIdx.x};
             Pf<1, 1> pf = {};
             pf_load<1, 1>(A, r, pf);
-            run_all<1, 1>(A, r, pf, lds, sums);
+            run_all<1, 1>(A, r, pf, lds, sums, dmask);
         }
     } else {
         for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
             const int64_t r[1] = {i};
             Pf<1, 1> pf = {};
             pf_load<1, 1>(A, r, pf);
-            run_all<1, 1>(A, r, pf, lds, sums);
+            run_all<1, 1>(A, r, pf, lds, sums, dmask);
         }
     }
     __syncthreads();
@@ -2416,14 +2445,25 @@ struct BatchShape {
     bool double_buffer;
     bool wide = false;
     bool stage = false; // the smallest mask tables copied into LDS per block
+    bool dict = false;  // sparse mask tables as rank dictionaries in LDS
     int id() const { return block * 2 + (double_buffer ? 1 : 0); } // the compiled source
-    int key() const { return (id() * 2 + (wide ? 1 : 0)) * 2 + (stage ? 1 : 0); }
+    int key() const { return ((id() * 2 + (wide ? 1 : 0)) * 2 + (stage ? 1 : 0)) * 2 + (dict ? 1 : 0); }
 };
 // autotuner candidates; the first is the default below the tuning size
 const BatchShape BATCH_CANDIDATES[] = {{512, false, false, false}, {256, false, false, false},
                                        {1024, false, false, false}, {1024, true, false, false},
                                        {256, false, true, false},  {512, false, true, false},
-                                       {256, false, false, true},  {1024, false, false, true}};
+                                       {256, false, false, true},  {1024, false, false, true},
+                                       {512, false, false, false, true}, {256, false, false, false, true}};
+// a mask table becomes a rank dictionary (with the dict layout) when it spans
+// at least this many keys (and at most F_DICT_MAX_LEN): a sparse key range;
+// SDB_BATCH_DICT_MIN_LEN sets another bound
+constexpr uint32_t BATCH_DICT_MIN_LEN = 32768;
+uint32_t batch_dict_min_len()
+{
+    const char *e = getenv("SDB_BATCH_DICT_MIN_LEN");
+    return e && *e ? (uint32_t)std::max(1L, atol(e)) : BATCH_DICT_MIN_LEN;
+}
 // LDS a batched block may take with staged mask tables (gfx950: 160 KB per CU)
 constexpr uint64_t BATCH_STAGE_LDS_MAX = 144 * 1024;
 constexpr int N_BATCH_CAND = (int)(sizeof(BATCH_CANDIDATES) / sizeof(BATCH_CANDIDATES[0]));
@@ -2569,6 +2609,7 @@ struct MaskPlan {
     std::vector<FMaskEntry> ents;
     uint32_t shift = 0, wbytes = 4; // the window of the word the table stores
     uint32_t lds = 0;               // 1 + byte offset of its LDS copy (0: read from memory)
+    uint32_t dict = 0;              // 1 + byte offset of its rank dictionary's LDS copy (0: none)
     uint64_t bytes() const { return ((uint64_t)std::max<int64_t>(hi - lo + 1, 1) * wbytes + 15) / 16 * 16; }
 };
 
@@ -2589,19 +2630,22 @@ struct BatchPlan {
 };
 
 // A launch shape fixed by SDB_BATCH_BLOCK / SDB_BATCH_DOUBLE_BUFFER /
-// SDB_BATCH_WIDE_MASKS / SDB_BATCH_STAGE_MASKS (no tuning), if any is set.
+// SDB_BATCH_WIDE_MASKS / SDB_BATCH_STAGE_MASKS / SDB_BATCH_DICT_MASKS (no
+// tuning), if any is set.
 bool batch_shape_fixed(BatchShape &out)
 {
     const char *b = getenv("SDB_BATCH_BLOCK");
     const char *d = getenv("SDB_BATCH_DOUBLE_BUFFER");
     const char *w = getenv("SDB_BATCH_WIDE_MASKS");
     const char *g = getenv("SDB_BATCH_STAGE_MASKS");
-    if (!(b && *b) && !(d && *d) && !(w && *w) && !(g && *g)) return false;
+    const char *r = getenv("SDB_BATCH_DICT_MASKS");
+    if (!(b && *b) && !(d && *d) && !(w && *w) && !(g && *g) && !(r && *r)) return false;
     const int blk = b && *b ? atoi(b) : BATCH_BLOCK;
     out.block = blk == 256 || blk == 512 || blk == 1024 ? blk : BATCH_BLOCK;
     out.double_buffer = d && *d == '1';
     out.wide = w && *w == '1';
     out.stage = g && *g == '1';
+    out.dict = r && *r == '1' && !out.stage;
     return true;
 }
 
@@ -2623,8 +2667,8 @@ std::string batch_source(const BatchLibrary &lib, int unroll, bool vec, const Ba
              "const uint32_t (&act)[UU])\n{\n"
           << lib.bodies[k] << "}\n";
     s << "template <int R, int UU>\n__device__ __forceinline__ void run_all(const BatchArgs &A, const int64_t "
-         "(&row0)[UU], const Pf<R, UU> &pf, uint8_t *lds, unsigned long long *sums)\n{\n"
-         "    uint32_t mw[UU][R];\n    mask_rows<R, UU>(A, pf, lds, mw);\n    uint32_t any = 0;\n"
+         "(&row0)[UU], const Pf<R, UU> &pf, uint8_t *lds, unsigned long long *sums, uint32_t dmask)\n{\n"
+         "    uint32_t mw[UU][R];\n    mask_rows<R, UU>(A, pf, lds, dmask, mw);\n    uint32_t any = 0;\n"
          "#pragma unroll\n    for (int u = 0; u < UU; u++)\n#pragma unroll\n"
          "        for (int k = 0; k < R; k++) any |= mw[u][k];\n"
          "    // programs some row of the wave keeps (read from the first lane so\n"
@@ -2885,6 +2929,20 @@ BatchPlan plan_batch(const std::vector<FProgram> &in, int unroll, const std::str
         }
         if (at > b.lds) b.lds = at;
     }
+    if (shape.dict) {
+        // tables over a long, sparse key range as rank dictionaries in LDS,
+        // while the block's LDS stays within BATCH_STAGE_LDS_MAX
+        uint32_t at = (b.lds + 15u) & ~15u;
+        for (MaskPlan &m : b.masks) {
+            const int64_t len = m.hi - m.lo + 1;
+            if (len < (int64_t)batch_dict_min_len() || len > (int64_t)F_DICT_MAX_LEN) continue;
+            const uint32_t need = dict_lds_bytes((uint32_t)len);
+            if ((uint64_t)at + need > BATCH_STAGE_LDS_MAX) continue;
+            m.dict = at + 1u;
+            at += need;
+        }
+        if (at > b.lds) b.lds = at;
+    }
     BatchLibrary::Src &src = lib.srcs[shape.id()];
     if (src.state[0] != lib.bodies.size() || src.state[1] != lib.slots.size() ||
         src.state[2] != lib.mask_slots.size()) {
@@ -3053,22 +3111,29 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     const int nmask = (int)b.mask_slots.size();
     const int nshape = (int)b.first.size() - 1;
     size_t nents = 0, tab_bytes = 16; // the first 16 bytes: the all-ones word
+    int ndicts = 0;
     for (const MaskPlan &m : b.masks) {
         nents += m.ents.size();
         tab_bytes += m.bytes(); // padded to 16 bytes, so a staged table copies in uint4s
     }
+    for (const MaskPlan &m : b.masks) // rank dictionaries after the tables
+        if (m.dict) {
+            tab_bytes += 16 + dict_lds_bytes((uint32_t)(m.hi - m.lo + 1));
+            ndicts++;
+        }
     // argument block (the kernel's BatchArgs): progs, n, nprog, sums_off,
-    // full, lazy, first[NSHAPE+1], pf[NPF], mtab[NMASK], mlo[NMASK],
-    // mlen[NMASK], mdflt[NMASK], mshift[NMASK], mwbytes[NMASK], mlds[NMASK],
-    // morder[NMASK], bitpos[32]
+    // full, lazy, first[NSHAPE+1], pf[NPF], mtab[NMASK], mdg[NMASK],
+    // mlo[NMASK], mlen[NMASK], mdflt[NMASK], mshift[NMASK], mwbytes[NMASK],
+    // mlds[NMASK], mdict[NMASK], morder[NMASK], bitpos[32]
     size_t args_bytes = 0;
     {
         auto grow = [&](size_t bytes, size_t align) { args_bytes = (args_bytes + align - 1) / align * align + bytes; };
         grow(8, 8), grow(8, 8), grow(4, 4), grow(4, 4), grow(4, 4), grow(4, 4);
         for (int i = 0; i <= nshape; i++) grow(4, 4);
         for (size_t i = 0; i < b.pf.size(); i++) grow(8, 8);
-        for (int i = 0; i < std::max(nmask, 1); i++) grow(8, 8);
-        for (int f = 0; f < 7; f++)
+        for (int f = 0; f < 2; f++)
+            for (int i = 0; i < std::max(nmask, 1); i++) grow(8, 8);
+        for (int f = 0; f < 8; f++)
             for (int i = 0; i < std::max(nmask, 1); i++) grow(4, 4);
         for (int i = 0; i < 32; i++) grow(4, 4);
         args_bytes = (args_bytes + 7) / 8 * 8;
@@ -3076,7 +3141,8 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     const size_t off_progs = 0;
     const size_t off_cols = off_progs + al256(b.progs.size() * sizeof(FProgram));
     const size_t off_ents = off_cols + al256(std::max<size_t>(b.masks.size(), 1) * sizeof(FMaskCol));
-    const size_t off_tabs = off_ents + al256(std::max<size_t>(nents, 1) * sizeof(FMaskEntry));
+    const size_t off_dicts = off_ents + al256(std::max<size_t>(nents, 1) * sizeof(FMaskEntry));
+    const size_t off_tabs = off_dicts + al256((size_t)std::max(ndicts, 1) * sizeof(FMaskDict));
     const size_t copy_bytes = off_tabs + 16, block_bytes = off_tabs + tab_bytes;
 
     // tuning trials and graphs off: the block from the caller's allocator
@@ -3123,12 +3189,14 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     std::memcpy(host + off_progs, b.progs.data(), b.progs.size() * sizeof(FProgram));
     FMaskCol *d_cols = reinterpret_cast<FMaskCol *>(dblock + off_cols);
     FMaskEntry *d_ents = reinterpret_cast<FMaskEntry *>(dblock + off_ents);
+    FMaskDict *d_dicts = reinterpret_cast<FMaskDict *>(dblock + off_dicts);
     uint8_t *tabs = dblock + off_tabs;
     std::vector<const uint8_t *> mtab(std::max(nmask, 1), tabs); // unused slots (mlen 0) are skipped
     std::vector<int32_t> mlo(std::max(nmask, 1), 0);
     std::vector<uint32_t> mlen(std::max(nmask, 1), 0), mdflt(std::max(nmask, 1), ~0u); // mlen 0: unused
     std::vector<uint32_t> mshift(std::max(nmask, 1), 0), mwbytes(std::max(nmask, 1), 4);
-    std::vector<uint32_t> mlds(std::max(nmask, 1), 0);
+    std::vector<uint32_t> mlds(std::max(nmask, 1), 0), mdict(std::max(nmask, 1), 0);
+    std::vector<const uint8_t *> mdg(std::max(nmask, 1), nullptr);
     uint32_t max_len = 0;
     {
         size_t at = 16, ei = 0;
@@ -3162,6 +3230,23 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
             mlds[mi] = m.lds;
             at += m.bytes();
         }
+        // the rank dictionaries' global buffers, after the tables
+        int di = 0;
+        for (size_t ci = 0; ci < b.masks.size(); ci++) {
+            const MaskPlan &m = b.masks[ci];
+            if (!m.dict) continue;
+            const int mi = (int)(std::find(b.mask_slots.begin(), b.mask_slots.end(), m.slot) - b.mask_slots.begin());
+            FMaskDict d{};
+            d.tab = mtab[mi];
+            d.dict = tabs + at;
+            d.len = (uint32_t)(m.hi - m.lo + 1);
+            d.wbytes = m.wbytes;
+            std::memcpy(host + off_dicts + di * sizeof(FMaskDict), &d, sizeof(FMaskDict));
+            di++;
+            mdict[mi] = m.dict;
+            mdg[mi] = d.dict;
+            at += 16 + dict_lds_bytes(d.len);
+        }
         const uint32_t ones[4] = {~0u, ~0u, ~0u, ~0u};
         std::memcpy(host + off_tabs, ones, 16);
     }
@@ -3188,12 +3273,14 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
         }
         for (const int32_t *c : b.pf) put(&c, 8, 8);
         for (const uint8_t *t : mtab) put(&t, 8, 8);
+        for (const uint8_t *t : mdg) put(&t, 8, 8);
         for (int32_t v : mlo) put(&v, 4, 4);
         for (uint32_t v : mlen) put(&v, 4, 4);
         for (uint32_t v : mdflt) put(&v, 4, 4);
         for (uint32_t v : mshift) put(&v, 4, 4);
         for (uint32_t v : mwbytes) put(&v, 4, 4);
         for (uint32_t v : mlds) put(&v, 4, 4);
+        for (uint32_t v : mdict) put(&v, 4, 4);
         for (int i = 0; i < std::max(nmask, 1); i++) {
             const int32_t m = i < (int)b.morder.size() ? b.morder[i] : -1;
             put(&m, 4, 4);
@@ -3232,6 +3319,7 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     auto enqueue = [&]() {
         SDB_HIP(hipMemcpyAsync(dblock, host, copy_bytes, hipMemcpyHostToDevice, s));
         if (nmask > 0 && !b.masks.empty()) launch_batch_masks(d_cols, (int)b.masks.size(), max_len, d_ents, s);
+        launch_batch_dict(d_dicts, ndicts, s);
         SDB_HIP(hipModuleLaunchKernel(k.fn, grid, 1, 1, shape.block, 1, 1, b.lds, s, nullptr, config));
     };
     static const bool debug = [] {
@@ -3246,7 +3334,7 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
         fprintf(stderr, "; mask windows");
         for (const MaskPlan &m : b.masks)
             fprintf(stderr, " [slot %d: %lld keys x %u B at bit %u%s]", m.slot, (long long)(m.hi - m.lo + 1), m.wbytes,
-                    m.shift, m.lds ? ", in LDS" : "");
+                    m.shift, m.lds ? ", in LDS" : m.dict ? ", rank dictionary in LDS" : "");
         fprintf(stderr, "\n");
     }
     if (G) {
@@ -3308,6 +3396,7 @@ void launch_fused_batch(const std::vector<FProgram> &in, int64_t n, hipStream_t 
     // row and program, since batches of one library differ in size
     SDB_HIP(hipMemcpyAsync(dblock, host, copy_bytes, hipMemcpyHostToDevice, s));
     if (nmask > 0 && !b.masks.empty()) launch_batch_masks(d_cols, (int)b.masks.size(), max_len, d_ents, s);
+    launch_batch_dict(d_dicts, ndicts, s);
     hipEvent_t ea, eb;
     SDB_HIP(hipEventCreate(&ea));
     SDB_HIP(hipEventCreate(&eb));
@@ -3362,13 +3451,15 @@ std::string jit_tuning_report()
             o << "block " << BATCH_CANDIDATES[t.chosen].block
               << (BATCH_CANDIDATES[t.chosen].double_buffer ? ", double-buffered" : ", one tile in registers")
               << (BATCH_CANDIDATES[t.chosen].wide ? ", wide mask words" : ", narrow mask windows")
-              << (BATCH_CANDIDATES[t.chosen].stage ? ", small mask tables in LDS" : "");
+              << (BATCH_CANDIDATES[t.chosen].stage ? ", small mask tables in LDS" : "")
+              << (BATCH_CANDIDATES[t.chosen].dict ? ", sparse mask tables as rank dictionaries in LDS" : "");
         else
             o << "tuning (" << t.trials << " trials)";
         o << "; ns per row and program";
         for (int c = 0; c < N_BATCH_CAND; c++) {
             o << " [b" << BATCH_CANDIDATES[c].block << (BATCH_CANDIDATES[c].double_buffer ? "d" : "")
-              << (BATCH_CANDIDATES[c].wide ? "w" : "") << (BATCH_CANDIDATES[c].stage ? "s" : "") << "] ";
+              << (BATCH_CANDIDATES[c].wide ? "w" : "") << (BATCH_CANDIDATES[c].stage ? "s" : "")
+              << (BATCH_CANDIDATES[c].dict ? "r" : "") << "] ";
             if (t.best[c] < 1e299) o << t.best[c];
             else o << "-";
         }

@@ -924,6 +924,77 @@ __global__ void __launch_bounds__(BLOCK) k_batch_masks(const FMaskCol *cols, con
     }
 }
 
+// Rank dictionaries of batched mask tables (FMaskDict), one block per table:
+// the presence bitmap of the keys with a nonzero window (a ballot per 64
+// keys), the exclusive counts before each bitmap word (each thread a run of
+// words, scanned across the block), then every set key's window at its rank.
+constexpr int DICT_BLOCK = 256;
+constexpr uint32_t DICT_MAX_WORDS = F_DICT_MAX_LEN / 32;
+
+__global__ void __launch_bounds__(DICT_BLOCK) k_batch_dict(const FMaskDict *dicts)
+{
+    __shared__ uint32_t sbits[DICT_MAX_WORDS];
+    __shared__ uint32_t spre[DICT_MAX_WORDS];
+    __shared__ uint32_t wave_total[DICT_BLOCK / 64];
+    const FMaskDict d = dicts[blockIdx.x];
+    const uint32_t words = dict_words(d.len);
+    auto window = [&](uint32_t i) {
+        const uint32_t byte = i * d.wbytes;
+        const uint32_t raw = *reinterpret_cast<const uint32_t *>(d.tab + (byte & ~3u)) >> ((byte & 3u) * 8u);
+        return d.wbytes >= 4u ? raw : raw & ((1u << (8u * d.wbytes)) - 1u);
+    };
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t w0 = 2u * wave; w0 < words; w0 += 2u * (DICT_BLOCK / 64)) {
+        const uint32_t i = w0 * 32u + lane;
+        const uint64_t m = __ballot(i < d.len && window(i) != 0u);
+        if (lane == 0) {
+            sbits[w0] = (uint32_t)m;
+            if (w0 + 1 < words) sbits[w0 + 1] = (uint32_t)(m >> 32);
+        }
+    }
+    __syncthreads();
+    const uint32_t per = (words + DICT_BLOCK - 1) / DICT_BLOCK;
+    const uint32_t a = min(words, threadIdx.x * per), b = min(words, a + per);
+    uint32_t local = 0;
+    for (uint32_t w = a; w < b; w++) local += __popc(sbits[w]);
+    uint32_t incl = local;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) wave_total[wave] = incl;
+    __syncthreads();
+    uint32_t base = incl - local, total = 0;
+    for (int v = 0; v < DICT_BLOCK / 64; v++) {
+        if (v < wave) base += wave_total[v];
+        total += wave_total[v];
+    }
+    for (uint32_t w = a; w < b; w++) {
+        spre[w] = base;
+        base += __popc(sbits[w]);
+    }
+    __syncthreads();
+    uint32_t *hdr = reinterpret_cast<uint32_t *>(d.dict);
+    uint32_t *obits = reinterpret_cast<uint32_t *>(d.dict + 16);
+    uint16_t *opre = reinterpret_cast<uint16_t *>(d.dict + 16 + dict_prefix_off(d.len));
+    uint32_t *ovals = reinterpret_cast<uint32_t *>(d.dict + 16 + dict_vals_off(d.len));
+    for (uint32_t w = threadIdx.x; w < words; w += DICT_BLOCK) {
+        obits[w] = sbits[w];
+        opre[w] = (uint16_t)spre[w];
+    }
+    const bool ok = total <= F_DICT_CAP;
+    if (ok)
+        for (uint32_t i = threadIdx.x; i < d.len; i += DICT_BLOCK) {
+            const uint32_t w = i >> 5, bit = i & 31u;
+            if ((sbits[w] >> bit) & 1u) ovals[spre[w] + __popc(sbits[w] & ((1u << bit) - 1u))] = window(i);
+        }
+    if (threadIdx.x == 0) {
+        hdr[0] = ok ? 1u : 0u;
+        hdr[1] = total;
+    }
+}
+
 // ------------------------------------------------------------ compaction
 // Three passes over tiles of COMPACT_TILE rows: count per tile, exclusive
 // scan of the counts (one block), then write ids in row order.
@@ -1390,6 +1461,13 @@ void launch_batch_masks(const FMaskCol *cols, int ncols, uint32_t max_len, const
     const unsigned gx = (unsigned)std::min<uint64_t>(((uint64_t)max_len + BLOCK - 1) / BLOCK, 4096);
     hipLaunchKernelGGL(k_batch_masks, dim3(gx, (unsigned)ncols), dim3(BLOCK), 0, s, cols, ents);
     check_launch("batch_masks");
+}
+
+void launch_batch_dict(const FMaskDict *dicts, int ndicts, hipStream_t s)
+{
+    if (ndicts <= 0) return;
+    hipLaunchKernelGGL(k_batch_dict, dim3((unsigned)ndicts), dim3(DICT_BLOCK), 0, s, dicts);
+    check_launch("batch_dict");
 }
 
 int64_t compact_tiles(int64_t n) { return n > 0 ? (n + COMPACT_TILE - 1) / COMPACT_TILE : 0; }

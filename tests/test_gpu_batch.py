@@ -107,13 +107,23 @@ def test_batch_without_shared_masks(pipelines, expected, monkeypatch):
         eng.close()
 
 
-@pytest.mark.parametrize("lazy", ["", "1", "0"])
-def test_batch_mask_edge_cases(cpu_tables, monkeypatch, lazy):
+MASK_READS = [{}, {"SDB_BATCH_LAZY_MASKS": "1"}, {"SDB_BATCH_LAZY_MASKS": "0"},
+              {"SDB_BATCH_DICT_MASKS": "1"}, {"SDB_BATCH_DICT_MASKS": "1", "SDB_BATCH_LAZY_MASKS": "1"},
+              {"SDB_BATCH_DICT_MASKS": "1", "SDB_BATCH_DICT_MIN_LEN": "1024"}]
+
+
+@pytest.mark.parametrize("env", MASK_READS)
+def test_batch_mask_edge_cases(cpu_tables, monkeypatch, env):
     """Shared probe masks at the extremes: a pipeline no row passes (a year
     outside the date table), near-empty ones, identical pipelines side by
     side, and every template's widest predicates; with the mask tables read
-    as the estimate decides, always lazily, and always eagerly."""
-    monkeypatch.setenv("SDB_BATCH_LAZY_MASKS", lazy)
+    as the estimate decides, always lazily, always eagerly, and through rank
+    dictionaries (the date table's; with a lower length bound also the
+    supplier's, and the customer's, whose set keys overflow the dictionary so
+    that the kernel reads the table instead)."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    lazy = env
     pipes = [
         ("t1", {"date": ("year", 2050), "disc": (1, 3), "qty": ("lt", 25)}),
         ("t2", {"part": ("brand", 999), "s_region": 4}),
@@ -130,7 +140,7 @@ def test_batch_mask_edge_cases(cpu_tables, monkeypatch, lazy):
     eng = _engine()
     try:
         plans = [PL.build(t, **p) for t, p in pipes]
-        print("mask edge batch, lazy=%r" % lazy, flush=True)
+        print("mask edge batch", lazy, flush=True)
         results, batched = eng.execute_batch(plans)
         assert batched == len(plans)
         for i, r in enumerate(results):
@@ -139,13 +149,17 @@ def test_batch_mask_edge_cases(cpu_tables, monkeypatch, lazy):
         eng.close()
 
 
-@pytest.mark.parametrize("lazy", ["", "1"])
-def test_batch_irregular_date_keys(cpu_tables, pipelines, monkeypatch, lazy):
+@pytest.mark.parametrize("env", [{}, {"SDB_BATCH_LAZY_MASKS": "1"}, {"SDB_BATCH_DICT_MASKS": "1"},
+                                 {"SDB_BATCH_DICT_MASKS": "1", "SDB_BATCH_LAZY_MASKS": "1"}])
+def test_batch_irregular_date_keys(cpu_tables, pipelines, monkeypatch, env):
     """Fact date keys that are no dates (day 32, month 13, zero, negative,
     far years), inside and outside the probed key range, and a date table
     holding such a key: the shared masks' range tests and default words must
-    still match the oracle (mask tables also read lazily)."""
-    monkeypatch.setenv("SDB_BATCH_LAZY_MASKS", lazy)
+    still match the oracle (mask tables also read lazily, and the date table
+    as a rank dictionary)."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    lazy = env
     import numpy as np
     from sycldb_amd.engine import Engine
     tables = {t: {c: np.array(a, dtype=np.int32, copy=True) for c, a in cols.items()} for t, cols in cpu_tables.items()}
@@ -163,7 +177,7 @@ def test_batch_irregular_date_keys(cpu_tables, pipelines, monkeypatch, lazy):
     try:
         eng.load_tables(tables)
         plans = [PL.build(t, **p) for t, p in pipes]
-        print("calendar batch, lazy=%r" % lazy, flush=True)
+        print("calendar batch", lazy, flush=True)
         results, batched = eng.execute_batch(plans)
         assert batched == len(plans)
         for i, r in enumerate(results):
@@ -235,7 +249,10 @@ def test_batch_graph_replays_match_oracle(pipelines, expected, cpu_tables, kw):
                                  {"SDB_BATCH_STAGE_MASKS": "1", "SDB_BATCH_WIDE_MASKS": "1"},
                                  {"SDB_BATCH_LAZY_MASKS": "1"},
                                  {"SDB_BATCH_LAZY_MASKS": "1", "SDB_BATCH_STAGE_MASKS": "1", "SDB_BATCH_BLOCK": "1024"},
-                                 {"SDB_BATCH_LAZY_MASKS": "1", "SDB_BATCH_WIDE_MASKS": "1"}])
+                                 {"SDB_BATCH_LAZY_MASKS": "1", "SDB_BATCH_WIDE_MASKS": "1"},
+                                 {"SDB_BATCH_DICT_MASKS": "1"}, {"SDB_BATCH_DICT_MASKS": "1", "SDB_BATCH_BLOCK": "1024"},
+                                 {"SDB_BATCH_DICT_MASKS": "1", "SDB_BATCH_WIDE_MASKS": "1"},
+                                 {"SDB_BATCH_DICT_MASKS": "1", "SDB_BATCH_DICT_MIN_LEN": "1024"}])
 def test_batch_layouts_match_oracle(pipelines, expected, monkeypatch, env):
     """The autotuner's other candidates, forced: whole 32-bit mask words
     instead of narrow windows, mask tables staged in LDS, other launch
@@ -254,13 +271,15 @@ def test_batch_layouts_match_oracle(pipelines, expected, monkeypatch, env):
         eng.close()
 
 
-@pytest.mark.parametrize("lazy", ["", "1", "0"])
-def test_batch_per_template_matches_oracle(pipelines, expected, monkeypatch, lazy):
+@pytest.mark.parametrize("env", MASK_READS)
+def test_batch_per_template_matches_oracle(pipelines, expected, monkeypatch, env):
     """Batches of one template each (the lazy form's case: every program
     probes every masked column), in one engine whose shape library already
     holds all four templates, so masked slots the batch does not use are
     skipped."""
-    monkeypatch.setenv("SDB_BATCH_LAZY_MASKS", lazy)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    lazy = env
     eng = _engine()
     try:
         plans = [PL.build(t, **p) for t, p in pipelines]
@@ -270,7 +289,7 @@ def test_batch_per_template_matches_oracle(pipelines, expected, monkeypatch, laz
             idx = [i for i, (t, _) in enumerate(pipelines) if t == tmpl]
             if len(idx) < 2:
                 continue
-            print("template batch", tmpl, "lazy=%r" % lazy, flush=True)
+            print("template batch", tmpl, lazy, flush=True)
             results, batched = eng.execute_batch([plans[i] for i in idx])
             assert batched == len(idx)
             for j, r in zip(idx, results):
