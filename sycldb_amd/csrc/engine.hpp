@@ -53,9 +53,9 @@ uint64_t groupby_lds_max_slots();
 // uint16 value offsets from a key-value pair table; counts set bits into
 // *count and sets *bad when a value does not fit 16 bits.
 void launch_bits_from_keys(const uint8_t *ht, int32_t len, uint32_t *bits, uint32_t *count, hipStream_t s);
-// Rank dictionary (FV_RANK16 layout) of `words` presence words and their
-// per-slot 16-bit values.
-void launch_rank_dict(const uint32_t *bits, int32_t words, const uint16_t *vals, uint8_t *dict, uint32_t dict_cap,
+// Rank dictionary (FV_RANK16 layout) of the presence bitmap of `len` slots
+// and their per-slot 16-bit values.
+void launch_rank_dict(const uint32_t *bits, int32_t len, const uint16_t *vals, uint8_t *dict, uint32_t dict_cap,
                       hipStream_t s);
 void launch_bits_from_pairs(const int32_t *ht, int32_t len, uint32_t *bits, uint16_t *vals, int32_t vmin,
                             uint32_t *count, uint32_t *bad, hipStream_t s);

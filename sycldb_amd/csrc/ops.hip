@@ -826,32 +826,81 @@ __global__ void __launch_bounds__(BLOCK) k_bits_from_pairs(const int2 *ht, int32
     if (!ok) atomicOr(bad, 1u);
 }
 
+// The windows of 32 consecutive keys (one bitmap word) of a table of WB-byte
+// windows (1, 2 or 4 bytes), 16-byte aligned and padded to 16 bytes, in
+// 2 * WB vector loads issued together.
+template <uint32_t WB> struct DictWord {
+    static constexpr uint32_t NQ = 2 * WB; // 16-byte chunks per 32 windows
+    static constexpr uint32_t FIELD = WB >= 4 ? ~0u : (1u << (8u * (WB % 4u))) - 1u;
+    uint4 q[NQ];
+    // the windows of keys [32 w, 32 w + 32) that are below len; a chunk that
+    // starts past the table's last window is not read
+    __device__ __forceinline__ void load(const uint8_t *tab, uint32_t len, uint32_t w)
+    {
+        const uint4 *p = reinterpret_cast<const uint4 *>(tab) + (size_t)w * NQ;
+        const uint32_t nbytes = min(len - w * 32u, 32u) * WB;
+#pragma unroll
+        for (uint32_t k = 0; k < NQ; k++) q[k] = 16u * k < nbytes ? p[k] : make_uint4(0u, 0u, 0u, 0u);
+    }
+    __device__ __forceinline__ uint32_t window(uint32_t e) const // e: compile-time after unrolling
+    {
+        const uint32_t byte = e * WB, k = byte >> 4, j = (byte >> 2) & 3u;
+        const uint32_t c = j == 0 ? q[k].x : j == 1 ? q[k].y : j == 2 ? q[k].z : q[k].w;
+        return (c >> (8u * (byte & 3u))) & FIELD;
+    }
+    __device__ __forceinline__ uint32_t bits(uint32_t len, uint32_t w) const
+    {
+        uint32_t m = 0;
+#pragma unroll
+        for (uint32_t e = 0; e < 32u; e++) m |= (uint32_t)(window(e) != 0u) << e;
+        const uint32_t n = min(len - w * 32u, 32u);
+        return n == 32u ? m : m & ((1u << n) - 1u);
+    }
+    // the windows of the keys set in m, at ranks rank, rank + 1, ... (those
+    // at or past cap are dropped)
+    template <class T> __device__ __forceinline__ void write(uint32_t m, uint32_t rank, T *vals, uint32_t cap) const
+    {
+#pragma unroll
+        for (uint32_t e = 0; e < 32u; e++)
+            if ((m >> e) & 1u) {
+                if (rank < cap) vals[rank] = (T)window(e);
+                rank++;
+            }
+    }
+};
+
 // Rank dictionary of a presence bitmap with per-slot 16-bit values (the
 // layout FV_RANK16 describes in fused_types.hpp), in one block of
-// RANK_BLOCK threads.  Each thread owns a run of consecutive words; the block
-// scans the run totals (within each wave by shuffles, then across the
-// waves) and every thread writes its words and their prefix counts.  Then,
-// for a bitmap of at most RANK_LDS_WORDS words (kept in LDS with its
-// prefixes), one thread per slot writes the values of the set slots at their
-// ranks: coalesced reads of the value table, every load independent.  A
-// larger bitmap writes them per word, in its owner thread.
+// RANK_BLOCK threads.  A bitmap of at most RANK_LDS_WORDS words comes into
+// LDS with coalesced loads; each thread then counts a run of consecutive
+// words, the block scans the run totals (within each wave by shuffles, then
+// across the waves), and the words and their prefix counts go out coalesced.
+// The values: each thread takes whole nonempty words, loads the word's 32
+// values in four vector loads issued together and writes the set slots' at
+// their ranks.  A larger bitmap is read and written per word by the thread
+// that owns the word's run.
 // (one 256-thread block: it fits beside a long batched scan of another
 // engine, where a 1024-thread block would wait for that scan to end)
 constexpr int RANK_BLOCK = 256;
 constexpr int RANK_LDS_WORDS = 4096;
 
 __global__ void __launch_bounds__(RANK_BLOCK) k_rank_dict(const uint32_t *__restrict__ bits, int32_t words,
-                                                          const uint16_t *__restrict__ vals, uint8_t *__restrict__ dict,
-                                                          uint32_t cap_vals)
+                                                          const uint16_t *__restrict__ vals, uint32_t len,
+                                                          uint8_t *__restrict__ dict, uint32_t cap_vals)
 {
     __shared__ uint32_t wave_total[RANK_BLOCK / 64];
     __shared__ uint32_t s_bits[RANK_LDS_WORDS];
     __shared__ uint32_t s_prefix[RANK_LDS_WORDS];
     const bool in_lds = words <= RANK_LDS_WORDS;
+    if (in_lds) {
+#pragma unroll 4
+        for (int32_t w = threadIdx.x; w < words; w += RANK_BLOCK) s_bits[w] = bits[w];
+        __syncthreads();
+    }
     const int32_t per = (words + RANK_BLOCK - 1) / RANK_BLOCK;
     const int32_t w0 = min(words, (int32_t)threadIdx.x * per), w1 = min(words, w0 + per);
     uint32_t local = 0;
-    for (int32_t w = w0; w < w1; w++) local += __popc(bits[w]);
+    for (int32_t w = w0; w < w1; w++) local += __popc(in_lds ? s_bits[w] : bits[w]);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t incl = local;
 #pragma unroll
@@ -866,31 +915,33 @@ __global__ void __launch_bounds__(RANK_BLOCK) k_rank_dict(const uint32_t *__rest
     uint32_t *out_bits = reinterpret_cast<uint32_t *>(dict);
     uint16_t *out_prefix = reinterpret_cast<uint16_t *>(dict + rank_prefix_off((uint32_t)words));
     uint16_t *out_vals = reinterpret_cast<uint16_t *>(dict + rank_values_off((uint32_t)words));
-    for (int32_t w = w0; w < w1; w++) {
-        const uint32_t m = bits[w];
-        out_bits[w] = m;
-        out_prefix[w] = (uint16_t)base;
-        if (in_lds) {
-            s_bits[w] = m;
-            s_prefix[w] = base;
-            base += __popc(m);
-        } else {
+    if (!in_lds) {
+        for (int32_t w = w0; w < w1; w++) {
+            const uint32_t m = bits[w];
+            out_bits[w] = m;
+            out_prefix[w] = (uint16_t)base;
             // values past the buffer's capacity are dropped (the dictionary
             // is then not used)
             for (uint32_t r = m; r; r &= r - 1, base++)
                 if (base < cap_vals) out_vals[base] = vals[(int64_t)w * 32 + __builtin_ctz(r)];
         }
+        return;
     }
-    if (!in_lds) return;
+    for (int32_t w = w0; w < w1; w++) {
+        s_prefix[w] = base;
+        base += __popc(s_bits[w]);
+    }
     __syncthreads();
-    const int64_t slots = (int64_t)words * 32;
-    for (int64_t sl = threadIdx.x; sl < slots; sl += RANK_BLOCK) {
-        const uint32_t w = (uint32_t)(sl >> 5), b = (uint32_t)(sl & 31);
+    for (int32_t w = threadIdx.x; w < words; w += RANK_BLOCK) {
+        out_bits[w] = s_bits[w];
+        out_prefix[w] = (uint16_t)s_prefix[w];
+    }
+    for (int32_t w = threadIdx.x; w < words; w += RANK_BLOCK) {
         const uint32_t m = s_bits[w];
-        if ((m >> b) & 1u) {
-            const uint32_t r = s_prefix[w] + __popc(m & ((1u << b) - 1u));
-            if (r < cap_vals) out_vals[r] = vals[sl];
-        }
+        if (!m) continue;
+        DictWord<2> x;
+        x.load(reinterpret_cast<const uint8_t *>(vals), len, (uint32_t)w);
+        x.write(m, s_prefix[w], out_vals, cap_vals);
     }
 }
 
@@ -924,12 +975,37 @@ __global__ void __launch_bounds__(BLOCK) k_batch_masks(const FMaskCol *cols, con
     }
 }
 
-// Rank dictionaries of batched mask tables (FMaskDict), one block per table:
-// the presence bitmap of the keys with a nonzero window (a ballot per 64
-// keys), the exclusive counts before each bitmap word (each thread a run of
-// words, scanned across the block), then every set key's window at its rank.
+// Rank dictionaries of batched mask tables (FMaskDict), one block per table.
+// Each thread takes whole bitmap words: the word's 32 windows (32 x 1, 2 or 4
+// bytes, 16-byte aligned -- every table starts on 16 bytes and is padded to
+// 16) come in with 2, 4 or 8 vector loads issued together, and the word's
+// bit per key is whether the key's window is nonzero.  The exclusive counts
+// before each word are scanned across the block; then each thread reloads its
+// words' windows and writes every set key's window at its rank.
 constexpr int DICT_BLOCK = 256;
 constexpr uint32_t DICT_MAX_WORDS = F_DICT_MAX_LEN / 32;
+
+template <uint32_t WB> __device__ void dict_bits(const FMaskDict &d, uint32_t words, uint32_t *sbits)
+{
+    for (uint32_t w = threadIdx.x; w < words; w += DICT_BLOCK) {
+        DictWord<WB> x;
+        x.load(d.tab, d.len, w);
+        sbits[w] = x.bits(d.len, w);
+    }
+}
+
+template <uint32_t WB>
+__device__ void dict_values(const FMaskDict &d, uint32_t words, const uint32_t *sbits, const uint32_t *spre,
+                            uint32_t *vals)
+{
+    for (uint32_t w = threadIdx.x; w < words; w += DICT_BLOCK) {
+        const uint32_t m = sbits[w];
+        if (!m) continue;
+        DictWord<WB> x;
+        x.load(d.tab, d.len, w);
+        x.write(m, spre[w], vals, ~0u);
+    }
+}
 
 __global__ void __launch_bounds__(DICT_BLOCK) k_batch_dict(const FMaskDict *dicts)
 {
@@ -938,21 +1014,11 @@ __global__ void __launch_bounds__(DICT_BLOCK) k_batch_dict(const FMaskDict *dict
     __shared__ uint32_t wave_total[DICT_BLOCK / 64];
     const FMaskDict d = dicts[blockIdx.x];
     const uint32_t words = dict_words(d.len);
-    auto window = [&](uint32_t i) {
-        const uint32_t byte = i * d.wbytes;
-        const uint32_t raw = *reinterpret_cast<const uint32_t *>(d.tab + (byte & ~3u)) >> ((byte & 3u) * 8u);
-        return d.wbytes >= 4u ? raw : raw & ((1u << (8u * d.wbytes)) - 1u);
-    };
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t w0 = 2u * wave; w0 < words; w0 += 2u * (DICT_BLOCK / 64)) {
-        const uint32_t i = w0 * 32u + lane;
-        const uint64_t m = __ballot(i < d.len && window(i) != 0u);
-        if (lane == 0) {
-            sbits[w0] = (uint32_t)m;
-            if (w0 + 1 < words) sbits[w0 + 1] = (uint32_t)(m >> 32);
-        }
-    }
+    if (d.wbytes == 1) dict_bits<1>(d, words, sbits);
+    else if (d.wbytes == 2) dict_bits<2>(d, words, sbits);
+    else dict_bits<4>(d, words, sbits);
     __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t per = (words + DICT_BLOCK - 1) / DICT_BLOCK;
     const uint32_t a = min(words, threadIdx.x * per), b = min(words, a + per);
     uint32_t local = 0;
@@ -984,11 +1050,11 @@ __global__ void __launch_bounds__(DICT_BLOCK) k_batch_dict(const FMaskDict *dict
         opre[w] = (uint16_t)spre[w];
     }
     const bool ok = total <= F_DICT_CAP;
-    if (ok)
-        for (uint32_t i = threadIdx.x; i < d.len; i += DICT_BLOCK) {
-            const uint32_t w = i >> 5, bit = i & 31u;
-            if ((sbits[w] >> bit) & 1u) ovals[spre[w] + __popc(sbits[w] & ((1u << bit) - 1u))] = window(i);
-        }
+    if (ok) {
+        if (d.wbytes == 1) dict_values<1>(d, words, sbits, spre, ovals);
+        else if (d.wbytes == 2) dict_values<2>(d, words, sbits, spre, ovals);
+        else dict_values<4>(d, words, sbits, spre, ovals);
+    }
     if (threadIdx.x == 0) {
         hdr[0] = ok ? 1u : 0u;
         hdr[1] = total;
@@ -1445,13 +1511,15 @@ void launch_bits_from_pairs(const int32_t *ht, int32_t len, uint32_t *bits, uint
     check_launch("bits_from_pairs");
 }
 
-void launch_rank_dict(const uint32_t *bits, int32_t words, const uint16_t *vals, uint8_t *dict, uint32_t dict_cap,
+void launch_rank_dict(const uint32_t *bits, int32_t len, const uint16_t *vals, uint8_t *dict, uint32_t dict_cap,
                       hipStream_t s)
 {
-    if (words <= 0) return;
+    if (len <= 0) return;
+    const int32_t words = (int32_t)(((int64_t)len + 31) / 32);
     if (dict_cap < rank_values_off((uint32_t)words)) fail(SDB_ERR_RUNTIME, "rank dictionary buffer too small");
     const uint32_t cap_vals = (dict_cap - rank_values_off((uint32_t)words)) / 2u;
-    hipLaunchKernelGGL(k_rank_dict, dim3(1), dim3(RANK_BLOCK), 0, s, bits, words, vals, dict, cap_vals);
+    hipLaunchKernelGGL(k_rank_dict, dim3(1), dim3(RANK_BLOCK), 0, s, bits, words, vals, (uint32_t)len, dict,
+                       cap_vals);
     check_launch("rank_dict");
 }
 

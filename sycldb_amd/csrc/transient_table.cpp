@@ -564,7 +564,7 @@ static std::shared_ptr<CompactState> build_compact(Engine &eng, const void *ht, 
     if (narrow && words <= 0xFFFFu && rank_values_off((uint32_t)words) < F_RANK_DICT_MAX) {
         c->dict = eng.mm->alloc<uint8_t>(F_RANK_DICT_MAX);
         c->dict_cap = F_RANK_DICT_MAX;
-        launch_rank_dict(bits, (int32_t)words, vals, c->dict, c->dict_cap, eng.stream);
+        launch_rank_dict(bits, (int32_t)len, vals, c->dict, c->dict_cap, eng.stream);
         eng.launches++;
     }
     eng.pending_compact.push_back(c);
