@@ -1,77 +1,100 @@
 // Minimal JSON reader for plan documents (objects, arrays, strings, integers,
 // booleans, null).  Numbers keep their integer value (plans carry int64
 // literals); fractional numbers are rejected.
+//
+// A Document parses the whole text into one contiguous array of nodes: the
+// members of an object (or the elements of an array) sit next to each other,
+// and strings are views into the text, so parsing allocates a few vectors
+// rather than a string and two vectors per value.  A string with escapes is
+// unescaped into storage the Document owns.  Nodes and views stay valid while
+// the Document and the text live.
 #pragma once
 
 #include <cstdint>
-#include <memory>
+#include <deque>
 #include <stdexcept>
 #include <string>
-#include <utility>
+#include <string_view>
 #include <vector>
 
 namespace sdb {
 namespace json {
 
 struct Value {
-    enum Kind { Null, Bool, Int, String, Array, Object } kind = Null;
+    enum Kind : uint8_t { Null, Bool, Int, String, Array, Object } kind = Null;
     bool b = false;
     int64_t i = 0;
-    std::string s;
-    std::vector<Value> a;
-    // object members in document order (plan objects have a handful of
-    // keys: a linear search beats a tree); a repeated key keeps its last value
-    std::vector<std::pair<std::string, Value>> o;
+    std::string_view s;   // String: the unescaped text
+    std::string_view key; // the member's key when the value sits in an object
+    const Value *kids = nullptr; // Array elements / Object members, contiguous
+    uint32_t first = 0, n = 0;   // (first: the kids' index while parsing)
 
-    const Value *find(const char *k) const
+    struct Range {
+        const Value *b, *e;
+        const Value *begin() const { return b; }
+        const Value *end() const { return e; }
+        size_t size() const { return (size_t)(e - b); }
+    };
+
+    // object members in document order (plan objects have a handful of
+    // keys: a linear search); a repeated key keeps its last value
+    const Value *find(std::string_view k) const
     {
         if (kind != Object) return nullptr;
-        for (auto it = o.rbegin(); it != o.rend(); ++it)
-            if (it->first == k) return &it->second;
+        for (uint32_t j = n; j-- > 0;)
+            if (kids[j].key == k) return &kids[j];
         return nullptr;
     }
-    bool has(const char *k) const { return find(k) != nullptr; }
-    bool has(const std::string &k) const { return find(k.c_str()) != nullptr; }
-    const Value &at(const char *k) const
+    bool has(std::string_view k) const { return find(k) != nullptr; }
+    const Value &at(std::string_view k) const
     {
         const Value *v = find(k);
-        if (!v) throw std::invalid_argument(std::string("plan: missing field '") + k + "'");
+        if (!v) throw std::invalid_argument("plan: missing field '" + std::string(k) + "'");
         return *v;
     }
-    const Value &at(const std::string &k) const { return at(k.c_str()); }
     int64_t as_int() const
     {
         if (kind != Int) throw std::invalid_argument("plan: expected an integer");
         return i;
     }
-    const std::string &as_str() const
+    std::string_view as_str() const
     {
         if (kind != String) throw std::invalid_argument("plan: expected a string");
         return s;
     }
-    const std::vector<Value> &as_arr() const
+    Range as_arr() const
     {
         if (kind != Array) throw std::invalid_argument("plan: expected an array");
-        return a;
+        return {kids, kids + n};
     }
 };
 
-class Parser {
+class Document {
   public:
-    explicit Parser(const std::string &t) : t_(t) {}
-    Value parse()
+    explicit Document(std::string_view text) : t_(text)
     {
-        Value v = value();
+        nodes_.reserve(t_.size() / 8 + 4);
+        pending_.reserve(64);
+        value(0);
         ws();
         if (p_ != t_.size()) err("trailing characters");
-        return v;
+        nodes_.push_back(pending_.back());
+        for (Value &v : nodes_)
+            if (v.kind == Value::Array || v.kind == Value::Object) v.kids = nodes_.data() + v.first;
     }
+    Document(const Document &) = delete;
+    Document &operator=(const Document &) = delete;
+    const Value &root() const { return nodes_.back(); }
 
   private:
-    const std::string &t_;
+    static constexpr int MAX_DEPTH = 256;
+    std::string_view t_;
     size_t p_ = 0;
+    std::vector<Value> nodes_;   // finished containers' children, then the root
+    std::vector<Value> pending_; // values whose container is still open
+    std::deque<std::string> unescaped_;
 
-    [[noreturn]] void err(const char *m)
+    [[noreturn]] void err(const char *m) const
     {
         throw std::invalid_argument(std::string("plan JSON: ") + m + " at offset " + std::to_string(p_));
     }
@@ -79,88 +102,94 @@ class Parser {
     {
         while (p_ < t_.size() && (t_[p_] == ' ' || t_[p_] == '\n' || t_[p_] == '\t' || t_[p_] == '\r')) p_++;
     }
-    bool lit(const char *w)
+    bool lit(std::string_view w)
     {
-        size_t n = std::char_traits<char>::length(w);
-        if (t_.compare(p_, n, w) == 0) {
-            p_ += n;
+        if (t_.compare(p_, w.size(), w) == 0) {
+            p_ += w.size();
             return true;
         }
         return false;
     }
-    Value value()
+    // the children pending since `start` move to the node array, contiguous
+    void close(Value &v, size_t start)
+    {
+        v.first = (uint32_t)nodes_.size();
+        v.n = (uint32_t)(pending_.size() - start);
+        nodes_.insert(nodes_.end(), pending_.begin() + (std::ptrdiff_t)start, pending_.end());
+        pending_.resize(start);
+    }
+    // parses one value and leaves it on top of the pending list
+    void value(int depth)
     {
         ws();
         if (p_ >= t_.size()) err("unexpected end");
-        char c = t_[p_];
+        if (depth > MAX_DEPTH) err("nesting too deep");
+        const char c = t_[p_];
         Value v;
         if (c == '{') {
             v.kind = Value::Object;
             p_++;
             ws();
+            const size_t start = pending_.size();
             if (p_ < t_.size() && t_[p_] == '}') {
                 p_++;
-                return v;
-            }
-            for (;;) {
-                ws();
-                if (p_ >= t_.size() || t_[p_] != '"') err("expected key");
-                std::string k = str();
-                ws();
-                if (p_ >= t_.size() || t_[p_] != ':') err("expected ':'");
-                p_++;
-                v.o.emplace_back(std::move(k), value());
-                ws();
-                if (p_ < t_.size() && t_[p_] == ',') {
+            } else {
+                for (;;) {
+                    ws();
+                    if (p_ >= t_.size() || t_[p_] != '"') err("expected key");
+                    const std::string_view k = str();
+                    ws();
+                    if (p_ >= t_.size() || t_[p_] != ':') err("expected ':'");
                     p_++;
-                    continue;
+                    value(depth + 1);
+                    pending_.back().key = k;
+                    ws();
+                    if (p_ < t_.size() && t_[p_] == ',') {
+                        p_++;
+                        continue;
+                    }
+                    if (p_ < t_.size() && t_[p_] == '}') {
+                        p_++;
+                        break;
+                    }
+                    err("expected ',' or '}'");
                 }
-                if (p_ < t_.size() && t_[p_] == '}') {
-                    p_++;
-                    return v;
-                }
-                err("expected ',' or '}'");
             }
-        }
-        if (c == '[') {
+            close(v, start);
+        } else if (c == '[') {
             v.kind = Value::Array;
             p_++;
             ws();
+            const size_t start = pending_.size();
             if (p_ < t_.size() && t_[p_] == ']') {
                 p_++;
-                return v;
-            }
-            for (;;) {
-                v.a.push_back(value());
-                ws();
-                if (p_ < t_.size() && t_[p_] == ',') {
-                    p_++;
-                    continue;
+            } else {
+                for (;;) {
+                    value(depth + 1);
+                    ws();
+                    if (p_ < t_.size() && t_[p_] == ',') {
+                        p_++;
+                        continue;
+                    }
+                    if (p_ < t_.size() && t_[p_] == ']') {
+                        p_++;
+                        break;
+                    }
+                    err("expected ',' or ']'");
                 }
-                if (p_ < t_.size() && t_[p_] == ']') {
-                    p_++;
-                    return v;
-                }
-                err("expected ',' or ']'");
             }
-        }
-        if (c == '"') {
+            close(v, start);
+        } else if (c == '"') {
             v.kind = Value::String;
             v.s = str();
-            return v;
-        }
-        if (lit("true")) {
+        } else if (lit("true")) {
             v.kind = Value::Bool;
             v.b = true;
-            return v;
-        }
-        if (lit("false")) {
+        } else if (lit("false")) {
             v.kind = Value::Bool;
-            return v;
-        }
-        if (lit("null")) return v;
-        if (c == '-' || (c >= '0' && c <= '9')) {
-            size_t s = p_;
+        } else if (lit("null")) {
+        } else if (c == '-' || (c >= '0' && c <= '9')) {
+            const size_t s = p_;
             if (t_[p_] == '-') p_++;
             while (p_ < t_.size() && t_[p_] >= '0' && t_[p_] <= '9') p_++;
             if (p_ < t_.size() && (t_[p_] == '.' || t_[p_] == 'e' || t_[p_] == 'E')) err("non-integer number");
@@ -178,19 +207,20 @@ class Parser {
                 mag = mag * 10 + d;
             }
             v.i = neg ? (int64_t)(uint64_t(0) - mag) : (int64_t)mag;
-            return v;
+        } else {
+            err("unexpected character");
         }
-        err("unexpected character");
+        pending_.push_back(v);
     }
-    std::string str()
+    std::string_view str()
     {
         p_++; // opening quote
-        // fast path: no escape before the closing quote
+        // fast path: no escape before the closing quote -- a view of the text
         {
             size_t q = p_;
             while (q < t_.size() && t_[q] != '"' && t_[q] != '\\') q++;
             if (q < t_.size() && t_[q] == '"') {
-                std::string out(t_, p_, q - p_);
+                const std::string_view out = t_.substr(p_, q - p_);
                 p_ = q + 1;
                 return out;
             }
@@ -220,11 +250,10 @@ class Parser {
         }
         if (p_ >= t_.size()) err("unterminated string");
         p_++;
-        return out;
+        unescaped_.push_back(std::move(out));
+        return unescaped_.back();
     }
 };
-
-inline Value parse(const std::string &text) { return Parser(text).parse(); }
 
 } // namespace json
 } // namespace sdb

@@ -9,7 +9,7 @@ namespace {
 
 using json::Value;
 
-RelNodeType rel_type(const std::string &s)
+RelNodeType rel_type(std::string_view s)
 {
     if (s == "TABLE_SCAN") return RelNodeType::TABLE_SCAN;
     if (s == "FILTER") return RelNodeType::FILTER;
@@ -30,25 +30,28 @@ std::vector<int64_t> int_list(const Value &v)
 ExprType expr_from(const Value &v)
 {
     ExprType e;
-    const std::string &k = v.at("exprType").as_str();
+    const std::string_view k = v.at("exprType").as_str();
     if (k == "COLUMN") {
         e.exprType = ExprOption::COLUMN;
         e.input = v.at("input").as_int();
-        if (v.has("name") && v.at("name").kind == Value::String) e.name = v.at("name").s;
+        if (v.has("name") && v.at("name").kind == Value::String) e.name = std::string(v.at("name").s);
     } else if (k == "EXPR") {
         e.exprType = ExprOption::EXPR;
-        e.op = v.at("op").as_str();
-        if (v.has("operands"))
-            for (const Value &o : v.at("operands").as_arr()) e.operands.push_back(expr_from(o));
+        e.op = std::string(v.at("op").as_str());
+        if (v.has("operands")) {
+            const Value::Range ops = v.at("operands").as_arr();
+            e.operands.reserve(ops.size());
+            for (const Value &o : ops) e.operands.push_back(expr_from(o));
+        }
     } else if (k == "LITERAL") {
         e.exprType = ExprOption::LITERAL;
         const Value &l = v.at("literal");
-        const std::string &lo = l.has("literalOption") ? l.at("literalOption").as_str() : "LITERAL";
+        const std::string_view lo = l.has("literalOption") ? l.at("literalOption").as_str() : "LITERAL";
         if (lo == "RANGE") {
             e.literal.literalOption = LiteralOption::RANGE;
             for (const Value &r : l.at("rangeSet").as_arr()) {
                 std::vector<std::string> row;
-                for (const Value &x : r.as_arr()) row.push_back(x.kind == Value::Int ? std::to_string(x.i) : x.as_str());
+                for (const Value &x : r.as_arr()) row.push_back(x.kind == Value::Int ? std::to_string(x.i) : std::string(x.as_str()));
                 e.literal.rangeSet.push_back(row);
             }
         } else {
@@ -56,9 +59,9 @@ ExprType expr_from(const Value &v)
             e.literal.value = l.at("value").as_int();
         }
     } else {
-        throw std::invalid_argument("plan: unknown exprType " + k);
+        throw std::invalid_argument("plan: unknown exprType " + std::string(k));
     }
-    if (v.has("type") && v.at("type").kind == Value::String) e.type = v.at("type").s;
+    if (v.has("type") && v.at("type").kind == Value::String) e.type = std::string(v.at("type").s);
     return e;
 }
 
@@ -66,32 +69,38 @@ ExprType expr_from(const Value &v)
 
 PlanResult parse_plan(const std::string &text)
 {
-    const Value root = json::parse(text);
+    const json::Document doc(text);
+    const Value &root = doc.root();
     PlanResult plan;
-    for (const Value &r : root.at("rels").as_arr()) {
+    const Value::Range rels = root.at("rels").as_arr();
+    plan.rels.reserve(rels.size());
+    for (const Value &r : rels) {
         RelNode rel;
         rel.id = r.at("id").as_int();
         rel.relOp = rel_type(r.at("relOp").as_str());
         if (r.has("tables"))
-            for (const Value &t : r.at("tables").as_arr()) rel.tables.push_back(t.as_str());
+            for (const Value &t : r.at("tables").as_arr()) rel.tables.emplace_back(t.as_str());
         if (r.has("inputs")) rel.inputs = int_list(r.at("inputs"));
         if (r.has("condition") && r.at("condition").kind == Value::Object) rel.condition = expr_from(r.at("condition"));
-        if (r.has("joinType") && r.at("joinType").kind == Value::String) rel.joinType = r.at("joinType").s;
+        if (r.has("joinType") && r.at("joinType").kind == Value::String) rel.joinType = std::string(r.at("joinType").s);
         if (r.has("fields"))
-            for (const Value &f : r.at("fields").as_arr()) rel.fields.push_back(f.kind == Value::String ? f.s : "");
-        if (r.has("exprs"))
-            for (const Value &e : r.at("exprs").as_arr()) rel.exprs.push_back(expr_from(e));
+            for (const Value &f : r.at("fields").as_arr()) rel.fields.emplace_back(f.kind == Value::String ? f.s : std::string_view());
+        if (r.has("exprs")) {
+            const Value::Range es = r.at("exprs").as_arr();
+            rel.exprs.reserve(es.size());
+            for (const Value &e : es) rel.exprs.push_back(expr_from(e));
+        }
         if (r.has("group")) rel.group = int_list(r.at("group"));
         if (r.has("aggs")) {
             for (const Value &a : r.at("aggs").as_arr()) {
                 AggType agg;
                 if (a.has("agg")) {
                     const Value &n = a.at("agg");
-                    agg.agg = n.kind == Value::Object ? n.at("name").as_str() : n.as_str();
+                    agg.agg = std::string(n.kind == Value::Object ? n.at("name").as_str() : n.as_str());
                 }
                 if (a.has("operands")) agg.operands = int_list(a.at("operands"));
-                if (a.has("name") && a.at("name").kind == Value::String) agg.name = a.at("name").s;
-                if (a.has("type") && a.at("type").kind == Value::String) agg.type = a.at("type").s;
+                if (a.has("name") && a.at("name").kind == Value::String) agg.name = std::string(a.at("name").s);
+                if (a.has("type") && a.at("type").kind == Value::String) agg.type = std::string(a.at("type").s);
                 if (a.has("distinct") && a.at("distinct").kind == Value::Bool) agg.distinct = a.at("distinct").b;
                 rel.aggs.push_back(agg);
             }
@@ -100,8 +109,8 @@ PlanResult parse_plan(const std::string &text)
             for (const Value &c : r.at("collation").as_arr()) {
                 CollationType ct;
                 ct.field = c.at("field").as_int();
-                if (c.has("direction")) ct.direction = c.at("direction").as_str();
-                if (c.has("nulls")) ct.nulls = c.at("nulls").as_str();
+                if (c.has("direction")) ct.direction = std::string(c.at("direction").as_str());
+                if (c.has("nulls")) ct.nulls = std::string(c.at("nulls").as_str());
                 rel.collation.push_back(ct);
             }
         }

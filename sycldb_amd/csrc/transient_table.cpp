@@ -580,11 +580,11 @@ void Engine::resolve_compact()
     pending_compact.clear();
     if (todo.empty()) return;
     // through the pinned staging buffer: asynchronous copies, one sync
-    // (several tables' statistics packed on the device first: one transfer)
+    // (several tables' statistics packed on the device first, PACK_MAX_PARTS
+    // per launch: one transfer)
     uint32_t *staged = static_cast<uint32_t *>(pinned_buffer(8 * todo.size()));
-    if (todo.size() == 1 || todo.size() > (size_t)PACK_MAX_PARTS) {
-        for (size_t i = 0; i < todo.size(); i++)
-            SDB_HIP(hipMemcpyAsync(&staged[2 * i], todo[i]->stats, 8, hipMemcpyDeviceToHost, stream));
+    if (todo.size() == 1) {
+        SDB_HIP(hipMemcpyAsync(staged, todo[0]->stats, 8, hipMemcpyDeviceToHost, stream));
     } else {
         uint8_t *dev = mm->alloc<uint8_t>(8 * todo.size());
         PackParts pk{};
@@ -592,10 +592,12 @@ void Engine::resolve_compact()
             pk.src[pk.n] = reinterpret_cast<const uint8_t *>(todo[i]->stats);
             pk.off[pk.n] = 8 * i;
             pk.bytes[pk.n] = 8;
-            pk.n++;
+            if (++pk.n == PACK_MAX_PARTS || i + 1 == todo.size()) {
+                launch_pack_parts(pk, dev, stream);
+                launches++;
+                pk = PackParts{};
+            }
         }
-        launch_pack_parts(pk, dev, stream);
-        launches++;
         SDB_HIP(hipMemcpyAsync(staged, dev, 8 * todo.size(), hipMemcpyDeviceToHost, stream));
     }
     mark("stats wait");
