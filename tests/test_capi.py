@@ -2,6 +2,7 @@
 include/sycldb.h declares, and its host-only entry points (operator-string
 parsing, plan parsing and DAG ordering) agree with the oracle."""
 import ctypes
+import json
 import os
 import re
 
@@ -54,6 +55,30 @@ def test_bad_plans_raise_value_error():
     with pytest.raises(ValueError):
         E.dag_order('{"rels": [{"id": 0, "relOp": "FILTER", "condition": {"exprType": "EXPR", "op": "=", '
                     '"operands": []}}]}')
+
+
+def test_plan_json_reading():
+    """The plan reader: key order is free, a repeated key keeps its last
+    value, escaped strings read, deep nesting and malformed text raise."""
+    base = PL.build("t1")
+    want = PO.dag_topological_sort(base["rels"])
+    # members in reverse order, whitespace around every token
+    flipped = {"rels": [dict(reversed(list(r.items()))) for r in base["rels"]]}
+    assert E.dag_order(json.dumps(flipped, indent=2)) == want
+    # a repeated "id" key: the last one counts (the first would break the
+    # ids' order, which raises)
+    text = json.dumps(base)
+    rel1 = json.dumps(base["rels"][1])
+    doubled = rel1.replace('"id": 1', '"id": 7, "id": 1', 1)
+    assert doubled != rel1
+    assert E.dag_order(text.replace(rel1, doubled)) == want
+    # escaped strings (an output name: read, not used, by the ordering)
+    esc = text.replace('"name": "revenue"', '"name": "rev\\"en\\\\ue\\u0041"', 1)
+    assert esc != text and E.dag_order(esc) == want
+    for bad in ["[" * 100000, '{"rels": [' + "{" * 300, text[:-1], text + "x", '{"rels": [{"id": 1.5}]}',
+                '{"rels": [{"id": 99999999999999999999}]}', '{"rels": "x"}', '{"rels": [{"id": 0, "relOp": 3}]}']:
+        with pytest.raises(ValueError):
+            E.dag_order(bad)
 
 
 def test_version_and_last_error():
