@@ -50,15 +50,18 @@ void launch_seen_to_flags(const uint32_t *seen, uint8_t *flags, uint64_t prod, h
 void minmax_i32(const int32_t *col, int64_t n, int32_t *mn, int32_t *mx, hipStream_t s);
 uint64_t groupby_lds_max_slots();
 // Compact probe tables (ops.hip): bitmap from a uint8 key table, bitmap and
-// uint16 value offsets from a key-value pair table; counts set bits into
-// *count and sets *bad when a value does not fit 16 bits.
-void launch_bits_from_keys(const uint8_t *ht, int32_t len, uint32_t *bits, uint32_t *count, hipStream_t s);
+// uint16 value offsets from a key-value pair table; the set bits counted per
+// block into partials[0, bits_partials(len)) (launch_stats_reduce sums
+// them), *bad set when a value does not fit 16 bits.
+uint32_t bits_partials(int32_t len);
+void launch_bits_from_keys(const uint8_t *ht, int32_t len, uint32_t *bits, uint32_t *partials, hipStream_t s);
 // Rank dictionary (FV_RANK16 layout) of the presence bitmap of `len` slots
-// and their per-slot 16-bit values.
-void launch_rank_dict(const uint32_t *bits, int32_t len, const uint16_t *vals, uint8_t *dict, uint32_t dict_cap,
-                      hipStream_t s);
+// and their per-slot 16-bit values; `partials` (or null): the per-block
+// counts launch_bits_from_pairs left for the same table.
+void launch_rank_dict(const uint32_t *bits, int32_t len, const uint16_t *vals, const uint32_t *partials,
+                      uint8_t *dict, uint32_t dict_cap, hipStream_t s);
 void launch_bits_from_pairs(const int32_t *ht, int32_t len, uint32_t *bits, uint16_t *vals, int32_t vmin,
-                            uint32_t *count, uint32_t *bad, hipStream_t s);
+                            uint32_t *partials, uint32_t *bad, hipStream_t s);
 
 // --------------------------------------------------------- allocation map
 
@@ -172,7 +175,8 @@ struct GroupBySpec {
 struct CompactState {
     // built
     const uint32_t *bits = nullptr;
-    const uint32_t *stats = nullptr; // device: set slots, value overflow
+    const uint32_t *stats = nullptr; // device: value overflow, then nparts per-block counts of set slots
+    uint32_t nparts = 0;
     bool pairs = false;
     const void *ht = nullptr;       // reference-layout table
     const uint16_t *narrow = nullptr; // 16-bit values, when the value range fits

@@ -730,12 +730,14 @@ __global__ void __launch_bounds__(BLOCK) k_seen_to_flags(const uint32_t *seen, u
 
 // ------------------------------------------------------ compact probe tables
 // One thread per 32-slot bitmap word; deterministic (no atomics on the
-// tables), one atomic per word for the population count.
+// tables).  The population count is left per block and summed when the
+// statistics are read (k_stats_reduce).
 
-// Adds every wave's count (held by its lane 0) into *count with one atomic
-// per block: same-address atomics from every wave serialise in L2 (22K waves
-// over the 1.4M-slot part table took 95 us).
-__device__ __forceinline__ void block_count_add(uint32_t local, uint32_t *count)
+// Stores the block's count (the sum of every wave's, held by its lane 0) in
+// partials[blockIdx.x].  One same-address atomic per block instead
+// serialises: 1,368 blocks over the 1.4M-slot part table took 21 us, 7 us
+// with the partial counts.
+__device__ __forceinline__ void block_count_store(uint32_t local, uint32_t *partials)
 {
     __shared__ uint32_t wave_sum[BLOCK / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -745,13 +747,14 @@ __device__ __forceinline__ void block_count_add(uint32_t local, uint32_t *count)
         uint32_t t = 0;
 #pragma unroll
         for (int w = 0; w < BLOCK / 64; w++) t += wave_sum[w];
-        if (t) atomicAdd(count, t);
+        partials[blockIdx.x] = t;
     }
 }
 
 // Presence bitmaps of the reference tables: one thread per slot, each wave
 // covering 64 consecutive slots (coalesced reads), its ballot giving two
-// bitmap words; the set slots are counted per wave, then per block.
+// bitmap words; the set slots are counted per wave, then per block (one
+// partial count per block, bits_partials(len) of them).
 // Each wave takes BITS_GROUPS consecutive 64-slot groups per step, their
 // loads issued together, on a grid of at most BITS_MAX_BLOCKS blocks.  From
 // the pairs, the 16-bit value table is written too: value - vmin in the set
@@ -761,7 +764,7 @@ constexpr int BITS_GROUPS = 4;
 constexpr unsigned BITS_MAX_BLOCKS = 2048;
 
 __global__ void __launch_bounds__(BLOCK) k_bits_from_keys(const uint8_t *ht, int32_t len, uint32_t *bits,
-                                                          uint32_t *count)
+                                                          uint32_t *partials)
 {
     const int64_t words = ((int64_t)len + 31) / 32;
     const int lane = threadIdx.x & 63;
@@ -786,11 +789,11 @@ __global__ void __launch_bounds__(BLOCK) k_bits_from_keys(const uint8_t *ht, int
             }
         }
     }
-    block_count_add(local, count);
+    block_count_store(local, partials);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_bits_from_pairs(const int2 *ht, int32_t len, uint32_t *bits,
-                                                           uint16_t *vals, int32_t vmin, uint32_t *count,
+                                                           uint16_t *vals, int32_t vmin, uint32_t *partials,
                                                            uint32_t *bad)
 {
     const int64_t words = ((int64_t)len + 31) / 32;
@@ -822,7 +825,7 @@ __global__ void __launch_bounds__(BLOCK) k_bits_from_pairs(const int2 *ht, int32
             }
         }
     }
-    block_count_add(local, count);
+    block_count_store(local, partials);
     if (!ok) atomicOr(bad, 1u);
 }
 
@@ -942,6 +945,46 @@ __global__ void __launch_bounds__(RANK_BLOCK) k_rank_dict(const uint32_t *__rest
         DictWord<2> x;
         x.load(reinterpret_cast<const uint8_t *>(vals), len, (uint32_t)w);
         x.write(m, s_prefix[w], out_vals, cap_vals);
+    }
+}
+
+// The same dictionary from the per-block counts of k_bits_from_pairs, when
+// its block b counted exactly the slots [1024 b, 1024 b + 1024), i.e. the
+// bitmap words [32 b, 32 b + 32): one wave per such range.  The wave's set
+// slots before its range are the partial counts of the blocks before it;
+// lane l < 32 takes word 32 b + l, the wave scans the words' counts, and each
+// lane writes its word, its prefix and its set slots' values (four vector
+// loads issued together).
+constexpr uint32_t RANK_PART_WORDS = 32;
+
+__global__ void __launch_bounds__(64) k_rank_dict_parts(const uint32_t *__restrict__ bits, int32_t words,
+                                                        const uint32_t *__restrict__ partials,
+                                                        const uint16_t *__restrict__ vals, uint32_t len,
+                                                        uint8_t *__restrict__ dict, uint32_t cap_vals)
+{
+    const int lane = threadIdx.x;
+    const uint32_t b = blockIdx.x;
+    const int32_t w = (int32_t)(b * RANK_PART_WORDS) + (lane & 31);
+    const uint32_t m = lane < 32 && w < words ? bits[w] : 0u;
+    uint32_t before = 0;
+    for (uint32_t j = lane; j < b; j += 64) before += partials[j];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) before += __shfl_xor(before, off, 64);
+    const uint32_t c = __popc(m);
+    uint32_t incl = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    if (lane >= 32 || w >= words) return;
+    const uint32_t base = before + incl - c;
+    reinterpret_cast<uint32_t *>(dict)[w] = m;
+    reinterpret_cast<uint16_t *>(dict + rank_prefix_off((uint32_t)words))[w] = (uint16_t)base;
+    if (m) {
+        DictWord<2> x;
+        x.load(reinterpret_cast<const uint8_t *>(vals), len, (uint32_t)w);
+        x.write(m, base, reinterpret_cast<uint16_t *>(dict + rank_values_off((uint32_t)words)), cap_vals);
     }
 }
 
@@ -1162,6 +1205,30 @@ __global__ void __launch_bounds__(BLOCK) k_pack_parts(PackParts p, uint8_t *dst)
             reinterpret_cast<uint4 *>(out)[j] = reinterpret_cast<const uint4 *>(src)[j];
     }
     for (uint64_t j = head + (uint64_t)blockIdx.x * BLOCK + threadIdx.x; j < bytes; j += stride) out[j] = src[j];
+}
+
+// The statistics of compact probe tables (one block per table): the sum of
+// the table's partial counts and its overflow flag, as the pair dst[2 y],
+// dst[2 y + 1].
+__global__ void __launch_bounds__(BLOCK) k_stats_reduce(StatsParts p, uint32_t *dst)
+{
+    __shared__ uint32_t wave_sum[BLOCK / 64];
+    const int y = blockIdx.y;
+    const uint32_t *st = p.stats[y];
+    uint32_t t = 0;
+    for (uint32_t i = threadIdx.x; i < p.nparts[y]; i += BLOCK) t += st[1 + i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) wave_sum[wave] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t sum = 0;
+#pragma unroll
+        for (int w = 0; w < BLOCK / 64; w++) sum += wave_sum[w];
+        dst[2 * y] = sum;
+        dst[2 * y + 1] = st[0];
+    }
 }
 
 __global__ void __launch_bounds__(BLOCK) k_gather_i32(const int32_t *src, const int32_t *ids, int64_t count,
@@ -1486,40 +1553,56 @@ void launch_seen_to_flags(const uint32_t *seen, uint8_t *flags, uint64_t prod, h
     check_launch("seen_to_flags");
 }
 
-void launch_bits_from_keys(const uint8_t *ht, int32_t len, uint32_t *bits, uint32_t *count, hipStream_t s)
+uint32_t bits_partials(int32_t len)
+{
+    if (len <= 0) return 0;
+    const int64_t per_block = 64 * BITS_GROUPS * (BLOCK / 64);
+    return (uint32_t)std::min<int64_t>(BITS_MAX_BLOCKS, ((int64_t)len + per_block - 1) / per_block);
+}
+
+void launch_bits_from_keys(const uint8_t *ht, int32_t len, uint32_t *bits, uint32_t *partials, hipStream_t s)
 {
     if (len <= 0) return;
-    const int64_t words = ((int64_t)len + 31) / 32;
-    (void)words;
-    const unsigned grid = std::min<unsigned>(
-        BITS_MAX_BLOCKS, (unsigned)(((int64_t)len + 64 * BITS_GROUPS * (BLOCK / 64) - 1) / (64 * BITS_GROUPS * (BLOCK / 64))));
-    hipLaunchKernelGGL(k_bits_from_keys, dim3(grid), dim3(BLOCK), 0, s, ht, len, bits, count);
+    hipLaunchKernelGGL(k_bits_from_keys, dim3(bits_partials(len)), dim3(BLOCK), 0, s, ht, len, bits, partials);
     check_launch("bits_from_keys");
 }
 
 void launch_bits_from_pairs(const int32_t *ht, int32_t len, uint32_t *bits, uint16_t *vals, int32_t vmin,
-                            uint32_t *count, uint32_t *bad, hipStream_t s)
+                            uint32_t *partials, uint32_t *bad, hipStream_t s)
 {
     if (len <= 0) return;
     if (!aligned(ht, 8)) fail(SDB_ERR_INVALID_ARG, "key-value hash table must be 8-byte aligned");
-    const int64_t words = ((int64_t)len + 31) / 32;
-    (void)words;
-    const unsigned grid = std::min<unsigned>(
-        BITS_MAX_BLOCKS, (unsigned)(((int64_t)len + 64 * BITS_GROUPS * (BLOCK / 64) - 1) / (64 * BITS_GROUPS * (BLOCK / 64))));
-    hipLaunchKernelGGL(k_bits_from_pairs, dim3(grid), dim3(BLOCK), 0, s, reinterpret_cast<const int2 *>(ht), len, bits,
-                       vals, vmin, count, bad);
+    hipLaunchKernelGGL(k_bits_from_pairs, dim3(bits_partials(len)), dim3(BLOCK), 0, s,
+                       reinterpret_cast<const int2 *>(ht), len, bits, vals, vmin, partials, bad);
     check_launch("bits_from_pairs");
 }
 
-void launch_rank_dict(const uint32_t *bits, int32_t len, const uint16_t *vals, uint8_t *dict, uint32_t dict_cap,
-                      hipStream_t s)
+void launch_stats_reduce(const StatsParts &p, uint32_t *dst, hipStream_t s)
+{
+    if (p.n <= 0) return;
+    if (p.n > PACK_MAX_PARTS) fail(SDB_ERR_INVALID_ARG, "stats reduce: too many tables");
+    hipLaunchKernelGGL(k_stats_reduce, dim3(1, (unsigned)p.n), dim3(BLOCK), 0, s, p, dst);
+    check_launch("stats_reduce");
+}
+
+void launch_rank_dict(const uint32_t *bits, int32_t len, const uint16_t *vals, const uint32_t *partials,
+                      uint8_t *dict, uint32_t dict_cap, hipStream_t s)
 {
     if (len <= 0) return;
     const int32_t words = (int32_t)(((int64_t)len + 31) / 32);
     if (dict_cap < rank_values_off((uint32_t)words)) fail(SDB_ERR_RUNTIME, "rank dictionary buffer too small");
     const uint32_t cap_vals = (dict_cap - rank_values_off((uint32_t)words)) / 2u;
-    hipLaunchKernelGGL(k_rank_dict, dim3(1), dim3(RANK_BLOCK), 0, s, bits, words, vals, (uint32_t)len, dict,
-                       cap_vals);
+    // the bitmap kernel's blocks each counted one run of 32 words when its
+    // grid covers the table without wrapping
+    static_assert(64 * BITS_GROUPS * (BLOCK / 64) == 32 * RANK_PART_WORDS, "one bitmap block per 32 words");
+    const uint32_t ranges = (uint32_t)(((int64_t)words + RANK_PART_WORDS - 1) / RANK_PART_WORDS);
+    if (partials && ranges == bits_partials(len)) {
+        hipLaunchKernelGGL(k_rank_dict_parts, dim3(ranges), dim3(64), 0, s, bits, words, partials, vals, (uint32_t)len,
+                           dict, cap_vals);
+    } else {
+        hipLaunchKernelGGL(k_rank_dict, dim3(1), dim3(RANK_BLOCK), 0, s, bits, words, vals, (uint32_t)len, dict,
+                           cap_vals);
+    }
     check_launch("rank_dict");
 }
 

@@ -71,4 +71,13 @@ struct PackParts {
 };
 void launch_pack_parts(const PackParts &p, uint8_t *dst, hipStream_t s);
 
+// Statistics of compact probe tables, each stats[i] laid out as [overflow
+// flag, nparts[i] partial counts]: the sums and flags go to dst as pairs.
+struct StatsParts {
+    const uint32_t *stats[PACK_MAX_PARTS];
+    uint32_t nparts[PACK_MAX_PARTS];
+    int32_t n;
+};
+void launch_stats_reduce(const StatsParts &p, uint32_t *dst, hipStream_t s);
+
 } // namespace sdb

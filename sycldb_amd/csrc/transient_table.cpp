@@ -543,17 +543,20 @@ static std::shared_ptr<CompactState> build_compact(Engine &eng, const void *ht, 
     const int64_t len = (int64_t)ht_max - ht_min + 1;
     const uint64_t words = (uint64_t)(len + 31) / 32;
     uint32_t *bits = eng.mm->alloc<uint32_t>((words + 3) / 4 * 4);
-    uint32_t *stats = eng.mm->alloc_zero<uint32_t>(2); // set bits, value overflow
+    // value overflow (cleared), then one count of set slots per block
+    const uint32_t nparts = bits_partials((int32_t)len);
+    uint32_t *stats = eng.mm->alloc_zero<uint32_t>(1 + (uint64_t)nparts);
     const bool narrow = pairs && (int64_t)vmax - vmin < 0xFFFF;
     uint16_t *vals = narrow ? eng.mm->alloc<uint16_t>((uint64_t)len) : nullptr;
     if (pairs)
-        launch_bits_from_pairs(static_cast<const int32_t *>(ht), (int32_t)len, bits, vals, vmin, stats, stats + 1,
+        launch_bits_from_pairs(static_cast<const int32_t *>(ht), (int32_t)len, bits, vals, vmin, stats + 1, stats,
                                eng.stream);
     else
-        launch_bits_from_keys(static_cast<const uint8_t *>(ht), (int32_t)len, bits, stats, eng.stream);
+        launch_bits_from_keys(static_cast<const uint8_t *>(ht), (int32_t)len, bits, stats + 1, eng.stream);
     eng.launches++;
     c->bits = bits;
     c->stats = stats;
+    c->nparts = nparts;
     c->pairs = pairs;
     c->ht = ht;
     c->narrow = vals;
@@ -564,7 +567,7 @@ static std::shared_ptr<CompactState> build_compact(Engine &eng, const void *ht, 
     if (narrow && words <= 0xFFFFu && rank_values_off((uint32_t)words) < F_RANK_DICT_MAX) {
         c->dict = eng.mm->alloc<uint8_t>(F_RANK_DICT_MAX);
         c->dict_cap = F_RANK_DICT_MAX;
-        launch_rank_dict(bits, (int32_t)len, vals, c->dict, c->dict_cap, eng.stream);
+        launch_rank_dict(bits, (int32_t)len, vals, stats + 1, c->dict, c->dict_cap, eng.stream);
         eng.launches++;
     }
     eng.pending_compact.push_back(c);
@@ -579,27 +582,24 @@ void Engine::resolve_compact()
             if (!c->resolved) todo.push_back(c);
     pending_compact.clear();
     if (todo.empty()) return;
-    // through the pinned staging buffer: asynchronous copies, one sync
-    // (several tables' statistics packed on the device first, PACK_MAX_PARTS
-    // per launch: one transfer)
+    // each table's counts summed on the device (PACK_MAX_PARTS tables per
+    // launch) into one buffer of (set slots, overflow) pairs: one transfer,
+    // one sync
     uint32_t *staged = static_cast<uint32_t *>(pinned_buffer(8 * todo.size()));
-    if (todo.size() == 1) {
-        SDB_HIP(hipMemcpyAsync(staged, todo[0]->stats, 8, hipMemcpyDeviceToHost, stream));
-    } else {
-        uint8_t *dev = mm->alloc<uint8_t>(8 * todo.size());
-        PackParts pk{};
-        for (size_t i = 0; i < todo.size(); i++) {
-            pk.src[pk.n] = reinterpret_cast<const uint8_t *>(todo[i]->stats);
-            pk.off[pk.n] = 8 * i;
-            pk.bytes[pk.n] = 8;
-            if (++pk.n == PACK_MAX_PARTS || i + 1 == todo.size()) {
-                launch_pack_parts(pk, dev, stream);
-                launches++;
-                pk = PackParts{};
-            }
+    uint32_t *dev = mm->alloc<uint32_t>(2 * todo.size());
+    StatsParts sp{};
+    size_t base = 0;
+    for (size_t i = 0; i < todo.size(); i++) {
+        sp.stats[sp.n] = todo[i]->stats;
+        sp.nparts[sp.n] = todo[i]->nparts;
+        if (++sp.n == PACK_MAX_PARTS || i + 1 == todo.size()) {
+            launch_stats_reduce(sp, dev + 2 * base, stream);
+            launches++;
+            base = i + 1;
+            sp = StatsParts{};
         }
-        SDB_HIP(hipMemcpyAsync(staged, dev, 8 * todo.size(), hipMemcpyDeviceToHost, stream));
     }
+    SDB_HIP(hipMemcpyAsync(staged, dev, 8 * todo.size(), hipMemcpyDeviceToHost, stream));
     mark("stats wait");
     wait_stream();
     mark("stats ready");
