@@ -98,6 +98,23 @@ void KernelBundle::execute(const ExecContext &ctx) const
             validate_program(p, kernels_.front().n, eng.registry);
             dump_program(p, kernels_.front().n, ctx.table ? *ctx.table : std::string("?"));
             const int64_t n = kernels_.front().n;
+            // the table's all-true flags, in use after the bundles: a program
+            // storing them for every row writes them; one storing a part of
+            // them needs the rest filled first (its reads of them stay true)
+            bool stores_whole = false, stores_part = false;
+            if (ctx.virgin_live && !ctx.virgin_filled)
+                for (int j = 0; j < F_MAX_FLAGS; j++) {
+                    const uint8_t *st = p.flag_store[j];
+                    if (!st || st < ctx.virgin_begin || st >= ctx.virgin_end) continue;
+                    if (st == ctx.virgin_begin && n == ctx.virgin_end - ctx.virgin_begin) stores_whole = true;
+                    else stores_part = true;
+                }
+            if (stores_part) {
+                eng.registry.check(ctx.virgin_begin, (uint64_t)(ctx.virgin_end - ctx.virgin_begin), "initial flags");
+                launch_fill_u8(const_cast<uint8_t *>(ctx.virgin_begin), 1, ctx.virgin_end - ctx.virgin_begin, ctx.stream);
+                eng.launches++;
+                ctx.virgin_filled = true;
+            }
             const bool jit = eng.codegen == SDB_CODEGEN_COMPILE || (eng.codegen == SDB_CODEGEN_AUTO && jit_available());
             // the kernel is chosen, compiled and tuned before the profiling
             // events, which then bracket the program's own launch only
@@ -111,6 +128,7 @@ void KernelBundle::execute(const ExecContext &ctx) const
             else launch_fused(p, n, ctx.stream);
             if (timed) eng.prof_end(rec);
             eng.launches++;
+            if (stores_whole) ctx.virgin_filled = true;
             eng.mark("launched");
             return;
         }
@@ -141,7 +159,8 @@ void KernelBundle::execute_unfused(const ExecContext &ctx) const
     hipStream_t s = ctx.stream;
     Engine &eng = *ctx.eng;
     // a flags buffer that is still logically all-true gets its values now
-    bool need_fill = false;
+    // (or one still in use after the bundles: its kernels may write a part)
+    bool need_fill = ctx.virgin_live;
     for (const KernelData &k : kernels_)
         need_fill |= reads_virgin(ctx, k.in_flags) || reads_virgin(ctx, k.local_flags);
     if (need_fill && !ctx.virgin_filled) {

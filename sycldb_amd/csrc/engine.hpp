@@ -250,7 +250,10 @@ struct ExecContext {
     mutable uint64_t segment = 0;       // segment the bundle covers
     // flags buffer the table has not written yet: logically all true
     const uint8_t *virgin_begin = nullptr, *virgin_end = nullptr;
-    mutable bool virgin_filled = false; // set once an unfused bundle filled it
+    mutable bool virgin_filled = false; // set once it holds values (filled, or stored whole by a program)
+    // the all-true buffer stays in use after the bundles: it must end up
+    // holding values (stored whole by a program, else filled)
+    bool virgin_live = false;
 };
 
 // KernelBundle, models/execution.hpp:500-651: the operators queued for one
@@ -413,6 +416,9 @@ class TransientTable {
     const uint8_t *flags_device();
 
   private:
+    // throws unless the flags buffer is registered device memory (a captured
+    // fact table's placeholder is not: NotBatchable)
+    void check_flags_memory() const;
     uint64_t segment_num() const { return nrows_ == 0 ? 0 : (nrows_ + eng_.segment_size - 1) / eng_.segment_size; }
     uint64_t seg_rows(uint64_t s) const { return std::min(eng_.segment_size, nrows_ - s * eng_.segment_size); }
     void add_pending_kernels(std::vector<KernelBundle> ops); // 347-381

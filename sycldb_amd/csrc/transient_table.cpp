@@ -126,17 +126,22 @@ void TransientTable::add_pending_kernels(std::vector<KernelBundle> ops)
         for (const KernelData &k : ops[i].get_kernels()) last[i].add_kernel(k);
 }
 
+void TransientTable::check_flags_memory() const
+{
+    try {
+        eng_.registry.check(flags_, std::max<uint64_t>(nrows_, 1), "initial flags");
+    } catch (const Error &) {
+        // a row-sized placeholder of a captured fact table
+        if (eng_.capture.active && name_ == eng_.capture.table)
+            throw NotBatchable("the fact table's flags are needed in memory");
+        throw;
+    }
+}
+
 const uint8_t *TransientTable::flags_device()
 {
     if (virgin_ == flags_) {
-        try {
-            eng_.registry.check(flags_, std::max<uint64_t>(nrows_, 1), "initial flags");
-        } catch (const Error &) {
-            // a row-sized placeholder of a captured fact table
-            if (eng_.capture.active && name_ == eng_.capture.table)
-                throw NotBatchable("the fact table's flags are needed in memory");
-            throw;
-        }
+        check_flags_memory();
         launch_fill_u8(flags_, 1, (int64_t)nrows_, eng_.stream);
         eng_.launches++;
         virgin_ = nullptr;
@@ -148,9 +153,15 @@ void TransientTable::execute_pending_kernels(bool dead_after)
 {
     if (pending_kernels_.empty()) return;
     LiveSet live;
+    bool virgin_live = false;
     if (!dead_after) {
-        // flags and columns the table still exposes must hold their values
-        if (virgin_ == flags_) flags_device();
+        // flags and columns the table still exposes must hold their values;
+        // flags still all true are written by a program that stores them
+        // whole, else filled (KernelBundle::execute, and below)
+        if (virgin_ == flags_) {
+            check_flags_memory();
+            virgin_live = true;
+        }
         live.add(flags_, nrows_);
         for (Column *c : current_columns_)
             if (c && c->loaded && c->data) live.add(c->data, c->nrows * (c->is_aggregate ? 8 : 4));
@@ -162,12 +173,17 @@ void TransientTable::execute_pending_kernels(bool dead_after)
         ctx.virgin_begin = virgin_;
         ctx.virgin_end = virgin_ + nrows_;
     }
+    ctx.virgin_live = virgin_live;
     const uint64_t nseg = segment_num();
     for (const auto &phase : pending_kernels_)
         if (phase.size() != nseg) throw std::runtime_error("Pending kernels segment number mismatch.");
     for (uint64_t s = 0; s < nseg; s++) {
         ctx.segment = s;
         for (const auto &phase : pending_kernels_) phase[s].execute(ctx);
+    }
+    if (virgin_live && !ctx.virgin_filled) { // no bundle wrote them
+        launch_fill_u8(flags_, 1, (int64_t)nrows_, eng_.stream);
+        eng_.launches++;
     }
     pending_kernels_.clear();
     virgin_ = nullptr;
