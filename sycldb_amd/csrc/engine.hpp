@@ -346,6 +346,10 @@ struct Engine {
     // (measured up to ~0.5 ms per query step).
     hipEvent_t sync_ev = nullptr;
     void wait_stream();
+    // While a plan executes, the waits record ev_stop instead of sync_ev: the
+    // plan's last wait (its result transfer) then marks its end on the device
+    // with no further event round trip (stop_recorded: one has been).
+    bool stop_at_wait = false, stop_recorded = false;
 
     // Optional per-launch timing with HIP events on the engine stream.
     struct KernelStat {

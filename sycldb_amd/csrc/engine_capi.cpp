@@ -299,14 +299,27 @@ int sdb_engine_execute(sdb_engine *e, const char *plan_json, sdb_result **out)
         const auto w0 = std::chrono::steady_clock::now();
         SDB_HIP(hipEventRecord(eng.ev_start, eng.stream));
         auto r = std::make_unique<sdb_result>();
-        r->t = execute_ddor_plan(eng, plan);
-        eng.mark("plan done");
-        SDB_HIP(hipEventRecord(eng.ev_stop, eng.stream));
-        // the result wait has drained the stream: polled, this returns at once
-        hipError_t q;
-        while ((q = hipEventQuery(eng.ev_stop)) == hipErrorNotReady) {
+        eng.stop_at_wait = true;
+        eng.stop_recorded = false;
+        try {
+            r->t = execute_ddor_plan(eng, plan);
+        } catch (...) {
+            eng.stop_at_wait = false;
+            throw;
         }
-        SDB_HIP(q);
+        eng.stop_at_wait = false;
+        eng.mark("plan done");
+        // the result wait recorded ev_stop; a stream with work since (still
+        // running) gets its end recorded now
+        const hipError_t idle = eng.stop_recorded ? hipStreamQuery(eng.stream) : hipErrorNotReady;
+        if (idle != hipSuccess) {
+            if (idle != hipErrorNotReady) SDB_HIP(idle);
+            SDB_HIP(hipEventRecord(eng.ev_stop, eng.stream));
+            hipError_t q;
+            while ((q = hipEventQuery(eng.ev_stop)) == hipErrorNotReady) {
+            }
+            SDB_HIP(q);
+        }
         if (eng.profile) eng.prof_collect();
         eng.mark("end");
         eng.print_marks();

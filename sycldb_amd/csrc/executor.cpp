@@ -240,11 +240,13 @@ void Engine::print_marks()
 void Engine::wait_stream()
 {
     if (!sync_ev) SDB_HIP(hipEventCreateWithFlags(&sync_ev, hipEventDisableTiming));
-    SDB_HIP(hipEventRecord(sync_ev, stream));
+    hipEvent_t ev = stop_at_wait ? ev_stop : sync_ev;
+    SDB_HIP(hipEventRecord(ev, stream));
     hipError_t e;
-    while ((e = hipEventQuery(sync_ev)) == hipErrorNotReady) {
+    while ((e = hipEventQuery(ev)) == hipErrorNotReady) {
     }
     SDB_HIP(e);
+    if (stop_at_wait) stop_recorded = true;
 }
 
 void *Engine::pinned_buffer(size_t bytes)
