@@ -439,6 +439,10 @@ class TransientTable {
     uint8_t *flags_ = nullptr, *flags_secondary_ = nullptr;
     uint8_t *virgin_ = nullptr;          // flags buffer that is logically all
                                          // true and not yet written
+    // estimated fraction of rows the filters so far keep (from the column
+    // statistics; it orders fused predicates, never decides a result)
+    float est_kept_ = 1.0f;
+    void note_filter(int logic, float sel);
     std::vector<Column *> current_columns_;
     std::deque<Column> materialized_columns_;
     Column *group_by_column_ = nullptr;

@@ -37,6 +37,17 @@ int32_t to_int32(int64_t v) { return (int32_t)(uint32_t)(uint64_t)v; }
 
 float clamp01(double x) { return (float)(x < 0 ? 0 : (x > 1 ? 1 : x)); }
 
+// SDB_SEMI_STATS=1: semi joins read their key tables' statistics too (one
+// stream sync) instead of taking the build side's estimated selectivity.
+bool semi_estimates()
+{
+    static const bool on = [] {
+        const char *e = getenv("SDB_SEMI_STATS");
+        return !(e && *e == '1');
+    }();
+    return on;
+}
+
 // Fraction of a uniformly distributed column in [mn, mx] a comparison keeps
 // (used only to order fused predicates, never for results).
 float estimate_cmp(int cmp, int32_t lit, std::pair<int32_t, int32_t> mm)
@@ -191,6 +202,15 @@ void TransientTable::execute_pending_kernels(bool dead_after)
 
 // -------------------------------------------------------------------- filter
 
+// The kept fraction after a filter that combines its own estimate with the
+// table's flags by `logic` (predicates taken as independent).
+void TransientTable::note_filter(int logic, float sel)
+{
+    if (logic == SDB_AND) est_kept_ *= sel;
+    else if (logic == SDB_OR) est_kept_ = est_kept_ + sel - est_kept_ * sel;
+    else est_kept_ = sel;
+}
+
 void TransientTable::apply_filter(const ExprType &expr, const std::string &parent_op)
 {
     if (expr.exprType != ExprOption::EXPR) {
@@ -231,20 +251,23 @@ void TransientTable::apply_filter(const ExprType &expr, const std::string &paren
     }
     if (!cols[0]) throw std::invalid_argument("filter: the first comparison operand must be a column");
     ensure_secondary_flags();
+    const int cmp = get_comp_op(expr.op), logic = get_logical_op(parent_op);
+    const float sel = literal ? estimate_cmp(cmp, literal_value, cols[0]->get_min_max()) : 0.5f;
+    note_filter(logic, sel);
     std::vector<KernelBundle> ops(segment_num());
     for (uint64_t s = 0; s < segment_num(); s++) {
         const uint64_t off = s * eng_.segment_size;
         KernelData k;
         k.type = literal ? KernelType::SelectionKernelLiteral : KernelType::SelectionKernelColumns;
         k.n = (int64_t)seg_rows(s);
-        k.cmp = get_comp_op(expr.op);
-        k.logic = get_logical_op(parent_op);
+        k.cmp = cmp;
+        k.logic = logic;
         k.in_flags = flags_ + off;
         k.out_flags = flags_secondary_ + off;
         k.a = cols[0]->i32(s);
         if (literal) k.literal = literal_value;
         else k.b = cols[1]->i32(s);
-        k.sel = literal ? estimate_cmp(k.cmp, literal_value, cols[0]->get_min_max()) : 0.5f;
+        k.sel = sel;
         ops[s].add_kernel(k);
     }
     add_pending_kernels(std::move(ops));
@@ -279,6 +302,7 @@ void TransientTable::apply_search(const ExprType &expr, const std::string &paren
         sel = range ? clamp01(((double)std::min(v1, mm.second) - std::max(v0, mm.first) + 1) / width)
                     : clamp01(2.0 / width);
     }
+    note_filter(logic, sel);
     std::vector<KernelBundle> ops(segment_num());
     for (uint64_t s = 0; s < segment_num(); s++) {
         const uint64_t off = s * eng_.segment_size;
@@ -552,8 +576,12 @@ std::tuple<int32_t *, int32_t, int32_t> TransientTable::build_key_vals_hash_tabl
 // Results never depend on them: a bit is set exactly when the reference
 // table reports a match.  Only the launches happen here; the statistics are
 // read on first use (Engine::resolve_compact).
+// A key table (semi join) takes `est_sel` (>= 0) as its selectivity, with no
+// statistics read: the value only orders fused predicates.  A pair table's
+// statistics are read (its value layout and dictionary depend on them).
 static std::shared_ptr<CompactState> build_compact(Engine &eng, const void *ht, bool pairs, int32_t ht_min,
-                                                   int32_t ht_max, int32_t vmin, int32_t vmax, uint64_t build_rows)
+                                                   int32_t ht_max, int32_t vmin, int32_t vmax, uint64_t build_rows,
+                                                   float est_sel = -1.0f)
 {
     auto c = std::make_shared<CompactState>();
     const int64_t len = (int64_t)ht_max - ht_min + 1;
@@ -586,7 +614,12 @@ static std::shared_ptr<CompactState> build_compact(Engine &eng, const void *ht, 
         launch_rank_dict(bits, (int32_t)len, vals, stats + 1, c->dict, c->dict_cap, eng.stream);
         eng.launches++;
     }
-    eng.pending_compact.push_back(c);
+    if (!pairs && est_sel >= 0.0f) {
+        c->sel = std::min(1.0f, est_sel);
+        c->resolved = true;
+    } else {
+        eng.pending_compact.push_back(c);
+    }
     return c;
 }
 
@@ -660,7 +693,8 @@ void TransientTable::apply_join(TransientTable &right, const RelNode &rel)
         right.execute_pending_kernels(false);
         std::shared_ptr<CompactState> ct;
         if (eng_.fusion)
-            ct = build_compact(eng_, std::get<0>(ht), false, std::get<1>(ht), std::get<2>(ht), 0, 0, right.nrows_);
+            ct = build_compact(eng_, std::get<0>(ht), false, std::get<1>(ht), std::get<2>(ht), 0, 0, right.nrows_,
+                               semi_estimates() ? right.est_kept_ : -1.0f);
         std::vector<KernelBundle> ops(segment_num());
         for (uint64_t s = 0; s < segment_num(); s++) {
             const uint64_t off = s * eng_.segment_size;
