@@ -81,10 +81,10 @@ def batched_measure(eng, n_pipes, seed, reps):
         if codegen_pending() == 0:
             break
         codegen_wait(300)
-    for p in plans:
-        eng.execute(p)
-    torch.cuda.synchronize()
     with no_gc():
+        for p in plans:
+            eng.execute(p)
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
             for p in plans:
@@ -101,23 +101,24 @@ def batched_measure(eng, n_pipes, seed, reps):
     torch.cuda.synchronize()
     from sycldb_amd.engine import batch_graph_stats, set_batch_graphs
 
-    def time_batches():
+    def time_batches(warm):
         with no_gc():
+            for _ in range(warm):
+                eng.execute_batch(plans)
+            torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(reps):
                 eng.execute_batch(plans)
             torch.cuda.synchronize()
             return (time.perf_counter() - t0) / reps
     # the launches as plain stream operations (the default), then as
-    # replayed hipGraphs; each mode warmed first
+    # replayed hipGraphs; each mode warmed first, right before its timing
     set_batch_graphs(False)
-    eng.execute_batch(plans)
-    bat_s = time_batches()
+    bat_s = time_batches(1)
     set_batch_graphs(True)
     eng.execute_batch(plans)
-    eng.execute_batch(plans)
     c0, r0 = batch_graph_stats()
-    graph_s = time_batches()
+    graph_s = time_batches(1)
     c1, r1 = batch_graph_stats()
     set_batch_graphs(None)
     # per template: its 8 pipelines one by one and as one batch
@@ -136,12 +137,17 @@ def batched_measure(eng, n_pipes, seed, reps):
                 break
         torch.cuda.synchronize()
         with no_gc():
+            for p in sub:
+                eng.execute(p)
+            torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(reps):
                 for p in sub:
                     eng.execute(p)
             torch.cuda.synchronize()
             t_seq = (time.perf_counter() - t0) / reps
+            eng.execute_batch(sub)
+            torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(reps):
                 eng.execute_batch(sub)
@@ -231,10 +237,10 @@ def served_measure(eng_big, seed, n_requests, rates, max_batch):
 def config_measure(name, seed, steps, cpu_seconds):
     """One of the smaller plain configurations at its own size: q1.1 on the
     6M-row table (C1) or the q3.1 join on the 60M-row table (C2).  GPU: the
-    fused plan, settled first, then timed steps with HIP events around the
-    fact kernel.  CPU: the oracle (the reference's unfused operator
-    sequence) on the same seeded table at full size, on this host's cores.
-    The two results must agree."""
+    fused plan, settled first, then timed steps, then as many steps with HIP
+    events around the fact kernel for its launch time.  CPU: the oracle (the
+    reference's unfused operator sequence) on the same seeded table at full
+    size, on this host's cores.  The two results must agree."""
     import torch
     from oracle import datagen, oracle as O, plan_oracle as PO
     from sycldb_amd import plans as PL, schema as S
@@ -251,20 +257,27 @@ def config_measure(name, seed, steps, cpu_seconds):
             codegen_wait(300)
             res = eng.execute(text)
             runs += 1
-        for _ in range(3):
-            eng.execute(text)
-        eng.set_profiling(True, table="lineorder")
-        eng.profile_reset()
-        torch.cuda.synchronize()
         with no_gc():
+            for _ in range(10):
+                eng.execute(text)
+            torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(steps):
                 eng.execute(text)
             torch.cuda.synchronize()
             wall = time.perf_counter() - t0
+        # the fact kernel's launch time from HIP events around its launches,
+        # in steps of their own: at these sizes the events' host time is a
+        # visible part of a step
+        eng.set_profiling(True, table="lineorder")
+        eng.profile_reset()
+        for _ in range(steps):
+            eng.execute(text)
+        torch.cuda.synchronize()
         launches, krows, kms = eng.profile().get("fused:lineorder", (0, 0, 0.0))
+        eng.set_profiling(False)
         gpu_rows = res.rows()
-    out["gpu"] = {"ms_per_step": wall * 1e3 / steps, "fact_rows_per_s": rows * steps / wall,
+    out["gpu"] = {"steps": steps, "ms_per_step": wall * 1e3 / steps, "fact_rows_per_s": rows * steps / wall,
                   "kernel_ms": kms / max(launches, 1),
                   "kernel_hbm_frac": (4 * len(cols) * krows / (kms / 1e3)) / (HBM_PEAK_GBS * 1e9) if kms else None}
     tables = datagen.generate_tables(S, rows, seed, fact_columns=cols)
@@ -409,8 +422,6 @@ def main():
     tuning_s = time.perf_counter() - t_tune
     if codegen_pending() > 0:
         log("warning: the autotuner has not settled after %d executions" % tuning_runs)
-    for _ in range(args.warmup):
-        res = eng.execute(plan_text)
     groups = len(res.rows())
     tuning = codegen_tuning_report()
     if tuning:
@@ -418,10 +429,15 @@ def main():
     # HIP events around the fact-table launches only (the dimension
     # bundles run untimed, without the events' host cost)
     eng.set_profiling(True, table="lineorder")
-    eng.profile_reset()
-    torch.cuda.synchronize()
     dev_ms = []
     with no_gc():
+        # the warmup steps right before the timed ones, after the set-up
+        # above: the device idling meanwhile (logging, a collection) made
+        # the first steps after it up to 0.2 ms slower
+        for _ in range(args.warmup):
+            eng.execute(plan_text)
+        eng.profile_reset()
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             eng.execute(plan_text)
@@ -477,7 +493,7 @@ def main():
     if not args.no_configs and args.workload == "c3":
         configs = {}
         for name in ("c1", "c2"):
-            configs[name] = config_measure(name, args.seed, 20, 3.0 if args.no_cpu else 6.0)
+            configs[name] = config_measure(name, args.seed, 100, 3.0 if args.no_cpu else 6.0)
             log("%s: %s" % (name, configs[name]))
 
     cpu = None
