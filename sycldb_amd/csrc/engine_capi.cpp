@@ -48,7 +48,12 @@ void Engine::prof_end(ProfRec &rec)
 void Engine::prof_collect()
 {
     for (ProfRec &r : prof_pending) {
-        SDB_HIP(hipEventSynchronize(r.b));
+        // polled, as the engine's other waits: a blocking wait's wake-up
+        // cost tens of microseconds per step even for a completed event
+        hipError_t q;
+        while ((q = hipEventQuery(r.b)) == hipErrorNotReady) {
+        }
+        SDB_HIP(q);
         float ms = 0;
         SDB_HIP(hipEventElapsedTime(&ms, r.a, r.b));
         KernelStat &st = prof_stats[r.name];
