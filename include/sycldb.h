@@ -468,6 +468,15 @@ int sdb_codegen_tuning_report(char *buf, int cap);
  * runs its final configuration. */
 int sdb_codegen_wait(double timeout_s);
 int sdb_codegen_pending(void);
+/* Stops the background compiler: queued compilations are dropped (their
+ * program shapes run on the interpreter kernel in SDB_CODEGEN_AUTO mode, or
+ * fail in SDB_CODEGEN_COMPILE mode) and the one in progress is waited for.
+ * Call it before the process exits while kernels may still be compiling: a
+ * compilation still running when the process tears down its static objects
+ * can crash or hang the exit.  The Python package calls it when the
+ * interpreter exits.  No counterpart in the reference (its kernels are
+ * compiled ahead of time). */
+int sdb_codegen_shutdown(void);
 /* Compiles, without launching (no device needed), the runtime-generated
  * kernels of programs covering every instruction form for GPU architecture
  * `arch` (e.g. "gfx950"); *count = kernels compiled. */

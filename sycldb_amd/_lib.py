@@ -3,6 +3,7 @@
 The library is the product: there is no fallback.  Importing this module
 raises ImportError when the built library is missing.
 """
+import atexit
 import ctypes
 import os
 
@@ -164,6 +165,7 @@ SIGNATURES = {
     "sdb_codegen_check": (INT, [CSTR, ctypes.POINTER(INT)]),
     "sdb_codegen_wait": (INT, [ctypes.c_double]),
     "sdb_codegen_pending": (INT, []),
+    "sdb_codegen_shutdown": (INT, []),
     "sdb_set_batch_graphs": (INT, [INT]),
     "sdb_batch_graph_stats": (INT, [ctypes.POINTER(I64), ctypes.POINTER(I64)]),
     "sdb_engine_set_profiling": (INT, [P, INT]),
@@ -189,6 +191,11 @@ for _name, (_res, _args) in SIGNATURES.items():
     _fn = getattr(lib, _name)
     _fn.restype = _res
     _fn.argtypes = _args
+
+# The background kernel compiler stops while the interpreter exits, before
+# the process tears down the compiler libraries a compilation in progress
+# still uses (a no-op when nothing was compiled in the background).
+atexit.register(lib.sdb_codegen_shutdown)
 
 
 class SdbError(RuntimeError):

@@ -609,6 +609,11 @@ int sdb_codegen_tuning_report(char *buf, int cap)
 
 int sdb_codegen_wait(double timeout_s) { return jit_wait_idle(timeout_s) ? 1 : 0; }
 
+int sdb_codegen_shutdown(void)
+{
+    return guarded([&] { jit_shutdown(); });
+}
+
 int sdb_set_batch_graphs(int on)
 {
     return guarded([&] {

@@ -50,6 +50,10 @@ void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll);
 // Waits until the background compiler has built every kernel queued so far
 // (true), or until timeout_s seconds have passed (false).
 bool jit_wait_idle(double timeout_s);
+// Stops the background compiler (queued compilations dropped, the one in
+// progress finished); call before the process exits while kernels may still
+// be compiling.
+void jit_shutdown();
 // Program shapes whose kernels are still compiling, or whose launches over
 // at least the tuning size have not yet settled on a configuration, plus
 // batched-kernel libraries still tuning.

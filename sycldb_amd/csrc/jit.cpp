@@ -1671,9 +1671,15 @@ struct CompileJob {
 
 // A thread that compiles queued kernels one at a time (the default
 // configurations first).  It only generates source and runs hipRTC, never
-// the HIP runtime; the launching thread loads the code objects.  Destroyed
-// (queue dropped, the compilation in progress finished) at process exit,
-// before the code cache and the hipRTC library it uses.
+// the HIP runtime; the launching thread loads the code objects.
+//
+// It must be stopped before the process tears down its static objects: the
+// compiler libraries behind hipRTC set up their own statics during the first
+// compilation, after this object, so at exit they are destroyed before this
+// object's destructor could wait for a compilation still running on the
+// thread (a CPU reproduction crashed or hung at exit 3 times in 3).
+// jit_shutdown (sdb_codegen_shutdown; the Python package calls it when the
+// interpreter exits) stops it first.
 class BackgroundCompiler {
 public:
     static BackgroundCompiler &get()
@@ -1681,33 +1687,63 @@ public:
         static BackgroundCompiler c;
         return c;
     }
+    static std::atomic<bool> &created()
+    {
+        static std::atomic<bool> made{false};
+        return made;
+    }
     void submit(std::vector<CompileJob> jobs)
     {
         {
             std::lock_guard<std::mutex> lk(mu_);
-            for (CompileJob &j : jobs) {
-                if (j.kind == JOB_DEFAULT) q_.push_front(std::move(j));
-                else q_.push_back(std::move(j));
+            if (!stop_) {
+                for (CompileJob &j : jobs) {
+                    if (j.kind == JOB_DEFAULT) q_.push_front(std::move(j));
+                    else q_.push_back(std::move(j));
+                }
+                jobs.clear();
             }
         }
         cv_.notify_all();
+        drop(jobs);
     }
-    ~BackgroundCompiler()
+    // Drops the queued jobs and waits for the compilation in progress; later
+    // submissions are dropped too.  A dropped job's program shape records an
+    // error: it runs on the interpreter kernel (SDB_CODEGEN_AUTO), or its
+    // execution fails where compiled kernels are required.
+    void shutdown()
     {
+        std::vector<CompileJob> dropped;
         {
             std::lock_guard<std::mutex> lk(mu_);
             stop_ = true;
+            for (CompileJob &j : q_) dropped.push_back(std::move(j));
             q_.clear();
         }
         cv_.notify_all();
         if (th_.joinable()) th_.join();
+        drop(dropped);
     }
+    ~BackgroundCompiler() { shutdown(); }
 
 private:
     BackgroundCompiler()
     {
         rtc(); // the hipRTC library is loaded (and its exit handlers set) before this object's
         th_ = std::thread([this] { run(); });
+        created() = true;
+    }
+    static void drop(const std::vector<CompileJob> &jobs)
+    {
+        if (jobs.empty()) return;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            for (const CompileJob &j : jobs) {
+                Tune &t = g_tune[j.key];
+                if (t.error.empty()) t.error = "the background compiler has stopped";
+            }
+        }
+        g_tune_cv.notify_all();
     }
     void run()
     {
@@ -2081,6 +2117,11 @@ void run_fused_jit(const FusedLaunch &L, const FProgram &p, int64_t n, hipStream
 void launch_fused_jit(const FProgram &p, int64_t n, hipStream_t s, int unroll)
 {
     run_fused_jit(prepare_fused_jit(p, n, s, unroll, false), p, n, s);
+}
+
+void jit_shutdown()
+{
+    if (BackgroundCompiler::created()) BackgroundCompiler::get().shutdown();
 }
 
 bool jit_wait_idle(double timeout_s)

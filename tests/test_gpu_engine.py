@@ -2,6 +2,8 @@
 unfused bundles, several segment sizes) against the plan oracle on the same
 seeded tables; the device generator against the CPU one; reference error
 behaviour."""
+import os
+
 import numpy as np
 import pytest
 
@@ -315,3 +317,40 @@ def test_value_table_width_boundary(cpu_tables, span):
         assert eng.execute(plan).rows() == want
     finally:
         eng.close()
+
+
+@pytest.mark.gpu
+def test_exit_while_kernels_compile():
+    """A process that exits while its plans' kernels still compile in the
+    background exits cleanly (the compiler is stopped when the interpreter
+    exits, before the compiler libraries are torn down), and a plan shape
+    first met after the compiler stopped runs on the interpreter kernel with
+    the interpreter's result."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "import torch\n"
+        "from sycldb_amd import plans, _lib as L\n"
+        "from sycldb_amd.engine import Engine, codegen_pending\n"
+        "with Engine() as eng:\n"
+        "    eng.generate(200_000, 7)\n"
+        "    a = eng.execute(plans.t2_plan(('category', 3), 2)).rows()\n"
+        "    busy = codegen_pending()\n"
+        "    L.check(L.lib.sdb_codegen_shutdown())\n"
+        "    b = eng.execute(plans.t4_plan(2, 3, (0, 1))).rows()\n"
+        "with Engine(codegen='interpret') as eng:\n"
+        "    eng.generate(200_000, 7)\n"
+        "    want_a = eng.execute(plans.t2_plan(('category', 3), 2)).rows()\n"
+        "    want_b = eng.execute(plans.t4_plan(2, 3, (0, 1))).rows()\n"
+        "assert a == want_a and b == want_b and len(a) > 0 and len(b) > 0\n"
+        "print('ok', busy, flush=True)\n"
+        "with Engine() as eng:\n"
+        "    eng.generate(200_000, 7)\n"
+        "    eng.execute(plans.t3_plan()).rows()\n"
+    ) % root
+    print("exit while compiling: child process", flush=True)
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240)
+    assert out.returncode == 0, (out.returncode, out.stderr[-3000:])
+    assert out.stdout.split()[0] == "ok", out.stdout
