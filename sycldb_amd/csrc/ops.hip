@@ -873,88 +873,13 @@ template <uint32_t WB> struct DictWord {
 };
 
 // Rank dictionary of a presence bitmap with per-slot 16-bit values (the
-// layout FV_RANK16 describes in fused_types.hpp), in one block of
-// RANK_BLOCK threads.  A bitmap of at most RANK_LDS_WORDS words comes into
-// LDS with coalesced loads; each thread then counts a run of consecutive
-// words, the block scans the run totals (within each wave by shuffles, then
-// across the waves), and the words and their prefix counts go out coalesced.
-// The values: each thread takes whole nonempty words, loads the word's 32
-// values in four vector loads issued together and writes the set slots' at
-// their ranks.  A larger bitmap is read and written per word by the thread
-// that owns the word's run.
-// (one 256-thread block: it fits beside a long batched scan of another
-// engine, where a 1024-thread block would wait for that scan to end)
-constexpr int RANK_BLOCK = 256;
-constexpr int RANK_LDS_WORDS = 4096;
-
-__global__ void __launch_bounds__(RANK_BLOCK) k_rank_dict(const uint32_t *__restrict__ bits, int32_t words,
-                                                          const uint16_t *__restrict__ vals, uint32_t len,
-                                                          uint8_t *__restrict__ dict, uint32_t cap_vals)
-{
-    __shared__ uint32_t wave_total[RANK_BLOCK / 64];
-    __shared__ uint32_t s_bits[RANK_LDS_WORDS];
-    __shared__ uint32_t s_prefix[RANK_LDS_WORDS];
-    const bool in_lds = words <= RANK_LDS_WORDS;
-    if (in_lds) {
-#pragma unroll 4
-        for (int32_t w = threadIdx.x; w < words; w += RANK_BLOCK) s_bits[w] = bits[w];
-        __syncthreads();
-    }
-    const int32_t per = (words + RANK_BLOCK - 1) / RANK_BLOCK;
-    const int32_t w0 = min(words, (int32_t)threadIdx.x * per), w1 = min(words, w0 + per);
-    uint32_t local = 0;
-    for (int32_t w = w0; w < w1; w++) local += __popc(in_lds ? s_bits[w] : bits[w]);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t incl = local;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += y;
-    }
-    if (lane == 63) wave_total[wave] = incl;
-    __syncthreads();
-    uint32_t base = incl - local;
-    for (int v = 0; v < wave; v++) base += wave_total[v];
-    uint32_t *out_bits = reinterpret_cast<uint32_t *>(dict);
-    uint16_t *out_prefix = reinterpret_cast<uint16_t *>(dict + rank_prefix_off((uint32_t)words));
-    uint16_t *out_vals = reinterpret_cast<uint16_t *>(dict + rank_values_off((uint32_t)words));
-    if (!in_lds) {
-        for (int32_t w = w0; w < w1; w++) {
-            const uint32_t m = bits[w];
-            out_bits[w] = m;
-            out_prefix[w] = (uint16_t)base;
-            // values past the buffer's capacity are dropped (the dictionary
-            // is then not used)
-            for (uint32_t r = m; r; r &= r - 1, base++)
-                if (base < cap_vals) out_vals[base] = vals[(int64_t)w * 32 + __builtin_ctz(r)];
-        }
-        return;
-    }
-    for (int32_t w = w0; w < w1; w++) {
-        s_prefix[w] = base;
-        base += __popc(s_bits[w]);
-    }
-    __syncthreads();
-    for (int32_t w = threadIdx.x; w < words; w += RANK_BLOCK) {
-        out_bits[w] = s_bits[w];
-        out_prefix[w] = (uint16_t)s_prefix[w];
-    }
-    for (int32_t w = threadIdx.x; w < words; w += RANK_BLOCK) {
-        const uint32_t m = s_bits[w];
-        if (!m) continue;
-        DictWord<2> x;
-        x.load(reinterpret_cast<const uint8_t *>(vals), len, (uint32_t)w);
-        x.write(m, s_prefix[w], out_vals, cap_vals);
-    }
-}
-
-// The same dictionary from the per-block counts of k_bits_from_pairs, when
-// its block b counted exactly the slots [1024 b, 1024 b + 1024), i.e. the
-// bitmap words [32 b, 32 b + 32): one wave per such range.  The wave's set
-// slots before its range are the partial counts of the blocks before it;
-// lane l < 32 takes word 32 b + l, the wave scans the words' counts, and each
-// lane writes its word, its prefix and its set slots' values (four vector
-// loads issued together).
+// layout FV_RANK16 describes in fused_types.hpp), from the per-block counts
+// of k_bits_from_pairs: its block b counted exactly the slots
+// [1024 b, 1024 b + 1024), i.e. the bitmap words [32 b, 32 b + 32).  One wave
+// per such range: its set slots before the range are the partial counts of
+// the blocks before it; lane l < 32 takes word 32 b + l, the wave scans the
+// words' counts, and each lane writes its word, its prefix and its set
+// slots' values (four vector loads issued together).
 constexpr uint32_t RANK_PART_WORDS = 32;
 
 __global__ void __launch_bounds__(64) k_rank_dict_parts(const uint32_t *__restrict__ bits, int32_t words,
@@ -1593,16 +1518,14 @@ void launch_rank_dict(const uint32_t *bits, int32_t len, const uint16_t *vals, c
     if (dict_cap < rank_values_off((uint32_t)words)) fail(SDB_ERR_RUNTIME, "rank dictionary buffer too small");
     const uint32_t cap_vals = (dict_cap - rank_values_off((uint32_t)words)) / 2u;
     // the bitmap kernel's blocks each counted one run of 32 words when its
-    // grid covers the table without wrapping
+    // grid covers the table without wrapping (below 2^21 slots; dictionaries
+    // are built for tables far smaller)
     static_assert(64 * BITS_GROUPS * (BLOCK / 64) == 32 * RANK_PART_WORDS, "one bitmap block per 32 words");
     const uint32_t ranges = (uint32_t)(((int64_t)words + RANK_PART_WORDS - 1) / RANK_PART_WORDS);
-    if (partials && ranges == bits_partials(len)) {
-        hipLaunchKernelGGL(k_rank_dict_parts, dim3(ranges), dim3(64), 0, s, bits, words, partials, vals, (uint32_t)len,
-                           dict, cap_vals);
-    } else {
-        hipLaunchKernelGGL(k_rank_dict, dim3(1), dim3(RANK_BLOCK), 0, s, bits, words, vals, (uint32_t)len, dict,
-                           cap_vals);
-    }
+    if (!partials || ranges != bits_partials(len))
+        fail(SDB_ERR_RUNTIME, "rank dictionary: the bitmap's per-block counts do not cover 32 words each");
+    hipLaunchKernelGGL(k_rank_dict_parts, dim3(ranges), dim3(64), 0, s, bits, words, partials, vals, (uint32_t)len,
+                       dict, cap_vals);
     check_launch("rank_dict");
 }
 
