@@ -729,9 +729,8 @@ __global__ void __launch_bounds__(BLOCK) k_seen_to_flags(const uint32_t *seen, u
 }
 
 // ------------------------------------------------------ compact probe tables
-// One thread per 32-slot bitmap word; deterministic (no atomics on the
-// tables).  The population count is left per block and summed when the
-// statistics are read (k_stats_reduce).
+// Deterministic (no atomics on the tables).  The population count is left
+// per block and summed when the statistics are read (k_stats_reduce).
 
 // Stores the block's count (the sum of every wave's, held by its lane 0) in
 // partials[blockIdx.x].  One same-address atomic per block instead
