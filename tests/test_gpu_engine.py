@@ -354,3 +354,39 @@ def test_exit_while_kernels_compile():
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240)
     assert out.returncode == 0, (out.returncode, out.stderr[-3000:])
     assert out.stdout.split()[0] == "ok", out.stdout
+
+
+@pytest.mark.gpu
+def test_semi_joins_with_exact_statistics():
+    """SDB_SEMI_STATS=1 (read once per process, hence a child process): semi
+    joins read their key tables' set-slot counts instead of the build side's
+    estimate; the q1.x plans' results stay those of the plan oracle."""
+    import json
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    variants = [dict(date=("year", 1993)), dict(date=("yearmonthnum", 199401), disc=(4, 6), qty=("between", 26, 35)),
+                dict(date=("year_range", 1994, 1995), disc=(5, 7), qty=("between", 26, 35))]
+    code = (
+        "import sys, json; sys.path.insert(0, %r)\n"
+        "import torch\n"
+        "from sycldb_amd import plans\n"
+        "from sycldb_amd.engine import Engine\n"
+        "out = []\n"
+        "with Engine() as eng:\n"
+        "    eng.generate(300_001, 11)\n"
+        "    for kw in json.loads(sys.argv[1]):\n"
+        "        kw = {k: tuple(v) for k, v in kw.items()}\n"
+        "        out.append([list(map(int, r)) for r in eng.execute(plans.t1_plan(**kw)).rows()])\n"
+        "print(json.dumps(out), flush=True)\n"
+    ) % root
+    env = dict(os.environ, SDB_SEMI_STATS="1")
+    print("semi joins with exact statistics: child process", flush=True)
+    out = subprocess.run([sys.executable, "-c", code, json.dumps(variants)], capture_output=True, text=True,
+                         timeout=240, env=env)
+    assert out.returncode == 0, out.stderr[-3000:]
+    got = json.loads(out.stdout.strip().splitlines()[-1])
+    tables = datagen.generate_tables(S, 300_001, 11)
+    for kw, rows in zip(variants, got):
+        want = [list(map(int, r)) for r in PO.execute_plan(PL.t1_plan(**kw), tables).rows()]
+        assert rows == want, kw
