@@ -217,13 +217,17 @@ static int device_checks()
 int main(int argc, char **argv)
 {
     const std::string mode = argc > 1 ? argv[1] : "host";
+    int rc = 0;
     try {
-        if (host_checks()) return 1;
-        if (mode == "device" && device_checks()) return 1;
+        if (host_checks()) rc = 1;
+        else if (mode == "device" && device_checks()) rc = 1;
     } catch (const std::exception &e) {
         std::fprintf(stderr, "exception: %s\n", e.what());
-        return 1;
+        rc = 1;
     }
-    std::printf("ok\n");
-    return 0;
+    // the bundle entry may have queued kernels with the background compiler:
+    // it is stopped before the process exits
+    sdb_codegen_shutdown();
+    if (rc == 0) std::printf("ok\n");
+    return rc;
 }
