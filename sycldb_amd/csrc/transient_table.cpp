@@ -37,12 +37,13 @@ int32_t to_int32(int64_t v) { return (int32_t)(uint32_t)(uint64_t)v; }
 
 float clamp01(double x) { return (float)(x < 0 ? 0 : (x > 1 ? 1 : x)); }
 
-// SDB_SEMI_STATS=1: semi joins read their key tables' statistics too (one
-// stream sync) instead of taking the build side's estimated selectivity.
-bool semi_estimates()
+// SDB_JOIN_STATS=1: joins read their compact tables' statistics (one stream
+// sync per plan) instead of taking the build side's estimated selectivity
+// and bounds.
+bool join_estimates()
 {
     static const bool on = [] {
-        const char *e = getenv("SDB_SEMI_STATS");
+        const char *e = getenv("SDB_JOIN_STATS");
         return !(e && *e == '1');
     }();
     return on;
@@ -576,9 +577,31 @@ std::tuple<int32_t *, int32_t, int32_t> TransientTable::build_key_vals_hash_tabl
 // Results never depend on them: a bit is set exactly when the reference
 // table reports a match.  Only the launches happen here; the statistics are
 // read on first use (Engine::resolve_compact).
-// A key table (semi join) takes `est_sel` (>= 0) as its selectivity, with no
-// statistics read: the value only orders fused predicates.  A pair table's
-// statistics are read (its value layout and dictionary depend on them).
+// The value layout of a pair table, from its number of set slots (or a bound
+// on it) and its overflow flag: the 16-bit values when they fit, with their
+// rank dictionary staged for `set` values when that fits its buffer.
+static void settle_values(CompactState &c, uint32_t set, uint32_t overflow)
+{
+    if (c.narrow && overflow == 0) {
+        c.vals = c.narrow;
+        c.vkind = FV_U16;
+        const uint32_t dict_bytes = (rank_values_off((uint32_t)c.words) + 2u * set + 15u) / 16u * 16u;
+        if (c.dict && set <= 0xFFFFu && dict_bytes <= c.dict_cap) {
+            c.rank = c.dict;
+            c.rank_bytes = dict_bytes;
+        }
+    } else {
+        c.vals = c.ht;
+        c.vkind = FV_PAIR;
+    }
+}
+
+// With `est_sel` >= 0 the table is settled at once, with no statistics read:
+// its selectivity (which only orders fused predicates) is the estimate, and
+// a pair table's layout needs no count.  Its 16-bit values are taken only
+// when the group-by column's range fits them (vmax - vmin < 0xFFFF), so no
+// value overflows; its set slots are at most the build rows (each inserts one
+// key), so a dictionary sized for that many values holds them all.
 static std::shared_ptr<CompactState> build_compact(Engine &eng, const void *ht, bool pairs, int32_t ht_min,
                                                    int32_t ht_max, int32_t vmin, int32_t vmax, uint64_t build_rows,
                                                    float est_sel = -1.0f)
@@ -614,8 +637,9 @@ static std::shared_ptr<CompactState> build_compact(Engine &eng, const void *ht, 
         launch_rank_dict(bits, (int32_t)len, vals, stats + 1, c->dict, c->dict_cap, eng.stream);
         eng.launches++;
     }
-    if (!pairs && est_sel >= 0.0f) {
+    if (est_sel >= 0.0f) {
         c->sel = std::min(1.0f, est_sel);
+        if (pairs) settle_values(*c, (uint32_t)std::min<uint64_t>(build_rows, (uint64_t)len), 0);
         c->resolved = true;
     } else {
         eng.pending_compact.push_back(c);
@@ -659,20 +683,7 @@ void Engine::resolve_compact()
         // fraction of the build rows selected: the share of uniformly drawn
         // foreign keys that find a match
         c.sel = std::min(1.0f, (float)set / (float)std::max<uint64_t>(c.build_rows, 1));
-        if (c.pairs) {
-            if (c.narrow && overflow == 0) {
-                c.vals = c.narrow;
-                c.vkind = FV_U16;
-                const uint32_t dict_bytes = (rank_values_off((uint32_t)c.words) + 2u * set + 15u) / 16u * 16u;
-                if (c.dict && set <= 0xFFFFu && dict_bytes <= c.dict_cap) {
-                    c.rank = c.dict;
-                    c.rank_bytes = dict_bytes;
-                }
-            } else {
-                c.vals = c.ht;
-                c.vkind = FV_PAIR;
-            }
-        }
+        if (c.pairs) settle_values(c, set, overflow);
         c.resolved = true;
     }
 }
@@ -694,7 +705,7 @@ void TransientTable::apply_join(TransientTable &right, const RelNode &rel)
         std::shared_ptr<CompactState> ct;
         if (eng_.fusion)
             ct = build_compact(eng_, std::get<0>(ht), false, std::get<1>(ht), std::get<2>(ht), 0, 0, right.nrows_,
-                               semi_estimates() ? right.est_kept_ : -1.0f);
+                               join_estimates() ? right.est_kept_ : -1.0f);
         std::vector<KernelBundle> ops(segment_num());
         for (uint64_t s = 0; s < segment_num(); s++) {
             const uint64_t off = s * eng_.segment_size;
@@ -731,7 +742,7 @@ void TransientTable::apply_join(TransientTable &right, const RelNode &rel)
     std::shared_ptr<CompactState> ct;
     if (eng_.fusion)
         ct = build_compact(eng_, std::get<0>(ht), true, std::get<1>(ht), std::get<2>(ht), gb_mm.first, gb_mm.second,
-                           right.nrows_);
+                           right.nrows_, join_estimates() ? right.est_kept_ : -1.0f);
     std::vector<KernelBundle> ops(segment_num());
     for (uint64_t s = 0; s < segment_num(); s++) {
         const uint64_t off = s * eng_.segment_size;

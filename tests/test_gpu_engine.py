@@ -357,16 +357,20 @@ def test_exit_while_kernels_compile():
 
 
 @pytest.mark.gpu
-def test_semi_joins_with_exact_statistics():
-    """SDB_SEMI_STATS=1 (read once per process, hence a child process): semi
-    joins read their key tables' set-slot counts instead of the build side's
-    estimate; the q1.x plans' results stay those of the plan oracle."""
+def test_joins_with_exact_statistics():
+    """SDB_JOIN_STATS=1 (read once per process, hence a child process): joins
+    read their compact tables' set-slot counts (one stream sync) instead of
+    the build side's estimate and bounds; semi-join (q1.x) and full-join
+    (q2.x, q3.x) plans keep the plan oracle's results."""
     import json
     import subprocess
     import sys
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    variants = [dict(date=("year", 1993)), dict(date=("yearmonthnum", 199401), disc=(4, 6), qty=("between", 26, 35)),
-                dict(date=("year_range", 1994, 1995), disc=(5, 7), qty=("between", 26, 35))]
+    variants = [("t1", dict(date=("year", 1993))),
+                ("t1", dict(date=("yearmonthnum", 199401), disc=(4, 6), qty=("between", 26, 35))),
+                ("t1", dict(date=("year_range", 1994, 1995), disc=(5, 7), qty=("between", 26, 35))),
+                ("t2", dict(part=("category", 3), s_region=2)),
+                ("t3", dict(cust=("nation", 7), supp=("nation", 7), date=("year_range", 1992, 1997)))]
     code = (
         "import sys, json; sys.path.insert(0, %r)\n"
         "import torch\n"
@@ -375,18 +379,18 @@ def test_semi_joins_with_exact_statistics():
         "out = []\n"
         "with Engine() as eng:\n"
         "    eng.generate(300_001, 11)\n"
-        "    for kw in json.loads(sys.argv[1]):\n"
-        "        kw = {k: tuple(v) for k, v in kw.items()}\n"
-        "        out.append([list(map(int, r)) for r in eng.execute(plans.t1_plan(**kw)).rows()])\n"
+        "    for t, kw in json.loads(sys.argv[1]):\n"
+        "        kw = {k: tuple(v) if isinstance(v, list) else v for k, v in kw.items()}\n"
+        "        out.append([list(map(int, r)) for r in eng.execute(plans.build(t, **kw)).rows()])\n"
         "print(json.dumps(out), flush=True)\n"
     ) % root
-    env = dict(os.environ, SDB_SEMI_STATS="1")
-    print("semi joins with exact statistics: child process", flush=True)
+    env = dict(os.environ, SDB_JOIN_STATS="1")
+    print("joins with exact statistics: child process", flush=True)
     out = subprocess.run([sys.executable, "-c", code, json.dumps(variants)], capture_output=True, text=True,
                          timeout=240, env=env)
     assert out.returncode == 0, out.stderr[-3000:]
     got = json.loads(out.stdout.strip().splitlines()[-1])
     tables = datagen.generate_tables(S, 300_001, 11)
-    for kw, rows in zip(variants, got):
-        want = [list(map(int, r)) for r in PO.execute_plan(PL.t1_plan(**kw), tables).rows()]
-        assert rows == want, kw
+    for (t, kw), rows in zip(variants, got):
+        want = [list(map(int, r)) for r in PO.execute_plan(PL.build(t, **kw), tables).rows()]
+        assert rows == want, (t, kw)
