@@ -81,6 +81,15 @@ def test_plan_json_reading():
             E.dag_order(bad)
 
 
+def test_codegen_shutdown_without_a_compiler():
+    """The Python package stops the background compiler at every interpreter
+    exit: with none started (no device here) the call does nothing and
+    succeeds, also twice."""
+    assert L.lib.sdb_codegen_shutdown() == 0
+    assert L.lib.sdb_codegen_shutdown() == 0
+    assert L.lib.sdb_codegen_pending() == 0
+
+
 def test_version_and_last_error():
     assert b"sycldb" in L.lib.sdb_version()
     assert isinstance(L.lib.sdb_last_error(), bytes)
